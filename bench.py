@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Ingest benchmark of the HIP RemoLOD path on one MI355X.
+
+A step is one pass of the hot path over one batch: Update (the octree build of
+one 1,000,000-point batch), Accumulate and Analysis. The workload is the
+100,000,000-point seed-generated lidar-like scene (ground 40 x 40 m with 2 cm
+noise, 20 boxes, 50 cylinders, scan-line order, seed 2) in 100 batches of
+1,000,000, resident in HBM before timing starts. `value` is ingested points per
+second over the K timed steps (steps run from batch 0 of a fresh tree).
+
+Prints exactly one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "M pts/s ingest (Update+Accumulate+Analysis), plain vs learned score; % HBM BW"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(scope: str, t: dict) -> float:
+    """Compulsory HBM bytes of one scope over the timed steps.
+
+    t: totals over the timed region -- points (batch points ingested), spilled,
+    voxels (new voxels), splits, rounds (count launches), folded (points folded by
+    Accumulate), nodes (tree size at the end).
+    """
+    pts, sp, vox = t["points"], t["spilled"], t["voxels"]
+    if scope == "remolod.construct.count":
+        # round 0 reads every batch point once; later rounds read batch + spill again
+        extra = max(0, t["rounds"] - t["steps"])
+        return 16.0 * (pts + extra * (pts / max(1, t["steps"]) + sp / max(1, t["steps"])))
+    if scope == "remolod.construct.split":
+        return 262144.0 * t["splits"] + 32.0 * sp
+    if scope == "remolod.construct.voxelSampling":
+        return 16.0 * (pts + sp) + 20.0 * vox
+    if scope == "remolod.construct.insertPoints":
+        return 32.0 * (pts + sp)
+    if scope == "remolod.construct.insertVoxels":
+        return 36.0 * vox
+    if scope == "remolod.construct.allocChunks":
+        return 64.0 * t["nodes"] * t["steps"]
+    if scope == "remolod.accumulate":
+        return 16.0 * t["folded"] + 192.0 * t["nodes"] * t["steps"]
+    if scope == "remolod.analysis":
+        return 400.0 * t["nodes"] * t["steps"]
+    return 0.0
+
+
+def cpu_baseline(points, meta, batch: int, sample_batches: int) -> dict:
+    """The CPU oracle (a serial transcription of the reference kernels) on the
+    first `sample_batches` batches of the same workload."""
+    import numpy as np
+
+    from oracle.oracle import OracleTree
+
+    n = min(len(points), sample_batches * batch)
+    nb = (n + batch - 1) // batch
+    sizes = np.array([min(batch, n - i * batch) for i in range(nb)], dtype=np.uint32)
+    o = OracleTree(meta.box_size, batches_per_step=1)
+    slots = points[:n]
+    t0 = time.perf_counter()
+    done = 0
+    while done < nb:
+        done = o.build_step(slots, sizes, nb, batch, nb)
+        o.accumulate(done)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": round(n / dt / 1e6, 3), "unit": "M pts/s", "cores": 1, "kind": "port",
+            "sample": f"first {nb} of the workload's batches ({n:,} points), Update+Accumulate, "
+                      f"serial C++ oracle, {dt:.1f} s on {platform.processor() or 'host CPU'}"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=100_000_000)
+    ap.add_argument("--batch", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--scene", default="lidar")
+    ap.add_argument("--cpu-sample-batches", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--analysis", choices=["analytic", "learned", "off"], default="analytic")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        log("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
+            "in DESIGN.md); run with --gpus 1")
+        return 2
+
+    from remobench_amd import lib as L
+    from remobench_amd.pipeline import PointRing, Tree
+
+    if L.device_count() < 1:
+        log("no GPU visible")
+        return 1
+
+    t = time.perf_counter()
+    threads = min(16, os.cpu_count() or 1)
+    points, meta = L.generate_scene(args.points, args.seed, args.scene, threads=threads)
+    nb = (len(points) + args.batch - 1) // args.batch
+    log(f"scene: {len(points):,} points, {nb} batches, box {list(meta.box_size)} "
+        f"({time.perf_counter() - t:.1f} s)")
+    steps = args.steps
+    if steps > nb:
+        log(f"--steps {steps} > {nb} batches in the scene: timing {nb}")
+        steps = nb
+
+    ring = PointRing(args.batch, nb)
+    ring.set_cloud(points, args.batch)
+    ring.set_batches_consumed(0)
+    ring.pump()
+    view = ring.view()
+    tree = Tree(meta, slot_capacity=args.batch, ring_slots=nb, batches_per_step=1,
+                step_budget_ms=-1.0)
+    log(f"uploaded; tree context created ({time.perf_counter() - t:.1f} s)")
+
+    uniforms = L.SharedUniforms()
+    analysis_mode = {"analytic": L.REMO_ANALYSIS_ANALYTIC, "learned": L.REMO_ANALYSIS_LEARNED}.get(
+        args.analysis)
+    analysis_ok = analysis_mode is not None
+
+    def step():
+        nonlocal analysis_ok
+        _, consumed = tree.build_step(view)
+        tree.accumulate(consumed)
+        if analysis_ok:
+            try:
+                tree.analysis(uniforms, analysis_mode)
+            except L.RemoError as e:
+                if e.code != -5:
+                    raise
+                analysis_ok = False
+        return consumed
+
+    for i in range(min(args.warmup, nb)):
+        step()
+    tree.sync()
+    tree.reset()
+    tree.profile_enable(True)
+    tree.profile_clear()
+    tree.sync()
+
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step()
+    tree.sync()
+    dt = time.perf_counter() - t0
+    st = tree.stats()
+    prof = tree.profile()
+    ingested = st["num_points_processed"]
+    value = ingested / dt / 1e6
+    log(f"timed {steps} steps: {dt * 1e3:.1f} ms, {value:.1f} M pts/s; nodes {st['num_nodes']}, "
+        f"voxels {st['num_voxels']:,}")
+
+    totals = {"points": ingested, "spilled": st["total_spilled_points"],
+              "voxels": st["total_new_voxels"], "splits": st["total_nodes_split"],
+              "rounds": st["total_expand_iters"], "folded": st["num_accumulated"],
+              "nodes": st["num_nodes"], "steps": steps}
+    scopes = {}
+    for name, (ms, launches) in prof.items():
+        if launches:
+            scopes[name] = {"ms": round(ms, 3), "launches": launches,
+                            "avg_us": round(1e3 * ms / launches, 2),
+                            "algorithmic_bytes": algorithmic_bytes(name, totals)}
+    dominant = max(scopes, key=lambda k: scopes[k]["ms"]) if scopes else None
+    roofline = None
+    if dominant:
+        d = scopes[dominant]
+        per_launch = d["algorithmic_bytes"] / d["launches"]
+        achieved = per_launch / (d["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": None, "bytes_per_launch": round(per_launch)}
+    path_bytes = sum(s["algorithmic_bytes"] for s in scopes.values())
+    path_gbs = path_bytes / dt / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle on {args.cpu_sample_batches} batches")
+        cpu = cpu_baseline(points, meta, args.batch, args.cpu_sample_batches)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "M pts/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u32+int64",
+        "data": f"synthetic: seed-generated lidar-like scene (seed {args.seed}), resident in HBM",
+        "config": {
+            "workload": f"{args.scene}-{len(points) // 1_000_000}M: {len(points):,} points in {nb} "
+                        f"batches of {args.batch:,}; Update+Accumulate"
+                        + ("+Analysis (analytic)" if analysis_ok and args.analysis == "analytic" else
+                           "+Analysis (learned)" if analysis_ok else ""),
+            "points": len(points), "batch": args.batch, "batches_timed": steps,
+            "analysis": args.analysis if analysis_ok else "unavailable",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "path_hbm_gbs": round(path_gbs, 2),
+        "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 5),
+        "scopes": scopes,
+        "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points",
+                                    "num_voxels", "max_depth", "allocated_bytes_persistent")},
+    }
+    print(json.dumps(out), flush=True)
+    tree.close()
+    ring.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,299 @@
+/*
+ * remolod.h -- C ABI of the MI355X-native RemoLOD progressive-LOD path.
+ *
+ * One context = one octree on one GPU, externally synchronised (one host thread
+ * at a time), every call stream-ordered on the context's own HIP stream. No C++
+ * types, no HIP types and no exceptions cross this boundary: device buffers are
+ * plain pointers obtained from remo_dev_alloc() (or any hipMalloc'd memory).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to
+ * the RemoBench repository).
+ */
+#ifndef REMOLOD_H
+#define REMOLOD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* The reference reports init/allocate failure as bool + std::string*
+ * (include/remo/ILodPipeline.h:124-127) and calls reportDeadContextAndExit(70)
+ * on a sticky device fault (src/cuda/CudaCheck.cpp:8-35). Here every call returns
+ * one of these; a sticky fault is REMO_ERR_DEVICE and the context is dead. */
+#define REMO_OK              0
+#define REMO_DONE            1   /* build_step: every batch of the cloud consumed  */
+#define REMO_ERR_INVALID    -1   /* bad argument or shape                          */
+#define REMO_ERR_OOM        -2   /* device allocation failed / budget too small    */
+#define REMO_ERR_DEVICE     -3   /* sticky device fault; the context is dead       */
+#define REMO_ERR_RING       -4   /* ring addressing does not match the context     */
+#define REMO_ERR_STATE      -5   /* call out of lifecycle order                    */
+#define REMO_ERR_CAPACITY   -6   /* a fixed capacity (spill, backlog, dir) was hit */
+
+/* ---- constants of the path (kernels/remolod/remolod_structures.cuh:12-25) -- */
+#define REMO_MAX_POINTS_PER_NODE   50000u
+#define REMO_POINTS_PER_CHUNK      1000u
+#define REMO_GRID_SIZE             128u
+#define REMO_GRID_WORDS            65536u     /* 128^3 bits / 32 */
+#define REMO_MAX_DEPTH             20u
+#define REMO_DEFAULT_MAX_NODES     200000u
+#define REMO_DEFAULT_RING_SLOTS    50u
+#define REMO_DEFAULT_SLOT_CAPACITY 1000000u
+
+/* ---- plain data -------------------------------------------------------- */
+
+/* == remo::Point (include/remo/HostDeviceCommon.h:15-18): 16 B AoS,
+ *    color = r | g<<8 | b<<16 | a<<24. */
+typedef struct remo_point {
+    float x, y, z;
+    uint32_t color;
+} remo_point;
+
+/* == remo::mat4 (include/remo/HostDeviceCommon.h:20-22): row-major rows[r][c]. */
+typedef struct remo_mat4 {
+    float rows[4][4];
+} remo_mat4;
+
+typedef struct remo_vec3f {
+    float x, y, z;
+} remo_vec3f;
+
+/* == remo::SharedUniforms (include/remo/HostDeviceCommon.h:28-60), same layout. */
+typedef struct remo_shared_uniforms {
+    float width, height, fovy_rad, time;
+    remo_mat4 transform, view, proj;
+    remo_mat4 transform_frozen, transform_frozen_inv;
+    remo_vec3f box_min, box_max;
+    uint64_t frame_counter;
+    float lod_pixel_budget, min_node_size, lod_scale;
+    int32_t point_size, color_mode;
+    float edl_strength;
+    int32_t do_update_visibility, use_high_quality_shading, enable_edl;
+    int32_t show_bounding_box, show_points, pad0;
+} remo_shared_uniforms;
+
+/* CloudMeta subset the path uses (include/remo/PointSource.h:16-32). Points are
+ * already translated so the box minimum is 0 (src/io/PointSource.cpp:360-379). */
+typedef struct remo_cloud_meta {
+    uint64_t num_points;
+    double translation[3];
+    float box_size[3];
+    uint32_t pad0;
+} remo_cloud_meta;
+
+/* == remo::BatchView (include/remo/PointSource.h:43-67). Device pointers.
+ * Batch B lives in slot B % num_slots at slots + slot * slot_capacity points,
+ * with batch_sizes[slot] valid points; *num_batches_uploaded is the producer
+ * cursor. */
+typedef struct remo_batch_view {
+    const remo_point* slots;
+    const uint32_t* batch_sizes;
+    const uint32_t* num_batches_uploaded;
+    uint32_t slot_capacity;
+    uint32_t num_slots;
+    uint32_t num_batches_total;
+    uint32_t pad0;
+} remo_batch_view;
+
+/* Sizing and policy of one context: what RemolodPipeline::allocate derives from
+ * CloudMeta + DeviceBudget (src/pipelines/RemolodPipeline.cpp:235-325). Zero
+ * fields take the reference defaults. */
+typedef struct remo_config {
+    uint64_t persistent_bytes;      /* octree store (grids + chunks); 0: 48 B/pt, >= 512 MiB */
+    uint32_t max_nodes;             /* node pool, MAX_NODES_CAPACITY (200,000)            */
+    uint32_t max_batches_per_step;  /* batches folded per build_step (20)                 */
+    float    step_budget_ms;        /* device-time budget per build_step (10); < 0: none   */
+    uint32_t slot_capacity;         /* ring slot stride in points (1,000,000)             */
+    uint32_t ring_slots;            /* ring depth the batch index wraps at (50)           */
+    uint32_t spill_capacity;        /* spilled points per batch (10,000,000)              */
+    uint32_t backlog_capacity;      /* new voxels per batch (10,000,000)                  */
+    uint32_t flags;                 /* REMO_FLAG_*                                        */
+} remo_config;
+
+#define REMO_FLAG_NO_ACCUM   1u     /* --remolod-no-accum                       */
+
+/* == Stats (kernels/simlod/HostDeviceInterface.h:51-75) + PipelineStats
+ * (include/remo/ILodPipeline.h:88-111) + the DeviceTimeline counter block
+ * (include/remo/HostDeviceCommon.h:102-166) + AccumGlobals
+ * (include/remo/RemoAccum.h:33-40) + DeviceDiagnostics. */
+typedef struct remo_stats {
+    uint32_t num_nodes, num_inner, num_leaves, num_nonempty_leaves;
+    uint64_t num_points;            /* sum of leaf numPoints      */
+    uint64_t num_voxels;            /* sum of inner-node numVoxels */
+    uint64_t num_points_processed;
+    uint32_t batchlet_index;
+    uint32_t num_chunks_points, num_chunks_voxels;
+    uint32_t max_points_per_node;
+    uint32_t counter_underflows;    /* leaves with counter < numPoints */
+    uint32_t max_depth;
+    uint64_t num_allocated_chunks, chunk_pool_size;
+    uint64_t allocated_bytes_persistent, persistent_capacity;
+    uint32_t mem_capacity_reached, node_capacity_reached, node_pool_overflow, alloc_overflow;
+    uint32_t spill_overflow, backlog_overflow, dir_overflow, pad0;
+    /* per-step counters (DeviceTimeline counter block semantics, summed over launches) */
+    uint64_t batches, expand_iters, nodes_split, spilled_points, new_voxels;
+    /* the same five, cumulative since the last reset */
+    uint64_t total_batches, total_expand_iters, total_nodes_split, total_spilled_points, total_new_voxels;
+    /* AccumGlobals */
+    uint64_t morton_watermark, num_accumulated, sum_leaf_counts;
+    uint32_t points_folded, leaves_folded, inner_with_sums, pad1;
+    /* render diagnostics (DeviceDiagnostics) */
+    uint32_t draw_items, draw_list_overflow;
+    uint64_t draw_samples;
+} remo_stats;
+
+/* Canonical node record for dumps. Pool index order is not part of parity;
+ * compare by (level, x, y, z). */
+typedef struct remo_node_record {
+    uint32_t level, x, y, z;
+    int32_t first_child;            /* -1: leaf; else children are first_child + 0..7,
+                                       child i = (cx<<2)|(cy<<1)|cz */
+    uint32_t counter, num_points, num_voxels, num_voxels_stored;
+    int32_t grid;                   /* occupancy-grid number, -1: none */
+    uint32_t pad0, pad1;
+} remo_node_record;
+
+/* Per-node Accumulate record. g/c/count/last_touched_batch/state/score have the
+ * meaning of remo::NodeAccum (include/remo/RemoAccum.h:18-30); q1/q2 are the
+ * exact sums of q = floor(l * 2^20) - 2^19, l the node-local coordinate in
+ * [0,1] (node-centred fixed point, 2^-20 of the node extent). */
+typedef struct remo_node_accum {
+    int64_t q1[3];                  /* sum qx, qy, qz                    */
+    int64_t q2[6];                  /* sum qxqx qxqy qxqz qyqy qyqz qzqz */
+    double g[9];                    /* FP64 sums x y z xx xy xz yy yz zz */
+    float c[6];                     /* FP32 colour sums r g b rr gg bb   */
+    uint32_t count, last_touched_batch, state;
+    float score;
+    uint64_t morton_max;
+} remo_node_accum;
+
+/* Per-node Analysis result. Eigenvalues ascending, of the covariance in node
+ * extent units; planarity = (l1 - l0) / l2, linearity = (l2 - l1) / l2,
+ * surface variation = l0 / (l0 + l1 + l2). */
+typedef struct remo_node_analysis {
+    float lambda[3];
+    float surface_variation, planarity, linearity;
+    float coverage;                 /* projected extent, pixels^2 (clamped to the screen) */
+    float score;
+    uint64_t n;                     /* points under the node (rolled up) */
+    uint32_t state;                 /* 0 open, 1 closed, 2 finalized     */
+    uint32_t flags;                 /* bit0: FP64 fallback used; bit1: learned score used */
+} remo_node_analysis;
+
+#define REMO_ANALYSIS_ANALYTIC 0
+#define REMO_ANALYSIS_LEARNED  1
+
+typedef struct remo_ctx remo_ctx;
+typedef struct remo_ring remo_ring;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* ILodPipeline::initPrograms + allocate (include/remo/ILodPipeline.h:124-127,
+ * src/pipelines/RemolodPipeline.cpp:194-325): code objects are linked ahead of
+ * time, so this only allocates every device buffer. */
+int remo_create(const remo_config* cfg, const remo_cloud_meta* meta, remo_ctx** out);
+
+/* ILodPipeline::release (src/pipelines/RemolodPipeline.cpp:327-342). */
+void remo_destroy(remo_ctx* ctx);
+
+/* ILodPipeline::reset + the reset kernel (kernels/remolod/remolod_reset.cu:23-88,
+ * src/pipelines/RemolodPipeline.cpp:340-389): empty tree, root with a zeroed grid,
+ * zeroed accumulators. The caller rewinds its source, as the reference does. */
+int remo_reset(remo_ctx* ctx);
+
+/* ILodPipeline::build -> kernel_construct (kernels/remolod/remolod_octree.cu:1062-1320):
+ * folds batches [batchletIndex, min(+max_batches_per_step, uploaded)) from the ring,
+ * stopping early on the step budget or the persistent-memory margin. Returns REMO_OK
+ * while batches remain, REMO_DONE once num_batches_total are consumed. */
+int remo_build_step(remo_ctx* ctx, const remo_batch_view* view, uint32_t* batches_consumed);
+
+/* runAccumulator -> kernel_accumulate + kernel_accum_verify
+ * (src/pipelines/RemolodPipeline.cpp:242-294, kernels/remolod/remolod_accum.cu:47-233). */
+int remo_accumulate(remo_ctx* ctx, uint32_t batch_index);
+
+/* Analysis (README.md:80, wiki/02_KernelPipeline.md:205-224; no reference code):
+ * roll-up of the exact sums, 3x3 eigen-solve, score = geometric x coverage,
+ * work queue. mode: REMO_ANALYSIS_ANALYTIC or REMO_ANALYSIS_LEARNED. */
+int remo_analysis(remo_ctx* ctx, const remo_shared_uniforms* u, int mode);
+
+/* ILodPipeline::render -> kernel_render selection + shared rasteriser
+ * (kernels/remolod/remolod_render.cu:72-181, kernels/shared/remo_draw.cuh:118-167,
+ * kernels/shared/remo_framebuffer.cuh:214-248). fb: device u64[width*height],
+ * cleared and filled with (float_bits(depth) << 32) | rgba. */
+int remo_render(remo_ctx* ctx, const remo_shared_uniforms* u, uint64_t* fb);
+
+/* ILodPipeline::stats / diagnostics (src/pipelines/RemolodPipeline.cpp:440-470, :603-726). */
+int remo_get_stats(remo_ctx* ctx, remo_stats* out);
+const char* remo_last_error(const remo_ctx* ctx);
+int remo_sync(remo_ctx* ctx);
+
+/* ---- timing scopes (GpuProfiler / GpuScope, include/remo/GpuProfiler.h:62-158) ----
+ * When enabled, every phase launch is bracketed by a pair of HIP events on the
+ * context's stream; remo_profile_read harvests finished pairs into per-scope
+ * totals. Scope names follow the reference's (src/pipelines/RemolodPipeline.cpp:474-479):
+ * remolod.construct.{count,split,voxelSampling,allocChunks,insertPoints,insertVoxels},
+ * remolod.accumulate, remolod.analysis, remolod.render. */
+typedef struct remo_profile_entry {
+    char name[48];
+    double total_ms;
+    uint64_t launches;
+} remo_profile_entry;
+
+int remo_profile_enable(remo_ctx* ctx, int on);
+int remo_profile_read(remo_ctx* ctx, remo_profile_entry* out, uint32_t cap, uint32_t* num);
+int remo_profile_clear(remo_ctx* ctx);
+
+/* ---- canonical dumps (parity; host buffers) ----------------------------- */
+int remo_dump_nodes(remo_ctx* ctx, remo_node_record* out, uint32_t cap, uint32_t* num);
+/* Points of node `node` in storage order (leaf points, or voxels when voxels != 0). */
+int remo_dump_node_samples(remo_ctx* ctx, uint32_t node, int voxels,
+                           remo_point* out, uint32_t cap, uint32_t* num);
+/* The 65,536-word occupancy grid of node `node` (REMO_ERR_INVALID if it has none). */
+int remo_dump_grid(remo_ctx* ctx, uint32_t node, uint32_t* out_words);
+int remo_dump_accums(remo_ctx* ctx, remo_node_accum* out, uint32_t cap, uint32_t* num);
+int remo_dump_analysis(remo_ctx* ctx, remo_node_analysis* out, uint32_t cap, uint32_t* num);
+
+/* ---- the input ring (PointSource, include/remo/PointSource.h:69-115) ---- */
+int remo_ring_create(uint32_t slot_capacity, uint32_t num_slots, remo_ring** out);
+void remo_ring_destroy(remo_ring* ring);
+/* Synchronous upload of batch `batch_index` into slot batch_index % num_slots;
+ * advances the uploaded cursor to batch_index + 1 if it was below. */
+int remo_ring_upload(remo_ring* ring, uint32_t batch_index, const remo_point* host, uint32_t n);
+int remo_ring_view(const remo_ring* ring, uint32_t num_batches_total, remo_batch_view* out);
+/* PointSource::rewind: uploaded cursor back to 0. */
+int remo_ring_rewind(remo_ring* ring);
+
+/* ---- device memory plumbing -------------------------------------------- */
+int remo_dev_alloc(size_t bytes, void** out);
+int remo_dev_free(void* p);
+int remo_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int remo_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int remo_device_count(int* out);
+
+/* ---- synthetic scenes (makeSyntheticSource's role, src/io/PointSource.cpp:535-584) ----
+ * Seed-generated lidar-like scene, scan-line ordered, translated to box min 0. */
+typedef struct remo_scene_spec {
+    double extent;                  /* ground square side, metres (40)          */
+    uint32_t num_boxes;             /* axis-aligned boxes, 2-10 m sides (20)    */
+    uint32_t num_cylinders;         /* r 0.15 m, h 6 m (50)                     */
+    double cluster_fraction;        /* share of points in Gaussian clusters (0) */
+    double cluster_volume_fraction; /* volume the clusters cover (0.05)         */
+    double cluster_sigma;           /* 0.5 m                                    */
+    double ground_sigma;            /* 0.02 m                                   */
+    double scanline;                /* scan-line pitch, metres (0.05)           */
+    uint32_t shuffle_window;        /* 4096                                     */
+    uint32_t threads;               /* 0: all hardware threads                  */
+} remo_scene_spec;
+
+void remo_scene_default(remo_scene_spec* spec);
+int remo_scene_generate(const remo_scene_spec* spec, uint64_t seed, uint64_t num_points,
+                        remo_point* out, remo_cloud_meta* meta);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

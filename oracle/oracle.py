@@ -1,0 +1,128 @@
+"""ctypes binding of the CPU oracle (oracle/remolod_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from remobench_amd.lib import ACCUM_DTYPE, NODE_DTYPE, POINT_DTYPE, Stats
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-s", "-C", str(HERE)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout.decode(errors="replace"))
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    sig = {
+        "oracle_create": (C.c_void_p, [P(C.c_float), C.c_uint32, C.c_uint64, C.c_uint32]),
+        "oracle_destroy": (None, [C.c_void_p]),
+        "oracle_reset": (None, [C.c_void_p]),
+        "oracle_build_step": (C.c_uint32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                           C.c_uint32, C.c_uint32]),
+        "oracle_accumulate": (None, [C.c_void_p, C.c_uint32]),
+        "oracle_stats": (None, [C.c_void_p, P(Stats)]),
+        "oracle_num_nodes": (C.c_uint32, [C.c_void_p]),
+        "oracle_dump_nodes": (None, [C.c_void_p, C.c_void_p]),
+        "oracle_node_samples": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_uint32]),
+        "oracle_grid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+        "oracle_dump_accums": (None, [C.c_void_p, C.c_void_p]),
+        "oracle_quantise": (None, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),
+        "oracle_morton": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class OracleTree:
+    """The reference's construct/accumulate path, restated on the CPU."""
+
+    def __init__(self, box_size, max_nodes=200_000, persistent_bytes=1 << 62, batches_per_step=20):
+        self.lib = load()
+        bs = (C.c_float * 3)(*[float(v) for v in box_size])
+        self.h = self.lib.oracle_create(bs, max_nodes, persistent_bytes, batches_per_step)
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def build_step(self, slots: np.ndarray, batch_sizes: np.ndarray, num_slots: int,
+                   slot_capacity: int, num_uploaded: int) -> int:
+        slots = np.ascontiguousarray(slots, dtype=POINT_DTYPE)
+        sizes = np.ascontiguousarray(batch_sizes, dtype=np.uint32)
+        return self.lib.oracle_build_step(self.h, slots.ctypes.data, sizes.ctypes.data, num_slots,
+                                          slot_capacity, num_uploaded)
+
+    def accumulate(self, batch_index: int):
+        self.lib.oracle_accumulate(self.h, batch_index)
+
+    def stats(self) -> dict:
+        s = Stats()
+        self.lib.oracle_stats(self.h, C.byref(s))
+        return s.as_dict()
+
+    def nodes(self) -> np.ndarray:
+        n = self.lib.oracle_num_nodes(self.h)
+        out = np.zeros(n, dtype=NODE_DTYPE)
+        self.lib.oracle_dump_nodes(self.h, out.ctypes.data)
+        return out
+
+    def samples(self, node: int, voxels: bool = False) -> np.ndarray:
+        n = self.lib.oracle_node_samples(self.h, node, int(voxels), None, 0)
+        out = np.zeros(n, dtype=POINT_DTYPE)
+        if n:
+            self.lib.oracle_node_samples(self.h, node, int(voxels), out.ctypes.data, n)
+        return out
+
+    def grid(self, node: int):
+        out = np.zeros(65536, dtype=np.uint32)
+        if self.lib.oracle_grid(self.h, node, out.ctypes.data) != 0:
+            return None
+        return out
+
+    def accums(self) -> np.ndarray:
+        n = self.lib.oracle_num_nodes(self.h)
+        out = np.zeros(n, dtype=ACCUM_DTYPE)
+        self.lib.oracle_dump_accums(self.h, out.ctypes.data)
+        return out
+
+
+def quantise(point, size: float):
+    p = np.zeros(1, dtype=POINT_DTYPE)
+    p[0] = point
+    q20 = np.zeros(3, dtype=np.uint32)
+    q28 = np.zeros(3, dtype=np.uint32)
+    load().oracle_quantise(p.ctypes.data, float(size), q20.ctypes.data, q28.ctypes.data)
+    return q20, q28
+
+
+def morton(x: int, y: int, z: int) -> int:
+    return int(load().oracle_morton(x, y, z))

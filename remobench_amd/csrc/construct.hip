@@ -1,0 +1,767 @@
+// Update: the per-batch progressive octree build on gfx950.
+//
+// Follows kernel_construct / addBatch (kernels/remolod/remolod_octree.cu:940-1320)
+// phase for phase -- count, split, voxel-sample, allocate chunks, insert points,
+// insert voxels -- with three changes that remove work without changing the
+// tree (node set, counters, occupied cells and per-leaf point multisets are
+// the reference's):
+//
+//  * Every leaf keeps an octant histogram of its stored points (storedOct). A
+//    split therefore knows its children's counters at once (stored + this
+//    batch's octant counts), so the reference's re-count of the spilled points
+//    (expand iterations 1..N, 21.5% of construct, wiki/03_ExpandStageFindings.md)
+//    only happens for a child that itself holds > 50,000 points.
+//  * Counting and inserting aggregate a wave64 by (leaf, octant) with
+//    ballot/readlane, one atomic per group (the reference's labeled_partition,
+//    remolod_octree.cu:205-219, for 64-wide waves).
+//  * Chunks are found through a per-node directory, not a list walk.
+//
+// Each phase is its own kernel on one stream; kernel boundaries are the
+// reference's grid.sync()s.
+
+#include "remolod_internal.h"
+
+namespace remo_hip {
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kPerThread = 4;
+constexpr uint32_t kTile = kBlock * kPerThread;
+
+// Point i of the batch-then-spill stream.
+__device__ __forceinline__ remo_point streamPoint(const DevTree& t, BatchRef b, uint32_t batchSize,
+                                                  uint32_t i) {
+    return i < batchSize ? b.points[i] : t.spill[i - batchSize];
+}
+
+// Spilled points actually held (the counter runs past the capacity on overflow).
+__device__ __forceinline__ uint32_t spilledCount(const DevTree& t) {
+    return min(t.g->numSpilled, t.spillCap);
+}
+
+__device__ __forceinline__ uint32_t batchSizeOf(const DevTree& t, BatchRef b) {
+    return t.g->stop ? 0u : *b.sizePtr;
+}
+
+// Wave-aggregated atomicAdd(base[key], 1) over the lanes with `valid`; returns
+// this lane's pre-increment value (base + rank in its group). Must be reached
+// by every lane of the wave.
+__device__ __forceinline__ uint32_t waveAggInc(uint32_t* base, uint32_t key, bool valid) {
+    uint64_t pend = __ballot(valid);
+    uint32_t out = 0;
+    const uint32_t lane = laneId();
+    while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const uint64_t grp = __ballot(valid && key == lk) & pend;
+        uint32_t old = 0;
+        if (lane == uint32_t(leader)) old = atomicAdd(&base[lk], uint32_t(__popcll(grp)));
+        old = __shfl(old, leader);
+        if ((grp >> lane) & 1ull) out = old + uint32_t(__popcll(grp & lanesBelow()));
+        pend &= ~grp;
+    }
+    return out;
+}
+
+// Wave-aggregated atomicAdd(&ctr, 1) on one global counter; returns the slot.
+__device__ __forceinline__ uint32_t waveAppend(uint32_t* ctr, bool valid) {
+    const uint64_t m = __ballot(valid);
+    if (m == 0) return 0;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (laneId() == uint32_t(leader)) base = atomicAdd(ctr, uint32_t(__popcll(m)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(m & lanesBelow()));
+}
+
+// Blocks cover contiguous tile ranges, XCD-aware, so a wave sees 64
+// consecutive (scan-line coherent) points.
+struct TileRange {
+    uint32_t begin, end;
+};
+__device__ __forceinline__ TileRange myTiles(uint32_t total) {
+    const uint32_t numTiles = (total + kTile - 1) / kTile;
+    const uint32_t lb = xcdBlock();
+    const uint64_t b = uint64_t(numTiles) * lb / gridDim.x;
+    const uint64_t e = uint64_t(numTiles) * (lb + 1) / gridDim.x;
+    return {uint32_t(b), uint32_t(e)};
+}
+
+// ---------------------------------------------------------------------------
+// reset (kernels/remolod/remolod_reset.cu:23-88): root at level 0 with a grid.
+__global__ void kResetRoot(DevTree t) {
+    Globals* g = t.g;
+    t.firstChild[0] = -1;
+    t.level[0] = 0;
+    t.nx[0] = t.ny[0] = t.nz[0] = 0;
+    t.counter[0] = 0;
+    t.numPoints[0] = 0;
+    t.numVoxels[0] = 0;
+    t.numVoxelsStored[0] = 0;
+    t.pending[0] = 0;
+    for (int i = 0; i < 8; ++i) {
+        t.storedOct[i] = 0;
+        t.batchOct[i] = 0;
+    }
+    for (uint32_t i = 0; i < kPointDir; ++i) t.pointDir[i] = 0;
+    g->numNodes = 1;
+    g->arenaOffset = kArenaFirstOffset;
+    g->numGrids = 1;
+    t.gridOffset[0] = g->arenaOffset;
+    g->arenaOffset += kGridAllocBytes;
+    t.voxelChunks[0] = 0;
+    t.gridNo[0] = 0;
+}
+
+// Zeroes `count` 256 KiB grids listed by offsets (uint4 stores).
+__global__ void kZeroRootGrid(DevTree t) {
+    uint4* w = reinterpret_cast<uint4*>(t.arena + t.gridOffset[0]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kGridWords / 4;
+         i += gridDim.x * blockDim.x)
+        w[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void kBeginStep(DevTree t) {
+    Globals* g = t.g;
+    g->stepStart = wall_clock64();
+    g->batches = g->expandIters = g->nodesSplit = g->spilledPoints = g->newVoxels = 0;
+    g->stop = 0;
+    g->budgetStop = 0;
+}
+
+__global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t checkBudget,
+                            uint64_t budgetTicks) {
+    Globals* g = t.g;
+    // remolod_octree.cu:1230-1240: after a batch, stop once the step has used its
+    // device-time budget. Checked here, before the next batch touches the tree.
+    if (checkBudget && budgetTicks && wall_clock64() - g->stepStart > budgetTicks) {
+        g->stop = 1;
+        g->budgetStop = 1;
+        return;
+    }
+    g->numSpilled = 0;
+    g->numBacklog = 0;
+    g->numCandidates = 0;
+    g->numAccepted = 0;
+    g->numPending = 0;
+    // remolod_octree.cu:1181-1210: stop before a batch once the persistent store
+    // is within the safety margin of its capacity.
+    const bool reached = g->arenaOffset + kSafetyMargin >= capacity;
+    g->memCapacityReached = reached ? 1u : 0u;
+    g->stop = reached ? 1u : 0u;
+    (void)b;
+}
+
+// ---------------------------------------------------------------------------
+// Count (doCounting, remolod_octree.cu:142-301). Round 0 counts this batch's
+// points into the leaves that existed before it; round r > 0 counts batch and
+// spilled points into the children created in round r-1 that hold > 50,000.
+// Counts go to the octant histogram batchOct[leaf][octant].
+__global__ __launch_bounds__(kBlock) void kCount(DevTree t, BatchRef b, uint32_t round) {
+    const uint32_t batchSize = batchSizeOf(t, b);
+    if (t.g->stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // per-round counters; nothing in this kernel reads them
+        t.g->numCandidates = 0;
+        t.g->numAccepted = 0;
+        t.g->numPending = 0;
+    }
+    const uint32_t total = batchSize + (round ? spilledCount(t) : 0u);
+    const TileRange tr = myTiles(total);
+    const float size = t.octreeSize;
+    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            bool valid = i < total;
+            uint32_t key = 0;
+            if (valid) {
+                const remo_point p = streamPoint(t, b, batchSize, i);
+                const uint32_t X = quant28(p.x, size) >> 8;
+                const uint32_t Y = quant28(p.y, size) >> 8;
+                const uint32_t Z = quant28(p.z, size) >> 8;
+                uint32_t level;
+                const uint32_t leaf = descend(t.firstChild, X, Y, Z, level);
+                if (round != 0 && t.pending[leaf] == 0) valid = false;
+                // a leaf at MAX_DEPTH never splits; its points count in octant 0
+                key = leaf * 8u + ((valid && level < uint32_t(kMaxDepth)) ? octantAt(X, Y, Z, level) : 0u);
+            }
+            waveAggInc(t.batchOct, key, valid);
+        }
+    }
+}
+
+// Split decision per node. Round 0: a leaf whose counter crosses 50,000 with
+// this batch is a candidate (remolod_octree.cu:213-218); every touched leaf's
+// counter becomes counter + its batch count. Round r > 0: the pending children
+// are the candidates (their counters are already their region counts).
+__global__ __launch_bounds__(kBlock) void kSplitDecide(DevTree t, uint32_t round) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t n = g->numNodes;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (t.firstChild[i] >= 0) continue;
+        bool cand = false;
+        if (round != 0 && t.pending[i] == 0) continue;
+        if (round == 0) {
+            const uint4* bo = reinterpret_cast<const uint4*>(t.batchOct + 8ull * i);
+            const uint4 a = bo[0], c = bo[1];
+            const uint32_t add = a.x + a.y + a.z + a.w + c.x + c.y + c.z + c.w;
+            if (add == 0) continue;
+            const uint32_t old = t.counter[i];
+            t.counter[i] = old + add;
+            cand = old <= kMaxPointsPerNode && old + add > kMaxPointsPerNode;
+        } else {
+            cand = t.pending[i] != 0;
+            t.pending[i] = 0;
+        }
+        if (cand && t.level[i] < uint32_t(kMaxDepth)) {
+            const uint32_t slot = atomicAdd(&g->numCandidates, 1u);
+            if (slot < t.candidateCap)
+                t.candidates[slot] = i;
+            else
+                g->candidateOverflow = 1;
+        } else {
+            // not splitting: this round's octant counts are spent
+            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * i);
+            bo[0] = make_uint4(0, 0, 0, 0);
+            bo[1] = make_uint4(0, 0, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ bool keyLess(const DevTree& t, uint32_t a, uint32_t b) {
+    const uint32_t la = t.level[a], lb = t.level[b];
+    if (la != lb) return la < lb;
+    if (t.nx[a] != t.nx[b]) return t.nx[a] < t.nx[b];
+    if (t.ny[a] != t.ny[b]) return t.ny[a] < t.ny[b];
+    return t.nz[a] < t.nz[b];
+}
+
+// doSplitting (remolod_octree.cu:303-400). Candidates are ranked by canonical
+// key (level, X, Y, Z); the node pool accepts the lowest-ranked ones that fit,
+// so child indices and refusals are deterministic. Children take their
+// counters from the octant histograms; a child still above 50,000 is pending.
+__global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t nc = min(g->numCandidates, t.candidateCap);
+    const uint32_t base = g->numNodes;
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
+        const uint32_t node = t.candidates[ci];
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < nc; ++k) rank += keyLess(t, t.candidates[k], node) ? 1u : 0u;
+        const uint64_t childOffset = uint64_t(base) + 8ull * rank;
+        if (childOffset + 8 > t.maxNodes) {
+            g->nodePoolOverflow = 1;  // stays a leaf; its points stay where they are
+            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
+            bo[0] = make_uint4(0, 0, 0, 0);
+            bo[1] = make_uint4(0, 0, 0, 0);
+            continue;
+        }
+        const uint32_t c0 = uint32_t(childOffset);
+        const uint32_t lvl = t.level[node] + 1;
+        const uint32_t X = t.nx[node], Y = t.ny[node], Z = t.nz[node];
+        for (uint32_t i = 0; i < 8; ++i) {
+            const uint32_t c = c0 + i;
+            const uint32_t cnt = t.storedOct[8ull * node + i] + t.batchOct[8ull * node + i];
+            t.firstChild[c] = -1;
+            t.level[c] = lvl;
+            t.nx[c] = 2 * X + ((i >> 2) & 1u);
+            t.ny[c] = 2 * Y + ((i >> 1) & 1u);
+            t.nz[c] = 2 * Z + (i & 1u);
+            t.counter[c] = cnt;
+            t.numPoints[c] = 0;
+            t.numVoxels[c] = 0;
+            t.numVoxelsStored[c] = 0;
+            t.gridNo[c] = kNone;
+            const bool pend = cnt > kMaxPointsPerNode && lvl < uint32_t(kMaxDepth);
+            t.pending[c] = pend ? 1u : 0u;
+            if (pend) atomicAdd(&g->numPending, 1u);
+            uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * c);
+            so[0] = so[1] = make_uint4(0, 0, 0, 0);
+            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * c);
+            bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
+            uint4* pd = reinterpret_cast<uint4*>(t.pointDir + uint64_t(c) * kPointDir);
+            for (uint32_t k = 0; k < kPointDir / 4; ++k) pd[k] = make_uint4(0, 0, 0, 0);
+            remo_node_accum* a = t.accums + c;
+            *a = remo_node_accum{};
+        }
+        uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
+        bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
+        uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * node);
+        so[0] = so[1] = make_uint4(0, 0, 0, 0);
+
+        // stored points -> spill list (copied by kSpillCopy); chunks -> queue
+        const uint32_t np = t.numPoints[node];
+        uint32_t spillOff = kNone;
+        if (np > 0) {
+            const uint32_t off = atomicAdd(&g->numSpilled, np);
+            if (uint64_t(off) + np <= t.spillCap)
+                spillOff = off;
+            else
+                g->spillOverflow = 1;
+            const uint32_t nChunks = (np + kPointsPerChunk - 1) / kPointsPerChunk;
+            const int64_t top = atomicAdd((unsigned long long*)&g->numAllocatedChunks,
+                                          (unsigned long long)(-int64_t(nChunks)));
+            for (uint32_t k = 0; k < nChunks; ++k) {
+                const int64_t qi = top - 1 - int64_t(k);
+                if (qi >= 0 && uint64_t(qi) < t.chunkQueueCap)
+                    t.chunkQueue[qi] = t.pointDir[uint64_t(node) * kPointDir + k];
+            }
+        }
+        const uint32_t slot = atomicAdd(&g->numAccepted, 1u);
+        t.acceptedSpill[2 * slot] = node;
+        t.acceptedSpill[2 * slot + 1] = spillOff;
+
+        if (t.gridNo[node] == kNone) {
+            const uint32_t gi = atomicAdd(&g->numGrids, 1u);
+            const unsigned long long off =
+                atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes);
+            if (gi < t.maxGrids && off + kGridAllocBytes <= t.arenaCapacity) {
+                t.gridOffset[gi] = off;
+                t.voxelChunks[gi] = 0;
+                t.gridNo[node] = gi;
+            } else {
+                g->allocOverflow = 1;
+            }
+        }
+        atomicAdd((unsigned long long*)&g->nodesSplit, 1ull);
+        atomicAdd((unsigned long long*)&g->totNodesSplit, 1ull);
+        // publish the children last: later kernels descend through them
+        t.firstChild[node] = int32_t(c0);
+    }
+}
+
+// Bumps numNodes by the accepted splits (separate launch: kSplitApply ranks
+// against the value numNodes had when it started).
+__global__ void kSplitCommit(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    g->numNodes += 8u * g->numAccepted;
+    g->expandIters += 1;
+    g->totExpandIters += 1;
+}
+
+// The spill loop of doCounting (remolod_octree.cu:250-285): one block per
+// accepted split copies that node's stored points, chunk by chunk.
+__global__ __launch_bounds__(kBlock) void kSpillCopy(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t na = g->numAccepted;
+    for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+        const uint32_t node = t.acceptedSpill[2 * a];
+        const uint32_t off = t.acceptedSpill[2 * a + 1];
+        const uint32_t np = t.numPoints[node];
+        if (off != kNone) {
+            for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
+                if (s / kPointsPerChunk >= kPointDir) break;
+                const uint32_t ref = t.pointDir[uint64_t(node) * kPointDir + s / kPointsPerChunk];
+                t.spill[off + s] = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) t.numPoints[node] = 0;
+    }
+}
+
+// Zero every accepted split's grid (remolod_octree.cu:372-387), including a grid
+// the node already had (the root's, from reset).
+__global__ __launch_bounds__(kBlock) void kZeroGrids(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t na = g->numAccepted;
+    const uint64_t perGrid = kGridWords / 4;
+    const uint64_t total = uint64_t(na) * perGrid;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t node = t.acceptedSpill[2 * (i / perGrid)];
+        const uint32_t gi = t.gridNo[node];
+        if (gi == kNone) continue;
+        reinterpret_cast<uint4*>(t.arena + t.gridOffset[gi])[i % perGrid] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// voxelSampling / sampleVoxel (remolod_octree.cu:677-740, :95-140): at every
+// gridded node on the root->leaf path, set the point's 128^3 cell; the lane
+// that sets a bit appends the cell-centre voxel, coloured by its point.
+__global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
+    const uint32_t batchSize = batchSizeOf(t, b);
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t total = batchSize + spilledCount(t);
+    const TileRange tr = myTiles(total);
+    const float size = t.octreeSize;
+    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            const bool valid = i < total;
+            remo_point p = {0, 0, 0, 0};
+            uint32_t pX = 0, pY = 0, pZ = 0;
+            if (valid) {
+                p = streamPoint(t, b, batchSize, i);
+                pX = quant28(p.x, size);
+                pY = quant28(p.y, size);
+                pZ = quant28(p.z, size);
+            }
+            const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
+            uint32_t cur = 0;
+            bool walking = valid;
+            for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
+                if (__ballot(walking) == 0) break;
+                bool isNew = false;
+                uint32_t cx = 0, cy = 0, cz = 0, node = cur;
+                if (walking) {
+                    const uint32_t gi = t.gridNo[cur];
+                    if (gi != kNone) {
+                        const uint32_t sh = (kMaxDepth + 1) - level;
+                        cx = (pX >> sh) & (kGridSize - 1);
+                        cy = (pY >> sh) & (kGridSize - 1);
+                        cz = (pZ >> sh) & (kGridSize - 1);
+                        const uint32_t vi = cx + cy * kGridSize + cz * kGridSize * kGridSize;
+                        uint32_t* word = reinterpret_cast<uint32_t*>(t.arena + t.gridOffset[gi]) + (vi >> 5);
+                        const uint32_t mask = 1u << (vi & 31u);
+                        // plain peek first: a stale word only costs an extra atomicOr
+                        if ((*word & mask) == 0) {
+                            const uint32_t old = atomicOr(word, mask);
+                            isNew = (old & mask) == 0;
+                        }
+                    }
+                }
+                // one backlog slot per new voxel, one atomic per wave
+                const uint32_t slot = waveAppend(&g->numBacklog, isNew);
+                if (isNew) {
+                    atomicAdd(&t.numVoxels[node], 1u);
+                    if (slot < t.backlogCap) {
+                        const uint32_t lvl = level;
+                        const float nodeSize = size / float(1u << lvl);
+                        const float minx = float(t.nx[node]) * nodeSize;
+                        const float miny = float(t.ny[node]) * nodeSize;
+                        const float minz = float(t.nz[node]) * nodeSize;
+                        remo_point v;
+                        v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
+                        v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
+                        v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
+                        v.color = p.color;
+                        t.backlogPts[slot] = v;
+                        t.backlogNode[slot] = node;
+                    } else {
+                        g->backlogOverflow = 1;
+                    }
+                }
+                if (walking) {
+                    const int32_t fc = t.firstChild[cur];
+                    if (fc < 0) {
+                        walking = false;
+                    } else {
+                        const uint32_t sh = kMaxDepth - level - 1;
+                        cur = uint32_t(fc) + ((((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) |
+                                              ((Z >> sh) & 1u));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// allocatePointChunks + allocateVoxelChunks (remolod_octree.cu:742-792, :885-916).
+// Point chunks come from the recycled queue while the in-use count is below the
+// pool's high-water mark, from the persistent store beyond it.
+__global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t n = g->numNodes;
+    const int64_t poolSize = g->chunkPoolSize;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (t.firstChild[i] < 0) {
+            const uint32_t np = t.numPoints[i], cnt = t.counter[i];
+            if (np < cnt) {
+                const uint32_t have = (np + kPointsPerChunk - 1) / kPointsPerChunk;
+                const uint32_t need = (cnt + kPointsPerChunk - 1) / kPointsPerChunk;
+                if (need > have) {
+                    const uint32_t add = need - have;
+                    const int64_t first = int64_t(atomicAdd(
+                        (unsigned long long*)&g->numAllocatedChunks, (unsigned long long)add));
+                    // indices at or above the pool size need fresh memory
+                    const int64_t freshFrom = first > poolSize ? first : poolSize;
+                    const int64_t last = first + add;
+                    const uint32_t fresh = last > freshFrom ? uint32_t(last - freshFrom) : 0u;
+                    uint64_t off = 0;
+                    if (fresh)
+                        off = atomicAdd((unsigned long long*)&g->arenaOffset,
+                                        (unsigned long long)(fresh * kChunkAllocBytes));
+                    uint32_t f = 0;
+                    for (uint32_t k = 0; k < add; ++k) {
+                        const int64_t qi = first + k;
+                        uint32_t ref = 0;
+                        if (qi >= poolSize) {
+                            const uint64_t o = off + uint64_t(f++) * kChunkAllocBytes;
+                            if (o + kChunkAllocBytes <= t.arenaCapacity)
+                                ref = uint32_t(o >> 4);
+                            else
+                                g->allocOverflow = 1;
+                        } else if (uint64_t(qi) < t.chunkQueueCap) {
+                            ref = t.chunkQueue[qi];
+                        }
+                        const uint32_t slot = have + k;
+                        if (slot < kPointDir)
+                            t.pointDir[uint64_t(i) * kPointDir + slot] = ref;
+                        else
+                            g->dirOverflow = 1;
+                    }
+                }
+            }
+        }
+        const uint32_t gi = t.gridNo[i];
+        if (gi != kNone) {
+            const uint32_t need = (t.numVoxels[i] + kPointsPerChunk - 1) / kPointsPerChunk;
+            const uint32_t have = t.voxelChunks[gi];
+            if (need > have) {
+                const uint32_t add = need - have;
+                const uint64_t off = atomicAdd((unsigned long long*)&g->arenaOffset,
+                                               (unsigned long long)(add * kChunkAllocBytes));
+                for (uint32_t k = 0; k < add; ++k) {
+                    const uint64_t o = off + uint64_t(k) * kChunkAllocBytes;
+                    const uint32_t slot = have + k;
+                    if (o + kChunkAllocBytes > t.arenaCapacity) {
+                        g->allocOverflow = 1;
+                    } else if (slot < kVoxelDir) {
+                        t.voxelDir[uint64_t(gi) * kVoxelDir + slot] = uint32_t(o >> 4);
+                    } else {
+                        g->dirOverflow = 1;
+                    }
+                }
+                t.voxelChunks[gi] = need;
+            }
+        }
+    }
+}
+
+// insertPoints (remolod_octree.cu:794-883): slot = atomicAdd(numPoints) per
+// (leaf, octant) group; the octant histogram of stored points grows with it.
+__global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
+    const uint32_t batchSize = batchSizeOf(t, b);
+    Globals* g = t.g;
+    if (g->stop) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        // allocatePointChunks' closing update (remolod_octree.cu:789-791)
+        if (g->numAllocatedChunks > g->chunkPoolSize) g->chunkPoolSize = g->numAllocatedChunks;
+    }
+    const uint32_t total = batchSize + spilledCount(t);
+    const TileRange tr = myTiles(total);
+    const float size = t.octreeSize;
+    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            const bool valid = i < total;
+            remo_point p = {0, 0, 0, 0};
+            uint32_t leaf = 0, oct = 0;
+            if (valid) {
+                p = streamPoint(t, b, batchSize, i);
+                const uint32_t X = quant28(p.x, size) >> 8;
+                const uint32_t Y = quant28(p.y, size) >> 8;
+                const uint32_t Z = quant28(p.z, size) >> 8;
+                uint32_t level;
+                leaf = descend(t.firstChild, X, Y, Z, level);
+                oct = level < uint32_t(kMaxDepth) ? octantAt(X, Y, Z, level) : 0u;
+            }
+            // one group per (leaf, octant): the slot counter and the histogram
+            uint64_t pend = __ballot(valid);
+            const uint32_t key = leaf * 8u + oct;
+            const uint32_t lane = laneId();
+            uint32_t slot = 0;
+            while (pend) {
+                const int leader = __ffsll((unsigned long long)pend) - 1;
+                const uint32_t lk = __shfl(key, leader);
+                const uint64_t grp = __ballot(valid && key == lk) & pend;
+                uint32_t old = 0;
+                if (lane == uint32_t(leader)) {
+                    const uint32_t c = uint32_t(__popcll(grp));
+                    old = atomicAdd(&t.numPoints[lk >> 3], c);
+                    atomicAdd(&t.storedOct[lk], c);
+                }
+                old = __shfl(old, leader);
+                if ((grp >> lane) & 1ull) slot = old + uint32_t(__popcll(grp & lanesBelow()));
+                pend &= ~grp;
+            }
+            if (valid) {
+                const uint32_t k = slot / kPointsPerChunk;
+                const uint32_t ref = k < kPointDir ? t.pointDir[uint64_t(leaf) * kPointDir + k] : 0u;
+                if (ref != 0)
+                    chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = p;
+                else
+                    g->dirOverflow = 1;
+            }
+        }
+    }
+}
+
+// insertVoxels (remolod_octree.cu:918-938).
+__global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t total = min(g->numBacklog, t.backlogCap);
+    const TileRange tr = myTiles(total);
+    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            const bool valid = i < total;
+            const uint32_t node = valid ? t.backlogNode[i] : 0u;
+            const uint32_t slot = waveAggInc(t.numVoxelsStored, node, valid);
+            if (valid) {
+                const uint32_t gi = t.gridNo[node];
+                const uint32_t k = slot / kPointsPerChunk;
+                const uint32_t ref = (gi != kNone && k < kVoxelDir)
+                                         ? t.voxelDir[uint64_t(gi) * kVoxelDir + k]
+                                         : 0u;
+                if (ref != 0)
+                    chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[i];
+                else
+                    g->dirOverflow = 1;
+            }
+        }
+    }
+}
+
+__global__ void kEndBatch(DevTree t, BatchRef b) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    g->batchletIndex += 1;
+    g->numPointsProcessed += *b.sizePtr;
+    const uint64_t spilled = min(g->numSpilled, t.spillCap);
+    const uint64_t voxels = min(g->numBacklog, t.backlogCap);
+    g->batches += 1;
+    g->spilledPoints += spilled;
+    g->newVoxels += voxels;
+    g->totBatches += 1;
+    g->totSpilledPoints += spilled;
+    g->totNewVoxels += voxels;
+}
+
+// The per-node stats sweep (remolod_octree.cu:1248-1319).
+__global__ __launch_bounds__(kBlock) void kStatsSweep(DevTree t) {
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    uint32_t inner = 0, leaves = 0, nonempty = 0, maxPts = 0, chunksP = 0, chunksV = 0, under = 0,
+             maxDepth = 0;
+    uint64_t pts = 0, vox = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        maxDepth = max(maxDepth, t.level[i]);
+        if (t.firstChild[i] < 0) {
+            const uint32_t np = t.numPoints[i];
+            leaves++;
+            pts += np;
+            chunksP += (np + kPointsPerChunk - 1) / kPointsPerChunk;
+            maxPts = max(maxPts, np);
+            if (t.counter[i] < np) under++;
+            if (np > 0) nonempty++;
+        } else {
+            const uint32_t nv = t.numVoxels[i];
+            inner++;
+            vox += nv;
+            chunksV += (nv + kPointsPerChunk - 1) / kPointsPerChunk;
+        }
+    }
+    // wave reductions, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        inner += __shfl_down(inner, off);
+        leaves += __shfl_down(leaves, off);
+        nonempty += __shfl_down(nonempty, off);
+        chunksP += __shfl_down(chunksP, off);
+        chunksV += __shfl_down(chunksV, off);
+        under += __shfl_down(under, off);
+        pts += __shfl_down(pts, off);
+        vox += __shfl_down(vox, off);
+        maxPts = max(maxPts, __shfl_down(maxPts, off));
+        maxDepth = max(maxDepth, __shfl_down(maxDepth, off));
+    }
+    if (laneId() == 0) {
+        atomicAdd(&g->sInner, inner);
+        atomicAdd(&g->sLeaves, leaves);
+        atomicAdd(&g->sNonempty, nonempty);
+        atomicAdd(&g->sChunksPoints, chunksP);
+        atomicAdd(&g->sChunksVoxels, chunksV);
+        atomicAdd(&g->sUnderflows, under);
+        atomicAdd((unsigned long long*)&g->sPoints, (unsigned long long)pts);
+        atomicAdd((unsigned long long*)&g->sVoxels, (unsigned long long)vox);
+        atomicMax(&g->sMaxPoints, maxPts);
+        atomicMax(&g->sMaxDepth, maxDepth);
+    }
+}
+
+__global__ void kStatsClear(DevTree t) {
+    Globals* g = t.g;
+    g->sInner = g->sLeaves = g->sNonempty = g->sMaxPoints = 0;
+    g->sPoints = g->sVoxels = 0;
+    g->sChunksPoints = g->sChunksVoxels = g->sUnderflows = g->sMaxDepth = 0;
+}
+
+}  // namespace
+
+// ---- launchers --------------------------------------------------------------
+
+void launchResetTree(const DevTree& t, hipStream_t s) {
+    hipLaunchKernelGGL(kResetRoot, dim3(1), dim3(1), 0, s, t);
+    hipLaunchKernelGGL(kZeroRootGrid, dim3(64), dim3(kBlock), 0, s, t);
+}
+
+void launchBeginStep(const DevTree& t, hipStream_t s) {
+    hipLaunchKernelGGL(kBeginStep, dim3(1), dim3(1), 0, s, t);
+}
+
+void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, hipStream_t s) {
+    hipLaunchKernelGGL(kBeginBatch, dim3(1), dim3(1), 0, s, t, b, t.arenaCapacity, checkBudget ? 1u : 0u,
+                       budgetTicks);
+}
+
+void launchCount(const DevTree& t, BatchRef b, uint32_t round, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kCount, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b, round);
+}
+
+void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kSplitDecide, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, round);
+}
+
+void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+    (void)ls;
+    hipLaunchKernelGGL(kSplitApply, dim3(16), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kSplitCommit, dim3(1), dim3(1), 0, s, t);
+}
+
+void launchSpillCopy(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+    (void)ls;
+    hipLaunchKernelGGL(kSpillCopy, dim3(256), dim3(kBlock), 0, s, t);
+}
+
+void launchZeroGrids(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+    (void)ls;
+    hipLaunchKernelGGL(kZeroGrids, dim3(1024), dim3(kBlock), 0, s, t);
+}
+
+void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
+}
+
+void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
+}
+
+void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
+}
+
+void launchInsertVoxels(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t);
+}
+
+void launchEndBatch(const DevTree& t, BatchRef b, hipStream_t s) {
+    hipLaunchKernelGGL(kEndBatch, dim3(1), dim3(1), 0, s, t, b);
+}
+
+void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kStatsClear, dim3(1), dim3(1), 0, s, t);
+    hipLaunchKernelGGL(kStatsSweep, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
+}
+
+}  // namespace remo_hip

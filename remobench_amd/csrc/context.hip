@@ -1,0 +1,757 @@
+// Host side of the C ABI: context lifecycle, the per-step batch loop, dumps,
+// the input ring and device-memory plumbing.
+//
+// Mirrors RemolodPipeline (src/pipelines/RemolodPipeline.cpp): allocate() sizes
+// every buffer up front, reset() rebuilds the root, build() folds up to 20
+// batches or a device-time budget per call, runAccumulator() folds new points.
+// Everything is ordered on one HIP stream owned by the context.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "remolod_internal.h"
+#include "remolod_analysis.h"
+
+using namespace remo_hip;
+
+namespace {
+
+enum ScopeId : int {
+    kScopeCount, kScopeSplit, kScopeVoxel, kScopeAlloc, kScopeInsertPoints, kScopeInsertVoxels,
+    kScopeAccumulate, kScopeAnalysis, kScopeRender, kNumScopes
+};
+const char* const kScopeNames[kNumScopes] = {
+    "remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
+    "remolod.construct.allocChunks", "remolod.construct.insertPoints",
+    "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
+};
+
+// HIP-event pairs per launch, harvested after the stream has passed them.
+struct Profiler {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    struct Pair {
+        int scope;
+        hipEvent_t a, b;
+    };
+    std::vector<Pair> pending;
+    double totalMs[kNumScopes] = {};
+    uint64_t launches[kNumScopes] = {};
+
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void harvest() {
+        for (Pair& p : pending) {
+            float ms = 0.0f;
+            if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                totalMs[p.scope] += ms;
+                launches[p.scope] += 1;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void destroy() {
+        harvest();
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+    }
+};
+
+// RAII scope: records a start event now and an end event when it closes.
+struct Scope {
+    Profiler& p;
+    int id;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    Scope(Profiler& prof, int scope, hipStream_t stream) : p(prof), id(scope), s(stream) {
+        if (p.enabled) {
+            a = p.get();
+            if (a) (void)hipEventRecord(a, s);
+        }
+    }
+    ~Scope() {
+        if (!a) return;
+        hipEvent_t b = p.get();
+        if (!b) return;
+        (void)hipEventRecord(b, s);
+        p.pending.push_back({id, a, b});
+    }
+};
+
+}  // namespace
+
+struct remo_ctx {
+    remo_config cfg;
+    remo_cloud_meta meta;
+    DevTree t;
+    hipStream_t stream = nullptr;
+    Globals* hostG = nullptr;  // pinned mirror
+    uint32_t* hostScratch = nullptr;  // pinned, 16 words after the mirror
+    uint64_t budgetTicks = 0;         // step budget in device wall-clock ticks
+    LaunchShape shape;
+    std::string err;
+    bool dead = false;
+    bool needsReset = true;
+    uint32_t batchletIndex = 0;
+    std::vector<void*> allocations;
+    AnalysisState analysis;
+    Profiler prof;
+};
+
+struct remo_ring {
+    remo_point* slots = nullptr;
+    uint32_t* sizes = nullptr;
+    uint32_t* uploaded = nullptr;
+    uint32_t slotCapacity = 0, numSlots = 0, hostUploaded = 0;
+    std::vector<uint32_t> hostSizes;
+};
+
+namespace {
+
+bool isSticky(hipError_t e) {
+    switch (e) {
+        case hipErrorIllegalAddress:
+        case hipErrorLaunchFailure:
+        case hipErrorAssert:
+        case hipErrorLaunchTimeOut:
+        case hipErrorECCNotCorrectable:
+            return true;
+        default:
+            return false;
+    }
+}
+
+int fail(remo_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return REMO_OK;
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    if (isSticky(e)) {
+        c->dead = true;
+        return REMO_ERR_DEVICE;
+    }
+    return e == hipErrorOutOfMemory ? REMO_ERR_OOM : REMO_ERR_INVALID;
+}
+
+#define CK(call, what)                              \
+    do {                                            \
+        const hipError_t e_ = (call);               \
+        if (e_ != hipSuccess) return fail(c, e_, what); \
+    } while (0)
+
+template <typename T>
+int devAlloc(remo_ctx* c, T** p, uint64_t count, const char* what) {
+    void* ptr = nullptr;
+    const uint64_t bytes = std::max<uint64_t>(count * sizeof(T), 16);
+    const hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) {
+        c->err = std::string("hipMalloc failed for ") + what + " (" +
+                 std::to_string(bytes >> 20) + " MiB): " + hipGetErrorString(e);
+        return REMO_ERR_OOM;
+    }
+    c->allocations.push_back(ptr);
+    *p = static_cast<T*>(ptr);
+    return REMO_OK;
+}
+
+void releaseAll(remo_ctx* c) {
+    for (void* p : c->allocations) (void)hipFree(p);
+    c->allocations.clear();
+    if (c->hostG) (void)hipHostFree(c->hostG);
+    c->hostG = nullptr;
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+int readGlobals(remo_ctx* c) {
+    CK(hipMemcpyAsync(c->hostG, c->t.g, sizeof(Globals), hipMemcpyDeviceToHost, c->stream),
+       "reading device counters");
+    CK(hipStreamSynchronize(c->stream), "device work");
+    return REMO_OK;
+}
+
+void fillStats(const remo_ctx* c, remo_stats* s) {
+    const Globals& g = *c->hostG;
+    std::memset(s, 0, sizeof(*s));
+    s->num_nodes = g.numNodes;
+    s->num_inner = g.sInner;
+    s->num_leaves = g.sLeaves;
+    s->num_nonempty_leaves = g.sNonempty;
+    s->num_points = g.sPoints;
+    s->num_voxels = g.sVoxels;
+    s->num_points_processed = g.numPointsProcessed;
+    s->batchlet_index = g.batchletIndex;
+    s->num_chunks_points = g.sChunksPoints;
+    s->num_chunks_voxels = g.sChunksVoxels;
+    s->max_points_per_node = g.sMaxPoints;
+    s->counter_underflows = g.sUnderflows;
+    s->max_depth = g.sMaxDepth;
+    s->num_allocated_chunks = uint64_t(g.numAllocatedChunks);
+    s->chunk_pool_size = uint64_t(g.chunkPoolSize);
+    s->allocated_bytes_persistent = g.arenaOffset;
+    s->persistent_capacity = c->t.arenaCapacity;
+    s->mem_capacity_reached = g.memCapacityReached;
+    s->node_capacity_reached = g.numNodes >= c->t.maxNodes ? 1u : 0u;
+    s->node_pool_overflow = g.nodePoolOverflow;
+    s->alloc_overflow = g.allocOverflow;
+    s->spill_overflow = g.spillOverflow;
+    s->backlog_overflow = g.backlogOverflow;
+    s->dir_overflow = g.dirOverflow | g.candidateOverflow;
+    s->batches = g.batches;
+    s->expand_iters = g.expandIters;
+    s->nodes_split = g.nodesSplit;
+    s->spilled_points = g.spilledPoints;
+    s->new_voxels = g.newVoxels;
+    s->total_batches = g.totBatches;
+    s->total_expand_iters = g.totExpandIters;
+    s->total_nodes_split = g.totNodesSplit;
+    s->total_spilled_points = g.totSpilledPoints;
+    s->total_new_voxels = g.totNewVoxels;
+    s->morton_watermark = g.mortonWatermark;
+    s->num_accumulated = g.numAccumulated;
+    s->sum_leaf_counts = g.sumLeafCounts;
+    s->points_folded = g.pointsFolded;
+    s->leaves_folded = g.leavesFolded;
+    s->inner_with_sums = g.innerWithSums;
+    s->draw_items = c->analysis.drawItems;
+    s->draw_list_overflow = c->analysis.drawListOverflow;
+    s->draw_samples = c->analysis.drawSamples;
+}
+
+}  // namespace
+
+extern "C" {
+
+int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx** out) {
+    if (!cfgIn || !meta || !out) return REMO_ERR_INVALID;
+    *out = nullptr;
+    remo_ctx* c = new (std::nothrow) remo_ctx();
+    if (!c) return REMO_ERR_OOM;
+    c->cfg = *cfgIn;
+    c->meta = *meta;
+    remo_config& cfg = c->cfg;
+    // reference defaults (src/pipelines/RemolodPipeline.cpp:154-167,
+    // remolod_octree.cu:26-40, :1138, :1169)
+    if (!cfg.max_nodes) cfg.max_nodes = REMO_DEFAULT_MAX_NODES;
+    if (!cfg.max_batches_per_step) cfg.max_batches_per_step = 20;
+    if (cfg.step_budget_ms == 0.0f) cfg.step_budget_ms = 10.0f;
+    if (!cfg.slot_capacity) cfg.slot_capacity = REMO_DEFAULT_SLOT_CAPACITY;
+    if (!cfg.ring_slots) cfg.ring_slots = REMO_DEFAULT_RING_SLOTS;
+    if (!cfg.spill_capacity) cfg.spill_capacity = 10000000u;
+    if (!cfg.backlog_capacity) cfg.backlog_capacity = 10000000u;
+    if (!cfg.persistent_bytes) {
+        cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20);
+    }
+    if (meta->num_points == 0) {
+        delete c;
+        return REMO_ERR_INVALID;  // "cloud is empty"
+    }
+    const float size = std::max(std::max(meta->box_size[0], meta->box_size[1]), meta->box_size[2]);
+    if (!(size > 0.0f)) {
+        delete c;
+        return REMO_ERR_INVALID;
+    }
+
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        delete c;
+        return REMO_ERR_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        delete c;
+        return REMO_ERR_DEVICE;
+    }
+    const uint32_t cus = std::max(1, prop.multiProcessorCount);
+    c->shape.pointBlocks = std::max(8u, (cus * 8u) & ~7u);
+    c->shape.nodeBlocks = std::max(8u, (cus * 2u) & ~7u);
+
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->hostG), sizeof(Globals) + 64, hipHostMallocDefault) !=
+            hipSuccess) {
+        releaseAll(c);
+        delete c;
+        return REMO_ERR_DEVICE;
+    }
+    std::memset(c->hostG, 0, sizeof(Globals) + 64);
+    c->hostScratch = reinterpret_cast<uint32_t*>(c->hostG + 1);
+    int clockKHz = 0;
+    if (hipDeviceGetAttribute(&clockKHz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || clockKHz <= 0)
+        clockKHz = 100000;
+    c->budgetTicks = cfg.step_budget_ms > 0.0f ? uint64_t(double(cfg.step_budget_ms) * double(clockKHz)) : 0;
+
+    DevTree& t = c->t;
+    std::memset(&t, 0, sizeof(t));
+    const uint64_t N = cfg.max_nodes;
+    t.maxNodes = cfg.max_nodes;
+    t.maxGrids = cfg.max_nodes / 8 + 2;
+    t.octreeSize = size;
+    t.arenaCapacity = cfg.persistent_bytes;
+    t.chunkQueueCap = cfg.persistent_bytes / kChunkAllocBytes + 1;
+    t.spillCap = cfg.spill_capacity;
+    t.backlogCap = cfg.backlog_capacity;
+    t.candidateCap = cfg.max_nodes;
+    int rc = REMO_OK;
+    auto A = [&](auto** p, uint64_t n, const char* what) {
+        if (rc == REMO_OK) rc = devAlloc(c, p, n, what);
+    };
+    A(&t.firstChild, N, "node children");
+    A(&t.level, N, "node levels");
+    A(&t.nx, N, "node X");
+    A(&t.ny, N, "node Y");
+    A(&t.nz, N, "node Z");
+    A(&t.counter, N, "node counters");
+    A(&t.numPoints, N, "node point counts");
+    A(&t.numVoxels, N, "node voxel counts");
+    A(&t.numVoxelsStored, N, "node stored voxel counts");
+    A(&t.gridNo, N, "node grids");
+    A(&t.storedOct, N * 8, "stored octant histograms");
+    A(&t.batchOct, N * 8, "batch octant histograms");
+    A(&t.pending, N, "pending flags");
+    A(&t.pointDir, N * kPointDir, "point chunk directories");
+    A(&t.gridOffset, t.maxGrids, "grid offsets");
+    A(&t.voxelDir, uint64_t(t.maxGrids) * kVoxelDir, "voxel chunk directories");
+    A(&t.voxelChunks, t.maxGrids, "voxel chunk counts");
+    A(&t.arena, t.arenaCapacity, "the persistent octree store");
+    A(&t.chunkQueue, t.chunkQueueCap, "the chunk queue");
+    A(&t.spill, t.spillCap, "spilled points");
+    A(&t.backlogPts, t.backlogCap, "the voxel backlog");
+    A(&t.backlogNode, t.backlogCap, "the voxel backlog targets");
+    A(&t.candidates, t.candidateCap, "split candidates");
+    A(&t.acceptedSpill, uint64_t(t.candidateCap) * 2, "accepted splits");
+    A(&t.g, 1, "device counters");
+    A(&t.accums, N, "the accumulator side array");
+    if (rc == REMO_OK) rc = analysisAllocate(c->analysis, N, [&](void** p, uint64_t bytes, const char* what) {
+        uint8_t* q = nullptr;
+        const int r = devAlloc(c, &q, bytes, what);
+        *p = q;
+        return r;
+    });
+    if (rc != REMO_OK) {
+        std::string e = c->err;
+        releaseAll(c);
+        delete c;
+        (void)e;
+        return rc;
+    }
+    *out = c;
+    return remo_reset(c);
+}
+
+void remo_destroy(remo_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->prof.destroy();
+    releaseAll(c);
+    delete c;
+}
+
+const char* remo_last_error(const remo_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int remo_sync(remo_ctx* c) {
+    if (!c) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    CK(hipStreamSynchronize(c->stream), "device work");
+    return REMO_OK;
+}
+
+int remo_reset(remo_ctx* c) {
+    if (!c) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    DevTree& t = c->t;
+    CK(hipMemsetAsync(t.g, 0, sizeof(Globals), c->stream), "reset");
+    CK(hipMemsetAsync(t.accums, 0, sizeof(remo_node_accum) * uint64_t(t.maxNodes), c->stream), "reset");
+    launchResetTree(t, c->stream);
+    CK(hipGetLastError(), "reset kernel");
+    const int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    c->batchletIndex = 0;
+    c->needsReset = false;
+    analysisReset(c->analysis);
+    return REMO_OK;
+}
+
+int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
+    if (!c || !v) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (!v->slots || !v->batch_sizes || !v->num_batches_uploaded) return REMO_ERR_INVALID;
+    // The ring check of RemolodPipeline::build (src/pipelines/RemolodPipeline.cpp:315-328):
+    // batch B is read from slot B % ring_slots, so the producer's ring must be that
+    // deep, or hold the whole cloud without wrapping.
+    const uint32_t R = c->cfg.ring_slots;
+    const bool ringOk = v->num_slots == R || (v->num_batches_total <= R && v->num_slots >= v->num_batches_total);
+    if (!ringOk || v->slot_capacity != c->cfg.slot_capacity) {
+        c->err = "ring of " + std::to_string(v->num_slots) + " slots x " +
+                 std::to_string(v->slot_capacity) + " points does not match the context (" +
+                 std::to_string(R) + " x " + std::to_string(c->cfg.slot_capacity) + ")";
+        return REMO_ERR_RING;
+    }
+    uint32_t uploaded = 0;
+    CK(hipMemcpyAsync(c->hostScratch, v->num_batches_uploaded, 4, hipMemcpyDeviceToHost, c->stream),
+       "reading the uploaded-batch cursor");
+    CK(hipStreamSynchronize(c->stream), "device work");
+    uploaded = c->hostScratch[0];
+    DevTree& t = c->t;
+    hipStream_t s = c->stream;
+    // step start time and per-step counters (DeviceTimeline counter block: per launch)
+    launchBeginStep(t, s);
+
+    const uint32_t first = c->batchletIndex;
+    const uint32_t numBatches = uploaded > first ? std::min(uploaded - first, c->cfg.max_batches_per_step) : 0u;
+    for (uint32_t bi = first; bi < first + numBatches; ++bi) {
+        const uint32_t slot = bi % v->num_slots;
+        BatchRef b{v->slots + uint64_t(slot) * v->slot_capacity, v->batch_sizes + slot};
+        launchBeginBatch(t, b, bi != first, c->budgetTicks, s);
+        uint32_t round = 0;
+        for (;;) {
+            {
+                Scope sc(c->prof, kScopeCount, s);
+                launchCount(t, b, round, c->shape, s);
+            }
+            {
+                Scope sc(c->prof, kScopeSplit, s);
+                launchSplitDecide(t, round, c->shape, s);
+                launchSplitApply(t, c->shape, s);
+                if (round == 0) launchSpillCopy(t, c->shape, s);
+                launchZeroGrids(t, c->shape, s);
+            }
+            CK(hipGetLastError(), "expand kernels");
+            int rc = readGlobals(c);
+            if (rc != REMO_OK) return rc;
+            if (c->hostG->stop || c->hostG->numPending == 0) break;
+            if (++round >= kMaxRounds) break;
+        }
+        if (c->hostG->stop) break;
+        {
+            Scope sc(c->prof, kScopeVoxel, s);
+            launchVoxelSample(t, b, c->shape, s);
+        }
+        {
+            Scope sc(c->prof, kScopeAlloc, s);
+            launchAllocChunks(t, c->shape, s);
+        }
+        {
+            Scope sc(c->prof, kScopeInsertPoints, s);
+            launchInsertPoints(t, b, c->shape, s);
+        }
+        {
+            Scope sc(c->prof, kScopeInsertVoxels, s);
+            launchInsertVoxels(t, c->shape, s);
+        }
+        launchEndBatch(t, b, s);
+        CK(hipGetLastError(), "insert kernels");
+        c->batchletIndex = bi + 1;
+    }
+    launchStatsSweep(t, c->shape, s);
+    CK(hipGetLastError(), "stats sweep");
+    const int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    c->batchletIndex = c->hostG->batchletIndex;
+    if (consumed) *consumed = c->batchletIndex;
+    return (v->num_batches_total > 0 && c->batchletIndex >= v->num_batches_total) ? REMO_DONE : REMO_OK;
+}
+
+int remo_accumulate(remo_ctx* c, uint32_t batchIndex) {
+    if (!c) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (c->cfg.flags & REMO_FLAG_NO_ACCUM) return REMO_OK;
+    {
+        Scope sc(c->prof, kScopeAccumulate, c->stream);
+        launchAccumulate(c->t, batchIndex, c->shape, c->stream);
+    }
+    CK(hipGetLastError(), "accumulate kernels");
+    return readGlobals(c);
+}
+
+int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {
+    if (!c || !u) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (mode != REMO_ANALYSIS_ANALYTIC && mode != REMO_ANALYSIS_LEARNED) return REMO_ERR_INVALID;
+    int rc;
+    {
+        Scope sc(c->prof, kScopeAnalysis, c->stream);
+        rc = runAnalysis(c->analysis, c->t, *u, mode, c->shape, c->stream, &c->err);
+    }
+    if (rc != REMO_OK) return rc;
+    CK(hipGetLastError(), "analysis kernels");
+    return REMO_OK;
+}
+
+int remo_render(remo_ctx* c, const remo_shared_uniforms* u, uint64_t* fb) {
+    if (!c || !u || !fb) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (!(u->width >= 1.0f) || !(u->height >= 1.0f)) return REMO_ERR_INVALID;
+    int rc;
+    {
+        Scope sc(c->prof, kScopeRender, c->stream);
+        rc = runRender(c->analysis, c->t, *u, fb, c->shape, c->stream, &c->err);
+    }
+    if (rc != REMO_OK) return rc;
+    CK(hipGetLastError(), "render kernels");
+    return REMO_OK;
+}
+
+int remo_get_stats(remo_ctx* c, remo_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    const int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    analysisReadDiagnostics(c->analysis, c->stream);
+    fillStats(c, out);
+    return REMO_OK;
+}
+
+int remo_dump_nodes(remo_ctx* c, remo_node_record* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    const uint32_t n = c->hostG->numNodes;
+    *num = n;
+    if (!out || cap < n) return out ? REMO_ERR_INVALID : REMO_OK;
+    std::vector<int32_t> fc(n);
+    std::vector<uint32_t> lv(n), X(n), Y(n), Z(n), cnt(n), np(n), nv(n), nvs(n), gr(n);
+    auto D = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+    };
+    const DevTree& t = c->t;
+    CK(D(fc.data(), t.firstChild, 4ull * n), "dump");
+    CK(D(lv.data(), t.level, 4ull * n), "dump");
+    CK(D(X.data(), t.nx, 4ull * n), "dump");
+    CK(D(Y.data(), t.ny, 4ull * n), "dump");
+    CK(D(Z.data(), t.nz, 4ull * n), "dump");
+    CK(D(cnt.data(), t.counter, 4ull * n), "dump");
+    CK(D(np.data(), t.numPoints, 4ull * n), "dump");
+    CK(D(nv.data(), t.numVoxels, 4ull * n), "dump");
+    CK(D(nvs.data(), t.numVoxelsStored, 4ull * n), "dump");
+    CK(D(gr.data(), t.gridNo, 4ull * n), "dump");
+    CK(hipStreamSynchronize(c->stream), "dump");
+    for (uint32_t i = 0; i < n; ++i) {
+        remo_node_record& r = out[i];
+        std::memset(&r, 0, sizeof(r));
+        r.level = lv[i];
+        r.x = X[i];
+        r.y = Y[i];
+        r.z = Z[i];
+        r.first_child = fc[i];
+        r.counter = cnt[i];
+        r.num_points = np[i];
+        r.num_voxels = nv[i];
+        r.num_voxels_stored = nvs[i];
+        r.grid = gr[i] == kNone ? -1 : int32_t(gr[i]);
+    }
+    return REMO_OK;
+}
+
+int remo_dump_node_samples(remo_ctx* c, uint32_t node, int voxels, remo_point* out, uint32_t cap,
+                           uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    if (node >= c->hostG->numNodes) return REMO_ERR_INVALID;
+    const DevTree& t = c->t;
+    uint32_t n = 0, gi = kNone;
+    CK(hipMemcpy(&n, (voxels ? t.numVoxelsStored : t.numPoints) + node, 4, hipMemcpyDeviceToHost), "dump");
+    CK(hipMemcpy(&gi, t.gridNo + node, 4, hipMemcpyDeviceToHost), "dump");
+    *num = n;
+    if (!out) return REMO_OK;
+    if (cap < n) return REMO_ERR_INVALID;
+    if (n == 0) return REMO_OK;
+    std::vector<uint32_t> dir;
+    if (voxels) {
+        if (gi == kNone) return REMO_ERR_INVALID;
+        dir.resize(kVoxelDir);
+        CK(hipMemcpy(dir.data(), t.voxelDir + uint64_t(gi) * kVoxelDir, 4ull * kVoxelDir,
+                     hipMemcpyDeviceToHost), "dump");
+    } else {
+        dir.resize(kPointDir);
+        CK(hipMemcpy(dir.data(), t.pointDir + uint64_t(node) * kPointDir, 4ull * kPointDir,
+                     hipMemcpyDeviceToHost), "dump");
+    }
+    for (uint32_t k = 0; k * kPointsPerChunk < n; ++k) {
+        const uint32_t m = std::min(kPointsPerChunk, n - k * kPointsPerChunk);
+        if (k >= dir.size() || dir[k] == 0) return REMO_ERR_CAPACITY;
+        CK(hipMemcpy(out + uint64_t(k) * kPointsPerChunk, t.arena + (uint64_t(dir[k]) << 4),
+                     sizeof(remo_point) * m, hipMemcpyDeviceToHost), "dump");
+    }
+    return REMO_OK;
+}
+
+int remo_dump_grid(remo_ctx* c, uint32_t node, uint32_t* words) {
+    if (!c || !words) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    if (node >= c->hostG->numNodes) return REMO_ERR_INVALID;
+    uint32_t gi = kNone;
+    CK(hipMemcpy(&gi, c->t.gridNo + node, 4, hipMemcpyDeviceToHost), "dump");
+    if (gi == kNone) return REMO_ERR_INVALID;
+    uint64_t off = 0;
+    CK(hipMemcpy(&off, c->t.gridOffset + gi, 8, hipMemcpyDeviceToHost), "dump");
+    CK(hipMemcpy(words, c->t.arena + off, 4ull * kGridWords, hipMemcpyDeviceToHost), "dump");
+    return REMO_OK;
+}
+
+int remo_dump_accums(remo_ctx* c, remo_node_accum* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    const uint32_t n = c->hostG->numNodes;
+    *num = n;
+    if (!out) return REMO_OK;
+    if (cap < n) return REMO_ERR_INVALID;
+    CK(hipMemcpy(out, c->t.accums, sizeof(remo_node_accum) * n, hipMemcpyDeviceToHost), "dump");
+    return REMO_OK;
+}
+
+int remo_dump_analysis(remo_ctx* c, remo_node_analysis* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    const uint32_t n = c->hostG->numNodes;
+    *num = n;
+    if (!out) return REMO_OK;
+    if (cap < n) return REMO_ERR_INVALID;
+    CK(hipMemcpy(out, c->analysis.results, sizeof(remo_node_analysis) * n, hipMemcpyDeviceToHost), "dump");
+    return REMO_OK;
+}
+
+int remo_profile_enable(remo_ctx* c, int on) {
+    if (!c) return REMO_ERR_INVALID;
+    c->prof.enabled = on != 0;
+    return REMO_OK;
+}
+
+int remo_profile_read(remo_ctx* c, remo_profile_entry* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    c->prof.harvest();
+    *num = kNumScopes;
+    if (!out) return REMO_OK;
+    for (uint32_t i = 0; i < kNumScopes && i < cap; ++i) {
+        std::memset(&out[i], 0, sizeof(out[i]));
+        std::strncpy(out[i].name, kScopeNames[i], sizeof(out[i].name) - 1);
+        out[i].total_ms = c->prof.totalMs[i];
+        out[i].launches = c->prof.launches[i];
+    }
+    return REMO_OK;
+}
+
+int remo_profile_clear(remo_ctx* c) {
+    if (!c) return REMO_ERR_INVALID;
+    c->prof.harvest();
+    for (int i = 0; i < kNumScopes; ++i) {
+        c->prof.totalMs[i] = 0.0;
+        c->prof.launches[i] = 0;
+    }
+    return REMO_OK;
+}
+
+// ---- ring ---------------------------------------------------------------------
+
+int remo_ring_create(uint32_t slotCapacity, uint32_t numSlots, remo_ring** out) {
+    if (!out || !slotCapacity || !numSlots) return REMO_ERR_INVALID;
+    *out = nullptr;
+    remo_ring* r = new (std::nothrow) remo_ring();
+    if (!r) return REMO_ERR_OOM;
+    r->slotCapacity = slotCapacity;
+    r->numSlots = numSlots;
+    r->hostSizes.assign(numSlots, 0);
+    if (hipMalloc(&r->slots, sizeof(remo_point) * uint64_t(slotCapacity) * numSlots) != hipSuccess ||
+        hipMalloc(&r->sizes, 4ull * numSlots) != hipSuccess || hipMalloc(&r->uploaded, 4) != hipSuccess) {
+        remo_ring_destroy(r);
+        return REMO_ERR_OOM;
+    }
+    if (hipMemset(r->sizes, 0, 4ull * numSlots) != hipSuccess || hipMemset(r->uploaded, 0, 4) != hipSuccess) {
+        remo_ring_destroy(r);
+        return REMO_ERR_DEVICE;
+    }
+    *out = r;
+    return REMO_OK;
+}
+
+void remo_ring_destroy(remo_ring* r) {
+    if (!r) return;
+    if (r->slots) (void)hipFree(r->slots);
+    if (r->sizes) (void)hipFree(r->sizes);
+    if (r->uploaded) (void)hipFree(r->uploaded);
+    delete r;
+}
+
+int remo_ring_upload(remo_ring* r, uint32_t batchIndex, const remo_point* host, uint32_t n) {
+    if (!r || (!host && n) || n > r->slotCapacity) return REMO_ERR_INVALID;
+    const uint32_t slot = batchIndex % r->numSlots;
+    if (n && hipMemcpy(r->slots + uint64_t(slot) * r->slotCapacity, host, sizeof(remo_point) * uint64_t(n),
+                       hipMemcpyHostToDevice) != hipSuccess)
+        return REMO_ERR_DEVICE;
+    r->hostSizes[slot] = n;
+    if (hipMemcpy(r->sizes + slot, &n, 4, hipMemcpyHostToDevice) != hipSuccess) return REMO_ERR_DEVICE;
+    if (batchIndex + 1 > r->hostUploaded) {
+        r->hostUploaded = batchIndex + 1;
+        if (hipMemcpy(r->uploaded, &r->hostUploaded, 4, hipMemcpyHostToDevice) != hipSuccess)
+            return REMO_ERR_DEVICE;
+    }
+    return REMO_OK;
+}
+
+int remo_ring_view(const remo_ring* r, uint32_t numBatchesTotal, remo_batch_view* out) {
+    if (!r || !out) return REMO_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->slots = r->slots;
+    out->batch_sizes = r->sizes;
+    out->num_batches_uploaded = r->uploaded;
+    out->slot_capacity = r->slotCapacity;
+    out->num_slots = r->numSlots;
+    out->num_batches_total = numBatchesTotal;
+    return REMO_OK;
+}
+
+int remo_ring_rewind(remo_ring* r) {
+    if (!r) return REMO_ERR_INVALID;
+    r->hostUploaded = 0;
+    return hipMemset(r->uploaded, 0, 4) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
+}
+
+// ---- plumbing -----------------------------------------------------------------
+
+int remo_dev_alloc(size_t bytes, void** out) {
+    if (!out) return REMO_ERR_INVALID;
+    return hipMalloc(out, std::max<size_t>(bytes, 16)) == hipSuccess ? REMO_OK : REMO_ERR_OOM;
+}
+
+int remo_dev_free(void* p) { return hipFree(p) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE; }
+
+int remo_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
+}
+
+int remo_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
+}
+
+int remo_device_count(int* out) {
+    if (!out) return REMO_ERR_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *out = n;
+    return REMO_OK;
+}
+
+}  // extern "C"

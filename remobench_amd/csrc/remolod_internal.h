@@ -1,0 +1,201 @@
+// Internal device-side layout of one RemoLOD octree on a gfx950 GPU.
+//
+// Nodes are structure-of-arrays so that the hot descent reads one 4-byte word
+// per level (first_child) out of an array that stays L2-resident, instead of
+// chasing 152-byte Node records (kernels/remolod/remolod_structures.cuh:70-135).
+// All 8 children of a split are allocated contiguously, so child i of node n is
+// first_child[n] + i with i = (cx << 2) | (cy << 1) | cz, the reference's order.
+//
+// Point and voxel storage keeps the reference's 1000-point chunks and its
+// persistent-store byte strides (AllocatorGlobal, kernels/simlod/utils.h.cu:185-199:
+// 16,032 B per chunk, 262,160 B per 128^3 occupancy grid, first offset 16), but
+// replaces the singly linked chunk list with a per-node chunk directory, so slot
+// s of a node is directory[s / 1000] in one load rather than a list walk
+// (remolod_octree.cu:845-857).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/remolod.h"
+
+namespace remo_hip {
+
+constexpr uint32_t kMaxPointsPerNode = 50000;
+constexpr uint32_t kPointsPerChunk = 1000;
+constexpr uint32_t kGridSize = 128;
+constexpr uint32_t kGridWords = 65536;
+constexpr int kMaxDepth = 20;
+constexpr uint64_t kChunkAllocBytes = 16032;   // 16 * ((16,016 + 16) / 16)
+constexpr uint64_t kGridAllocBytes = 262160;   // 16 * ((262,144 + 16) / 16)
+constexpr uint64_t kArenaFirstOffset = 16;
+constexpr uint64_t kSafetyMargin = 200000000ull;  // remolod_octree.cu:1195
+constexpr uint32_t kPointDir = 64;     // chunk slots per leaf: 64,000 points
+constexpr uint32_t kVoxelDir = 2112;   // >= ceil(2^21 / 1000) chunk slots per grid
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kMaxRounds = 20;    // the expand loop bound, remolod_octree.cu:411
+
+// Device-resident counters; one instance per context.
+struct Globals {
+    uint32_t numNodes;
+    uint32_t numGrids;
+    uint64_t arenaOffset;          // persistent-store bump offset (reference strides)
+    int64_t numAllocatedChunks;    // point chunks in use == chunk-queue stack pointer
+    int64_t chunkPoolSize;         // high-water mark of numAllocatedChunks
+    uint32_t numSpilled;
+    uint32_t numBacklog;
+    uint32_t numCandidates;        // split candidates of the current round
+    uint32_t numAccepted;          // accepted splits of the current round
+    uint32_t numPending;           // new children that must split again this batch
+    uint32_t stop;                 // set when the margin or the step budget is reached
+    uint32_t batchletIndex;
+    uint32_t memCapacityReached;
+    uint64_t numPointsProcessed;
+    // health flags
+    uint32_t allocOverflow, nodePoolOverflow, spillOverflow, backlogOverflow;
+    uint32_t dirOverflow, candidateOverflow, budgetStop, pad1;
+    uint64_t stepStart;            // wall_clock64() at the start of the step
+    // per-step counters
+    uint64_t batches, expandIters, nodesSplit, spilledPoints, newVoxels;
+    // the same, cumulative since reset
+    uint64_t totBatches, totExpandIters, totNodesSplit, totSpilledPoints, totNewVoxels;
+    // AccumGlobals (include/remo/RemoAccum.h:33-40)
+    uint64_t mortonWatermark, numAccumulated, sumLeafCounts;
+    uint32_t pointsFolded, leavesFolded, innerWithSums, pad2;
+    // stats sweep
+    uint32_t sInner, sLeaves, sNonempty, sMaxPoints;
+    uint64_t sPoints, sVoxels;
+    uint32_t sChunksPoints, sChunksVoxels, sUnderflows, sMaxDepth;
+};
+
+// Everything a kernel needs, passed by value (one kernel argument).
+struct DevTree {
+    // node structure-of-arrays, capacity maxNodes
+    int32_t* firstChild;
+    uint32_t* level;
+    uint32_t* nx;
+    uint32_t* ny;
+    uint32_t* nz;
+    uint32_t* counter;
+    uint32_t* numPoints;
+    uint32_t* numVoxels;
+    uint32_t* numVoxelsStored;
+    uint32_t* gridNo;       // kNone: no grid
+    uint32_t* storedOct;    // [n * 8] octant histogram of the stored points (leaves)
+    uint32_t* batchOct;     // [n * 8] octant histogram counted this round
+    uint32_t* pending;      // 1: created this batch with > 50,000 points
+    uint32_t* pointDir;     // [n * kPointDir] chunk refs (arena offset >> 4), 0 = none
+    // grids, capacity maxGrids
+    uint64_t* gridOffset;   // arena offset of grid g
+    uint32_t* voxelDir;     // [g * kVoxelDir] chunk refs
+    uint32_t* voxelChunks;  // [g] chunks in the directory
+    // persistent store
+    uint8_t* arena;
+    uint64_t arenaCapacity;
+    uint32_t* chunkQueue;   // recycled point chunks (remolod_octree.cu:349-360, :763-770)
+    uint64_t chunkQueueCap;
+    // per-batch scratch
+    remo_point* spill;
+    uint32_t spillCap;
+    uint32_t backlogCap;
+    remo_point* backlogPts;
+    uint32_t* backlogNode;
+    uint32_t* candidates;   // split candidates of a round
+    uint32_t* acceptedSpill;// [candidateCap * 2] (node, spill offset) of accepted splits
+    uint32_t candidateCap;
+    uint32_t maxNodes;
+    uint32_t maxGrids;
+    float octreeSize;
+    Globals* g;
+    remo_node_accum* accums;
+};
+
+// A batch in the ring: batch B at slots + (B % numSlots) * slotCapacity,
+// batch_sizes[B % numSlots] points.
+struct BatchRef {
+    const remo_point* points;
+    const uint32_t* sizePtr;
+};
+
+// ---- device helpers -------------------------------------------------------
+
+// Saturating float -> u32 (NaN and negatives to 0), the device cvt's rule.
+__device__ __forceinline__ uint32_t f2u_sat(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return uint32_t(f);
+}
+
+// remolod_octree.cu:163-169 with octreeMin = 0: pX = u32(2^28 * x / size).
+// The 20-bit coordinate the descent uses is pX >> 8: 2^28 * x / size is
+// 2^8 * (2^20 * x / size) exactly, since scaling by a power of two commutes
+// with IEEE rounding, and floor(256 f) >> 8 == floor(f).
+__device__ __forceinline__ uint32_t quant28(float c, float size) {
+    const float num = 268435456.0f * c;
+    return f2u_sat(__fdiv_rn(num, size));
+}
+
+// Root -> leaf descent on the 20-bit coordinate (remolod_octree.cu:182-199).
+__device__ __forceinline__ uint32_t descend(const int32_t* __restrict__ firstChild, uint32_t X,
+                                            uint32_t Y, uint32_t Z, uint32_t& levelOut) {
+    uint32_t cur = 0;
+    uint32_t level = 0;
+    for (; level < uint32_t(kMaxDepth); ++level) {
+        const int32_t fc = firstChild[cur];
+        if (fc < 0) break;
+        const uint32_t sh = kMaxDepth - level - 1;
+        const uint32_t ci = (((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) | ((Z >> sh) & 1u);
+        cur = uint32_t(fc) + ci;
+    }
+    levelOut = level;
+    return cur;
+}
+
+__device__ __forceinline__ uint32_t octantAt(uint32_t X, uint32_t Y, uint32_t Z, uint32_t level) {
+    const uint32_t sh = kMaxDepth - level - 1;
+    return (((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) | ((Z >> sh) & 1u);
+}
+
+// XCD-aware block index: consecutive hardware block ids go round-robin to the 8
+// XCDs, so renumber blocks so that each XCD owns one contiguous range of the
+// logical index space (its L2 then holds that range's nodes and grid words).
+__device__ __forceinline__ uint32_t xcdBlock() {
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    if ((n & 7u) != 0u) return b;
+    return (b & 7u) * (n >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanesBelow() {
+    const uint32_t l = laneId();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ remo_point* chunkPtr(uint8_t* arena, uint32_t ref) {
+    return reinterpret_cast<remo_point*>(arena + (uint64_t(ref) << 4));
+}
+
+// ---- host launchers --------------------------------------------------------
+struct LaunchShape {
+    uint32_t pointBlocks;   // blocks of the point-parallel kernels
+    uint32_t nodeBlocks;    // blocks of the node-parallel kernels
+};
+
+void launchResetTree(const DevTree& t, hipStream_t s);
+void launchBeginStep(const DevTree& t, hipStream_t s);
+void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, hipStream_t s);
+void launchCount(const DevTree& t, BatchRef b, uint32_t round, const LaunchShape& ls, hipStream_t s);
+void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, hipStream_t s);
+void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchSpillCopy(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchZeroGrids(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
+void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
+void launchInsertVoxels(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchEndBatch(const DevTree& t, BatchRef b, hipStream_t s);
+void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchAccumulate(const DevTree& t, uint32_t batchIndex, const LaunchShape& ls, hipStream_t s);
+
+}  // namespace remo_hip

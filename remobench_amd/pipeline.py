@@ -1,0 +1,371 @@
+"""Python mirror of the reference's pipeline-plugin interface, over the C ABI.
+
+Same method set and lifecycle as remo::ILodPipeline (include/remo/ILodPipeline.h:118-145):
+info -> init_programs -> allocate -> reset -> per frame render, pump, build.
+`RemolodPipeline` is the `remolod` plugin served by the HIP path; `PointRing`
+plays PointSource's part (include/remo/PointSource.h:69-115): a device ring of
+batch slots that the host fills behind the consumer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lib as L
+
+
+@dataclass
+class PipelineInfo:
+    """remo::PipelineInfo (include/remo/ILodPipeline.h:34-63)."""
+    id: str
+    display_name: str
+    progressive: bool = False
+    needs_whole_cloud_resident: bool = False
+    ring_slots: int = 0
+    bytes_per_point_estimate: float = 0.0
+    min_bytes_per_point_estimate: float = 0.0
+    min_store_bytes: int = 0
+    fixed_bytes_estimate: int = 0
+
+
+@dataclass
+class DeviceBudget:
+    """remo::DeviceBudget (include/remo/ILodPipeline.h:65-69)."""
+    vram_total: int = 0
+    vram_free_at_startup: int = 0
+    bytes: int = 0
+
+
+@dataclass
+class FrameContext:
+    """remo::FrameContext (include/remo/ILodPipeline.h:77-86), without GL targets."""
+    uniforms: L.SharedUniforms = field(default_factory=L.SharedUniforms)
+    width: int = 0
+    height: int = 0
+    framebuffer: int = 0          # device pointer to u64[width * height], 0: no render
+    strict_timing: bool = False
+
+
+@dataclass
+class TimingScopes:
+    build: list
+    render: str
+
+
+class PointRing:
+    """Device ring of batch slots; batch B goes to slot B % num_slots."""
+
+    def __init__(self, slot_capacity: int = 1_000_000, num_slots: int = 50):
+        self.lib = L.load()
+        self.h = C.c_void_p()
+        L.check(self.lib.remo_ring_create(slot_capacity, num_slots, C.byref(self.h)), "ring")
+        self.slot_capacity = slot_capacity
+        self.num_slots = num_slots
+        self.num_batches_total = 0
+        self.produced = 0
+        self.consumed = 0
+        self.overran = False
+        self._points = None
+        self._sizes = []
+
+    def close(self):
+        if self.h:
+            self.lib.remo_ring_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def set_cloud(self, points: np.ndarray, batch_size: int):
+        """Streams `points` in batches of `batch_size` (<= slot capacity)."""
+        if batch_size > self.slot_capacity:
+            raise ValueError("batch larger than a ring slot")
+        self._points = np.ascontiguousarray(points, dtype=L.POINT_DTYPE)
+        n = len(self._points)
+        self._sizes = [min(batch_size, n - s) for s in range(0, n, batch_size)]
+        self.num_batches_total = len(self._sizes)
+        self.rewind()
+
+    def rewind(self):
+        L.check(self.lib.remo_ring_rewind(self.h), "ring rewind")
+        self.produced = 0
+        self.consumed = 0
+
+    def pump(self, max_batches: int | None = None):
+        """Fills as many slots as the consumer window allows (PointSource::pump)."""
+        limit = self.consumed + self.num_slots
+        off = sum(self._sizes[: self.produced])
+        done = 0
+        while self.produced < min(limit, self.num_batches_total):
+            if max_batches is not None and done >= max_batches:
+                break
+            n = self._sizes[self.produced]
+            chunk = self._points[off: off + n]
+            L.check(self.lib.remo_ring_upload(self.h, self.produced, chunk.ctypes.data, n), "upload")
+            off += n
+            self.produced += 1
+            done += 1
+
+    def set_batches_consumed(self, n: int):
+        self.consumed = n
+
+    def view(self) -> L.BatchView:
+        v = L.BatchView()
+        L.check(self.lib.remo_ring_view(self.h, self.num_batches_total, C.byref(v)), "ring view")
+        return v
+
+
+class Tree:
+    """One HIP octree context: the C ABI with numpy-returning dumps."""
+
+    def __init__(self, meta: L.CloudMeta, *, max_nodes=0, persistent_bytes=0, batches_per_step=0,
+                 step_budget_ms=-1.0, slot_capacity=0, ring_slots=0, spill_capacity=0,
+                 backlog_capacity=0, accumulate=True):
+        self.lib = L.load()
+        cfg = L.Config()
+        cfg.persistent_bytes = persistent_bytes
+        cfg.max_nodes = max_nodes
+        cfg.max_batches_per_step = batches_per_step
+        cfg.step_budget_ms = step_budget_ms
+        cfg.slot_capacity = slot_capacity
+        cfg.ring_slots = ring_slots
+        cfg.spill_capacity = spill_capacity
+        cfg.backlog_capacity = backlog_capacity
+        cfg.flags = 0 if accumulate else L.REMO_FLAG_NO_ACCUM
+        self.cfg = cfg
+        self.meta = meta
+        self.h = C.c_void_p()
+        L.check(self.lib.remo_create(C.byref(cfg), C.byref(meta), C.byref(self.h)), "remo_create")
+
+    def close(self):
+        if self.h:
+            self.lib.remo_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _ck(self, code, what):
+        return L.check(code, what, self.h)
+
+    def reset(self):
+        self._ck(self.lib.remo_reset(self.h), "reset")
+
+    def build_step(self, view: L.BatchView) -> tuple[int, int]:
+        consumed = C.c_uint32(0)
+        rc = self._ck(self.lib.remo_build_step(self.h, C.byref(view), C.byref(consumed)), "build_step")
+        return rc, consumed.value
+
+    def accumulate(self, batch_index: int):
+        self._ck(self.lib.remo_accumulate(self.h, batch_index), "accumulate")
+
+    def analysis(self, uniforms: L.SharedUniforms, mode: int = L.REMO_ANALYSIS_ANALYTIC):
+        self._ck(self.lib.remo_analysis(self.h, C.byref(uniforms), mode), "analysis")
+
+    def render(self, uniforms: L.SharedUniforms, fb_ptr: int):
+        self._ck(self.lib.remo_render(self.h, C.byref(uniforms), C.c_void_p(fb_ptr)), "render")
+
+    def sync(self):
+        self._ck(self.lib.remo_sync(self.h), "sync")
+
+    def profile_enable(self, on: bool = True):
+        self._ck(self.lib.remo_profile_enable(self.h, int(on)), "profile_enable")
+
+    def profile_clear(self):
+        self._ck(self.lib.remo_profile_clear(self.h), "profile_clear")
+
+    def profile(self) -> dict:
+        """{scope: (total_ms, launches)} from the HIP-event scopes."""
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_profile_read(self.h, None, 0, C.byref(n)), "profile_read")
+        arr = (L.ProfileEntry * n.value)()
+        self._ck(self.lib.remo_profile_read(self.h, arr, n.value, C.byref(n)), "profile_read")
+        return {e.name.decode(): (e.total_ms, e.launches) for e in arr}
+
+    def stats(self) -> dict:
+        s = L.Stats()
+        self._ck(self.lib.remo_get_stats(self.h, C.byref(s)), "stats")
+        return s.as_dict()
+
+    def nodes(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_dump_nodes(self.h, None, 0, C.byref(n)), "dump_nodes")
+        out = np.zeros(n.value, dtype=L.NODE_DTYPE)
+        self._ck(self.lib.remo_dump_nodes(self.h, out.ctypes.data, len(out), C.byref(n)), "dump_nodes")
+        return out
+
+    def samples(self, node: int, voxels: bool = False) -> np.ndarray:
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_dump_node_samples(self.h, node, int(voxels), None, 0, C.byref(n)), "samples")
+        out = np.zeros(n.value, dtype=L.POINT_DTYPE)
+        if n.value:
+            self._ck(self.lib.remo_dump_node_samples(self.h, node, int(voxels), out.ctypes.data,
+                                                     len(out), C.byref(n)), "samples")
+        return out
+
+    def grid(self, node: int) -> np.ndarray:
+        out = np.zeros(65536, dtype=np.uint32)
+        self._ck(self.lib.remo_dump_grid(self.h, node, out.ctypes.data), "grid")
+        return out
+
+    def accums(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_dump_accums(self.h, None, 0, C.byref(n)), "accums")
+        out = np.zeros(n.value, dtype=L.ACCUM_DTYPE)
+        self._ck(self.lib.remo_dump_accums(self.h, out.ctypes.data, len(out), C.byref(n)), "accums")
+        return out
+
+    def analysis_results(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_dump_analysis(self.h, None, 0, C.byref(n)), "analysis dump")
+        out = np.zeros(n.value, dtype=L.ANALYSIS_DTYPE)
+        self._ck(self.lib.remo_dump_analysis(self.h, out.ctypes.data, len(out), C.byref(n)),
+                 "analysis dump")
+        return out
+
+
+class RemolodPipeline:
+    """The `remolod` plugin (src/pipelines/RemolodPipeline.{h,cpp}) on the HIP path."""
+
+    MOMENTARY_BYTES = 512 << 20
+    BYTES_PER_POINT = 48.0
+    MIN_BYTES_PER_POINT = 26.0
+    MIN_PERSISTENT = 512 << 20
+    MAX_NODES = 200_000
+
+    def __init__(self, accumulate: bool = True, batches_per_step: int = 20, step_budget_ms: float = 10.0,
+                 analysis_mode: int | None = None):
+        self.accum_enabled = accumulate
+        self.batches_per_step = batches_per_step
+        self.step_budget_ms = step_budget_ms
+        self.analysis_mode = analysis_mode
+        self.tree: Tree | None = None
+        self.needs_reset = True
+        self.complete = False
+        self.batches_consumed = 0
+        self.batches_total = 0
+        self._stats = {}
+        self._ring_mismatch_reported = False
+
+    def info(self) -> PipelineInfo:
+        return PipelineInfo(
+            id="remolod", display_name="RemoLOD", progressive=True,
+            needs_whole_cloud_resident=False, ring_slots=50,
+            bytes_per_point_estimate=self.BYTES_PER_POINT,
+            min_bytes_per_point_estimate=self.MIN_BYTES_PER_POINT,
+            min_store_bytes=self.MIN_PERSISTENT,
+            fixed_bytes_estimate=self.MOMENTARY_BYTES + self.MAX_NODES * (152 + 112))
+
+    def init_programs(self) -> tuple[bool, str]:
+        # Code objects are linked ahead of time; loading the library is the whole job.
+        try:
+            L.load()
+        except RuntimeError as e:
+            return False, str(e)
+        return True, ""
+
+    def allocate(self, meta: L.CloudMeta, budget: DeviceBudget, ring: PointRing | None = None
+                 ) -> tuple[bool, str]:
+        self.release()
+        if meta.num_points == 0:
+            return False, "cloud is empty"
+        slot_cap = ring.slot_capacity if ring else 1_000_000
+        batches = (meta.num_points + slot_cap - 1) // slot_cap
+        slots = min(ring.num_slots if ring else 50, batches)
+        input_bytes = slots * slot_cap * 16
+        available = max(0, budget.bytes - input_bytes)
+        fixed = self.MAX_NODES * (152 + 112) + self.MOMENTARY_BYTES
+        want = max(int(self.BYTES_PER_POINT * meta.num_points), self.MIN_PERSISTENT)
+        if available < fixed + self.MIN_PERSISTENT:
+            return False, (f"not enough device memory: needs at least "
+                           f"{(fixed + self.MIN_PERSISTENT) >> 20} MB, {available >> 20} MB available")
+        persistent = min(want, available - fixed)
+        try:
+            self.tree = Tree(meta, persistent_bytes=persistent, batches_per_step=self.batches_per_step,
+                             step_budget_ms=self.step_budget_ms,
+                             slot_capacity=slot_cap, ring_slots=ring.num_slots if ring else 50,
+                             accumulate=self.accum_enabled)
+        except L.RemoError as e:
+            return False, str(e)
+        self.needs_reset = True
+        self.complete = False
+        return True, ""
+
+    def release(self):
+        if self.tree:
+            self.tree.close()
+        self.tree = None
+        self.complete = False
+        self.needs_reset = True
+
+    def reset(self):
+        self.needs_reset = True
+        self.complete = False
+        self.batches_consumed = 0
+
+    def build(self, source: PointRing, frame: FrameContext) -> bool:
+        """ILodPipeline::build (src/pipelines/RemolodPipeline.cpp:296-438)."""
+        if self.tree is None:
+            return False
+        if self.needs_reset:
+            self.tree.reset()
+            self.needs_reset = False
+            self.complete = False
+            self.batches_consumed = 0
+            source.rewind()
+            return True
+        if self.complete:
+            return False
+        view = source.view()
+        self.batches_total = view.num_batches_total
+        try:
+            rc, consumed = self.tree.build_step(view)
+        except L.RemoError as e:
+            if e.code == -4 and not self._ring_mismatch_reported:
+                self._ring_mismatch_reported = True
+            if e.code == -4:
+                return False
+            raise
+        self.batches_consumed = consumed
+        source.set_batches_consumed(consumed)
+        if self.accum_enabled:
+            self.tree.accumulate(consumed)
+        self._stats = self.tree.stats()
+        if self.batches_total > 0 and consumed >= self.batches_total:
+            self.complete = True
+            return False
+        return True
+
+    def render(self, frame: FrameContext):
+        if self.tree is None or self.needs_reset or not frame.framebuffer:
+            return
+        self.tree.render(frame.uniforms, frame.framebuffer)
+
+    def stats(self) -> dict:
+        return dict(self._stats)
+
+    def timing_scopes(self) -> TimingScopes:
+        return TimingScopes(build=["remolod.reset", "remolod.construct", "remolod.accumulate"],
+                            render="remolod.render")
+
+    def diagnostics(self) -> list[str]:
+        """RemolodPipeline::diagnostics (src/pipelines/RemolodPipeline.cpp:603-726)."""
+        s = self._stats
+        out = []
+        if s.get("node_pool_overflow"):
+            out.append("node pool\tOVERFLOW (raise MAX_NODES_CAPACITY)")
+        u = s.get("counter_underflows", 0)
+        out.append("leaf counter >= pts\t" + ("ok" if u == 0 else f"VIOLATED in {u} leaves (counter < numPoints)"))
+        if not self.accum_enabled:
+            out.append("accumulator\toff (--remolod-no-accum)")
+            return out
+        agree = s.get("sum_leaf_counts") == s.get("num_points")
+        out.append(f"accum sum/points\t{s.get('sum_leaf_counts')} / {s.get('num_points')}  "
+                   f"{'ok' if agree else 'MISMATCH'}")
+        iw = s.get("inner_with_sums", 0)
+        out.append(f"accum inner w/ sums\t{iw}  {'ok' if iw == 0 else 'MISMATCH'}")
+        out.append(f"accum folded total\t{s.get('num_accumulated')}")
+        out.append(f"accum watermark\t0x{s.get('morton_watermark', 0):x}")
+        return out

@@ -1,0 +1,107 @@
+"""GPU parity of Update + Accumulate against the CPU oracle, through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleTree
+from remobench_amd import lib as L
+from remobench_amd.pipeline import PointRing, Tree
+
+from .cases import lidar_scene, tiny_cases
+from .helpers import batch_arrays, compare_trees, keyed, stats_subset
+
+
+def build_hip(points, meta, batch, *, num_slots=None, batches_per_step=20, **kw):
+    nb = (len(points) + batch - 1) // batch
+    num_slots = num_slots or nb
+    ring = PointRing(batch, num_slots)
+    ring.set_cloud(points, batch)
+    tree = Tree(meta, slot_capacity=batch, ring_slots=num_slots, batches_per_step=batches_per_step,
+                step_budget_ms=-1.0, **kw)
+    steps = 0
+    while True:
+        ring.pump()
+        rc, consumed = tree.build_step(ring.view())
+        ring.set_batches_consumed(consumed)
+        tree.accumulate(consumed)
+        steps += 1
+        if rc == L.REMO_DONE or steps > 10_000:
+            break
+        if tree.stats()["mem_capacity_reached"]:
+            break
+    return tree, ring
+
+
+def build_oracle(points, meta, batch, batches_per_step=20, **kw):
+    slots, sizes, nb = batch_arrays(points, batch)
+    o = OracleTree(meta.box_size, batches_per_step=batches_per_step, **kw)
+    while True:
+        before = o.stats()["batchlet_index"] if o.stats()["num_nodes"] else 0
+        done = o.build_step(slots, sizes, nb, batch, nb)
+        o.accumulate(done)
+        if done >= nb or done == before:
+            break
+    return o
+
+
+def compare_accums(tree, oracle):
+    an, on = tree.nodes(), oracle.nodes()
+    ka, ko = keyed(an), keyed(on)
+    aa, oa = tree.accums(), oracle.accums()
+    for k, ia in ka.items():
+        x, y = aa[ia], oa[ko[k]]
+        assert x["count"] == y["count"], k
+        assert np.array_equal(x["q1"], y["q1"]), f"fixed-point sum differs at {k}"
+        assert np.array_equal(x["q2"], y["q2"]), f"fixed-point second moments differ at {k}"
+        assert x["morton_max"] == y["morton_max"], k
+        assert x["last_touched_batch"] == y["last_touched_batch"], k
+        # FP64: both round differently (thread order); relative 1e-9 of the magnitude
+        np.testing.assert_allclose(x["g"], y["g"], rtol=1e-9, atol=1e-9 * max(1, int(x["count"])))
+        # FP32 colour sums: the oracle sums serially (error ~ n eps); 5e-3 relative
+        np.testing.assert_allclose(x["c"], y["c"], rtol=5e-3, atol=1e-3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(tiny_cases().keys()))
+def test_tiny_case_parity(gpu, name):
+    points, meta, batch = tiny_cases()[name]
+    size = max(meta.box_size)
+    print(f"case {name}: {len(points)} points, batch {batch}", flush=True)
+    tree, _ = build_hip(points, meta, batch)
+    o = build_oracle(points, meta, batch)
+    compare_trees(tree, o, size, points)
+    assert stats_subset(tree.stats()) == stats_subset(o.stats())
+    compare_accums(tree, o)
+    s = tree.stats()
+    assert s["sum_leaf_counts"] == s["num_points"] == len(points)
+    assert s["inner_with_sums"] == 0
+
+
+@pytest.mark.gpu
+def test_lidar_2m_parity(gpu):
+    """The 2M-point lidar scene in 8 batches of 262,144: bit-exact structure."""
+    points, meta = lidar_scene()
+    size = max(meta.box_size)
+    print("lidar 2M: building on the GPU", flush=True)
+    tree, _ = build_hip(points, meta, 262_144)
+    print("lidar 2M: oracle", flush=True)
+    o = build_oracle(points, meta, 262_144)
+    compare_trees(tree, o, size, points)
+    assert stats_subset(tree.stats()) == stats_subset(o.stats())
+    compare_accums(tree, o)
+    s, so = tree.stats(), o.stats()
+    for k in ("morton_watermark", "num_accumulated", "sum_leaf_counts", "inner_with_sums"):
+        assert s[k] == so[k], k
+
+
+@pytest.mark.gpu
+def test_wrapping_ring_and_small_steps(gpu):
+    """A 3-slot ring that wraps, one batch per step: the same tree as one pass."""
+    points, meta = lidar_scene(600_000, seed=7)
+    size = max(meta.box_size)
+    tree, _ = build_hip(points, meta, 50_000, num_slots=3, batches_per_step=1)
+    o = build_oracle(points, meta, 50_000, batches_per_step=1)
+    compare_trees(tree, o, size, points)
+    assert stats_subset(tree.stats()) == stats_subset(o.stats())
+    compare_accums(tree, o)
