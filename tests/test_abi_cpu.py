@@ -1,0 +1,116 @@
+"""CPU checks of the product library: it loads, exports every C ABI entry point
+include/remolod.h declares, and its host-only parts (scene generator, plugin
+metadata, argument validation) behave. No kernel is launched here."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from remobench_amd import lib as L
+from remobench_amd.pipeline import DeviceBudget, RemolodPipeline
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "remolod.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(remo_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_the_header():
+    # the ctypes mirrors must agree with the C layout the library was built with
+    assert C.sizeof(L.Point) == 16
+    assert C.sizeof(L.SharedUniforms) == 4 * 4 + 5 * 64 + 2 * 12 + 8 + 3 * 4 + 3 * 4 + 6 * 4
+    assert L.NODE_DTYPE.itemsize == 48
+    assert L.ACCUM_DTYPE.itemsize == 72 + 72 + 24 + 16 + 8
+    assert L.ANALYSIS_DTYPE.itemsize == 48
+
+
+def test_scene_is_deterministic_across_thread_counts():
+    a, ma = L.generate_scene(300_000, 11, "lidar", threads=1)
+    b, mb = L.generate_scene(300_000, 11, "lidar", threads=7)
+    assert a.tobytes() == b.tobytes()
+    assert list(ma.box_size) == list(mb.box_size)
+    c, _ = L.generate_scene(300_000, 12, "lidar", threads=7)
+    assert a.tobytes() != c.tobytes()
+
+
+@pytest.mark.parametrize("kind", ["lidar", "skewed", "city"])
+def test_scene_bounds_and_colours(kind):
+    p, meta = L.generate_scene(200_000, 3, kind)
+    for a, ax in enumerate(("x", "y", "z")):
+        assert p[ax].min() == 0.0  # translated so the box minimum is 0
+        assert p[ax].max() == np.float32(meta.box_size[a])
+    assert ((p["color"] >> 24) == 255).all()
+    assert meta.num_points == len(p)
+
+
+def test_scene_is_scan_line_ordered():
+    p, _ = L.generate_scene(400_000, 5, "lidar")
+    lines = np.floor(p["y"].astype(np.float64) / 0.05)
+    w = 4096
+    nw = len(p) // w
+    lo = np.array([lines[i * w:(i + 1) * w].min() for i in range(nw)])
+    hi = np.array([lines[i * w:(i + 1) * w].max() for i in range(nw)])
+    # windows are shuffled internally but ordered between each other
+    assert (hi[:-1] <= lo[1:]).all()
+
+
+def test_skewed_scene_is_skewed():
+    p, meta = L.generate_scene(500_000, 3, "skewed")
+    size = max(meta.box_size)
+    cells = (np.floor(p["x"] / size * 16).astype(int) * 256 + np.floor(p["y"] / size * 16).astype(int) * 16 +
+             np.floor(p["z"] / size * 16).astype(int))
+    counts = np.sort(np.bincount(cells))[::-1]
+    top = counts[: max(1, len(counts) // 20)].sum() / len(p)
+    assert top > 0.5
+
+
+def test_plugin_metadata_and_budget_refusal():
+    pl = RemolodPipeline()
+    info = pl.info()
+    assert info.id == "remolod" and info.progressive and info.ring_slots == 50
+    assert pl.timing_scopes().build == ["remolod.reset", "remolod.construct", "remolod.accumulate"]
+    meta = L.CloudMeta()
+    meta.num_points = 0
+    ok, err = pl.allocate(meta, DeviceBudget(bytes=1 << 40))
+    assert not ok and err == "cloud is empty"
+    meta.num_points = 1000
+    ok, err = pl.allocate(meta, DeviceBudget(bytes=100 << 20))
+    assert not ok and err.startswith("not enough device memory")
+
+
+def test_diagnostics_format():
+    pl = RemolodPipeline()
+    pl._stats = {"counter_underflows": 0, "sum_leaf_counts": 10, "num_points": 10,
+                 "inner_with_sums": 0, "num_accumulated": 10, "morton_watermark": 255,
+                 "node_pool_overflow": 0}
+    d = pl.diagnostics()
+    assert "leaf counter >= pts\tok" in d
+    assert "accum sum/points\t10 / 10  ok" in d
+    assert "accum watermark\t0xff" in d
+
+
+def test_invalid_arguments_are_rejected_without_a_gpu():
+    lib = L.load()
+    h = C.c_void_p()
+    assert lib.remo_create(None, None, C.byref(h)) == -1
+    assert lib.remo_build_step(None, None, None) == -1
+    assert lib.remo_reset(None) == -1
+    spec = L.scene_spec("lidar")
+    spec.extent = 1.0  # too small for the primitives
+    meta = L.CloudMeta()
+    assert lib.remo_scene_generate(C.byref(spec), 1, 10, None, C.byref(meta)) == -1

@@ -1,0 +1,139 @@
+"""Pins the oracle's tree against what the mathematics fixes (CPU only).
+
+* an independent numpy closed form (oracle/closed_form.py): the node set, the
+  leaf counters, the occupied cells of every inner node;
+* hand-derived answers for tiny inputs;
+* batch-order and batch-size invariance of the integer structure;
+* numNodes = 1 + 8 * inner, the shape every published reference count has
+  (4,137 = 1 + 8 * 517, 40,689 = 1 + 8 * 5,086).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import closed_form as CF
+from oracle.oracle import OracleTree
+from remobench_amd import lib as L
+
+from .cases import lidar_scene, make_meta, tiny_cases, uniform_in_box
+from .helpers import batch_arrays, point_rows
+
+
+def oracle_build(points, meta, batch, **kw):
+    slots, sizes, nb = batch_arrays(points, batch)
+    o = OracleTree(meta.box_size, **kw)
+    o.build_step(slots, sizes, nb, batch, nb)
+    o.accumulate(nb)
+    return o
+
+
+def check_against_closed_form(o, points, size, root_quirk=False):
+    nodes = o.nodes()
+    cf = CF.build(points, size)
+    assert len(nodes) == len(cf)
+    for i, r in enumerate(nodes):
+        k = (int(r["level"]), int(r["x"]), int(r["y"]), int(r["z"]))
+        e = cf[k]
+        leaf = r["first_child"] < 0
+        assert leaf == e["leaf"], k
+        if leaf:
+            # a leaf counts every point of its region; an inner node stopped
+            # counting when it split, above MAX_POINTS_PER_NODE
+            assert r["counter"] == e["count"], (k, r["counter"], e["count"])
+        else:
+            assert 50_000 < r["counter"] <= e["count"], k
+        if leaf:
+            assert r["num_points"] == e["count"]
+            got = point_rows(o.samples(i))
+            want = point_rows(points[e["indices"]])
+            assert np.array_equal(got, want), k
+        else:
+            assert r["num_points"] == 0
+            g = o.grid(i)
+            cells = np.nonzero(np.unpackbits(g.view(np.uint8), bitorder="little"))[0]
+            assert np.array_equal(cells, e["cells"]), k
+            if not (root_quirk and k == (0, 0, 0, 0)):
+                assert r["num_voxels"] == len(e["cells"]) == r["num_voxels_stored"], k
+    inner = int((nodes["first_child"] >= 0).sum())
+    assert len(nodes) == 1 + 8 * inner
+
+
+def test_lidar_2m_matches_closed_form():
+    points, meta = lidar_scene()
+    o = oracle_build(points, meta, 262_144)
+    check_against_closed_form(o, points, max(meta.box_size))
+    s = o.stats()
+    assert s["num_points"] == len(points) and s["counter_underflows"] == 0
+
+
+@pytest.mark.parametrize("name", list(tiny_cases().keys()))
+def test_tiny_cases_match_closed_form(name):
+    points, meta, batch = tiny_cases()[name]
+    o = oracle_build(points, meta, batch)
+    # a first batch of <= 50,000 points leaves the root a leaf that samples voxels
+    check_against_closed_form(o, points, max(meta.box_size), root_quirk=batch <= 50_000)
+
+
+def test_hand_derived_counts():
+    # 50,000 points: the root never splits
+    p = uniform_in_box(50_000, (0, 0, 0), (1, 1, 1), 1, None)
+    o = oracle_build(p, make_meta(p, [1, 1, 1]), 50_000)
+    assert o.stats()["num_nodes"] == 1
+    # 50,001 points in [0, 0.5)^3: root splits, child 0 holds 50,001 and splits too
+    p = uniform_in_box(50_001, (0, 0, 0), (0.4999, 0.4999, 0.4999), 2, None)
+    o = oracle_build(p, make_meta(p, [1, 1, 1]), 50_001)
+    n = o.nodes()
+    assert len(n) == 17
+    assert n[0]["counter"] == 50_001 and n[1]["counter"] == 50_001
+    assert all(n[i]["counter"] == 0 for i in range(2, 9))
+    assert sum(int(n[i]["num_points"]) for i in range(9, 17)) == 50_001
+
+
+def test_root_quirk_duplicates_root_voxels():
+    """A root that samples voxels as a leaf and splits later re-samples into a
+    re-zeroed grid without resetting numVoxels (remolod_octree.cu:365-387)."""
+    points, meta, batch = tiny_cases()["root_leaf_voxels"]
+    o = oracle_build(points, meta, batch)
+    cf = CF.build(points, max(meta.box_size))
+    root = o.nodes()[0]
+    ncells = len(cf[(0, 0, 0, 0)]["cells"])
+    assert root["num_voxels"] > ncells
+
+
+def test_box_max_points_wrap_to_child_zero():
+    points, meta, batch = tiny_cases()["box_max_wrap"]
+    q20, _ = CF.quantise(points, 1.0)
+    assert (q20[:3000, 0] == 1 << 20).all()
+    o = oracle_build(points, meta, batch)
+    n = o.nodes()
+    # every x == 1.0 point lands in a child with X == 0 at level 1
+    for i, r in enumerate(n):
+        if r["first_child"] < 0 and r["num_points"]:
+            s = o.samples(i)
+            if (s["x"] == 1.0).any():
+                assert r["x"] == 0
+
+
+def test_batch_size_and_order_invariance():
+    points, meta = lidar_scene(1_500_000, seed=3)
+    size = max(meta.box_size)
+    a = oracle_build(points, meta, 300_000)
+    b = oracle_build(points, meta, 1_000_000)
+    rng = np.random.default_rng(0)
+    shuffled = points[rng.permutation(len(points))]
+    c = oracle_build(shuffled, meta, 77_777)
+    def canon(o):
+        n = o.nodes()
+        # an inner node's counter froze when it split, which depends on batching
+        return sorted((int(r["level"]), int(r["x"]), int(r["y"]), int(r["z"]), int(r["first_child"] < 0),
+                       int(r["counter"]) if r["first_child"] < 0 else -1, int(r["num_points"]),
+                       int(r["num_voxels"])) for r in n)
+    assert canon(a) == canon(b) == canon(c)
+    check_against_closed_form(c, shuffled, size)
+
+
+def test_published_count_shape():
+    # the reference's published trees satisfy numNodes = 1 + 8 * inner
+    for nodes, inner in ((4137, 517), (40689, 5086), (20633, 2579), (5665, 708)):
+        assert nodes == 1 + 8 * inner
