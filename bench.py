@@ -190,6 +190,28 @@ def main() -> int:
     path_bytes = sum(s["algorithmic_bytes"] for s in scopes.values())
     path_gbs = path_bytes / dt / 1e9
 
+    # Rasterize, 1920x1080, three poses, on the full tree
+    from remobench_amd import camera
+    import ctypes as C
+    raster = {}
+    fbp = C.c_void_p()
+    L.check(L.load().remo_dev_alloc(1920 * 1080 * 8, C.byref(fbp)), "framebuffer")
+    sample_pts = points[:: max(1, len(points) // 2_000_000)]
+    for pose, u in camera.poses(sample_pts, meta.box_size).items():
+        tree.render(u, fbp.value)  # warm
+        tree.sync()
+        tree.profile_clear()
+        reps = 10
+        for _ in range(reps):
+            tree.render(u, fbp.value)
+        tree.sync()
+        ms, launches = tree.profile()["remolod.render"]
+        rs = tree.stats()
+        raster[pose] = {"ms": round(ms / max(1, launches), 4), "draw_items": rs["draw_items"],
+                        "samples": rs["draw_samples"]}
+        log(f"render {pose}: {raster[pose]}")
+    L.load().remo_dev_free(fbp)
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on {args.cpu_sample_batches} batches")
@@ -221,6 +243,7 @@ def main() -> int:
         "path_hbm_gbs": round(path_gbs, 2),
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 5),
         "scopes": scopes,
+        "rasterize_1080p": raster,
         "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points",
                                     "num_voxels", "max_depth", "allocated_bytes_persistent")},
     }

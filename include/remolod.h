@@ -111,6 +111,9 @@ typedef struct remo_config {
     uint32_t spill_capacity;        /* spilled points per batch (10,000,000)              */
     uint32_t backlog_capacity;      /* new voxels per batch (10,000,000)                  */
     uint32_t flags;                 /* REMO_FLAG_*                                        */
+    uint32_t closure_batches;       /* Analysis: a leaf untouched for this many batches is
+                                       closed (8); the reference's named alternative to a
+                                       Morton watermark, plans/03_AccumulatorHook.md:507-509 */
 } remo_config;
 
 #define REMO_FLAG_NO_ACCUM   1u     /* --remolod-no-accum                       */

@@ -415,6 +415,12 @@ struct oracle_tree {
         return batchletIndex;
     }
 
+    void analyse(const remo_shared_uniforms& u, uint32_t closureMargin,
+                 std::vector<struct oracle_analysis_out>& out);
+
+    void render(const remo_shared_uniforms& u, uint64_t* fb, uint32_t* numItems, uint64_t* numSamples,
+                uint32_t capacity);
+
     void sweepStats() {
         remo_stats& s = lastStats;
         std::memset(&s, 0, sizeof(s));
@@ -543,6 +549,390 @@ struct oracle_tree {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// Analysis (spec only in the reference: README.md:80, wiki/02_KernelPipeline.md:205-224,
+// include/remo/RemoAccum.h:18-30). The build's definitions, restated here:
+//  * roll-up: a leaf's fixed-point sums (units 2^-(20+L) of the root extent,
+//    centred on the leaf) are rescaled to units U = 2^-(20+D), D the deepest
+//    node level, and re-centred on each ancestor with the exact child offset
+//    +-2^(18+D-L) per axis; everything in __int128;
+//  * covariance in node-extent units from num = n*S2 - S1*S1^T (exact int128 when
+//    it fits, FP64 otherwise), cov = num / n^2 / 4^(20+D-L);
+//  * eigenvalues: cyclic Jacobi on cov / trace, in FP32, redone in FP64 when the
+//    smallest gap is below 1e-4 of the trace; FP64 Jacobi is the reference;
+//  * surface variation l0/(l0+l1+l2), planarity (l1-l0)/l2, linearity (l2-l1)/l2;
+//  * coverage: projected 8-corner screen extent dx*dy, clamped to the screen;
+//    score = 3 * surface variation * coverage;
+//  * closure: a leaf is closed when the current batch is >= lastTouchedBatch +
+//    margin; an inner node when all its children are; closed nodes are Finalized.
+
+namespace anal {
+
+using i128 = __int128;
+
+inline double i128ToDouble(i128 x) {
+    const bool neg = x < 0;
+    unsigned __int128 u = neg ? (unsigned __int128)(-x) : (unsigned __int128)x;
+    const double hi = double(uint64_t(u >> 64));
+    const double lo = double(uint64_t(u));
+    const double d = hi * 18446744073709551616.0 + lo;
+    return neg ? -d : d;
+}
+
+inline int bitsU128(unsigned __int128 u) {
+    int b = 0;
+    while (u) {
+        ++b;
+        u >>= 1;
+    }
+    return b;
+}
+inline int bitsI128(i128 x) { return bitsU128(x < 0 ? (unsigned __int128)(-x) : (unsigned __int128)x); }
+
+template <typename T>
+void jacobi3(T a[6], T ev[3]) {
+    // a: a00 a01 a02 a11 a12 a22
+    T m[3][3] = {{a[0], a[1], a[2]}, {a[1], a[3], a[4]}, {a[2], a[4], a[5]}};
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        const T off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+        if (!(off > T(0))) break;
+        for (int pq = 0; pq < 3; ++pq) {
+            const int p = pq == 2 ? 1 : 0;
+            const int q = pq == 0 ? 1 : 2;
+            const T apq = m[p][q];
+            if (apq == T(0)) continue;
+            const T theta = (m[q][q] - m[p][p]) / (T(2) * apq);
+            const T at = theta < T(0) ? -theta : theta;
+            T t;
+            if (at > T(1e15))
+                t = T(0.5) / theta;
+            else
+                t = (theta < T(0) ? T(-1) : T(1)) / (at + std::sqrt(theta * theta + T(1)));
+            const T c = T(1) / std::sqrt(t * t + T(1));
+            const T sn = t * c;
+            const int r = 3 - p - q;
+            const T arp = m[r][p], arq = m[r][q];
+            m[p][p] = m[p][p] - t * apq;
+            m[q][q] = m[q][q] + t * apq;
+            m[p][q] = m[q][p] = T(0);
+            const T nrp = c * arp - sn * arq;
+            const T nrq = sn * arp + c * arq;
+            m[r][p] = m[p][r] = nrp;
+            m[r][q] = m[q][r] = nrq;
+        }
+    }
+    ev[0] = m[0][0];
+    ev[1] = m[1][1];
+    ev[2] = m[2][2];
+    // ascending
+    if (ev[0] > ev[1]) std::swap(ev[0], ev[1]);
+    if (ev[1] > ev[2]) std::swap(ev[1], ev[2]);
+    if (ev[0] > ev[1]) std::swap(ev[0], ev[1]);
+}
+
+struct Sums {
+    i128 s1[3], s2[6];
+    uint64_t n;
+};
+
+}  // namespace anal
+
+struct oracle_analysis_out {
+    remo_node_analysis r;
+    double lambda64[3];  // FP64 Jacobi on the same covariance (the reference)
+    double cov[6];       // covariance in node-extent units
+    double mean[3];      // mean offset from the node centre, node-extent units
+    int64_t s1_hi[3], s1_lo[3], s2_hi[6], s2_lo[6];  // the rolled-up int128 sums
+    uint64_t n;
+    uint32_t e;          // 20 + D - L: log2 of the node extent in units U
+    uint32_t exact;      // 1: covariance numerator computed exactly in int128
+};
+
+// Projected extent of a node's box (kernels/remolod/remolod_render.cu:30-70).
+static void nodeExtent(const ONode& nd, float cubeSize, const remo_shared_uniforms& u, float& dx, float& dy) {
+    const float size = cubeSize / std::ldexp(1.0f, int(nd.level));
+    const float mn[3] = {0.0f + float(nd.X) * size, 0.0f + float(nd.Y) * size, 0.0f + float(nd.Z) * size};
+    const float mx[3] = {mn[0] + size, mn[1] + size, mn[2] + size};
+    float minX = 1e30f, minY = 1e30f, maxX = -1e30f, maxY = -1e30f;
+    bool any = false;
+    dx = dy = 0.0f;
+    for (int corner = 0; corner < 8; ++corner) {
+        const float x = (corner & 1) ? mx[0] : mn[0];
+        const float y = (corner & 2) ? mx[1] : mn[1];
+        const float z = (corner & 4) ? mx[2] : mn[2];
+        const auto& m = u.transform_frozen.rows;
+        const float cx = m[0][0] * x + m[0][1] * y + m[0][2] * z + m[0][3] * 1.0f;
+        const float cy = m[1][0] * x + m[1][1] * y + m[1][2] * z + m[1][3] * 1.0f;
+        const float cw = m[3][0] * x + m[3][1] * y + m[3][2] * z + m[3][3] * 1.0f;
+        if (cw <= 0.0f) continue;
+        const float invW = 1.0f / cw;
+        const float sx = (cx * invW * 0.5f + 0.5f) * u.width;
+        const float sy = (cy * invW * 0.5f + 0.5f) * u.height;
+        minX = std::fmin(minX, sx);
+        maxX = std::fmax(maxX, sx);
+        minY = std::fmin(minY, sy);
+        maxY = std::fmax(maxY, sy);
+        any = true;
+    }
+    if (!any) return;
+    dx = maxX - minX;
+    dy = maxY - minY;
+}
+
+void oracle_tree::analyse(const remo_shared_uniforms& u, uint32_t closureMargin,
+                          std::vector<oracle_analysis_out>& out) {
+    using namespace anal;
+    const size_t N = nodes.size();
+    out.assign(N, oracle_analysis_out{});
+    uint32_t D = 0;
+    for (const ONode& nd : nodes) D = std::max(D, nd.level);
+    std::vector<Sums> sums(N);
+    std::vector<int> closed(N, 0);
+    // bottom-up: children always have larger indices than their parent here
+    std::vector<uint32_t> order(N);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return nodes[a].level > nodes[b].level; });
+    const uint32_t current = batchletIndex;
+    for (uint32_t id : order) {
+        const ONode& nd = nodes[id];
+        Sums& S = sums[id];
+        std::memset(&S, 0, sizeof(S));
+        if (nd.isLeaf()) {
+            const remo_node_accum& a = nd.acc;
+            const int sh = int(D - nd.level);
+            for (int i = 0; i < 3; ++i) S.s1[i] = i128(a.q1[i]) << sh;
+            for (int i = 0; i < 6; ++i) S.s2[i] = i128(a.q2[i]) << (2 * sh);
+            S.n = a.count;
+            closed[id] = (a.count == 0) || (current >= a.last_touched_batch + closureMargin);
+        } else {
+            const i128 h = i128(1) << (18 + D - nd.level);
+            closed[id] = 1;
+            for (int ci = 0; ci < 8; ++ci) {
+                const uint32_t c = uint32_t(nd.children[ci]);
+                const Sums& C = sums[c];
+                const i128 o[3] = {(ci & 4) ? h : -h, (ci & 2) ? h : -h, (ci & 1) ? h : -h};
+                const i128 n = i128(C.n);
+                for (int i = 0; i < 3; ++i) S.s1[i] += C.s1[i] + n * o[i];
+                const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+                for (int k = 0; k < 6; ++k) {
+                    const int a_ = pi[k], b_ = pj[k];
+                    S.s2[k] += C.s2[k] + o[a_] * C.s1[b_] + C.s1[a_] * o[b_] + n * o[a_] * o[b_];
+                }
+                S.n += C.n;
+                closed[id] &= closed[c];
+            }
+        }
+    }
+    const float cubeSize = octreeSize;
+    for (size_t id = 0; id < N; ++id) {
+        const ONode& nd = nodes[id];
+        const Sums& S = sums[id];
+        oracle_analysis_out& o = out[id];
+        o.n = S.n;
+        o.e = 20 + D - nd.level;
+        for (int i = 0; i < 3; ++i) {
+            o.s1_hi[i] = int64_t(S.s1[i] >> 64);
+            o.s1_lo[i] = int64_t(uint64_t(S.s1[i]));
+        }
+        for (int i = 0; i < 6; ++i) {
+            o.s2_hi[i] = int64_t(S.s2[i] >> 64);
+            o.s2_lo[i] = int64_t(uint64_t(S.s2[i]));
+        }
+        remo_node_analysis& r = o.r;
+        std::memset(&r, 0, sizeof(r));
+        r.n = S.n;
+        float dx, dy;
+        nodeExtent(nd, cubeSize, u, dx, dy);
+        const float cw = std::fmin(dx, u.width), ch = std::fmin(dy, u.height);
+        r.coverage = cw * ch;
+        r.state = closed[id] ? 2u : 0u;
+        if (S.n < 3) continue;
+        // covariance numerator
+        const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+        const i128 n = i128(S.n);
+        int maxS2 = 0, maxS1 = 0;
+        for (int k = 0; k < 6; ++k) maxS2 = std::max(maxS2, bitsI128(S.s2[k]));
+        for (int k = 0; k < 3; ++k) maxS1 = std::max(maxS1, bitsI128(S.s1[k]));
+        const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
+        o.exact = exact;
+        const double scale = std::ldexp(1.0, -int(o.e));
+        const double dn = double(S.n);
+        double cov[6];
+        for (int k = 0; k < 6; ++k) {
+            double num;
+            if (exact) {
+                num = i128ToDouble(n * S.s2[k] - S.s1[pi[k]] * S.s1[pj[k]]);
+            } else {
+                num = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
+            }
+            cov[k] = num / (dn * dn) * scale * scale;
+            o.cov[k] = cov[k];
+        }
+        for (int i = 0; i < 3; ++i) o.mean[i] = i128ToDouble(S.s1[i]) / dn * scale;
+        const double trace = cov[0] + cov[3] + cov[5];
+        if (!(trace > 0.0)) continue;
+        double a64[6];
+        float a32[6];
+        for (int k = 0; k < 6; ++k) {
+            a64[k] = cov[k] / trace;
+            a32[k] = float(a64[k]);
+        }
+        double ev64[3];
+        jacobi3<double>(a64, ev64);
+        float ev32[3];
+        jacobi3<float>(a32, ev32);
+        const float gap = std::fmin(ev32[1] - ev32[0], ev32[2] - ev32[1]);
+        double ev[3];
+        if (gap < 1e-4f) {
+            r.flags |= 1u;
+            for (int i = 0; i < 3; ++i) ev[i] = ev64[i];
+        } else {
+            for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
+        }
+        for (int i = 0; i < 3; ++i) {
+            o.lambda64[i] = ev64[i] * trace;
+            r.lambda[i] = float(ev[i] * trace);
+        }
+        const double sum = ev[0] + ev[1] + ev[2];
+        const float sv = sum > 0.0 ? float(ev[0] / sum) : 0.0f;
+        r.surface_variation = sv;
+        r.planarity = ev[2] > 0.0 ? float((ev[1] - ev[0]) / ev[2]) : 0.0f;
+        r.linearity = ev[2] > 0.0 ? float((ev[2] - ev[1]) / ev[2]) : 0.0f;
+        r.score = 3.0f * sv * r.coverage;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Rasterize: kernel_render's selection and the shared rasteriser
+// (kernels/remolod/remolod_render.cu:72-181, kernels/shared/remo_draw.cuh:118-167,
+// kernels/shared/remo_framebuffer.cuh:214-248, kernels/shared/remo_math.cuh:16-79).
+// Order-independent: a pixel keeps the minimum of (float_bits(depth) << 32) | rgba.
+
+namespace rast {
+
+struct Plane {
+    float nx, ny, nz, d;
+};
+
+inline void frustumPlanes(const remo_mat4& m, Plane P[6]) {
+    const float(*r)[4] = m.rows;
+    const float raw[6][4] = {
+        {r[3][0] - r[0][0], r[3][1] - r[0][1], r[3][2] - r[0][2], r[3][3] - r[0][3]},
+        {r[3][0] + r[0][0], r[3][1] + r[0][1], r[3][2] + r[0][2], r[3][3] + r[0][3]},
+        {r[3][0] + r[1][0], r[3][1] + r[1][1], r[3][2] + r[1][2], r[3][3] + r[1][3]},
+        {r[3][0] - r[1][0], r[3][1] - r[1][1], r[3][2] - r[1][2], r[3][3] - r[1][3]},
+        {r[3][0] - r[2][0], r[3][1] - r[2][1], r[3][2] - r[2][2], r[3][3] - r[2][3]},
+        {r[3][0] + r[2][0], r[3][1] + r[2][1], r[3][2] + r[2][2], r[3][3] + r[2][3]},
+    };
+    for (int i = 0; i < 6; ++i) {
+        const float a = raw[i][0], b = raw[i][1], c = raw[i][2], d = raw[i][3];
+        const float len = std::sqrt(a * a + b * b + c * c);
+        const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+        P[i] = {a * inv, b * inv, c * inv, d * inv};
+    }
+}
+
+inline int32_t f2i_sat(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return int32_t(0x80000000u);
+    return int32_t(f);
+}
+
+inline void drawPoint(uint64_t* fb, const remo_shared_uniforms& u, int32_t w, int32_t h, const remo_point& p) {
+    const auto& M = u.transform.rows;
+    const float cx = M[0][0] * p.x + M[0][1] * p.y + M[0][2] * p.z + M[0][3] * 1.0f;
+    const float cy = M[1][0] * p.x + M[1][1] * p.y + M[1][2] * p.z + M[1][3] * 1.0f;
+    const float cw = M[3][0] * p.x + M[3][1] * p.y + M[3][2] * p.z + M[3][3] * 1.0f;
+    if (cw <= 0.0f) return;
+    const float invW = 1.0f / cw;
+    const float px = (cx * invW * 0.5f + 0.5f) * u.width;
+    const float py = (cy * invW * 0.5f + 0.5f) * u.height;
+    const int64_t half = u.point_size / 2;
+    const int64_t ix = f2i_sat(px), iy = f2i_sat(py);
+    if (ix + half < 0 || iy + half < 0 || ix - half >= w || iy - half >= h) return;
+    uint32_t bits;
+    std::memcpy(&bits, &cw, 4);
+    const uint32_t color = u.color_mode == 3 ? 0xFFFFFFFFu : p.color;  // COLOR_WHITE or COLOR_RGB
+    const uint64_t packed = (uint64_t(bits) << 32) | color;
+    for (int64_t oy = -half; oy <= half; ++oy) {
+        const int64_t sy = iy + oy;
+        if (sy < 0 || sy >= h) continue;
+        for (int64_t ox = -half; ox <= half; ++ox) {
+            const int64_t sx = ix + ox;
+            if (sx < 0 || sx >= w) continue;
+            uint64_t& px64 = fb[sy * w + sx];
+            if (packed < px64) px64 = packed;
+        }
+    }
+}
+
+}  // namespace rast
+
+void oracle_tree::render(const remo_shared_uniforms& u, uint64_t* fb, uint32_t* numItems, uint64_t* numSamples,
+                         uint32_t capacity) {
+    const int32_t w = int32_t(u.width), h = int32_t(u.height);
+    for (int64_t i = 0; i < int64_t(w) * h; ++i) fb[i] = 0x7f80000000332211ull;
+    const uint32_t n = uint32_t(std::min<size_t>(nodes.size(), maxNodes));
+    std::vector<uint8_t> visible(n), large(n);
+    rast::Plane P[6];
+    rast::frustumPlanes(u.transform_frozen, P);
+    for (uint32_t i = 0; i < n; ++i) {
+        const ONode& nd = nodes[i];
+        const float size = octreeSize / std::ldexp(1.0f, int(nd.level));
+        const float mn[3] = {0.0f + float(nd.X) * size, 0.0f + float(nd.Y) * size, 0.0f + float(nd.Z) * size};
+        const float mx[3] = {mn[0] + size, mn[1] + size, mn[2] + size};
+        bool inside = true;
+        for (int k = 0; k < 6 && inside; ++k) {
+            const rast::Plane& p = P[k];
+            const float px = p.nx > 0.0f ? mx[0] : mn[0];
+            const float py = p.ny > 0.0f ? mx[1] : mn[1];
+            const float pz = p.nz > 0.0f ? mx[2] : mn[2];
+            if (p.nx * px + p.ny * py + p.nz * pz + p.d < 0.0f) inside = false;
+        }
+        const bool hasSamples = nd.numPoints > 0 || nd.numVoxels > 0;
+        visible[i] = hasSamples && inside;
+        float dx, dy;
+        nodeExtent(nd, octreeSize, u, dx, dy);
+        large[i] = (dx > u.lod_pixel_budget || dy > u.lod_pixel_budget) ? 1 : 0;
+    }
+    std::vector<uint32_t> items;
+    uint64_t samples = 0;
+    auto emit = [&](uint32_t id) {
+        const ONode& nd = nodes[id];
+        if (nd.numPoints == 0 && nd.numVoxels == 0) return;
+        if (items.size() >= capacity) return;
+        items.push_back(id);
+        samples += uint64_t(nd.numPoints) + nd.numVoxels;
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!large[i]) continue;
+        const ONode& nd = nodes[i];
+        if (!nd.isLeaf()) {
+            for (int c = 0; c < 8; ++c) {
+                const int32_t child = nd.children[c];
+                if (child < 0 || uint32_t(child) >= n) continue;
+                if (large[child] || !visible[child]) continue;
+                emit(uint32_t(child));
+            }
+        } else if (visible[i]) {
+            emit(i);
+        }
+    }
+    if (u.show_points != 0) {
+        for (uint32_t id : items) {
+            const ONode& nd = nodes[id];
+            for (uint32_t k = 0; k < nd.numPoints && k < nd.points.size(); ++k) rast::drawPoint(fb, u, w, h, nd.points[k]);
+            for (uint32_t k = 0; k < nd.numVoxels && k < nd.voxels.size(); ++k) rast::drawPoint(fb, u, w, h, nd.voxels[k]);
+        }
+    }
+    *numItems = uint32_t(items.size());
+    *numSamples = samples;
+}
+
 extern "C" {
 
 oracle_tree* oracle_create(const float boxSize[3], uint32_t maxNodes, uint64_t persistentCapacity,
@@ -606,6 +996,33 @@ int oracle_grid(const oracle_tree* t, uint32_t node, uint32_t* words) {
 
 void oracle_dump_accums(const oracle_tree* t, remo_node_accum* out) {
     for (size_t i = 0; i < t->nodes.size(); ++i) out[i] = t->nodes[i].acc;
+}
+
+// Analysis over the current tree; out has oracle_num_nodes() entries.
+uint32_t oracle_analysis_record_size() { return uint32_t(sizeof(oracle_analysis_out)); }
+
+void oracle_analyse(oracle_tree* t, const remo_shared_uniforms* u, uint32_t closureMargin,
+                    oracle_analysis_out* out) {
+    std::vector<oracle_analysis_out> v;
+    t->analyse(*u, closureMargin, v);
+    std::memcpy(out, v.data(), v.size() * sizeof(oracle_analysis_out));
+}
+
+void oracle_render(oracle_tree* t, const remo_shared_uniforms* u, uint64_t* fb, uint32_t* numItems,
+                   uint64_t* numSamples) {
+    t->render(*u, fb, numItems, numSamples, 131072u);
+}
+
+// The FP32 and FP64 Jacobi solvers on one symmetric matrix (a00 a01 a02 a11 a12 a22).
+void oracle_jacobi(const double a[6], double ev64[3], float ev32[3]) {
+    double a64[6];
+    float a32[6];
+    for (int k = 0; k < 6; ++k) {
+        a64[k] = a[k];
+        a32[k] = float(a[k]);
+    }
+    anal::jacobi3<double>(a64, ev64);
+    anal::jacobi3<float>(a32, ev32);
 }
 
 // Exposed so tests can pin the quantisation against exact rational arithmetic.

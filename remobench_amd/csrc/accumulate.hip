@@ -1,18 +1,23 @@
 // Accumulate: fold each leaf's new points into its running sums.
 //
 // Follows kernel_accumulate / kernel_accum_verify
-// (kernels/remolod/remolod_accum.cu:47-233): one workgroup per leaf whose
-// watermark count is below numPoints folds slots [count, numPoints); an inner
-// node holding sums is cleared; Morton watermark of the folded points.
+// (kernels/remolod/remolod_accum.cu:47-233): a leaf whose watermark count is
+// below numPoints folds slots [count, numPoints); an inner node holding sums is
+// cleared; the Morton watermark of the folded points advances; Σ leaf counts and
+// inner-with-sums are verified.
+//
+// Dirty leaves carry 1k-50k new points each, so one workgroup per leaf (the
+// reference's shape) leaves most of the GPU idle behind the largest leaves.
+// Here a plan pass cuts every dirty range into tiles of 2,048 points, one
+// persistent launch folds the tiles into partial sums, and a finish pass adds
+// each leaf's partials in tile order, which keeps the FP64 and FP32 sums
+// deterministic for a given point order.
 //
 // Besides the reference's FP64 geometry sums g[9] and FP32 colour sums c[6], every
 // point adds its node-centred fixed-point coordinate q = floor(l * 2^20) - 2^19
 // (l the node-local coordinate in [0,1], computed in the reference's FP64 order)
 // into int64 sums q1[3], q2[6]. Integer addition is associative, so those sums
 // are bit-exact and independent of thread order; they are what Analysis rolls up.
-//
-// The wave reduction is 64-wide (__shfl_down over 64 lanes), then one LDS slot
-// per wave, where the reference used 32-lane shuffles and 8 warps.
 
 #include "remolod_internal.h"
 
@@ -47,36 +52,59 @@ __device__ __forceinline__ uint64_t waveMax(uint64_t v) {
     return v;
 }
 
-__global__ void kAccumBegin(DevTree t) {
+// Plan: per node, clear an inner node's sums; cut a dirty leaf's new range into
+// tiles. A leaf whose tiles do not fit waits for the next launch (the watermark
+// makes the pass incremental, plans/03_AccumulatorHook.md:562-573).
+__global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w) {
     Globals* g = t.g;
-    g->pointsFolded = 0;
-    g->leavesFolded = 0;
-    g->sumLeafCounts = 0;
-    g->innerWithSums = 0;
+    const uint32_t n = g->numNodes;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g->pointsFolded = 0;
+        g->leavesFolded = 0;
+        g->sumLeafCounts = 0;
+        g->innerWithSums = 0;
+    }
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        remo_node_accum* a = t.accums + node;
+        w.nodeTiles[node] = 0;
+        if (t.firstChild[node] >= 0) {
+            if (a->count != 0) *a = remo_node_accum{};
+            continue;
+        }
+        const uint32_t from = a->count, to = t.numPoints[node];
+        if (to <= from) continue;
+        const uint32_t nt = (to - from + AccumWork::kTile - 1) / AccumWork::kTile;
+        const uint32_t base = atomicAdd(w.numTiles, nt);
+        if (uint64_t(base) + nt > w.tileCap) {
+            // no room: mark the reserved slots below the capacity empty
+            for (uint32_t k = 0; base + k < w.tileCap && k < nt; ++k)
+                w.tiles[base + k] = make_uint4(kNone, 0, 0, 0);
+            continue;
+        }
+        for (uint32_t k = 0; k < nt; ++k) {
+            const uint32_t b = from + k * AccumWork::kTile;
+            w.tiles[base + k] = make_uint4(node, b, min(to, b + AccumWork::kTile), k);
+        }
+        w.nodeTiles[node] = nt;
+        w.nodeBase[node] = base;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void kAccumulate(DevTree t, uint32_t batchIndex) {
+// Persistent fold of all tiles into partial sums.
+__global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
     __shared__ long long sq[9][kWaves];
     __shared__ double sg[9][kWaves];
     __shared__ float sc[6][kWaves];
     __shared__ unsigned long long sm[kWaves];
-
-    Globals* g = t.g;
-    const uint32_t numNodes = g->numNodes;
     const uint32_t lane = laneId(), wave = threadIdx.x >> 6;
     const double invSize = 1.0 / double(t.octreeSize);
     const double fGrid = double(1u << kMaxDepth);
+    const uint32_t numTiles = min(*w.numTiles, w.tileCap);
 
-    for (uint32_t node = xcdBlock(); node < numNodes; node += gridDim.x) {
-        remo_node_accum* a = t.accums + node;
-        if (t.firstChild[node] >= 0) {
-            if (threadIdx.x == 0 && a->count != 0) *a = remo_node_accum{};
-            continue;
-        }
-        const uint32_t from = a->count;
-        const uint32_t to = t.numPoints[node];
-        if (to <= from) continue;
-
+    for (uint32_t ti = xcdBlock(); ti < numTiles; ti += gridDim.x) {
+        const uint4 tile = w.tiles[ti];
+        const uint32_t node = tile.x;
+        if (node == kNone) continue;  // planned past the capacity
         const uint32_t lvl = t.level[node];
         const double levelScale = double(1ull << lvl);
         const double NX = double(t.nx[node]), NY = double(t.ny[node]), NZ = double(t.nz[node]);
@@ -86,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void kAccumulate(DevTree t, uint32_t batchI
         double lg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         float lc[6] = {0, 0, 0, 0, 0, 0};
         uint64_t lm = 0;
-        for (uint32_t s = from + threadIdx.x; s < to; s += kBlock) {
+        for (uint32_t s = tile.y + threadIdx.x; s < tile.z; s += kBlock) {
             const uint32_t k = s / kPointsPerChunk;
             const uint32_t ref = k < kPointDir ? dir[k] : 0u;
             if (ref == 0) continue;
@@ -139,65 +167,108 @@ __global__ __launch_bounds__(kBlock) void kAccumulate(DevTree t, uint32_t batchI
             if (lane == 0) sm[wave] = v;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int i = 0; i < 9; ++i) {
+        if (threadIdx.x < 25) {
+            AccumPartial* out = w.partials + ti;
+            const uint32_t f = threadIdx.x;
+            if (f < 9) {
                 long long v = 0;
                 double d = 0.0;
-                for (uint32_t w = 0; w < kWaves; ++w) {
-                    v += sq[i][w];
-                    d += sg[i][w];
+                for (uint32_t wv = 0; wv < kWaves; ++wv) {
+                    v += sq[f][wv];
+                    d += sg[f][wv];
                 }
-                if (i < 3)
-                    a->q1[i] += v;
-                else
-                    a->q2[i - 3] += v;
-                a->g[i] += d;
-            }
-            for (int i = 0; i < 6; ++i) {
+                out->q[f] = v;
+                out->g[f] = d;
+            } else if (f < 15) {
                 float v = 0.0f;
-                for (uint32_t w = 0; w < kWaves; ++w) v += sc[i][w];
-                a->c[i] += v;
+                for (uint32_t wv = 0; wv < kWaves; ++wv) v += sc[f - 9][wv];
+                out->c[f - 9] = v;
+            } else if (f == 15) {
+                unsigned long long mm = 0;
+                for (uint32_t wv = 0; wv < kWaves; ++wv) mm = sm[wv] > mm ? sm[wv] : mm;
+                out->morton = mm;
             }
-            unsigned long long mm = 0;
-            for (uint32_t w = 0; w < kWaves; ++w) mm = sm[w] > mm ? sm[w] : mm;
-            a->count = to;
-            a->last_touched_batch = batchIndex;
-            if (mm > a->morton_max) a->morton_max = mm;
-            atomicMax((unsigned long long*)&g->mortonWatermark, mm);
-            atomicAdd((unsigned long long*)&g->numAccumulated, (unsigned long long)(to - from));
-            atomicAdd(&g->pointsFolded, to - from);
-            atomicAdd(&g->leavesFolded, 1u);
         }
         __syncthreads();
     }
 }
 
-// kernel_accum_verify (remolod_accum.cu:214-233).
-__global__ __launch_bounds__(kBlock) void kAccumVerify(DevTree t) {
+// Finish: add each leaf's partials in tile order, advance its watermark count,
+// and verify (kernel_accum_verify, remolod_accum.cu:214-233).
+__global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, uint32_t batchIndex) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
-    unsigned long long sum = 0;
-    uint32_t inner = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (t.firstChild[i] < 0)
-            sum += t.accums[i].count;
-        else if (t.accums[i].count != 0)
-            inner++;
+    unsigned long long sumCounts = 0, folded = 0, wm = 0;
+    uint32_t inner = 0, leaves = 0;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        remo_node_accum* a = t.accums + node;
+        if (t.firstChild[node] >= 0) {
+            if (a->count != 0) inner++;
+            continue;
+        }
+        const uint32_t nt = w.nodeTiles[node];
+        if (nt) {
+            const uint32_t base = w.nodeBase[node];
+            long long q[9];
+            double gs[9];
+            float cs[6];
+            for (int i = 0; i < 9; ++i) {
+                q[i] = 0;
+                gs[i] = 0.0;
+            }
+            for (int i = 0; i < 6; ++i) cs[i] = 0.0f;
+            unsigned long long mm = 0;
+            for (uint32_t k = 0; k < nt; ++k) {
+                const AccumPartial& pp = w.partials[base + k];
+                for (int i = 0; i < 9; ++i) {
+                    q[i] += pp.q[i];
+                    gs[i] += pp.g[i];
+                }
+                for (int i = 0; i < 6; ++i) cs[i] += pp.c[i];
+                mm = pp.morton > mm ? pp.morton : mm;
+            }
+            for (int i = 0; i < 3; ++i) a->q1[i] += q[i];
+            for (int i = 0; i < 6; ++i) a->q2[i] += q[3 + i];
+            for (int i = 0; i < 9; ++i) a->g[i] += gs[i];
+            for (int i = 0; i < 6; ++i) a->c[i] += cs[i];
+            const uint4 last = w.tiles[base + nt - 1];
+            const uint32_t from = a->count, to = last.z;
+            a->count = to;
+            a->last_touched_batch = batchIndex;
+            if (mm > a->morton_max) a->morton_max = mm;
+            wm = mm > wm ? mm : wm;
+            folded += to - from;
+            leaves++;
+        }
+        sumCounts += a->count;
     }
-    sum = waveSum(sum);
+    sumCounts = waveSum(sumCounts);
+    folded = waveSum(folded);
     inner = waveSum(inner);
+    leaves = waveSum(leaves);
+    wm = waveMax(wm);
     if (laneId() == 0) {
-        if (sum) atomicAdd((unsigned long long*)&g->sumLeafCounts, sum);
+        if (sumCounts) atomicAdd((unsigned long long*)&g->sumLeafCounts, sumCounts);
         if (inner) atomicAdd(&g->innerWithSums, inner);
+        if (folded) {
+            atomicAdd((unsigned long long*)&g->numAccumulated, folded);
+            atomicAdd(&g->pointsFolded, uint32_t(folded));
+        }
+        if (leaves) atomicAdd(&g->leavesFolded, leaves);
+        if (wm) atomicMax((unsigned long long*)&g->mortonWatermark, wm);
     }
 }
 
+__global__ void kAccumBegin(AccumWork w) { *w.numTiles = 0; }
+
 }  // namespace
 
-void launchAccumulate(const DevTree& t, uint32_t batchIndex, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(kAccumBegin, dim3(1), dim3(1), 0, s, t);
-    hipLaunchKernelGGL(kAccumulate, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, batchIndex);
-    hipLaunchKernelGGL(kAccumVerify, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
+void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(kAccumBegin, dim3(1), dim3(1), 0, s, w);
+    hipLaunchKernelGGL(kAccumPlan, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w);
+    hipLaunchKernelGGL(kAccumTiles, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, w);
+    hipLaunchKernelGGL(kAccumFinish, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w, batchIndex);
 }
 
 }  // namespace remo_hip

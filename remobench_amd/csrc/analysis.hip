@@ -1,13 +1,534 @@
-// Analysis and Rasterize (filled in below).
+// Analysis and Rasterize: the per-frame passes over a built tree.
+//
+// Analysis has no reference code (kernels/remolod/remodlod_analysis.cu is empty);
+// it follows the spec in README.md:80 and wiki/02_KernelPipeline.md:205-224:
+// test closure, roll leaf accumulators up the tree and finalise, score =
+// geometric score x screen coverage, rebuild the work queue each frame.
+//
+//  * Roll-up is one bottom-up pass: every leaf thread writes its sums and walks
+//    up; the eighth child to arrive at a parent (acquire/release counter) adds
+//    the eight children in child order and continues. The sums are __int128 in
+//    units U = 2^-(20+D) of the root extent (D the deepest level), centred on
+//    each node: a child's sums move to its parent's centre with the exact offset
+//    +-2^(18+D-L) per axis, so the roll-up is exact and order-independent.
+//  * Finalise: covariance in node-extent units from num = n*S2 - S1*S1^T (exact
+//    int128 while it fits), cyclic Jacobi on cov/trace in FP32, redone in FP64
+//    when the smallest eigen-gap is below 1e-4 of the trace.
+//
+// Rasterize follows kernel_render's selection and the shared rasteriser
+// (kernels/remolod/remolod_render.cu:30-181, kernels/shared/remo_draw.cuh:118-167,
+// kernels/shared/remo_framebuffer.cuh:214-248, kernels/shared/remo_math.cuh:16-79):
+// per-node frustum + 8-corner screen extent -> large flags; each large inner
+// node emits its visible, non-large children and a large visible leaf emits
+// itself; one workgroup per draw item splats with a 64-bit atomicMin of
+// (float_bits(depth) << 32) | rgba. EDL, wireframe and the GL resolve are
+// presentation and not part of this path: the output is the u64 buffer.
+
+#include <cstring>
+
 #include "remolod_analysis.h"
+#include "remolod_mlp.h"
 
 namespace remo_hip {
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+using i128 = __int128;
+constexpr uint64_t kFbClearValue = 0x7f80000000332211ull;  // remo_framebuffer.cuh:11
+
+__device__ __forceinline__ double i128ToDouble(i128 x) {
+    const bool neg = x < 0;
+    const unsigned __int128 u = neg ? (unsigned __int128)(-x) : (unsigned __int128)x;
+    const double hi = double(uint64_t(u >> 64));
+    const double lo = double(uint64_t(u));
+    const double d = hi * 18446744073709551616.0 + lo;
+    return neg ? -d : d;
+}
+
+__device__ __forceinline__ int bitsU128(unsigned __int128 u) {
+    const uint64_t hi = uint64_t(u >> 64), lo = uint64_t(u);
+    return hi ? 128 - __clzll((long long)hi) : (lo ? 64 - __clzll((long long)lo) : 0);
+}
+__device__ __forceinline__ int bitsI128(i128 x) {
+    return bitsU128(x < 0 ? (unsigned __int128)(-x) : (unsigned __int128)x);
+}
+
+// Cyclic Jacobi on a symmetric 3x3 (a00 a01 a02 a11 a12 a22); eigenvalues ascending.
+template <typename T>
+__device__ __forceinline__ void jacobi3(const T a[6], T ev[3]) {
+    T m[3][3] = {{a[0], a[1], a[2]}, {a[1], a[3], a[4]}, {a[2], a[4], a[5]}};
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        const T off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+        if (!(off > T(0))) break;
+        for (int pq = 0; pq < 3; ++pq) {
+            const int p = pq == 2 ? 1 : 0;
+            const int q = pq == 0 ? 1 : 2;
+            const T apq = m[p][q];
+            if (apq == T(0)) continue;
+            const T theta = (m[q][q] - m[p][p]) / (T(2) * apq);
+            const T at = theta < T(0) ? -theta : theta;
+            T t;
+            if (at > T(1e15))
+                t = T(0.5) / theta;
+            else
+                t = (theta < T(0) ? T(-1) : T(1)) / (at + sqrt(theta * theta + T(1)));
+            const T c = T(1) / sqrt(t * t + T(1));
+            const T sn = t * c;
+            const int r = 3 - p - q;
+            const T arp = m[r][p], arq = m[r][q];
+            m[p][p] = m[p][p] - t * apq;
+            m[q][q] = m[q][q] + t * apq;
+            m[p][q] = m[q][p] = T(0);
+            const T nrp = c * arp - sn * arq;
+            const T nrq = sn * arp + c * arq;
+            m[r][p] = m[p][r] = nrp;
+            m[r][q] = m[q][r] = nrq;
+        }
+    }
+    ev[0] = m[0][0];
+    ev[1] = m[1][1];
+    ev[2] = m[2][2];
+    T tmp;
+    if (ev[0] > ev[1]) { tmp = ev[0]; ev[0] = ev[1]; ev[1] = tmp; }
+    if (ev[1] > ev[2]) { tmp = ev[1]; ev[1] = ev[2]; ev[2] = tmp; }
+    if (ev[0] > ev[1]) { tmp = ev[0]; ev[0] = ev[1]; ev[1] = tmp; }
+}
+
+// ---- roll-up -------------------------------------------------------------------
+
+__global__ void kFillU32(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__device__ __forceinline__ void leafSums(const DevTree& t, uint32_t node, uint32_t D, RollupSums& S) {
+    const remo_node_accum& a = t.accums[node];
+    const int sh = int(D - t.level[node]);
+    for (int i = 0; i < 3; ++i) S.s1[i] = i128(a.q1[i]) << sh;
+    for (int i = 0; i < 6; ++i) S.s2[i] = i128(a.q2[i]) << (2 * sh);
+    S.n = a.count;
+}
+
+__device__ __forceinline__ void parentSums(const DevTree& t, const RollupSums* R, uint32_t parent, uint32_t D,
+                                           RollupSums& S) {
+    const i128 h = i128(1) << (18 + D - t.level[parent]);
+    const uint32_t c0 = uint32_t(t.firstChild[parent]);
+    for (int i = 0; i < 3; ++i) S.s1[i] = 0;
+    for (int i = 0; i < 6; ++i) S.s2[i] = 0;
+    S.n = 0;
+    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+    for (int ci = 0; ci < 8; ++ci) {
+        const RollupSums& C = R[c0 + ci];
+        const i128 o[3] = {(ci & 4) ? h : -h, (ci & 2) ? h : -h, (ci & 1) ? h : -h};
+        const i128 n = i128(C.n);
+        for (int i = 0; i < 3; ++i) S.s1[i] += C.s1[i] + n * o[i];
+        for (int k = 0; k < 6; ++k) {
+            const int a_ = pi[k], b_ = pj[k];
+            S.s2[k] += C.s2[k] + o[a_] * C.s1[b_] + C.s1[a_] * o[b_] + n * o[a_] * o[b_];
+        }
+        S.n += C.n;
+    }
+}
+
+struct RollArgs {
+    RollupSums* rollup;
+    uint32_t* arrived;
+    uint8_t* closed;
+    uint32_t margin;
+};
+
+__global__ __launch_bounds__(kBlock) void kRollup(DevTree t, RollArgs ra) {
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    const uint32_t D = g->sMaxDepth;
+    const uint32_t current = g->batchletIndex;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        if (t.firstChild[node] >= 0) continue;
+        RollupSums S;
+        leafSums(t, node, D, S);
+        S.e = 20 + D - t.level[node];
+        ra.rollup[node] = S;
+        const remo_node_accum& a = t.accums[node];
+        ra.closed[node] = (a.count == 0 || current >= a.last_touched_batch + ra.margin) ? 1 : 0;
+        uint32_t cur = node;
+        for (;;) {
+            const uint32_t p = t.parent[cur];
+            if (p == kNone) break;
+            const uint32_t before =
+                __hip_atomic_fetch_add(&ra.arrived[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (before != 7) break;  // not the last child: its sibling continues
+            RollupSums P;
+            parentSums(t, ra.rollup, p, D, P);
+            P.e = 20 + D - t.level[p];
+            uint8_t cl = 1;
+            const uint32_t c0 = uint32_t(t.firstChild[p]);
+            for (int ci = 0; ci < 8; ++ci) cl &= ra.closed[c0 + ci];
+            ra.rollup[p] = P;
+            ra.closed[p] = cl;
+            cur = p;
+        }
+    }
+}
+
+// ---- finalise ------------------------------------------------------------------
+
+struct Bounds {
+    float mn[3], mx[3];
+};
+
+// nodeBounds (remolod_render.cu:30-38); cubeMin = 0, 2^level is exact.
+__device__ __forceinline__ Bounds nodeBounds(const DevTree& t, uint32_t node, float cubeSize) {
+    Bounds b;
+    const float size = cubeSize / float(1u << t.level[node]);
+    b.mn[0] = 0.0f + float(t.nx[node]) * size;
+    b.mn[1] = 0.0f + float(t.ny[node]) * size;
+    b.mn[2] = 0.0f + float(t.nz[node]) * size;
+    for (int i = 0; i < 3; ++i) b.mx[i] = b.mn[i] + size;
+    return b;
+}
+
+// projectedExtent (remolod_render.cu:40-70).
+__device__ __forceinline__ void projectedExtent(const Bounds& b, const remo_mat4& M, float width, float height,
+                                                float& dx, float& dy) {
+    dx = dy = 0.0f;
+    float minX = 1e30f, minY = 1e30f, maxX = -1e30f, maxY = -1e30f;
+    bool any = false;
+    for (int corner = 0; corner < 8; ++corner) {
+        const float x = (corner & 1) ? b.mx[0] : b.mn[0];
+        const float y = (corner & 2) ? b.mx[1] : b.mn[1];
+        const float z = (corner & 4) ? b.mx[2] : b.mn[2];
+        const float cx = M.rows[0][0] * x + M.rows[0][1] * y + M.rows[0][2] * z + M.rows[0][3] * 1.0f;
+        const float cy = M.rows[1][0] * x + M.rows[1][1] * y + M.rows[1][2] * z + M.rows[1][3] * 1.0f;
+        const float cw = M.rows[3][0] * x + M.rows[3][1] * y + M.rows[3][2] * z + M.rows[3][3] * 1.0f;
+        if (cw <= 0.0f) continue;
+        const float invW = 1.0f / cw;
+        const float sx = (cx * invW * 0.5f + 0.5f) * width;
+        const float sy = (cy * invW * 0.5f + 0.5f) * height;
+        minX = fminf(minX, sx);
+        maxX = fmaxf(maxX, sx);
+        minY = fminf(minY, sy);
+        maxY = fmaxf(maxY, sy);
+        any = true;
+    }
+    if (!any) return;
+    dx = maxX - minX;
+    dy = maxY - minY;
+}
+
+struct FinalArgs {
+    RollupSums* rollup;
+    uint8_t* closed;
+    remo_node_analysis* results;
+    uint32_t* queue;
+    uint32_t* queueCount;
+    const float* learned;   // [n * 2] predicted (surface variation, planarity), or null
+    uint32_t useLearned;
+};
+
+__global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, FinalArgs fa, remo_shared_uniforms u) {
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    const uint32_t D = g->sMaxDepth;
+    const float cubeSize = t.octreeSize;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        const RollupSums S = fa.rollup[node];
+        remo_node_analysis r;
+        memset(&r, 0, sizeof(r));
+        r.n = S.n;
+        float dx, dy;
+        projectedExtent(nodeBounds(t, node, cubeSize), u.transform_frozen, u.width, u.height, dx, dy);
+        r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
+        const bool closed = fa.closed[node] != 0;
+        r.state = closed ? 2u : 0u;
+        bool scored = false;
+        if (fa.useLearned && !closed && S.n >= 3) {
+            // learned score for open nodes; the analytic path is the fallback when the
+            // prediction says the node is not planar (surface variation > 0.05)
+            const float sv = fa.learned[2 * node], pl = fa.learned[2 * node + 1];
+            if (sv <= 0.05f) {
+                r.surface_variation = fmaxf(sv, 0.0f);
+                r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
+                r.linearity = 0.0f;
+                r.flags |= 2u;
+                scored = true;
+            }
+        }
+        if (!scored && S.n >= 3) {
+            const uint32_t e = 20 + D - t.level[node];
+            const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+            int maxS2 = 0, maxS1 = 0;
+            for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
+            for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
+            const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
+            const double scale = ldexp(1.0, -int(e));
+            const double dn = double(S.n);
+            double cov[6];
+            for (int k = 0; k < 6; ++k) {
+                double num;
+                if (exact)
+                    num = i128ToDouble(i128(S.n) * S.s2[k] - S.s1[pi[k]] * S.s1[pj[k]]);
+                else
+                    num = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
+                cov[k] = num / (dn * dn) * scale * scale;
+            }
+            const double trace = cov[0] + cov[3] + cov[5];
+            if (trace > 0.0) {
+                double a64[6];
+                float a32[6];
+                for (int k = 0; k < 6; ++k) {
+                    a64[k] = cov[k] / trace;
+                    a32[k] = float(a64[k]);
+                }
+                float ev32[3];
+                jacobi3<float>(a32, ev32);
+                const float gap = fminf(ev32[1] - ev32[0], ev32[2] - ev32[1]);
+                double ev[3];
+                if (gap < 1e-4f) {
+                    double ev64[3];
+                    jacobi3<double>(a64, ev64);
+                    for (int i = 0; i < 3; ++i) ev[i] = ev64[i];
+                    r.flags |= 1u;
+                } else {
+                    for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
+                }
+                for (int i = 0; i < 3; ++i) r.lambda[i] = float(ev[i] * trace);
+                const double sum = ev[0] + ev[1] + ev[2];
+                r.surface_variation = sum > 0.0 ? float(ev[0] / sum) : 0.0f;
+                r.planarity = ev[2] > 0.0 ? float((ev[1] - ev[0]) / ev[2]) : 0.0f;
+                r.linearity = ev[2] > 0.0 ? float((ev[2] - ev[1]) / ev[2]) : 0.0f;
+            }
+        }
+        r.score = 3.0f * r.surface_variation * r.coverage;
+        fa.results[node] = r;
+        remo_node_accum* a = t.accums + node;
+        a->state = r.state;
+        a->score = r.score;
+        // work queue: every node with a positive score, rebuilt each frame
+        const bool q = r.score > 0.0f;
+        const uint64_t m = __ballot(q);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if (laneId() == uint32_t(leader)) base = atomicAdd(fa.queueCount, uint32_t(__popcll(m)));
+            base = __shfl(base, leader);
+            if (q) fa.queue[base + __popcll(m & lanesBelow())] = node;
+        }
+    }
+}
+
+// ---- rasterize -----------------------------------------------------------------
+
+struct Plane {
+    float nx, ny, nz, d;
+};
+
+// Frustum::fromViewProj (remo_math.cuh:49-79).
+__device__ __forceinline__ void frustumPlanes(const remo_mat4& m, Plane P[6]) {
+    const float(*r)[4] = m.rows;
+    const float raw[6][4] = {
+        {r[3][0] - r[0][0], r[3][1] - r[0][1], r[3][2] - r[0][2], r[3][3] - r[0][3]},
+        {r[3][0] + r[0][0], r[3][1] + r[0][1], r[3][2] + r[0][2], r[3][3] + r[0][3]},
+        {r[3][0] + r[1][0], r[3][1] + r[1][1], r[3][2] + r[1][2], r[3][3] + r[1][3]},
+        {r[3][0] - r[1][0], r[3][1] - r[1][1], r[3][2] - r[1][2], r[3][3] - r[1][3]},
+        {r[3][0] - r[2][0], r[3][1] - r[2][1], r[3][2] - r[2][2], r[3][3] - r[2][3]},
+        {r[3][0] + r[2][0], r[3][1] + r[2][1], r[3][2] + r[2][2], r[3][3] + r[2][3]},
+    };
+    for (int i = 0; i < 6; ++i) {
+        const float a = raw[i][0], b = raw[i][1], c = raw[i][2], d = raw[i][3];
+        const float len = sqrtf(a * a + b * b + c * c);
+        const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+        P[i] = {a * inv, b * inv, c * inv, d * inv};
+    }
+}
+
+__device__ __forceinline__ bool intersectsBox(const Plane P[6], const Bounds& b) {
+    for (int i = 0; i < 6; ++i) {
+        const Plane& p = P[i];
+        const float px = p.nx > 0.0f ? b.mx[0] : b.mn[0];
+        const float py = p.ny > 0.0f ? b.mx[1] : b.mn[1];
+        const float pz = p.nz > 0.0f ? b.mx[2] : b.mn[2];
+        if (p.nx * px + p.ny * py + p.nz * pz + p.d < 0.0f) return false;
+    }
+    return true;
+}
+
+struct RenderArgs {
+    uint8_t* visible;
+    uint8_t* large;
+    DrawItem* items;
+    uint32_t* counters;  // [0] items, [1] overflow, [2..3] samples
+    uint32_t capacity;
+};
+
+__global__ void kFbClear(uint64_t* fb, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        fb[i] = kFbClearValue;
+}
+
+__global__ __launch_bounds__(kBlock) void kRenderFlags(DevTree t, RenderArgs ra, remo_shared_uniforms u) {
+    __shared__ Plane P[6];
+    if (threadIdx.x == 0) frustumPlanes(u.transform_frozen, P);
+    if (blockIdx.x == 0 && threadIdx.x < 4) ra.counters[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = min(t.g->numNodes, t.maxNodes);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Bounds b = nodeBounds(t, i, t.octreeSize);
+        const bool hasSamples = t.numPoints[i] > 0 || t.numVoxels[i] > 0;
+        ra.visible[i] = (hasSamples && intersectsBox(P, b)) ? 1 : 0;
+        float dx, dy;
+        projectedExtent(b, u.transform_frozen, u.width, u.height, dx, dy);
+        ra.large[i] = (dx > u.lod_pixel_budget || dy > u.lod_pixel_budget) ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ void emit(const DevTree& t, const RenderArgs& ra, uint32_t node) {
+    const uint32_t np = t.numPoints[node], nv = t.numVoxels[node];
+    if (np == 0 && nv == 0) return;
+    const uint32_t idx = atomicAdd(&ra.counters[0], 1u);
+    if (idx >= ra.capacity) {
+        ra.counters[1] = 1;
+        return;
+    }
+    ra.items[idx] = DrawItem{node, np, nv, t.level[node]};
+    atomicAdd(reinterpret_cast<unsigned long long*>(ra.counters + 2), (unsigned long long)(np + nv));
+}
+
+// Frontier selection (remolod_render.cu:134-179).
+__global__ __launch_bounds__(kBlock) void kRenderEmit(DevTree t, RenderArgs ra) {
+    const uint32_t n = min(t.g->numNodes, t.maxNodes);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (ra.large[i] == 0) continue;
+        const int32_t fc = t.firstChild[i];
+        if (fc >= 0) {
+            for (uint32_t c = 0; c < 8; ++c) {
+                const uint32_t child = uint32_t(fc) + c;
+                if (child >= n) continue;
+                if (ra.large[child] != 0) continue;
+                if (ra.visible[child] == 0) continue;
+                emit(t, ra, child);
+            }
+        } else if (ra.visible[i] != 0) {
+            emit(t, ra, i);
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t f2i_sat(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return int32_t(0x80000000u);
+    return int32_t(f);
+}
+
+__device__ __forceinline__ uint32_t hashColor(uint64_t key) {
+    uint64_t h = (key + 0x9E3779B97F4A7C15ull) * 0xD1B54A32D192ED03ull;
+    h ^= h >> 31;
+    h *= 0x94D049BB133111EBull;
+    h ^= h >> 29;
+    const uint32_t r = 96u + ((uint32_t(h) & 0xFFu) * 159u) / 255u;
+    const uint32_t gg = 96u + ((uint32_t(h >> 8) & 0xFFu) * 159u) / 255u;
+    const uint32_t b = 96u + ((uint32_t(h >> 16) & 0xFFu) * 159u) / 255u;
+    return r | (gg << 8) | (b << 16) | (255u << 24);
+}
+
+// remoDrawPoint + remoProject + remoFbPack (remo_framebuffer.cuh:33-61, remo_math.cuh:16-39).
+__device__ __forceinline__ void drawPoint(uint64_t* fb, const remo_shared_uniforms& u, int32_t w, int32_t h,
+                                          const remo_point& p, uint32_t color) {
+    const auto& M = u.transform.rows;
+    const float cx = M[0][0] * p.x + M[0][1] * p.y + M[0][2] * p.z + M[0][3] * 1.0f;
+    const float cy = M[1][0] * p.x + M[1][1] * p.y + M[1][2] * p.z + M[1][3] * 1.0f;
+    const float cw = M[3][0] * p.x + M[3][1] * p.y + M[3][2] * p.z + M[3][3] * 1.0f;
+    if (cw <= 0.0f) return;
+    const float invW = 1.0f / cw;
+    const float px = (cx * invW * 0.5f + 0.5f) * u.width;
+    const float py = (cy * invW * 0.5f + 0.5f) * u.height;
+    const int64_t half = u.point_size / 2;
+    const int64_t ix = f2i_sat(px), iy = f2i_sat(py);
+    if (ix + half < 0 || iy + half < 0 || ix - half >= w || iy - half >= h) return;
+    const uint64_t packed = (uint64_t(__float_as_uint(cw)) << 32) | color;
+    for (int64_t oy = -half; oy <= half; ++oy) {
+        const int64_t sy = iy + oy;
+        if (sy < 0 || sy >= h) continue;
+        for (int64_t ox = -half; ox <= half; ++ox) {
+            const int64_t sx = ix + ox;
+            if (sx < 0 || sx >= w) continue;
+            unsigned long long* px64 = reinterpret_cast<unsigned long long*>(fb + sy * w + sx);
+            // early depth test: the buffer only decreases, so a stale read is safe
+            if (packed < *px64) atomicMin(px64, (unsigned long long)packed);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kRenderRaster(DevTree t, RenderArgs ra, remo_shared_uniforms u,
+                                                        uint64_t* fb) {
+    if (u.show_points == 0) return;
+    const uint32_t numItems = min(ra.counters[0], ra.capacity);
+    const int32_t w = int32_t(u.width), h = int32_t(u.height);
+    for (uint32_t it = xcdBlock(); it < numItems; it += gridDim.x) {
+        const DrawItem item = ra.items[it];
+        uint32_t key = item.node;
+        if (u.color_mode == 2) key = uint32_t(item.level * 2654435761ull);
+        for (uint32_t pass = 0; pass < 2; ++pass) {
+            const uint32_t count = pass == 0 ? item.numPoints : item.numVoxels;
+            if (count == 0) continue;
+            const uint32_t* dir;
+            uint32_t dirCap;
+            if (pass == 0) {
+                dir = t.pointDir + uint64_t(item.node) * kPointDir;
+                dirCap = kPointDir;
+            } else {
+                const uint32_t gi = t.gridNo[item.node];
+                if (gi == kNone) continue;
+                dir = t.voxelDir + uint64_t(gi) * kVoxelDir;
+                dirCap = kVoxelDir;
+            }
+            for (uint32_t s = threadIdx.x; s < count; s += blockDim.x) {
+                const uint32_t k = s / kPointsPerChunk;
+                if (k >= dirCap) break;
+                const uint32_t ref = dir[k];
+                if (ref == 0) continue;
+                const remo_point p = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
+                uint32_t color = p.color;
+                if (u.color_mode == 3) color = 0xFFFFFFFFu;
+                else if (u.color_mode == 1 || u.color_mode == 2) color = hashColor(key);
+                drawPoint(fb, u, w, h, p, color);
+            }
+        }
+    }
+}
+
+}  // namespace
 
 int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) {
     a.maxNodes = uint32_t(maxNodes);
     void* p = nullptr;
-    int rc = alloc(&p, sizeof(remo_node_analysis) * maxNodes, "analysis results");
+    int rc = REMO_OK;
+    auto A = [&](void** dst, uint64_t bytes, const char* what) {
+        if (rc == REMO_OK) rc = alloc(dst, bytes, what);
+    };
+    A(&p, sizeof(remo_node_analysis) * maxNodes, "analysis results");
     a.results = static_cast<remo_node_analysis*>(p);
+    A(&p, sizeof(RollupSums) * maxNodes, "roll-up sums");
+    a.rollup = static_cast<RollupSums*>(p);
+    A(&p, 4 * maxNodes, "roll-up arrival counters");
+    a.arrived = static_cast<uint32_t*>(p);
+    A(&p, maxNodes, "closure flags");
+    a.closed = static_cast<uint8_t*>(p);
+    A(&p, 4 * maxNodes, "work queue");
+    a.queue = static_cast<uint32_t*>(p);
+    A(&p, 16, "work queue count");
+    a.queueCount = static_cast<uint32_t*>(p);
+    A(&p, maxNodes, "visible flags");
+    a.visible = static_cast<uint8_t*>(p);
+    A(&p, maxNodes, "large flags");
+    a.large = static_cast<uint8_t*>(p);
+    A(&p, sizeof(DrawItem) * uint64_t(a.drawCapacity), "draw list");
+    a.drawList = static_cast<DrawItem*>(p);
+    A(&p, 16, "draw counters");
+    a.drawCounters = static_cast<uint32_t*>(p);
+    A(&p, 8 * maxNodes, "learned predictions");
+    a.learned = static_cast<float*>(p);
+    if (rc == REMO_OK) rc = mlpAllocate(a, maxNodes, alloc);
     return rc;
 }
 
@@ -17,18 +538,45 @@ void analysisReset(AnalysisState& a) {
     a.drawSamples = 0;
 }
 
-int runAnalysis(AnalysisState&, const DevTree&, const remo_shared_uniforms&, int, const LaunchShape&,
-                hipStream_t, std::string* err) {
-    if (err) *err = "analysis not built";
-    return REMO_ERR_STATE;
+int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode,
+                const LaunchShape& ls, hipStream_t s, std::string* err) {
+    (void)err;
+    const uint32_t nb = ls.nodeBlocks;
+    hipLaunchKernelGGL(kFillU32, dim3(nb), dim3(kBlock), 0, s, a.arrived, t.maxNodes, 0u);
+    hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, s, a.queueCount, 1u, 0u);
+    RollArgs ra{a.rollup, a.arrived, a.closed, a.closureMargin};
+    hipLaunchKernelGGL(kRollup, dim3(nb), dim3(kBlock), 0, s, t, ra);
+    const bool learned = mode == REMO_ANALYSIS_LEARNED;
+    if (learned) {
+        const int rc = mlpPredict(a, t, ls, s, err);
+        if (rc != REMO_OK) return rc;
+    }
+    FinalArgs fa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.learned, learned ? 1u : 0u};
+    hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, fa, u);
+    return REMO_OK;
 }
 
-int runRender(AnalysisState&, const DevTree&, const remo_shared_uniforms&, uint64_t*, const LaunchShape&,
-              hipStream_t, std::string* err) {
-    if (err) *err = "render not built";
-    return REMO_ERR_STATE;
+int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb,
+              const LaunchShape& ls, hipStream_t s, std::string* err) {
+    (void)err;
+    const uint64_t pixels = uint64_t(u.width) * uint64_t(u.height);
+    hipLaunchKernelGGL(kFbClear, dim3(1024), dim3(kBlock), 0, s, fb, pixels);
+    RenderArgs ra{a.visible, a.large, a.drawList, a.drawCounters, a.drawCapacity};
+    hipLaunchKernelGGL(kRenderFlags, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, ra, u);
+    hipLaunchKernelGGL(kRenderEmit, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, ra);
+    hipLaunchKernelGGL(kRenderRaster, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, ra, u, fb);
+    return REMO_OK;
 }
 
-void analysisReadDiagnostics(AnalysisState&, hipStream_t) {}
+void analysisReadDiagnostics(AnalysisState& a, hipStream_t s) {
+    uint32_t c[4] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(c, a.drawCounters, sizeof(c), hipMemcpyDeviceToHost, s) != hipSuccess) return;
+    if (hipStreamSynchronize(s) != hipSuccess) return;
+    a.drawItems = c[0];
+    a.drawListOverflow = c[1];
+    uint64_t samples;
+    std::memcpy(&samples, c + 2, 8);
+    a.drawSamples = samples;
+}
 
 }  // namespace remo_hip

@@ -88,11 +88,49 @@ __device__ __forceinline__ TileRange myTiles(uint32_t total) {
     return {uint32_t(b), uint32_t(e)};
 }
 
+// Block-level aggregation table in LDS: open addressing on a 32-bit key, so a
+// tile of 1,024 scan-line-coherent points pays one global atomic per distinct
+// key instead of one per point (or per wave group).
+constexpr uint32_t kAggSlots = 2048;
+constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
+constexpr unsigned long long kEmpty64 = ~0ull;
+
+struct BlockAgg {
+    uint32_t key[kAggSlots];
+    uint32_t cnt[kAggSlots];
+    uint32_t base[kAggSlots];
+};
+
+__device__ __forceinline__ void aggClear(BlockAgg& a) {
+    for (uint32_t i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
+        a.key[i] = kEmpty32;
+        a.cnt[i] = 0;
+    }
+}
+
+__device__ __forceinline__ uint32_t aggHash(uint32_t key) { return (key * 2654435761u) >> 21; }
+
+// Finds or claims the slot of `key` and takes a rank in it; kEmpty32 when the
+// probe sequence is exhausted (the caller then goes to global memory directly).
+__device__ __forceinline__ uint32_t aggInsert(BlockAgg& a, uint32_t key, uint32_t* rank) {
+    uint32_t h = aggHash(key);
+    for (uint32_t probe = 0; probe < 64; ++probe) {
+        const uint32_t prev = atomicCAS(&a.key[h], kEmpty32, key);
+        if (prev == kEmpty32 || prev == key) {
+            *rank = atomicAdd(&a.cnt[h], 1u);
+            return h;
+        }
+        h = (h + 1) & (kAggSlots - 1);
+    }
+    return kEmpty32;
+}
+
 // ---------------------------------------------------------------------------
 // reset (kernels/remolod/remolod_reset.cu:23-88): root at level 0 with a grid.
 __global__ void kResetRoot(DevTree t) {
     Globals* g = t.g;
     t.firstChild[0] = -1;
+    t.parent[0] = kNone;
     t.level[0] = 0;
     t.nx[0] = t.ny[0] = t.nz[0] = 0;
     t.counter[0] = 0;
@@ -157,8 +195,10 @@ __global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t c
 // Count (doCounting, remolod_octree.cu:142-301). Round 0 counts this batch's
 // points into the leaves that existed before it; round r > 0 counts batch and
 // spilled points into the children created in round r-1 that hold > 50,000.
-// Counts go to the octant histogram batchOct[leaf][octant].
+// Counts go to the octant histogram batchOct[leaf][octant], aggregated per
+// tile in LDS.
 __global__ __launch_bounds__(kBlock) void kCount(DevTree t, BatchRef b, uint32_t round) {
+    __shared__ BlockAgg agg;
     const uint32_t batchSize = batchSizeOf(t, b);
     if (t.g->stop) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -171,23 +211,29 @@ __global__ __launch_bounds__(kBlock) void kCount(DevTree t, BatchRef b, uint32_t
     const TileRange tr = myTiles(total);
     const float size = t.octreeSize;
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        aggClear(agg);
+        __syncthreads();
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            bool valid = i < total;
-            uint32_t key = 0;
-            if (valid) {
-                const remo_point p = streamPoint(t, b, batchSize, i);
-                const uint32_t X = quant28(p.x, size) >> 8;
-                const uint32_t Y = quant28(p.y, size) >> 8;
-                const uint32_t Z = quant28(p.z, size) >> 8;
-                uint32_t level;
-                const uint32_t leaf = descend(t.firstChild, X, Y, Z, level);
-                if (round != 0 && t.pending[leaf] == 0) valid = false;
-                // a leaf at MAX_DEPTH never splits; its points count in octant 0
-                key = leaf * 8u + ((valid && level < uint32_t(kMaxDepth)) ? octantAt(X, Y, Z, level) : 0u);
-            }
-            waveAggInc(t.batchOct, key, valid);
+            if (i >= total) continue;
+            const remo_point p = streamPoint(t, b, batchSize, i);
+            const uint32_t X = quant28(p.x, size) >> 8;
+            const uint32_t Y = quant28(p.y, size) >> 8;
+            const uint32_t Z = quant28(p.z, size) >> 8;
+            uint32_t level;
+            const uint32_t leaf = descend(t.firstChild, X, Y, Z, level);
+            if (round != 0 && t.pending[leaf] == 0) continue;
+            // a leaf at MAX_DEPTH never splits; its points count in octant 0
+            const uint32_t key = leaf * 8u + (level < uint32_t(kMaxDepth) ? octantAt(X, Y, Z, level) : 0u);
+            uint32_t rank;
+            if (aggInsert(agg, key, &rank) == kEmpty32) atomicAdd(&t.batchOct[key], 1u);
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
+            const uint32_t key = agg.key[k];
+            if (key != kEmpty32) atomicAdd(&t.batchOct[key], agg.cnt[k]);
+        }
+        __syncthreads();
     }
 }
 
@@ -266,6 +312,7 @@ __global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
             const uint32_t c = c0 + i;
             const uint32_t cnt = t.storedOct[8ull * node + i] + t.batchOct[8ull * node + i];
             t.firstChild[c] = -1;
+            t.parent[c] = node;
             t.level[c] = lvl;
             t.nx[c] = 2 * X + ((i >> 2) & 1u);
             t.ny[c] = 2 * Y + ((i >> 1) & 1u);
@@ -333,22 +380,21 @@ __global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
     }
 }
 
-// Bumps numNodes by the accepted splits (separate launch: kSplitApply ranks
-// against the value numNodes had when it started).
-__global__ void kSplitCommit(DevTree t) {
-    Globals* g = t.g;
-    if (g->stop) return;
-    g->numNodes += 8u * g->numAccepted;
-    g->expandIters += 1;
-    g->totExpandIters += 1;
-}
-
-// The spill loop of doCounting (remolod_octree.cu:250-285): one block per
-// accepted split copies that node's stored points, chunk by chunk.
-__global__ __launch_bounds__(kBlock) void kSpillCopy(DevTree t) {
+// After the splits of a round (one launch): bump numNodes by the accepted
+// splits (kSplitApply ranked against the value it had when it started), copy
+// each accepted node's stored points to the spill list (the spill loop of
+// doCounting, remolod_octree.cu:250-285) and zero its grid
+// (remolod_octree.cu:372-387), including a grid the node already had (the
+// root's, from reset). Blocks [0, numAccepted) copy; all blocks zero.
+__global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t) {
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t na = g->numAccepted;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g->numNodes += 8u * na;
+        g->expandIters += 1;
+        g->totExpandIters += 1;
+    }
     for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
         const uint32_t node = t.acceptedSpill[2 * a];
         const uint32_t off = t.acceptedSpill[2 * a + 1];
@@ -363,14 +409,6 @@ __global__ __launch_bounds__(kBlock) void kSpillCopy(DevTree t) {
         __syncthreads();
         if (threadIdx.x == 0) t.numPoints[node] = 0;
     }
-}
-
-// Zero every accepted split's grid (remolod_octree.cu:372-387), including a grid
-// the node already had (the root's, from reset).
-__global__ __launch_bounds__(kBlock) void kZeroGrids(DevTree t) {
-    Globals* g = t.g;
-    if (g->stop) return;
-    const uint32_t na = g->numAccepted;
     const uint64_t perGrid = kGridWords / 4;
     const uint64_t total = uint64_t(na) * perGrid;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
@@ -384,9 +422,28 @@ __global__ __launch_bounds__(kBlock) void kZeroGrids(DevTree t) {
 
 // ---------------------------------------------------------------------------
 // voxelSampling / sampleVoxel (remolod_octree.cu:677-740, :95-140): at every
-// gridded node on the root->leaf path, set the point's 128^3 cell; the lane
-// that sets a bit appends the cell-centre voxel, coloured by its point.
+// gridded node on the root->leaf path, set the point's 128^3 cell; the lane that
+// sets a bit appends the cell-centre voxel, coloured by its point.
+//
+// A tile of 1,024 points keeps one LDS set of the (grid, cell) keys it has
+// already handled; a grid number names one node, so the set spans all levels
+// and needs no barrier between levels. Only the first lane to present a key
+// peeks the global word and, if the bit is clear, does the atomicOr.
+constexpr uint32_t kSeenSlots = 4096;
+
+__device__ __forceinline__ bool seenFirst4k(unsigned long long* seen, unsigned long long key) {
+    uint32_t h = uint32_t((key * 0x9E3779B97F4A7C15ull) >> 52);
+    for (uint32_t probe = 0; probe < 16; ++probe) {
+        const unsigned long long prev = atomicCAS(&seen[h], kEmpty64, key);
+        if (prev == kEmpty64) return true;
+        if (prev == key) return false;
+        h = (h + 1) & (kSeenSlots - 1);
+    }
+    return true;  // set full around this key: no de-duplication, still exact
+}
+
 __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
+    __shared__ unsigned long long seen[kSeenSlots];
     const uint32_t batchSize = batchSizeOf(t, b);
     Globals* g = t.g;
     if (g->stop) return;
@@ -394,48 +451,48 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
     const TileRange tr = myTiles(total);
     const float size = t.octreeSize;
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        for (uint32_t k = threadIdx.x; k < kSeenSlots; k += blockDim.x) seen[k] = kEmpty64;
+        __syncthreads();
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            const bool valid = i < total;
-            remo_point p = {0, 0, 0, 0};
-            uint32_t pX = 0, pY = 0, pZ = 0;
-            if (valid) {
-                p = streamPoint(t, b, batchSize, i);
+            bool walking = i < total;
+            uint32_t pX = 0, pY = 0, pZ = 0, col = 0, cur = 0;
+            if (walking) {
+                const remo_point p = streamPoint(t, b, batchSize, i);
                 pX = quant28(p.x, size);
                 pY = quant28(p.y, size);
                 pZ = quant28(p.z, size);
+                col = p.color;
             }
             const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
-            uint32_t cur = 0;
-            bool walking = valid;
             for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
                 if (__ballot(walking) == 0) break;
                 bool isNew = false;
-                uint32_t cx = 0, cy = 0, cz = 0, node = cur;
+                uint32_t vi = 0;
+                const uint32_t node = cur;
                 if (walking) {
-                    const uint32_t gi = t.gridNo[cur];
+                    const uint32_t gi = t.gridNo[node];
                     if (gi != kNone) {
                         const uint32_t sh = (kMaxDepth + 1) - level;
-                        cx = (pX >> sh) & (kGridSize - 1);
-                        cy = (pY >> sh) & (kGridSize - 1);
-                        cz = (pZ >> sh) & (kGridSize - 1);
-                        const uint32_t vi = cx + cy * kGridSize + cz * kGridSize * kGridSize;
-                        uint32_t* word = reinterpret_cast<uint32_t*>(t.arena + t.gridOffset[gi]) + (vi >> 5);
-                        const uint32_t mask = 1u << (vi & 31u);
-                        // plain peek first: a stale word only costs an extra atomicOr
-                        if ((*word & mask) == 0) {
-                            const uint32_t old = atomicOr(word, mask);
-                            isNew = (old & mask) == 0;
+                        vi = ((pX >> sh) & (kGridSize - 1)) + ((pY >> sh) & (kGridSize - 1)) * kGridSize +
+                             ((pZ >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
+                        if (seenFirst4k(seen, (uint64_t(gi) << 21) | vi)) {
+                            uint32_t* word =
+                                reinterpret_cast<uint32_t*>(t.arena + t.gridOffset[gi]) + (vi >> 5);
+                            const uint32_t mask = 1u << (vi & 31u);
+                            // plain peek first: a stale word only costs an extra atomicOr
+                            if ((*word & mask) == 0) isNew = (atomicOr(word, mask) & mask) == 0;
                         }
                     }
                 }
-                // one backlog slot per new voxel, one atomic per wave
-                const uint32_t slot = waveAppend(&g->numBacklog, isNew);
+                waveAggInc(t.numVoxels, node, isNew);
+                const uint32_t at = waveAppend(&g->numBacklog, isNew);
                 if (isNew) {
-                    atomicAdd(&t.numVoxels[node], 1u);
-                    if (slot < t.backlogCap) {
-                        const uint32_t lvl = level;
-                        const float nodeSize = size / float(1u << lvl);
+                    if (at < t.backlogCap) {
+                        const uint32_t cx = vi & (kGridSize - 1);
+                        const uint32_t cy = (vi >> 7) & (kGridSize - 1);
+                        const uint32_t cz = vi >> 14;
+                        const float nodeSize = size / float(1u << level);
                         const float minx = float(t.nx[node]) * nodeSize;
                         const float miny = float(t.ny[node]) * nodeSize;
                         const float minz = float(t.nz[node]) * nodeSize;
@@ -443,25 +500,26 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                         v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
                         v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
                         v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
-                        v.color = p.color;
-                        t.backlogPts[slot] = v;
-                        t.backlogNode[slot] = node;
+                        v.color = col;
+                        t.backlogPts[at] = v;
+                        t.backlogNode[at] = node;
                     } else {
                         g->backlogOverflow = 1;
                     }
                 }
                 if (walking) {
-                    const int32_t fc = t.firstChild[cur];
+                    const int32_t fc = t.firstChild[node];
                     if (fc < 0) {
                         walking = false;
                     } else {
                         const uint32_t sh = kMaxDepth - level - 1;
-                        cur = uint32_t(fc) + ((((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) |
-                                              ((Z >> sh) & 1u));
+                        cur = uint32_t(fc) +
+                              ((((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) | ((Z >> sh) & 1u));
                     }
                 }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -538,9 +596,12 @@ __global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
     }
 }
 
-// insertPoints (remolod_octree.cu:794-883): slot = atomicAdd(numPoints) per
-// (leaf, octant) group; the octant histogram of stored points grows with it.
+// insertPoints (remolod_octree.cu:794-883): slot = atomicAdd(numPoints), taken
+// once per (leaf, octant) key of a tile: the tile ranks its points in LDS, one
+// lane per key reserves the range (and grows the stored-octant histogram), then
+// every point lands at base + rank.
 __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
+    __shared__ BlockAgg agg;
     const uint32_t batchSize = batchSizeOf(t, b);
     Globals* g = t.g;
     if (g->stop) return;
@@ -552,75 +613,93 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
     const TileRange tr = myTiles(total);
     const float size = t.octreeSize;
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        aggClear(agg);
+        __syncthreads();
+        remo_point p[kPerThread];
+        uint32_t leaf[kPerThread], slotOf[kPerThread], rank[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            const bool valid = i < total;
-            remo_point p = {0, 0, 0, 0};
-            uint32_t leaf = 0, oct = 0;
-            if (valid) {
-                p = streamPoint(t, b, batchSize, i);
-                const uint32_t X = quant28(p.x, size) >> 8;
-                const uint32_t Y = quant28(p.y, size) >> 8;
-                const uint32_t Z = quant28(p.z, size) >> 8;
-                uint32_t level;
-                leaf = descend(t.firstChild, X, Y, Z, level);
-                oct = level < uint32_t(kMaxDepth) ? octantAt(X, Y, Z, level) : 0u;
-            }
-            // one group per (leaf, octant): the slot counter and the histogram
-            uint64_t pend = __ballot(valid);
-            const uint32_t key = leaf * 8u + oct;
-            const uint32_t lane = laneId();
-            uint32_t slot = 0;
-            while (pend) {
-                const int leader = __ffsll((unsigned long long)pend) - 1;
-                const uint32_t lk = __shfl(key, leader);
-                const uint64_t grp = __ballot(valid && key == lk) & pend;
-                uint32_t old = 0;
-                if (lane == uint32_t(leader)) {
-                    const uint32_t c = uint32_t(__popcll(grp));
-                    old = atomicAdd(&t.numPoints[lk >> 3], c);
-                    atomicAdd(&t.storedOct[lk], c);
-                }
-                old = __shfl(old, leader);
-                if ((grp >> lane) & 1ull) slot = old + uint32_t(__popcll(grp & lanesBelow()));
-                pend &= ~grp;
-            }
-            if (valid) {
-                const uint32_t k = slot / kPointsPerChunk;
-                const uint32_t ref = k < kPointDir ? t.pointDir[uint64_t(leaf) * kPointDir + k] : 0u;
-                if (ref != 0)
-                    chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = p;
-                else
-                    g->dirOverflow = 1;
+            slotOf[j] = kEmpty32;
+            leaf[j] = kNone;
+            if (i >= total) continue;
+            p[j] = streamPoint(t, b, batchSize, i);
+            const uint32_t X = quant28(p[j].x, size) >> 8;
+            const uint32_t Y = quant28(p[j].y, size) >> 8;
+            const uint32_t Z = quant28(p[j].z, size) >> 8;
+            uint32_t level;
+            leaf[j] = descend(t.firstChild, X, Y, Z, level);
+            const uint32_t key = leaf[j] * 8u + (level < uint32_t(kMaxDepth) ? octantAt(X, Y, Z, level) : 0u);
+            slotOf[j] = aggInsert(agg, key, &rank[j]);
+            if (slotOf[j] == kEmpty32) {
+                // table full: take a slot directly
+                rank[j] = atomicAdd(&t.numPoints[leaf[j]], 1u);
+                atomicAdd(&t.storedOct[key], 1u);
             }
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
+            const uint32_t key = agg.key[k];
+            if (key == kEmpty32) continue;
+            const uint32_t c = agg.cnt[k];
+            agg.base[k] = atomicAdd(&t.numPoints[key >> 3], c);
+            atomicAdd(&t.storedOct[key], c);
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            if (leaf[j] == kNone) continue;
+            const uint32_t slot = slotOf[j] == kEmpty32 ? rank[j] : agg.base[slotOf[j]] + rank[j];
+            const uint32_t k = slot / kPointsPerChunk;
+            const uint32_t ref = k < kPointDir ? t.pointDir[uint64_t(leaf[j]) * kPointDir + k] : 0u;
+            if (ref != 0)
+                chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = p[j];
+            else
+                g->dirOverflow = 1;
+        }
+        __syncthreads();
     }
 }
 
-// insertVoxels (remolod_octree.cu:918-938).
+// insertVoxels (remolod_octree.cu:918-938), ranked per tile in LDS by node like
+// insertPoints.
 __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t) {
+    __shared__ BlockAgg agg;
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t total = min(g->numBacklog, t.backlogCap);
     const TileRange tr = myTiles(total);
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        aggClear(agg);
+        __syncthreads();
+        uint32_t node[kPerThread], slotOf[kPerThread], rank[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            const bool valid = i < total;
-            const uint32_t node = valid ? t.backlogNode[i] : 0u;
-            const uint32_t slot = waveAggInc(t.numVoxelsStored, node, valid);
-            if (valid) {
-                const uint32_t gi = t.gridNo[node];
-                const uint32_t k = slot / kPointsPerChunk;
-                const uint32_t ref = (gi != kNone && k < kVoxelDir)
-                                         ? t.voxelDir[uint64_t(gi) * kVoxelDir + k]
-                                         : 0u;
-                if (ref != 0)
-                    chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[i];
-                else
-                    g->dirOverflow = 1;
-            }
+            node[j] = kNone;
+            slotOf[j] = kEmpty32;
+            if (i >= total) continue;
+            node[j] = t.backlogNode[i];
+            slotOf[j] = aggInsert(agg, node[j], &rank[j]);
+            if (slotOf[j] == kEmpty32) rank[j] = atomicAdd(&t.numVoxelsStored[node[j]], 1u);
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
+            const uint32_t key = agg.key[k];
+            if (key != kEmpty32) agg.base[k] = atomicAdd(&t.numVoxelsStored[key], agg.cnt[k]);
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            if (node[j] == kNone) continue;
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            const uint32_t slot = slotOf[j] == kEmpty32 ? rank[j] : agg.base[slotOf[j]] + rank[j];
+            const uint32_t gi = t.gridNo[node[j]];
+            const uint32_t k = slot / kPointsPerChunk;
+            const uint32_t ref =
+                (gi != kNone && k < kVoxelDir) ? t.voxelDir[uint64_t(gi) * kVoxelDir + k] : 0u;
+            if (ref != 0)
+                chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[i];
+            else
+                g->dirOverflow = 1;
+        }
+        __syncthreads();
     }
 }
 
@@ -726,17 +805,11 @@ void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, 
 void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
     (void)ls;
     hipLaunchKernelGGL(kSplitApply, dim3(16), dim3(kBlock), 0, s, t);
-    hipLaunchKernelGGL(kSplitCommit, dim3(1), dim3(1), 0, s, t);
 }
 
-void launchSpillCopy(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
+void launchSpillAndZero(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
     (void)ls;
-    hipLaunchKernelGGL(kSpillCopy, dim3(256), dim3(kBlock), 0, s, t);
-}
-
-void launchZeroGrids(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
-    (void)ls;
-    hipLaunchKernelGGL(kZeroGrids, dim3(1024), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t);
 }
 
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {

@@ -112,6 +112,7 @@ struct remo_ctx {
     uint32_t batchletIndex = 0;
     std::vector<void*> allocations;
     AnalysisState analysis;
+    AccumWork accWork;
     Profiler prof;
 };
 
@@ -254,6 +255,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     if (!cfg.ring_slots) cfg.ring_slots = REMO_DEFAULT_RING_SLOTS;
     if (!cfg.spill_capacity) cfg.spill_capacity = 10000000u;
     if (!cfg.backlog_capacity) cfg.backlog_capacity = 10000000u;
+    if (!cfg.closure_batches) cfg.closure_batches = 8u;
     if (!cfg.persistent_bytes) {
         cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20);
     }
@@ -311,6 +313,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         if (rc == REMO_OK) rc = devAlloc(c, p, n, what);
     };
     A(&t.firstChild, N, "node children");
+    A(&t.parent, N, "node parents");
     A(&t.level, N, "node levels");
     A(&t.nx, N, "node X");
     A(&t.ny, N, "node Y");
@@ -336,6 +339,18 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 2, "accepted splits");
     A(&t.g, 1, "device counters");
     A(&t.accums, N, "the accumulator side array");
+    {
+        AccumWork& w = c->accWork;
+        w.tileCap = N + (uint64_t(cfg.slot_capacity) * cfg.max_batches_per_step + cfg.spill_capacity) /
+                            AccumWork::kTile + 1;
+        w.tileCap = std::min<uint64_t>(w.tileCap, 1ull << 24);
+        A(&w.tiles, w.tileCap, "accumulate tiles");
+        A(&w.partials, w.tileCap, "accumulate partial sums");
+        A(&w.nodeTiles, N, "accumulate tiles per node");
+        A(&w.nodeBase, N, "accumulate first tile per node");
+        A(&w.numTiles, 1, "accumulate tile count");
+    }
+    c->analysis.closureMargin = cfg.closure_batches;
     if (rc == REMO_OK) rc = analysisAllocate(c->analysis, N, [&](void** p, uint64_t bytes, const char* what) {
         uint8_t* q = nullptr;
         const int r = devAlloc(c, &q, bytes, what);
@@ -427,8 +442,7 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
                 Scope sc(c->prof, kScopeSplit, s);
                 launchSplitDecide(t, round, c->shape, s);
                 launchSplitApply(t, c->shape, s);
-                if (round == 0) launchSpillCopy(t, c->shape, s);
-                launchZeroGrids(t, c->shape, s);
+                launchSpillAndZero(t, c->shape, s);
             }
             CK(hipGetLastError(), "expand kernels");
             int rc = readGlobals(c);
@@ -457,11 +471,11 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
         CK(hipGetLastError(), "insert kernels");
         c->batchletIndex = bi + 1;
     }
+    // The per-node stats sweep is queued, not waited for: the consumed-batch
+    // cursor is already known here (every batch either ran or hit `stop`, which
+    // the round readback reported), and remo_get_stats synchronises.
     launchStatsSweep(t, c->shape, s);
     CK(hipGetLastError(), "stats sweep");
-    const int rc = readGlobals(c);
-    if (rc != REMO_OK) return rc;
-    c->batchletIndex = c->hostG->batchletIndex;
     if (consumed) *consumed = c->batchletIndex;
     return (v->num_batches_total > 0 && c->batchletIndex >= v->num_batches_total) ? REMO_DONE : REMO_OK;
 }
@@ -472,10 +486,10 @@ int remo_accumulate(remo_ctx* c, uint32_t batchIndex) {
     if (c->cfg.flags & REMO_FLAG_NO_ACCUM) return REMO_OK;
     {
         Scope sc(c->prof, kScopeAccumulate, c->stream);
-        launchAccumulate(c->t, batchIndex, c->shape, c->stream);
+        launchAccumulate(c->t, c->accWork, batchIndex, c->shape, c->stream);
     }
     CK(hipGetLastError(), "accumulate kernels");
-    return readGlobals(c);
+    return REMO_OK;
 }
 
 int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {

@@ -34,8 +34,10 @@ struct MlpState;
 struct AnalysisState {
     remo_node_analysis* results = nullptr;
     RollupSums* rollup = nullptr;
-    uint32_t* levelLists = nullptr;   // nodes grouped by level
-    uint32_t* levelCounts = nullptr;  // [kMaxDepth + 2] counts + cursor
+    uint32_t* arrived = nullptr;      // children arrived, per node (bottom-up roll-up)
+    uint8_t* closed = nullptr;        // closure flag per node
+    uint32_t closureMargin = 8;       // batches untouched before a leaf is closed
+    float* learned = nullptr;         // [n * 2] MLP predictions
     uint32_t* queue = nullptr;        // work queue of nodes (score > 0)
     uint32_t* queueCount = nullptr;
     uint8_t* visible = nullptr;

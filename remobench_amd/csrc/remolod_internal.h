@@ -72,6 +72,7 @@ struct Globals {
 struct DevTree {
     // node structure-of-arrays, capacity maxNodes
     int32_t* firstChild;
+    uint32_t* parent;       // kNone for the root
     uint32_t* level;
     uint32_t* nx;
     uint32_t* ny;
@@ -108,6 +109,26 @@ struct DevTree {
     float octreeSize;
     Globals* g;
     remo_node_accum* accums;
+};
+
+// Partial sums of one Accumulate tile.
+struct AccumPartial {
+    long long q[9];
+    double g[9];
+    float c[6];
+    unsigned long long morton;
+    uint32_t pad[6];
+};
+
+// Work lists of one Accumulate launch.
+struct AccumWork {
+    static constexpr uint32_t kTile = 2048;  // points per tile
+    uint4* tiles;            // (node, begin, end, index within node)
+    AccumPartial* partials;  // one per tile
+    uint32_t* nodeTiles;     // [maxNodes] tiles of the node this launch
+    uint32_t* nodeBase;      // [maxNodes] first tile of the node
+    uint32_t* numTiles;
+    uint64_t tileCap;
 };
 
 // A batch in the ring: batch B at slots + (B % numSlots) * slotCapacity,
@@ -188,14 +209,14 @@ void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t b
 void launchCount(const DevTree& t, BatchRef b, uint32_t round, const LaunchShape& ls, hipStream_t s);
 void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, hipStream_t s);
 void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s);
-void launchSpillCopy(const DevTree& t, const LaunchShape& ls, hipStream_t s);
-void launchZeroGrids(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchSpillAndZero(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchInsertVoxels(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchEndBatch(const DevTree& t, BatchRef b, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
-void launchAccumulate(const DevTree& t, uint32_t batchIndex, const LaunchShape& ls, hipStream_t s);
+void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
+                      hipStream_t s);
 
 }  // namespace remo_hip
