@@ -245,7 +245,9 @@ typedef struct remo_profile_entry {
     uint64_t launches;
 } remo_profile_entry;
 
-int remo_profile_enable(remo_ctx* ctx, int on);
+/* scope_mask: bit i times scope i in the order listed above (count = bit 0 ...
+ * render = bit 8); 0 disables, 0xFFFFFFFF times all. */
+int remo_profile_enable(remo_ctx* ctx, uint32_t scope_mask);
 int remo_profile_read(remo_ctx* ctx, remo_profile_entry* out, uint32_t cap, uint32_t* num);
 int remo_profile_clear(remo_ctx* ctx);
 
@@ -274,7 +276,14 @@ int remo_dev_alloc(size_t bytes, void** out);
 int remo_dev_free(void* p);
 int remo_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int remo_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (remo_copy_kernel): the
+ * known-size reference that HBM byte counters are calibrated against. bytes must
+ * be a multiple of 16; synchronous. */
+int remo_dev_copy(void* dst, const void* src, size_t bytes);
 int remo_device_count(int* out);
+/* Free and total device memory: CudaContext::freeMemory / totalMemory, which
+ * App::computeBudget turns into DeviceBudget (src/shell/App.cpp:97-114). */
+int remo_device_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- synthetic scenes (makeSyntheticSource's role, src/io/PointSource.cpp:535-584) ----
  * Seed-generated lidar-like scene, scan-line ordered, translated to box min 0. */

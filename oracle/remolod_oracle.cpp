@@ -571,6 +571,9 @@ namespace anal {
 
 using i128 = __int128;
 
+// Left shift of a possibly negative __int128 (two's complement, no UB).
+inline i128 shl(i128 x, int k) { return i128((unsigned __int128)x << k); }
+
 inline double i128ToDouble(i128 x) {
     const bool neg = x < 0;
     unsigned __int128 u = neg ? (unsigned __int128)(-x) : (unsigned __int128)x;
@@ -702,12 +705,12 @@ void oracle_tree::analyse(const remo_shared_uniforms& u, uint32_t closureMargin,
         if (nd.isLeaf()) {
             const remo_node_accum& a = nd.acc;
             const int sh = int(D - nd.level);
-            for (int i = 0; i < 3; ++i) S.s1[i] = i128(a.q1[i]) << sh;
-            for (int i = 0; i < 6; ++i) S.s2[i] = i128(a.q2[i]) << (2 * sh);
+            for (int i = 0; i < 3; ++i) S.s1[i] = shl(i128(a.q1[i]), sh);
+            for (int i = 0; i < 6; ++i) S.s2[i] = shl(i128(a.q2[i]), 2 * sh);
             S.n = a.count;
             closed[id] = (a.count == 0) || (current >= a.last_touched_batch + closureMargin);
         } else {
-            const i128 h = i128(1) << (18 + D - nd.level);
+            const i128 h = shl(i128(1), int(18 + D - nd.level));
             closed[id] = 1;
             for (int ci = 0; ci < 8; ++ci) {
                 const uint32_t c = uint32_t(nd.children[ci]);

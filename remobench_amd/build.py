@@ -76,6 +76,33 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+HOST = PKG / "host"
+HOST_SOURCES = ["PointSource.cpp", "RemolodHipPipeline.cpp", "PipelineRegistry.cpp"]
+HOST_LIB = LIBDIR / "libremolod_host.so"
+HEADLESS = LIBDIR / "remolod_headless"
+
+
+def build_host() -> Path:
+    """The C++ host side (ILodPipeline mirror, point source, registry) over the C
+    ABI, plus the headless frame-loop driver. Plain g++: no HIP headers needed."""
+    inc = ["-I", str(ROOT / "include"), "-I", str(HOST)]
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    cmd = ["g++", *flags, *inc, "-shared", "-o", str(HOST_LIB) + ".tmp",
+           *[str(HOST / s) for s in HOST_SOURCES], "-L", str(LIBDIR), "-lremolod", rpath]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("host library build failed:\n" + r.stdout.decode(errors="replace"))
+    Path(str(HOST_LIB) + ".tmp").replace(HOST_LIB)
+    cmd = ["g++", *flags, *inc, "-o", str(HEADLESS) + ".tmp", str(HOST / "remolod_headless.cpp"),
+           "-L", str(LIBDIR), "-lremolod_host", "-lremolod", rpath]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("headless driver build failed:\n" + r.stdout.decode(errors="replace"))
+    Path(str(HEADLESS) + ".tmp").replace(HEADLESS)
+    return HEADLESS
+
+
 def build_oracle() -> Path:
     """Compiles the CPU oracle (test infrastructure) with its own Makefile."""
     r = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], stdout=subprocess.PIPE,
@@ -87,4 +114,5 @@ def build_oracle() -> Path:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_host())
     print(build_oracle())

@@ -198,6 +198,8 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
 __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, uint32_t batchIndex) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
+    // the tile counter starts at zero for the next launch (kAccumTiles is done with it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *w.numTiles = 0;
     unsigned long long sumCounts = 0, folded = 0, wm = 0;
     uint32_t inner = 0, leaves = 0;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
@@ -259,13 +261,10 @@ __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, u
     }
 }
 
-__global__ void kAccumBegin(AccumWork w) { *w.numTiles = 0; }
-
 }  // namespace
 
 void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
                       hipStream_t s) {
-    hipLaunchKernelGGL(kAccumBegin, dim3(1), dim3(1), 0, s, w);
     hipLaunchKernelGGL(kAccumPlan, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w);
     hipLaunchKernelGGL(kAccumTiles, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, w);
     hipLaunchKernelGGL(kAccumFinish, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w, batchIndex);

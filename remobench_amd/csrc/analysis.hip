@@ -5,9 +5,10 @@
 // test closure, roll leaf accumulators up the tree and finalise, score =
 // geometric score x screen coverage, rebuild the work queue each frame.
 //
-//  * Roll-up is one bottom-up pass: every leaf thread writes its sums and walks
-//    up; the eighth child to arrive at a parent (acquire/release counter) adds
-//    the eight children in child order and continues. The sums are __int128 in
+//  * Roll-up is level-synchronous, bottom-up: one launch writes every leaf's
+//    sums, then one launch per level adds each inner node's eight children in
+//    child order (a launch per level beats a one-workgroup walk with barriers
+//    and an arrival-counter chain on these tree sizes). The sums are __int128 in
 //    units U = 2^-(20+D) of the root extent (D the deepest level), centred on
 //    each node: a child's sums move to its parent's centre with the exact offset
 //    +-2^(18+D-L) per axis, so the roll-up is exact and order-independent.
@@ -97,21 +98,22 @@ __device__ __forceinline__ void jacobi3(const T a[6], T ev[3]) {
 
 // ---- roll-up -------------------------------------------------------------------
 
-__global__ void kFillU32(uint32_t* p, uint32_t n, uint32_t v) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
+// Left shift of a possibly negative __int128 (two's complement, no UB).
+__device__ __forceinline__ i128 shl(i128 x, int k) { return i128((unsigned __int128)x << k); }
 
 __device__ __forceinline__ void leafSums(const DevTree& t, uint32_t node, uint32_t D, RollupSums& S) {
     const remo_node_accum& a = t.accums[node];
     const int sh = int(D - t.level[node]);
-    for (int i = 0; i < 3; ++i) S.s1[i] = i128(a.q1[i]) << sh;
-    for (int i = 0; i < 6; ++i) S.s2[i] = i128(a.q2[i]) << (2 * sh);
+    for (int i = 0; i < 3; ++i) S.s1[i] = shl(i128(a.q1[i]), sh);
+    for (int i = 0; i < 6; ++i) S.s2[i] = shl(i128(a.q2[i]), 2 * sh);
     S.n = a.count;
 }
 
+// The child offsets are +-h with h = 2^k, so every product is a shift:
+// S1 += C1 + n o, S2 += C2 + o_a C1_b + C1_a o_b + n o_a o_b.
 __device__ __forceinline__ void parentSums(const DevTree& t, const RollupSums* R, uint32_t parent, uint32_t D,
                                            RollupSums& S) {
-    const i128 h = i128(1) << (18 + D - t.level[parent]);
+    const int k = int(18 + D - t.level[parent]);
     const uint32_t c0 = uint32_t(t.firstChild[parent]);
     for (int i = 0; i < 3; ++i) S.s1[i] = 0;
     for (int i = 0; i < 6; ++i) S.s2[i] = 0;
@@ -119,54 +121,18 @@ __device__ __forceinline__ void parentSums(const DevTree& t, const RollupSums* R
     const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
     for (int ci = 0; ci < 8; ++ci) {
         const RollupSums& C = R[c0 + ci];
-        const i128 o[3] = {(ci & 4) ? h : -h, (ci & 2) ? h : -h, (ci & 1) ? h : -h};
+        const bool pos[3] = {(ci & 4) != 0, (ci & 2) != 0, (ci & 1) != 0};
         const i128 n = i128(C.n);
-        for (int i = 0; i < 3; ++i) S.s1[i] += C.s1[i] + n * o[i];
-        for (int k = 0; k < 6; ++k) {
-            const int a_ = pi[k], b_ = pj[k];
-            S.s2[k] += C.s2[k] + o[a_] * C.s1[b_] + C.s1[a_] * o[b_] + n * o[a_] * o[b_];
+        const i128 nh = shl(n, k);
+        for (int i = 0; i < 3; ++i) S.s1[i] += C.s1[i] + (pos[i] ? nh : -nh);
+        for (int q = 0; q < 6; ++q) {
+            const int a_ = pi[q], b_ = pj[q];
+            const i128 t1 = shl(C.s1[b_], k), t2 = shl(C.s1[a_], k);
+            const i128 cross = (pos[a_] ? t1 : -t1) + (pos[b_] ? t2 : -t2);
+            const i128 nn = shl(n, 2 * k);
+            S.s2[q] += C.s2[q] + cross + (pos[a_] == pos[b_] ? nn : -nn);
         }
         S.n += C.n;
-    }
-}
-
-struct RollArgs {
-    RollupSums* rollup;
-    uint32_t* arrived;
-    uint8_t* closed;
-    uint32_t margin;
-};
-
-__global__ __launch_bounds__(kBlock) void kRollup(DevTree t, RollArgs ra) {
-    Globals* g = t.g;
-    const uint32_t n = g->numNodes;
-    const uint32_t D = g->sMaxDepth;
-    const uint32_t current = g->batchletIndex;
-    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
-        if (t.firstChild[node] >= 0) continue;
-        RollupSums S;
-        leafSums(t, node, D, S);
-        S.e = 20 + D - t.level[node];
-        ra.rollup[node] = S;
-        const remo_node_accum& a = t.accums[node];
-        ra.closed[node] = (a.count == 0 || current >= a.last_touched_batch + ra.margin) ? 1 : 0;
-        uint32_t cur = node;
-        for (;;) {
-            const uint32_t p = t.parent[cur];
-            if (p == kNone) break;
-            const uint32_t before =
-                __hip_atomic_fetch_add(&ra.arrived[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (before != 7) break;  // not the last child: its sibling continues
-            RollupSums P;
-            parentSums(t, ra.rollup, p, D, P);
-            P.e = 20 + D - t.level[p];
-            uint8_t cl = 1;
-            const uint32_t c0 = uint32_t(t.firstChild[p]);
-            for (int ci = 0; ci < 8; ++ci) cl &= ra.closed[c0 + ci];
-            ra.rollup[p] = P;
-            ra.closed[p] = cl;
-            cur = p;
-        }
     }
 }
 
@@ -215,104 +181,156 @@ __device__ __forceinline__ void projectedExtent(const Bounds& b, const remo_mat4
     dy = maxY - minY;
 }
 
-struct FinalArgs {
+struct AnalysisArgs {
     RollupSums* rollup;
+    uint32_t* arrived;      // arrivals per inner node; +8 every frame, never reset
     uint8_t* closed;
     remo_node_analysis* results;
     uint32_t* queue;
     uint32_t* queueCount;
     const float* learned;   // [n * 2] predicted (surface variation, planarity), or null
     uint32_t useLearned;
+    uint32_t margin;
 };
 
-__global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, FinalArgs fa, remo_shared_uniforms u) {
+// Closure, covariance, eigen-solve and score of one node from its rolled-up sums.
+__device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
+                             uint32_t node, const RollupSums& S, bool closed, uint32_t D) {
+    remo_node_analysis r;
+    memset(&r, 0, sizeof(r));
+    r.n = S.n;
+    float dx, dy;
+    projectedExtent(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, dx, dy);
+    r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
+    r.state = closed ? 2u : 0u;
+    bool scored = false;
+    if (aa.useLearned && !closed && S.n >= 3) {
+        // learned score for open nodes; the analytic path is the fallback when the
+        // prediction says the node is not planar (surface variation > 0.05)
+        const float sv = aa.learned[2 * node], pl = aa.learned[2 * node + 1];
+        if (sv <= 0.05f) {
+            r.surface_variation = fmaxf(sv, 0.0f);
+            r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
+            r.linearity = 0.0f;
+            r.flags |= 2u;
+            scored = true;
+        }
+    }
+    if (!scored && S.n >= 3) {
+        const uint32_t e = 20 + D - t.level[node];
+        const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+        int maxS2 = 0, maxS1 = 0;
+        for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
+        for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
+        const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
+        const double scale = ldexp(1.0, -int(e));
+        const double dn = double(S.n);
+        double cov[6];
+        for (int k = 0; k < 6; ++k) {
+            double num;
+            if (exact)
+                num = i128ToDouble(i128(S.n) * S.s2[k] - S.s1[pi[k]] * S.s1[pj[k]]);
+            else
+                num = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
+            cov[k] = num / (dn * dn) * scale * scale;
+        }
+        const double trace = cov[0] + cov[3] + cov[5];
+        if (trace > 0.0) {
+            double a64[6];
+            float a32[6];
+            for (int k = 0; k < 6; ++k) {
+                a64[k] = cov[k] / trace;
+                a32[k] = float(a64[k]);
+            }
+            float ev32[3];
+            jacobi3<float>(a32, ev32);
+            const float gap = fminf(ev32[1] - ev32[0], ev32[2] - ev32[1]);
+            double ev[3];
+            if (gap < 1e-4f) {
+                double ev64[3];
+                jacobi3<double>(a64, ev64);
+                for (int i = 0; i < 3; ++i) ev[i] = ev64[i];
+                r.flags |= 1u;
+            } else {
+                for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
+            }
+            for (int i = 0; i < 3; ++i) r.lambda[i] = float(ev[i] * trace);
+            const double sum = ev[0] + ev[1] + ev[2];
+            r.surface_variation = sum > 0.0 ? float(ev[0] / sum) : 0.0f;
+            r.planarity = ev[2] > 0.0 ? float((ev[1] - ev[0]) / ev[2]) : 0.0f;
+            r.linearity = ev[2] > 0.0 ? float((ev[2] - ev[1]) / ev[2]) : 0.0f;
+        }
+    }
+    r.score = 3.0f * r.surface_variation * r.coverage;
+    aa.results[node] = r;
+    remo_node_accum* a = t.accums + node;
+    a->state = r.state;
+    a->score = r.score;
+}
+
+// Leaves: sums from the fixed-point accumulators, closure from the batch age,
+// then their analysis.
+__global__ __launch_bounds__(kBlock) void kAnalysisLeaves(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
-    const uint32_t D = g->sMaxDepth;
-    const float cubeSize = t.octreeSize;
+    const uint32_t D = g->maxLevel;
+    const uint32_t current = g->batchletIndex;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *aa.queueCount = 0;  // kQueue runs after the levels
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
-        const RollupSums S = fa.rollup[node];
-        remo_node_analysis r;
-        memset(&r, 0, sizeof(r));
-        r.n = S.n;
-        float dx, dy;
-        projectedExtent(nodeBounds(t, node, cubeSize), u.transform_frozen, u.width, u.height, dx, dy);
-        r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
-        const bool closed = fa.closed[node] != 0;
-        r.state = closed ? 2u : 0u;
-        bool scored = false;
-        if (fa.useLearned && !closed && S.n >= 3) {
-            // learned score for open nodes; the analytic path is the fallback when the
-            // prediction says the node is not planar (surface variation > 0.05)
-            const float sv = fa.learned[2 * node], pl = fa.learned[2 * node + 1];
-            if (sv <= 0.05f) {
-                r.surface_variation = fmaxf(sv, 0.0f);
-                r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
-                r.linearity = 0.0f;
-                r.flags |= 2u;
-                scored = true;
-            }
-        }
-        if (!scored && S.n >= 3) {
-            const uint32_t e = 20 + D - t.level[node];
-            const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-            int maxS2 = 0, maxS1 = 0;
-            for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
-            for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
-            const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
-            const double scale = ldexp(1.0, -int(e));
-            const double dn = double(S.n);
-            double cov[6];
-            for (int k = 0; k < 6; ++k) {
-                double num;
-                if (exact)
-                    num = i128ToDouble(i128(S.n) * S.s2[k] - S.s1[pi[k]] * S.s1[pj[k]]);
-                else
-                    num = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
-                cov[k] = num / (dn * dn) * scale * scale;
-            }
-            const double trace = cov[0] + cov[3] + cov[5];
-            if (trace > 0.0) {
-                double a64[6];
-                float a32[6];
-                for (int k = 0; k < 6; ++k) {
-                    a64[k] = cov[k] / trace;
-                    a32[k] = float(a64[k]);
-                }
-                float ev32[3];
-                jacobi3<float>(a32, ev32);
-                const float gap = fminf(ev32[1] - ev32[0], ev32[2] - ev32[1]);
-                double ev[3];
-                if (gap < 1e-4f) {
-                    double ev64[3];
-                    jacobi3<double>(a64, ev64);
-                    for (int i = 0; i < 3; ++i) ev[i] = ev64[i];
-                    r.flags |= 1u;
-                } else {
-                    for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
-                }
-                for (int i = 0; i < 3; ++i) r.lambda[i] = float(ev[i] * trace);
-                const double sum = ev[0] + ev[1] + ev[2];
-                r.surface_variation = sum > 0.0 ? float(ev[0] / sum) : 0.0f;
-                r.planarity = ev[2] > 0.0 ? float((ev[1] - ev[0]) / ev[2]) : 0.0f;
-                r.linearity = ev[2] > 0.0 ? float((ev[2] - ev[1]) / ev[2]) : 0.0f;
-            }
-        }
-        r.score = 3.0f * r.surface_variation * r.coverage;
-        fa.results[node] = r;
-        remo_node_accum* a = t.accums + node;
-        a->state = r.state;
-        a->score = r.score;
-        // work queue: every node with a positive score, rebuilt each frame
-        const bool q = r.score > 0.0f;
+        if (t.firstChild[node] >= 0) continue;
+        RollupSums S;
+        leafSums(t, node, D, S);
+        S.e = 20 + D - t.level[node];
+        const remo_node_accum& a = t.accums[node];
+        const uint8_t cl = (a.count == 0 || current >= a.last_touched_batch + aa.margin) ? 1 : 0;
+        aa.rollup[node] = S;
+        aa.closed[node] = cl;
+    }
+}
+
+// Inner nodes of one level, bottom-up, one launch per level: the eight children
+// (one level deeper, finished by the previous launch) are added in child order.
+// Only the exact sums travel up the levels; the eigen-solve and score of every
+// node run afterwards in one node-parallel pass (kFinalize).
+__global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs aa, uint32_t level) {
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    const uint32_t D = g->maxLevel;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        if (t.level[node] != level) continue;
+        const int32_t fc = t.firstChild[node];
+        if (fc < 0) continue;
+        RollupSums P;
+        parentSums(t, aa.rollup, node, D, P);
+        P.e = 20 + D - level;
+        uint8_t pc = 1;
+        for (int ci = 0; ci < 8; ++ci) pc &= aa.closed[uint32_t(fc) + ci];
+        aa.rollup[node] = P;
+        aa.closed[node] = pc;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa, remo_shared_uniforms u) {
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    const uint32_t D = g->maxLevel;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
+        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
+}
+
+// Work queue: every node with a positive score, rebuilt each frame (one atomic per wave).
+__global__ __launch_bounds__(kBlock) void kQueue(DevTree t, AnalysisArgs aa) {
+    const uint32_t n = t.g->numNodes;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t node = base + threadIdx.x;
+        const bool q = node < n && aa.results[node].score > 0.0f;
         const uint64_t m = __ballot(q);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if (laneId() == uint32_t(leader)) base = atomicAdd(fa.queueCount, uint32_t(__popcll(m)));
-            base = __shfl(base, leader);
-            if (q) fa.queue[base + __popcll(m & lanesBelow())] = node;
-        }
+        if (m == 0) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t at = 0;
+        if (laneId() == uint32_t(leader)) at = atomicAdd(aa.queueCount, uint32_t(__popcll(m)));
+        at = __shfl(at, leader);
+        if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
     }
 }
 
@@ -538,21 +556,20 @@ void analysisReset(AnalysisState& a) {
     a.drawSamples = 0;
 }
 
-int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode,
+int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
                 const LaunchShape& ls, hipStream_t s, std::string* err) {
-    (void)err;
-    const uint32_t nb = ls.nodeBlocks;
-    hipLaunchKernelGGL(kFillU32, dim3(nb), dim3(kBlock), 0, s, a.arrived, t.maxNodes, 0u);
-    hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, s, a.queueCount, 1u, 0u);
-    RollArgs ra{a.rollup, a.arrived, a.closed, a.closureMargin};
-    hipLaunchKernelGGL(kRollup, dim3(nb), dim3(kBlock), 0, s, t, ra);
     const bool learned = mode == REMO_ANALYSIS_LEARNED;
     if (learned) {
         const int rc = mlpPredict(a, t, ls, s, err);
         if (rc != REMO_OK) return rc;
     }
-    FinalArgs fa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.learned, learned ? 1u : 0u};
-    hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, fa, u);
+    AnalysisArgs aa{a.rollup, a.arrived, a.closed, a.results, a.queue, a.queueCount,
+                    a.learned, learned ? 1u : 0u, a.closureMargin};
+    hipLaunchKernelGGL(kAnalysisLeaves, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa);
+    for (int level = int(maxLevel) - 1; level >= 0; --level)
+        hipLaunchKernelGGL(kAnalysisLevel, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, uint32_t(level));
+    hipLaunchKernelGGL(kFinalize, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, u);
+    hipLaunchKernelGGL(kQueue, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa);
     return REMO_OK;
 }
 

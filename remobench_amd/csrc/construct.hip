@@ -150,6 +150,7 @@ __global__ void kResetRoot(DevTree t) {
     g->arenaOffset += kGridAllocBytes;
     t.voxelChunks[0] = 0;
     t.gridNo[0] = 0;
+    t.walk[0] = make_uint2(kNone, uint32_t(t.gridOffset[0] >> 4));
 }
 
 // Zeroes `count` 256 KiB grids listed by offsets (uint4 stores).
@@ -288,15 +289,39 @@ __device__ __forceinline__ bool keyLess(const DevTree& t, uint32_t a, uint32_t b
 // key (level, X, Y, Z); the node pool accepts the lowest-ranked ones that fit,
 // so child indices and refusals are deterministic. Children take their
 // counters from the octant histograms; a child still above 50,000 is pending.
-__global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
+// Candidate keys staged in LDS for ranking (one workgroup handles a round).
+constexpr uint32_t kRankSlots = 2048;
+
+__device__ __forceinline__ bool keyLess4(const uint4& a, const uint4& b) {
+    if (a.x != b.x) return a.x < b.x;
+    if (a.y != b.y) return a.y < b.y;
+    if (a.z != b.z) return a.z < b.z;
+    return a.w < b.w;
+}
+
+__global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
+    __shared__ uint4 keys[kRankSlots];  // (level, X, Y, Z) of each candidate
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t nc = min(g->numCandidates, t.candidateCap);
     const uint32_t base = g->numNodes;
+    const bool staged = nc <= kRankSlots;
+    if (staged) {
+        for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+            const uint32_t c = t.candidates[k];
+            keys[k] = make_uint4(t.level[c], t.nx[c], t.ny[c], t.nz[c]);
+        }
+    }
+    __syncthreads();
     for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
         const uint32_t node = t.candidates[ci];
         uint32_t rank = 0;
-        for (uint32_t k = 0; k < nc; ++k) rank += keyLess(t, t.candidates[k], node) ? 1u : 0u;
+        if (staged) {
+            const uint4 mine = keys[ci];
+            for (uint32_t k = 0; k < nc; ++k) rank += keyLess4(keys[k], mine) ? 1u : 0u;
+        } else {
+            for (uint32_t k = 0; k < nc; ++k) rank += keyLess(t, t.candidates[k], node) ? 1u : 0u;
+        }
         const uint64_t childOffset = uint64_t(base) + 8ull * rank;
         if (childOffset + 8 > t.maxNodes) {
             g->nodePoolOverflow = 1;  // stays a leaf; its points stay where they are
@@ -307,12 +332,15 @@ __global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
         }
         const uint32_t c0 = uint32_t(childOffset);
         const uint32_t lvl = t.level[node] + 1;
+        atomicMax(&g->maxLevel, lvl);
         const uint32_t X = t.nx[node], Y = t.ny[node], Z = t.nz[node];
         for (uint32_t i = 0; i < 8; ++i) {
             const uint32_t c = c0 + i;
             const uint32_t cnt = t.storedOct[8ull * node + i] + t.batchOct[8ull * node + i];
             t.firstChild[c] = -1;
+            t.walk[c] = make_uint2(kNone, 0u);
             t.parent[c] = node;
+            t.arrivals[c] = 0;
             t.level[c] = lvl;
             t.nx[c] = 2 * X + ((i >> 2) & 1u);
             t.ny[c] = 2 * Y + ((i >> 1) & 1u);
@@ -358,8 +386,10 @@ __global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
             }
         }
         const uint32_t slot = atomicAdd(&g->numAccepted, 1u);
-        t.acceptedSpill[2 * slot] = node;
-        t.acceptedSpill[2 * slot + 1] = spillOff;
+        t.acceptedSpill[3 * slot] = node;
+        t.acceptedSpill[3 * slot + 1] = spillOff;
+        t.acceptedSpill[3 * slot + 2] = np;
+        t.numPoints[node] = 0;  // remolod_octree.cu:362; kSpillAndZero copies from the recorded count
 
         if (t.gridNo[node] == kNone) {
             const uint32_t gi = atomicAdd(&g->numGrids, 1u);
@@ -377,6 +407,8 @@ __global__ __launch_bounds__(kBlock) void kSplitApply(DevTree t) {
         atomicAdd((unsigned long long*)&g->totNodesSplit, 1ull);
         // publish the children last: later kernels descend through them
         t.firstChild[node] = int32_t(c0);
+        const uint32_t gno = t.gridNo[node];
+        t.walk[node] = make_uint2(c0, gno == kNone ? 0u : uint32_t(t.gridOffset[gno] >> 4));
     }
 }
 
@@ -395,25 +427,24 @@ __global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t) {
         g->expandIters += 1;
         g->totExpandIters += 1;
     }
-    for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
-        const uint32_t node = t.acceptedSpill[2 * a];
-        const uint32_t off = t.acceptedSpill[2 * a + 1];
-        const uint32_t np = t.numPoints[node];
-        if (off != kNone) {
-            for (uint32_t s = threadIdx.x; s < np; s += blockDim.x) {
-                if (s / kPointsPerChunk >= kPointDir) break;
-                const uint32_t ref = t.pointDir[uint64_t(node) * kPointDir + s / kPointsPerChunk];
-                t.spill[off + s] = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) t.numPoints[node] = 0;
+    // one (accepted node, chunk) pair per block step, so a large node's copy
+    // spreads over many workgroups
+    for (uint32_t pair = blockIdx.x; pair < na * kPointDir; pair += gridDim.x) {
+        const uint32_t a = pair / kPointDir, k = pair % kPointDir;
+        const uint32_t node = t.acceptedSpill[3 * a];
+        const uint32_t off = t.acceptedSpill[3 * a + 1];
+        const uint32_t np = t.acceptedSpill[3 * a + 2];
+        if (off == kNone || k * kPointsPerChunk >= np) continue;
+        const uint32_t ref = t.pointDir[uint64_t(node) * kPointDir + k];
+        const uint32_t m = min(kPointsPerChunk, np - k * kPointsPerChunk);
+        const remo_point* src = chunkPtr(t.arena, ref);
+        for (uint32_t s = threadIdx.x; s < m; s += blockDim.x) t.spill[off + k * kPointsPerChunk + s] = src[s];
     }
     const uint64_t perGrid = kGridWords / 4;
     const uint64_t total = uint64_t(na) * perGrid;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t node = t.acceptedSpill[2 * (i / perGrid)];
+        const uint32_t node = t.acceptedSpill[3 * (i / perGrid)];
         const uint32_t gi = t.gridNo[node];
         if (gi == kNone) continue;
         reinterpret_cast<uint4*>(t.arena + t.gridOffset[gi])[i % perGrid] = make_uint4(0, 0, 0, 0);
@@ -422,105 +453,180 @@ __global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t) {
 
 // ---------------------------------------------------------------------------
 // voxelSampling / sampleVoxel (remolod_octree.cu:677-740, :95-140): at every
-// gridded node on the root->leaf path, set the point's 128^3 cell; the lane that
-// sets a bit appends the cell-centre voxel, coloured by its point.
+// gridded node on the root->leaf path, set the point's 128^3 cell; whoever sets
+// a bit appends the cell-centre voxel, coloured by its point.
 //
-// A tile of 1,024 points keeps one LDS set of the (grid, cell) keys it has
-// already handled; a grid number names one node, so the set spans all levels
-// and needs no barrier between levels. Only the first lane to present a key
-// peeks the global word and, if the bit is clear, does the atomicOr.
-constexpr uint32_t kSeenSlots = 4096;
+// The walk goes bottom-up. A cell set at a node implies that the point's cells
+// at every gridded ancestor are set too: every point of a region is sampled into
+// every gridded node above it, and a node only gets (or re-zeroes) a grid in the
+// batch it splits, when all its region's points pass through the spill list. So
+// a point peeks its deepest cell first and climbs only while it keeps finding
+// clear bits; the lane whose atomicOr sets a bit owns that voxel and climbs on.
+// At steady state most points stop after one peek.
+//
+// New voxels are staged per wave in LDS and reach the global backlog with one
+// atomicAdd per flush; numVoxels is bumped once per (wave, node) group without a
+// return value.
+constexpr uint32_t kStage = 128;  // staged voxels per wave
 
-__device__ __forceinline__ bool seenFirst4k(unsigned long long* seen, unsigned long long key) {
-    uint32_t h = uint32_t((key * 0x9E3779B97F4A7C15ull) >> 52);
-    for (uint32_t probe = 0; probe < 16; ++probe) {
-        const unsigned long long prev = atomicCAS(&seen[h], kEmpty64, key);
-        if (prev == kEmpty64) return true;
-        if (prev == key) return false;
-        h = (h + 1) & (kSeenSlots - 1);
+// Wave-aggregated atomicAdd(base[key], 1) for the lanes with `valid`, without a
+// return value (the atomics are fire-and-forget).
+__device__ __forceinline__ void waveAggAdd(uint32_t* base, uint32_t key, bool valid) {
+    uint64_t pend = __ballot(valid);
+    while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const uint64_t grp = __ballot(valid && key == lk) & pend;
+        if (laneId() == uint32_t(leader)) atomicAdd(&base[lk], uint32_t(__popcll(grp)));
+        pend &= ~grp;
     }
-    return true;  // set full around this key: no de-duplication, still exact
+}
+
+struct VoxelStage {
+    remo_point pts[kStage];
+    uint32_t node[kStage];
+};
+
+// Moves a wave's staged voxels to the global backlog (one atomic).
+__device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uint32_t& count) {
+    if (count == 0) return;
+    // the wave's own LDS writes must land before other lanes read them
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t base = 0;
+    if (laneId() == 0) base = atomicAdd(&t.g->numBacklog, count);
+    base = __shfl(base, 0);
+    for (uint32_t k = laneId(); k < count; k += 64) {
+        const uint32_t at = base + k;
+        if (at < t.backlogCap) {
+            t.backlogPts[at] = st.pts[k];
+            t.backlogNode[at] = st.node[k];
+        } else {
+            t.g->backlogOverflow = 1;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    count = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
-    __shared__ unsigned long long seen[kSeenSlots];
+    __shared__ VoxelStage stage[kBlock / 64];
     const uint32_t batchSize = batchSizeOf(t, b);
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t total = batchSize + spilledCount(t);
     const TileRange tr = myTiles(total);
     const float size = t.octreeSize;
+    VoxelStage& st = stage[threadIdx.x >> 6];
+    uint32_t staged = 0;  // wave-uniform
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
-        for (uint32_t k = threadIdx.x; k < kSeenSlots; k += blockDim.x) seen[k] = kEmpty64;
-        __syncthreads();
+        uint32_t pX[kPerThread], pY[kPerThread], pZ[kPerThread], col[kPerThread];
+        uint32_t node[kPerThread], lvl[kPerThread];
+        bool active[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            bool walking = i < total;
-            uint32_t pX = 0, pY = 0, pZ = 0, col = 0, cur = 0;
-            if (walking) {
+            active[j] = i < total;
+            pX[j] = pY[j] = pZ[j] = col[j] = 0;
+            if (active[j]) {
                 const remo_point p = streamPoint(t, b, batchSize, i);
-                pX = quant28(p.x, size);
-                pY = quant28(p.y, size);
-                pZ = quant28(p.z, size);
-                col = p.color;
+                pX[j] = quant28(p.x, size);
+                pY[j] = quant28(p.y, size);
+                pZ[j] = quant28(p.z, size);
+                col[j] = p.color;
             }
-            const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
-            for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
-                if (__ballot(walking) == 0) break;
-                bool isNew = false;
-                uint32_t vi = 0;
-                const uint32_t node = cur;
-                if (walking) {
-                    const uint32_t gi = t.gridNo[node];
-                    if (gi != kNone) {
-                        const uint32_t sh = (kMaxDepth + 1) - level;
-                        vi = ((pX >> sh) & (kGridSize - 1)) + ((pY >> sh) & (kGridSize - 1)) * kGridSize +
-                             ((pZ >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
-                        if (seenFirst4k(seen, (uint64_t(gi) << 21) | vi)) {
-                            uint32_t* word =
-                                reinterpret_cast<uint32_t*>(t.arena + t.gridOffset[gi]) + (vi >> 5);
-                            const uint32_t mask = 1u << (vi & 31u);
-                            // plain peek first: a stale word only costs an extra atomicOr
-                            if ((*word & mask) == 0) isNew = (atomicOr(word, mask) & mask) == 0;
-                        }
-                    }
+        }
+        // descend to the leaf; the climb starts at the deepest node with a grid
+        // (the leaf's parent, or the root while it is still a leaf)
+        uint32_t parentOf[kPerThread];
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            node[j] = 0;
+            lvl[j] = 0;
+            parentOf[j] = kNone;
+        }
+        for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
+            bool any = false;
+            uint2 rec[kPerThread];
+            for (uint32_t j = 0; j < kPerThread; ++j) rec[j] = active[j] ? t.walk[node[j]] : make_uint2(kNone, 0u);
+            for (uint32_t j = 0; j < kPerThread; ++j) {
+                if (!active[j] || rec[j].x == kNone || lvl[j] != level) continue;
+                const uint32_t s2 = kMaxDepth - level - 1;
+                const uint32_t X = pX[j] >> 8, Y = pY[j] >> 8, Z = pZ[j] >> 8;
+                parentOf[j] = node[j];
+                node[j] = rec[j].x + ((((X >> s2) & 1u) << 2) | (((Y >> s2) & 1u) << 1) | ((Z >> s2) & 1u));
+                lvl[j] = level + 1;
+                any = true;
+            }
+            if (__ballot(any) == 0) break;
+        }
+        uint32_t gref[kPerThread];
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            gref[j] = active[j] ? t.walk[node[j]].y : 0u;
+            if (active[j] && gref[j] == 0) {
+                if (parentOf[j] == kNone) {
+                    active[j] = false;  // a root without a grid: nothing to sample
+                } else {
+                    node[j] = parentOf[j];
+                    lvl[j] -= 1;
                 }
-                waveAggInc(t.numVoxels, node, isNew);
-                const uint32_t at = waveAppend(&g->numBacklog, isNew);
-                if (isNew) {
-                    if (at < t.backlogCap) {
-                        const uint32_t cx = vi & (kGridSize - 1);
-                        const uint32_t cy = (vi >> 7) & (kGridSize - 1);
-                        const uint32_t cz = vi >> 14;
-                        const float nodeSize = size / float(1u << level);
-                        const float minx = float(t.nx[node]) * nodeSize;
-                        const float miny = float(t.ny[node]) * nodeSize;
-                        const float minz = float(t.nz[node]) * nodeSize;
+            }
+        }
+        // climb while new cells are set
+        for (uint32_t step = 0; step <= uint32_t(kMaxDepth); ++step) {
+            bool any = false;
+            for (uint32_t j = 0; j < kPerThread; ++j) any |= active[j];
+            if (__ballot(any) == 0) break;
+            uint32_t vi[kPerThread], mask[kPerThread], peek[kPerThread];
+            uint32_t* word[kPerThread];
+            for (uint32_t j = 0; j < kPerThread; ++j) {
+                gref[j] = active[j] ? t.walk[node[j]].y : 0u;
+                if (gref[j] == 0) active[j] = false;  // no grid here (allocation overflow)
+                const uint32_t sh = (kMaxDepth + 1) - lvl[j];
+                vi[j] = ((pX[j] >> sh) & (kGridSize - 1)) + ((pY[j] >> sh) & (kGridSize - 1)) * kGridSize +
+                        ((pZ[j] >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
+                mask[j] = 1u << (vi[j] & 31u);
+                word[j] = reinterpret_cast<uint32_t*>(t.arena + (uint64_t(gref[j]) << 4)) + (vi[j] >> 5);
+            }
+            // independent peeks, then independent atomics; a stale peek only costs an atomicOr
+            for (uint32_t j = 0; j < kPerThread; ++j) peek[j] = active[j] ? *word[j] : 0xFFFFFFFFu;
+            bool isNew[kPerThread];
+            for (uint32_t j = 0; j < kPerThread; ++j)
+                isNew[j] = (peek[j] & mask[j]) == 0 && (atomicOr(word[j], mask[j]) & mask[j]) == 0;
+            for (uint32_t j = 0; j < kPerThread; ++j) {
+                const uint32_t nd = node[j];
+                waveAggAdd(t.numVoxels, nd, isNew[j]);
+                const uint64_t nm = __ballot(isNew[j]);
+                const uint32_t cnt = uint32_t(__popcll(nm));
+                if (cnt) {
+                    if (staged + cnt > kStage) flushStage(t, st, staged);
+                    if (isNew[j]) {
+                        const uint32_t cx = vi[j] & (kGridSize - 1);
+                        const uint32_t cy = (vi[j] >> 7) & (kGridSize - 1);
+                        const uint32_t cz = vi[j] >> 14;
+                        const float nodeSize = size / float(1u << lvl[j]);
+                        const float minx = float(t.nx[nd]) * nodeSize;
+                        const float miny = float(t.ny[nd]) * nodeSize;
+                        const float minz = float(t.nz[nd]) * nodeSize;
                         remo_point v;
                         v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
                         v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
                         v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
-                        v.color = col;
-                        t.backlogPts[at] = v;
-                        t.backlogNode[at] = node;
-                    } else {
-                        g->backlogOverflow = 1;
+                        v.color = col[j];
+                        const uint32_t slot = staged + uint32_t(__popcll(nm & lanesBelow()));
+                        st.pts[slot] = v;
+                        st.node[slot] = nd;
                     }
+                    staged += cnt;
                 }
-                if (walking) {
-                    const int32_t fc = t.firstChild[node];
-                    if (fc < 0) {
-                        walking = false;
-                    } else {
-                        const uint32_t sh = kMaxDepth - level - 1;
-                        cur = uint32_t(fc) +
-                              ((((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) | ((Z >> sh) & 1u));
-                    }
+                // a set bit means every ancestor cell is set: stop; a new one climbs
+                if (!isNew[j] || lvl[j] == 0) {
+                    active[j] = false;
+                } else {
+                    node[j] = t.parent[nd];
+                    lvl[j] -= 1;
                 }
             }
         }
-        __syncthreads();
     }
+    flushStage(t, st, staged);
 }
 
 // allocatePointChunks + allocateVoxelChunks (remolod_octree.cu:742-792, :885-916).
@@ -661,11 +767,24 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
 
 // insertVoxels (remolod_octree.cu:918-938), ranked per tile in LDS by node like
 // insertPoints.
-__global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t) {
+__global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b) {
     __shared__ BlockAgg agg;
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t total = min(g->numBacklog, t.backlogCap);
+    __syncthreads();  // every thread has read the counters before block 0 ends the batch
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the batch loop's closing update (remolod_octree.cu:1223-1226)
+        const uint64_t spilled = min(g->numSpilled, t.spillCap);
+        g->batchletIndex += 1;
+        g->numPointsProcessed += *b.sizePtr;
+        g->batches += 1;
+        g->spilledPoints += spilled;
+        g->newVoxels += total;
+        g->totBatches += 1;
+        g->totSpilledPoints += spilled;
+        g->totNewVoxels += total;
+    }
     const TileRange tr = myTiles(total);
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
         aggClear(agg);
@@ -701,21 +820,6 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t) {
         }
         __syncthreads();
     }
-}
-
-__global__ void kEndBatch(DevTree t, BatchRef b) {
-    Globals* g = t.g;
-    if (g->stop) return;
-    g->batchletIndex += 1;
-    g->numPointsProcessed += *b.sizePtr;
-    const uint64_t spilled = min(g->numSpilled, t.spillCap);
-    const uint64_t voxels = min(g->numBacklog, t.backlogCap);
-    g->batches += 1;
-    g->spilledPoints += spilled;
-    g->newVoxels += voxels;
-    g->totBatches += 1;
-    g->totSpilledPoints += spilled;
-    g->totNewVoxels += voxels;
 }
 
 // The per-node stats sweep (remolod_octree.cu:1248-1319).
@@ -804,7 +908,7 @@ void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, 
 
 void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
     (void)ls;
-    hipLaunchKernelGGL(kSplitApply, dim3(16), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kSplitApply, dim3(1), dim3(1024), 0, s, t);
 }
 
 void launchSpillAndZero(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
@@ -824,12 +928,8 @@ void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hip
     hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
 }
 
-void launchInsertVoxels(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t);
-}
-
-void launchEndBatch(const DevTree& t, BatchRef b, hipStream_t s) {
-    hipLaunchKernelGGL(kEndBatch, dim3(1), dim3(1), 0, s, t, b);
+void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
 }
 
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s) {

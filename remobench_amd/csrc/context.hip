@@ -35,7 +35,7 @@ const char* const kScopeNames[kNumScopes] = {
 
 // HIP-event pairs per launch, harvested after the stream has passed them.
 struct Profiler {
-    bool enabled = false;
+    uint32_t mask = 0;  // scopes being timed
     std::vector<hipEvent_t> pool;
     struct Pair {
         int scope;
@@ -81,7 +81,7 @@ struct Scope {
     hipStream_t s;
     hipEvent_t a = nullptr;
     Scope(Profiler& prof, int scope, hipStream_t stream) : p(prof), id(scope), s(stream) {
-        if (p.enabled) {
+        if (p.mask & (1u << scope)) {
             a = p.get();
             if (a) (void)hipEventRecord(a, s);
         }
@@ -105,6 +105,8 @@ struct remo_ctx {
     Globals* hostG = nullptr;  // pinned mirror
     uint32_t* hostScratch = nullptr;  // pinned, 16 words after the mirror
     uint64_t budgetTicks = 0;         // step budget in device wall-clock ticks
+    const uint32_t* uploadedPtr = nullptr;  // ring cursor last read, and its value
+    uint32_t uploadedSeen = 0;
     LaunchShape shape;
     std::string err;
     bool dead = false;
@@ -313,6 +315,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         if (rc == REMO_OK) rc = devAlloc(c, p, n, what);
     };
     A(&t.firstChild, N, "node children");
+    A(&t.walk, N, "node walk records");
     A(&t.parent, N, "node parents");
     A(&t.level, N, "node levels");
     A(&t.nx, N, "node X");
@@ -336,7 +339,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.backlogPts, t.backlogCap, "the voxel backlog");
     A(&t.backlogNode, t.backlogCap, "the voxel backlog targets");
     A(&t.candidates, t.candidateCap, "split candidates");
-    A(&t.acceptedSpill, uint64_t(t.candidateCap) * 2, "accepted splits");
+    A(&t.acceptedSpill, uint64_t(t.candidateCap) * 3, "accepted splits");
     A(&t.g, 1, "device counters");
     A(&t.accums, N, "the accumulator side array");
     {
@@ -357,6 +360,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         *p = q;
         return r;
     });
+    t.arrivals = c->analysis.arrived;
     if (rc != REMO_OK) {
         std::string e = c->err;
         releaseAll(c);
@@ -391,11 +395,15 @@ int remo_reset(remo_ctx* c) {
     DevTree& t = c->t;
     CK(hipMemsetAsync(t.g, 0, sizeof(Globals), c->stream), "reset");
     CK(hipMemsetAsync(t.accums, 0, sizeof(remo_node_accum) * uint64_t(t.maxNodes), c->stream), "reset");
+    CK(hipMemsetAsync(t.arrivals, 0, sizeof(uint32_t) * uint64_t(t.maxNodes), c->stream), "reset");
+    CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
     launchResetTree(t, c->stream);
     CK(hipGetLastError(), "reset kernel");
     const int rc = readGlobals(c);
     if (rc != REMO_OK) return rc;
     c->batchletIndex = 0;
+    c->uploadedPtr = nullptr;
+    c->uploadedSeen = 0;
     c->needsReset = false;
     analysisReset(c->analysis);
     return REMO_OK;
@@ -416,11 +424,17 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
                  std::to_string(R) + " x " + std::to_string(c->cfg.slot_capacity) + ")";
         return REMO_ERR_RING;
     }
-    uint32_t uploaded = 0;
-    CK(hipMemcpyAsync(c->hostScratch, v->num_batches_uploaded, 4, hipMemcpyDeviceToHost, c->stream),
-       "reading the uploaded-batch cursor");
-    CK(hipStreamSynchronize(c->stream), "device work");
-    uploaded = c->hostScratch[0];
+    // The producer cursor only grows, so a value read earlier from the same ring is
+    // a valid lower bound; it is re-read only when it would hold this step back.
+    if (v->num_batches_uploaded != c->uploadedPtr ||
+        c->uploadedSeen < c->batchletIndex + c->cfg.max_batches_per_step) {
+        CK(hipMemcpyAsync(c->hostScratch, v->num_batches_uploaded, 4, hipMemcpyDeviceToHost, c->stream),
+           "reading the uploaded-batch cursor");
+        CK(hipStreamSynchronize(c->stream), "device work");
+        c->uploadedSeen = c->hostScratch[0];
+        c->uploadedPtr = v->num_batches_uploaded;
+    }
+    const uint32_t uploaded = c->uploadedSeen;
     DevTree& t = c->t;
     hipStream_t s = c->stream;
     // step start time and per-step counters (DeviceTimeline counter block: per launch)
@@ -465,17 +479,15 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
         }
         {
             Scope sc(c->prof, kScopeInsertVoxels, s);
-            launchInsertVoxels(t, c->shape, s);
+            launchInsertVoxels(t, b, c->shape, s);
         }
-        launchEndBatch(t, b, s);
         CK(hipGetLastError(), "insert kernels");
         c->batchletIndex = bi + 1;
     }
-    // The per-node stats sweep is queued, not waited for: the consumed-batch
-    // cursor is already known here (every batch either ran or hit `stop`, which
-    // the round readback reported), and remo_get_stats synchronises.
-    launchStatsSweep(t, c->shape, s);
-    CK(hipGetLastError(), "stats sweep");
+    // The per-node stats sweep (remolod_octree.cu:1248-1319) runs when stats are
+    // read (remo_get_stats): nothing on the path consumes it. The consumed-batch
+    // cursor is already known here: every batch either ran or hit `stop`, which
+    // the round readback reported.
     if (consumed) *consumed = c->batchletIndex;
     return (v->num_batches_total > 0 && c->batchletIndex >= v->num_batches_total) ? REMO_DONE : REMO_OK;
 }
@@ -499,7 +511,7 @@ int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {
     int rc;
     {
         Scope sc(c->prof, kScopeAnalysis, c->stream);
-        rc = runAnalysis(c->analysis, c->t, *u, mode, c->shape, c->stream, &c->err);
+        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, c->shape, c->stream, &c->err);
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "analysis kernels");
@@ -523,6 +535,8 @@ int remo_render(remo_ctx* c, const remo_shared_uniforms* u, uint64_t* fb) {
 int remo_get_stats(remo_ctx* c, remo_stats* out) {
     if (!c || !out) return REMO_ERR_INVALID;
     if (c->dead) return REMO_ERR_DEVICE;
+    launchStatsSweep(c->t, c->shape, c->stream);
+    CK(hipGetLastError(), "stats sweep");
     const int rc = readGlobals(c);
     if (rc != REMO_OK) return rc;
     analysisReadDiagnostics(c->analysis, c->stream);
@@ -648,9 +662,9 @@ int remo_dump_analysis(remo_ctx* c, remo_node_analysis* out, uint32_t cap, uint3
     return REMO_OK;
 }
 
-int remo_profile_enable(remo_ctx* c, int on) {
+int remo_profile_enable(remo_ctx* c, uint32_t mask) {
     if (!c) return REMO_ERR_INVALID;
-    c->prof.enabled = on != 0;
+    c->prof.mask = mask;
     return REMO_OK;
 }
 
@@ -760,11 +774,35 @@ int remo_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
 }
 
+}  // extern "C"
+
+__global__ void remo_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" {
+
+int remo_dev_copy(void* dst, const void* src, size_t bytes) {
+    if (!dst || !src || (bytes & 15)) return REMO_ERR_INVALID;
+    hipLaunchKernelGGL(remo_copy_kernel, dim3(4096), dim3(256), 0, 0, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), uint64_t(bytes / 16));
+    if (hipGetLastError() != hipSuccess) return REMO_ERR_DEVICE;
+    return hipDeviceSynchronize() == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
+}
+
 int remo_device_count(int* out) {
     if (!out) return REMO_ERR_INVALID;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
     *out = n;
+    return REMO_OK;
+}
+
+int remo_device_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return REMO_ERR_INVALID;
+    *free_bytes = *total_bytes = 0;
+    if (hipMemGetInfo(free_bytes, total_bytes) != hipSuccess) return REMO_ERR_DEVICE;
     return REMO_OK;
 }
 

@@ -56,7 +56,8 @@ using AllocFn = std::function<int(void**, uint64_t, const char*)>;
 
 int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc);
 void analysisReset(AnalysisState& a);
-int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode,
+// maxLevel: the deepest node level (Globals::maxLevel as last read back).
+int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
                 const LaunchShape& ls, hipStream_t s, std::string* err);
 int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb,
               const LaunchShape& ls, hipStream_t s, std::string* err);

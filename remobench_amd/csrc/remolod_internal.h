@@ -55,6 +55,8 @@ struct Globals {
     uint32_t allocOverflow, nodePoolOverflow, spillOverflow, backlogOverflow;
     uint32_t dirOverflow, candidateOverflow, budgetStop, pad1;
     uint64_t stepStart;            // wall_clock64() at the start of the step
+    uint32_t maxLevel;             // deepest node level (kept by the splits)
+    uint32_t pad3;
     // per-step counters
     uint64_t batches, expandIters, nodesSplit, spilledPoints, newVoxels;
     // the same, cumulative since reset
@@ -72,7 +74,9 @@ struct Globals {
 struct DevTree {
     // node structure-of-arrays, capacity maxNodes
     int32_t* firstChild;
+    uint2* walk;            // per node {first child or kNone, grid offset >> 4 or 0}: one load per level
     uint32_t* parent;       // kNone for the root
+    uint32_t* arrivals;     // Analysis roll-up arrival counters (zero for a new node)
     uint32_t* level;
     uint32_t* nx;
     uint32_t* ny;
@@ -102,7 +106,7 @@ struct DevTree {
     remo_point* backlogPts;
     uint32_t* backlogNode;
     uint32_t* candidates;   // split candidates of a round
-    uint32_t* acceptedSpill;// [candidateCap * 2] (node, spill offset) of accepted splits
+    uint32_t* acceptedSpill;// [candidateCap * 3] (node, spill offset, stored points) of accepted splits
     uint32_t candidateCap;
     uint32_t maxNodes;
     uint32_t maxGrids;
@@ -213,8 +217,7 @@ void launchSpillAndZero(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
-void launchInsertVoxels(const DevTree& t, const LaunchShape& ls, hipStream_t s);
-void launchEndBatch(const DevTree& t, BatchRef b, hipStream_t s);
+void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
                       hipStream_t s);

@@ -126,6 +126,12 @@ ANALYSIS_DTYPE = np.dtype([
 ])
 
 
+SCOPES = ["remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
+          "remolod.construct.allocChunks", "remolod.construct.insertPoints",
+          "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render"]
+PROFILE_ALL = 0xFFFFFFFF
+
+
 class ProfileEntry(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("launches", C.c_uint64)]
 
@@ -175,8 +181,10 @@ def load() -> C.CDLL:
         "remo_dev_free": (C.c_int, [C.c_void_p]),
         "remo_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "remo_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+        "remo_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "remo_device_count": (C.c_int, [P(C.c_int)]),
-        "remo_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+        "remo_device_mem_info": (C.c_int, [P(C.c_size_t), P(C.c_size_t)]),
+        "remo_profile_enable": (C.c_int, [C.c_void_p, C.c_uint32]),
         "remo_profile_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_profile_clear": (C.c_int, [C.c_void_p]),
         "remo_scene_default": (None, [P(SceneSpec)]),

@@ -170,8 +170,17 @@ class Tree:
     def sync(self):
         self._ck(self.lib.remo_sync(self.h), "sync")
 
-    def profile_enable(self, on: bool = True):
-        self._ck(self.lib.remo_profile_enable(self.h, int(on)), "profile_enable")
+    def profile_enable(self, on: bool = True, scopes=None):
+        """Times every scope, or only the named ones (see lib.SCOPES)."""
+        if not on:
+            mask = 0
+        elif scopes is None:
+            mask = L.PROFILE_ALL
+        else:
+            mask = 0
+            for name in scopes:
+                mask |= 1 << L.SCOPES.index(name)
+        self._ck(self.lib.remo_profile_enable(self.h, mask), "profile_enable")
 
     def profile_clear(self):
         self._ck(self.lib.remo_profile_clear(self.h), "profile_clear")
