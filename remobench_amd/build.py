@@ -17,7 +17,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libremolod.so"
 
-HIP_SOURCES = ["construct.hip", "accumulate.hip", "analysis.hip", "render.hip", "mlp.hip",
+HIP_SOURCES = ["construct.hip", "accumulate.hip", "analysis.hip", "mlp.hip",
                "context.hip"]
 CPP_SOURCES = ["scene.cpp"]
 

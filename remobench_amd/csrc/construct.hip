@@ -80,8 +80,9 @@ __device__ __forceinline__ uint32_t waveAppend(uint32_t* ctr, bool valid) {
 struct TileRange {
     uint32_t begin, end;
 };
+template <uint32_t Tile = kTile>
 __device__ __forceinline__ TileRange myTiles(uint32_t total) {
-    const uint32_t numTiles = (total + kTile - 1) / kTile;
+    const uint32_t numTiles = (total + Tile - 1) / Tile;
     const uint32_t lb = xcdBlock();
     const uint64_t b = uint64_t(numTiles) * lb / gridDim.x;
     const uint64_t e = uint64_t(numTiles) * (lb + 1) / gridDim.x;
@@ -508,22 +509,29 @@ __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uin
     count = 0;
 }
 
+// One point per lane: the voxel walk is a chain of dependent loads and
+// returning atomics, so more waves in flight beat more points per lane
+// (measured on the 100M-point scene: 232 us per 1M-point batch at one point per
+// lane, 256 us at four, 258 us at eight).
+constexpr uint32_t kVoxelPerThread = 1;
+constexpr uint32_t kVoxelTile = kBlock * kVoxelPerThread;
+
 __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
     __shared__ VoxelStage stage[kBlock / 64];
     const uint32_t batchSize = batchSizeOf(t, b);
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t total = batchSize + spilledCount(t);
-    const TileRange tr = myTiles(total);
+    const TileRange tr = myTiles<kVoxelTile>(total);
     const float size = t.octreeSize;
     VoxelStage& st = stage[threadIdx.x >> 6];
     uint32_t staged = 0;  // wave-uniform
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
-        uint32_t pX[kPerThread], pY[kPerThread], pZ[kPerThread], col[kPerThread];
-        uint32_t node[kPerThread], lvl[kPerThread];
-        bool active[kPerThread];
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+        uint32_t pX[kVoxelPerThread], pY[kVoxelPerThread], pZ[kVoxelPerThread], col[kVoxelPerThread];
+        uint32_t node[kVoxelPerThread], lvl[kVoxelPerThread];
+        bool active[kVoxelPerThread];
+        for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
+            const uint32_t i = tile * kVoxelTile + j * kBlock + threadIdx.x;
             active[j] = i < total;
             pX[j] = pY[j] = pZ[j] = col[j] = 0;
             if (active[j]) {
@@ -536,17 +544,17 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
         }
         // descend to the leaf; the climb starts at the deepest node with a grid
         // (the leaf's parent, or the root while it is still a leaf)
-        uint32_t parentOf[kPerThread];
-        for (uint32_t j = 0; j < kPerThread; ++j) {
+        uint32_t parentOf[kVoxelPerThread];
+        for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
             node[j] = 0;
             lvl[j] = 0;
             parentOf[j] = kNone;
         }
         for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
             bool any = false;
-            uint2 rec[kPerThread];
-            for (uint32_t j = 0; j < kPerThread; ++j) rec[j] = active[j] ? t.walk[node[j]] : make_uint2(kNone, 0u);
-            for (uint32_t j = 0; j < kPerThread; ++j) {
+            uint2 rec[kVoxelPerThread];
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j) rec[j] = active[j] ? t.walk[node[j]] : make_uint2(kNone, 0u);
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
                 if (!active[j] || rec[j].x == kNone || lvl[j] != level) continue;
                 const uint32_t s2 = kMaxDepth - level - 1;
                 const uint32_t X = pX[j] >> 8, Y = pY[j] >> 8, Z = pZ[j] >> 8;
@@ -557,8 +565,8 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
             }
             if (__ballot(any) == 0) break;
         }
-        uint32_t gref[kPerThread];
-        for (uint32_t j = 0; j < kPerThread; ++j) {
+        uint32_t gref[kVoxelPerThread];
+        for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
             gref[j] = active[j] ? t.walk[node[j]].y : 0u;
             if (active[j] && gref[j] == 0) {
                 if (parentOf[j] == kNone) {
@@ -572,11 +580,11 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
         // climb while new cells are set
         for (uint32_t step = 0; step <= uint32_t(kMaxDepth); ++step) {
             bool any = false;
-            for (uint32_t j = 0; j < kPerThread; ++j) any |= active[j];
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j) any |= active[j];
             if (__ballot(any) == 0) break;
-            uint32_t vi[kPerThread], mask[kPerThread], peek[kPerThread];
-            uint32_t* word[kPerThread];
-            for (uint32_t j = 0; j < kPerThread; ++j) {
+            uint32_t vi[kVoxelPerThread], mask[kVoxelPerThread], peek[kVoxelPerThread];
+            uint32_t* word[kVoxelPerThread];
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
                 gref[j] = active[j] ? t.walk[node[j]].y : 0u;
                 if (gref[j] == 0) active[j] = false;  // no grid here (allocation overflow)
                 const uint32_t sh = (kMaxDepth + 1) - lvl[j];
@@ -586,11 +594,11 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                 word[j] = reinterpret_cast<uint32_t*>(t.arena + (uint64_t(gref[j]) << 4)) + (vi[j] >> 5);
             }
             // independent peeks, then independent atomics; a stale peek only costs an atomicOr
-            for (uint32_t j = 0; j < kPerThread; ++j) peek[j] = active[j] ? *word[j] : 0xFFFFFFFFu;
-            bool isNew[kPerThread];
-            for (uint32_t j = 0; j < kPerThread; ++j)
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j) peek[j] = active[j] ? *word[j] : 0xFFFFFFFFu;
+            bool isNew[kVoxelPerThread];
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j)
                 isNew[j] = (peek[j] & mask[j]) == 0 && (atomicOr(word[j], mask[j]) & mask[j]) == 0;
-            for (uint32_t j = 0; j < kPerThread; ++j) {
+            for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
                 const uint32_t nd = node[j];
                 waveAggAdd(t.numVoxels, nd, isNew[j]);
                 const uint64_t nm = __ballot(isNew[j]);
