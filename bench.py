@@ -70,18 +70,56 @@ def cpu_baseline(points, meta, batch: int, sample_batches: int) -> dict:
     n = min(len(points), sample_batches * batch)
     nb = (n + batch - 1) // batch
     sizes = np.array([min(batch, n - i * batch) for i in range(nb)], dtype=np.uint32)
+    from remobench_amd import lib as L
+
     o = OracleTree(meta.box_size, batches_per_step=1)
     slots = points[:n]
+    uniforms = L.SharedUniforms()
     t0 = time.perf_counter()
     done = 0
     while done < nb:
         done = o.build_step(slots, sizes, nb, batch, nb)
         o.accumulate(done)
+        o.analyse(uniforms)
     dt = time.perf_counter() - t0
     o.close()
     return {"value": round(n / dt / 1e6, 3), "unit": "M pts/s", "cores": 1, "kind": "port",
-            "sample": f"first {nb} of the workload's batches ({n:,} points), Update+Accumulate, "
-                      f"serial C++ oracle, {dt:.1f} s on {platform.processor() or 'host CPU'}"}
+            "sample": f"first {nb} of the workload's batches ({n:,} points), one step per batch "
+                      f"(Update+Accumulate+Analysis), serial C++ oracle, {dt:.1f} s on "
+                      f"{platform.processor() or platform.machine() or 'host CPU'}"}
+
+
+# Kernels behind each timing scope (a scope launch may run several kernels).
+SCOPE_KERNELS = {
+    "remolod.construct.count": ["kCount"],
+    "remolod.construct.split": ["kSplitDecide", "kSplitApply", "kSpillAndZero"],
+    "remolod.construct.voxelSampling": ["kVoxelSample"],
+    "remolod.construct.allocChunks": ["kAllocChunks"],
+    "remolod.construct.insertPoints": ["kInsertPoints"],
+    "remolod.construct.insertVoxels": ["kInsertVoxels"],
+    "remolod.accumulate": ["kAccumPlan", "kAccumTiles", "kAccumFinish"],
+    "remolod.analysis": ["kAnalysisLeaves", "kAnalysisLevel", "kFinalize", "kQueue"],
+}
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_lidar100M.txt"
+
+
+def measured_traffic(scope: str, workload: str):
+    """HBM bytes per launch of `scope` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated against a copy kernel of
+    known size), when it was measured on this same workload; else None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    rows = {}
+    wl = None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        f = line.split()
+        if len(f) >= 2 and f[0] == "workload":
+            wl = line.split(None, 1)[1].strip()
+        elif len(f) == 4 and f[0] == "scope":
+            rows[f[1]] = (float(f[2]), float(f[3]))
+    if wl != workload or scope not in rows:
+        return None
+    return rows[scope][0] + rows[scope][1]
 
 
 def main() -> int:
@@ -141,7 +179,9 @@ def main() -> int:
     ring = PointRing(args.batch, nb)
     ring.set_cloud(points, args.batch)
     ring.set_batches_consumed(0)
+    t_up = time.perf_counter()
     ring.pump()
+    upload_s = time.perf_counter() - t_up
     view = ring.view()
     tree = Tree(meta, slot_capacity=args.batch, ring_slots=nb, batches_per_step=1,
                 step_budget_ms=-1.0)
@@ -205,6 +245,10 @@ def main() -> int:
                             "avg_us": round(1e3 * ms / launches, 2),
                             "algorithmic_bytes": algorithmic_bytes(name, totals)}
     dominant = dominant_name if dominant_name in scopes else None
+    workload = (f"{args.scene}-{len(points) // 1_000_000}M: {len(points):,} points in {nb} "
+                f"batches of {args.batch:,}; Update+Accumulate"
+                + ("+Analysis (analytic)" if analysis_ok and args.analysis == "analytic" else
+                   "+Analysis (learned)" if analysis_ok else ""))
     roofline = None
     if dominant:
         d = scopes[dominant]
@@ -212,10 +256,14 @@ def main() -> int:
         avg_us = 1e3 * ms_dom / max(1, launches_dom)  # from the timed pass
         per_launch = d["algorithmic_bytes"] / d["launches"]
         achieved = per_launch / (avg_us * 1e-6) / 1e9
+        traffic = measured_traffic(dominant, workload)
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                    "traffic": None, "bytes_per_launch": round(per_launch),
-                    "avg_launch_us": round(avg_us, 2)}
+                    "traffic": round(traffic) if traffic else None, "bytes_per_launch": round(per_launch),
+                    "avg_launch_us": round(avg_us, 2),
+                    "kernels": SCOPE_KERNELS.get(dominant, [])}
+        if traffic:
+            roofline["traffic_source"] = f"{TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc)"
     path_bytes = sum(s["algorithmic_bytes"] for s in scopes.values())
     path_gbs = path_bytes / dt / 1e9
 
@@ -262,15 +310,14 @@ def main() -> int:
         "dtype": "f32+u32+int64",
         "data": f"synthetic: seed-generated lidar-like scene (seed {args.seed}), resident in HBM",
         "config": {
-            "workload": f"{args.scene}-{len(points) // 1_000_000}M: {len(points):,} points in {nb} "
-                        f"batches of {args.batch:,}; Update+Accumulate"
-                        + ("+Analysis (analytic)" if analysis_ok and args.analysis == "analytic" else
-                           "+Analysis (learned)" if analysis_ok else ""),
+            "workload": workload,
             "points": len(points), "batch": args.batch, "batches_timed": steps,
             "analysis": args.analysis if analysis_ok else "unavailable",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "upload_gbs": round(16.0 * len(points) / upload_s / 1e9, 2),
+        "pcie_inclusive_mpts": round(ingested / (dt + upload_s * ingested / len(points)) / 1e6, 2),
         "path_hbm_gbs": round(path_gbs, 2),
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 5),
         "scopes": scopes,

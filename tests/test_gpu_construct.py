@@ -12,13 +12,13 @@ from .cases import lidar_scene, tiny_cases
 from .helpers import batch_arrays, compare_trees, keyed, stats_subset
 
 
-def build_hip(points, meta, batch, *, num_slots=None, batches_per_step=20, **kw):
+def build_hip(points, meta, batch, *, num_slots=None, batches_per_step=20, step_budget_ms=-1.0, **kw):
     nb = (len(points) + batch - 1) // batch
     num_slots = num_slots or nb
     ring = PointRing(batch, num_slots)
     ring.set_cloud(points, batch)
     tree = Tree(meta, slot_capacity=batch, ring_slots=num_slots, batches_per_step=batches_per_step,
-                step_budget_ms=-1.0, **kw)
+                step_budget_ms=step_budget_ms, **kw)
     steps = 0
     while True:
         ring.pump()
@@ -30,6 +30,7 @@ def build_hip(points, meta, batch, *, num_slots=None, batches_per_step=20, **kw)
             break
         if tree.stats()["mem_capacity_reached"]:
             break
+    tree.steps_taken = steps
     return tree, ring
 
 
@@ -105,3 +106,17 @@ def test_wrapping_ring_and_small_steps(gpu):
     compare_trees(tree, o, size, points)
     assert stats_subset(tree.stats()) == stats_subset(o.stats())
     compare_accums(tree, o)
+
+
+@pytest.mark.gpu
+def test_step_budget_cuts_steps_short(gpu):
+    """A 1 us device-time budget (kernel_construct's %globaltimer check,
+    remolod_octree.cu:1230-1240): every step folds exactly one batch, and the tree
+    is the one a single unbudgeted pass builds."""
+    points, meta = lidar_scene(600_000, seed=8)
+    size = max(meta.box_size)
+    tree, _ = build_hip(points, meta, 50_000, batches_per_step=20, step_budget_ms=0.001)
+    assert tree.steps_taken == 12
+    o = build_oracle(points, meta, 50_000)
+    compare_trees(tree, o, size, points)
+    assert stats_subset(tree.stats()) == stats_subset(o.stats())
