@@ -191,9 +191,16 @@ def main() -> int:
     analysis_mode = {"analytic": L.REMO_ANALYSIS_ANALYTIC, "learned": L.REMO_ANALYSIS_LEARNED}.get(
         args.analysis)
     analysis_ok = analysis_mode is not None
+    # the learned score, when a trained model is committed beside the package
+    from remobench_amd.score_model import DEFAULT_MODEL, ScoreMLP
+    model = None
+    if DEFAULT_MODEL.exists():
+        model = ScoreMLP.load(DEFAULT_MODEL)
+        tree.set_score_model(model)
+        log(f"learned score model: {DEFAULT_MODEL.relative_to(ROOT)}")
 
     def step():
-        nonlocal analysis_ok
+        nonlocal analysis_ok, analysis_mode
         _, consumed = tree.build_step(view)
         tree.accumulate(consumed)
         if analysis_ok:
@@ -229,6 +236,22 @@ def main() -> int:
     prof_dom = tree.profile()
     tree.profile_enable(False)
     st = tree.stats()
+    learned = None
+    if model is not None and analysis_ok and analysis_mode == L.REMO_ANALYSIS_ANALYTIC:
+        # the same steps with the learned score (the metric's "plain vs learned")
+        analysis_mode = L.REMO_ANALYSIS_LEARNED
+        dt_learned = run(["remolod.analysis"])
+        ms_l, n_l = tree.profile()["remolod.analysis"]
+        analysis_mode = L.REMO_ANALYSIS_ANALYTIC
+        tree.profile_enable(False)
+        ms_a, n_a = prof["remolod.analysis"]
+        learned = {"value": round(tree.stats()["num_points_processed"] / dt_learned / 1e6, 2),
+                   "ms_per_step": round(dt_learned * 1e3 / steps, 4),
+                   "analysis_avg_us": round(1e3 * ms_l / max(1, n_l), 2),
+                   "analysis_avg_us_analytic": round(1e3 * ms_a / max(1, n_a), 2),
+                   "analysis_speedup": round((ms_a / max(1, n_a)) / max(1e-9, ms_l / max(1, n_l)), 3),
+                   "model": str(DEFAULT_MODEL.relative_to(ROOT))}
+        log(f"learned score: {learned}")
     ingested = st["num_points_processed"]
     value = ingested / dt / 1e6
     log(f"timed {steps} steps: {dt * 1e3:.1f} ms, {value:.1f} M pts/s ({ingested / dt_all / 1e6:.1f} with "
@@ -315,6 +338,7 @@ def main() -> int:
             "analysis": args.analysis if analysis_ok else "unavailable",
         },
         "roofline": roofline,
+        "learned_score": learned,
         "cpu_baseline": cpu,
         "upload_gbs": round(16.0 * len(points) / upload_s / 1e9, 2),
         "pcie_inclusive_mpts": round(ingested / (dt + upload_s * ingested / len(points)) / 1e6, 2),

@@ -189,8 +189,18 @@ typedef struct remo_node_analysis {
 #define REMO_ANALYSIS_ANALYTIC 0
 #define REMO_ANALYSIS_LEARNED  1
 
+/* The learned score (no reference code: plans/04_NeuralPass.md is empty):
+ * 13 -> 64 -> 64 -> 2 MLP with exact GELU, BF16 weights, FP32 accumulation,
+ * predicting a node's final (surface variation, planarity). Parameters are FP32
+ * W1[64][13], b1[64], W2[64][64], b2[64], W3[2][64], b3[2], in that order. */
+#define REMO_MLP_INPUTS   13u
+#define REMO_MLP_HIDDEN   64u
+#define REMO_MLP_OUTPUTS  2u
+#define REMO_MLP_PARAMS   5186u
+
 typedef struct remo_ctx remo_ctx;
 typedef struct remo_ring remo_ring;
+typedef struct remo_mlp remo_mlp;
 
 /* ---- lifecycle ---------------------------------------------------------- */
 
@@ -232,6 +242,29 @@ int remo_render(remo_ctx* ctx, const remo_shared_uniforms* u, uint64_t* fb);
 int remo_get_stats(remo_ctx* ctx, remo_stats* out);
 const char* remo_last_error(const remo_ctx* ctx);
 int remo_sync(remo_ctx* ctx);
+
+/* ---- learned score ------------------------------------------------------ */
+/* A training object: FP32 master parameters (Glorot-uniform from `seed`, zero
+ * biases) with Adam state; batches of at most max_batch samples. */
+int remo_mlp_create(uint64_t seed, uint32_t max_batch, remo_mlp** out);
+void remo_mlp_destroy(remo_mlp* mlp);
+int remo_mlp_get_params(remo_mlp* mlp, float* host, uint32_t n);        /* n = REMO_MLP_PARAMS */
+int remo_mlp_set_params(remo_mlp* mlp, const float* host, uint32_t n);  /* resets Adam */
+/* Inference on device rows x[n][13] -> y[n][2] with the BF16 weights. */
+int remo_mlp_forward(remo_mlp* mlp, const float* x, uint32_t n, float* y);
+/* Loss (mean squared error over n x 2 outputs) and its gradient w.r.t. the
+ * parameters (host, REMO_MLP_PARAMS floats, may be NULL) without an update. */
+int remo_mlp_gradient(remo_mlp* mlp, const float* x, const float* t, uint32_t n, float* loss, float* grad_host);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected) on device rows
+ * x[n][13], targets t[n][2]; *loss is the batch loss before the step. */
+int remo_mlp_train_step(remo_mlp* mlp, const float* x, const float* t, uint32_t n, float lr, float* loss);
+/* Loads the model into a context: remo_analysis(..., REMO_ANALYSIS_LEARNED) then
+ * scores open nodes with it (closed nodes, and predictions above 0.05 surface
+ * variation, take the analytic eigen-solve). */
+int remo_set_score_model(remo_ctx* ctx, const remo_mlp* mlp);
+/* The 13 inputs of every node from the last remo_analysis roll-up, as device
+ * rows out[num][13] (training pairs are logged from these). */
+int remo_mlp_features(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* num);
 
 /* ---- timing scopes (GpuProfiler / GpuScope, include/remo/GpuProfiler.h:62-158) ----
  * When enabled, every phase launch is bracketed by a pair of HIP events on the

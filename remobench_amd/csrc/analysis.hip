@@ -38,15 +38,6 @@ constexpr uint32_t kBlock = 256;
 using i128 = __int128;
 constexpr uint64_t kFbClearValue = 0x7f80000000332211ull;  // remo_framebuffer.cuh:11
 
-__device__ __forceinline__ double i128ToDouble(i128 x) {
-    const bool neg = x < 0;
-    const unsigned __int128 u = neg ? (unsigned __int128)(-x) : (unsigned __int128)x;
-    const double hi = double(uint64_t(u >> 64));
-    const double lo = double(uint64_t(u));
-    const double d = hi * 18446744073709551616.0 + lo;
-    return neg ? -d : d;
-}
-
 __device__ __forceinline__ int bitsU128(unsigned __int128 u) {
     const uint64_t hi = uint64_t(u >> 64), lo = uint64_t(u);
     return hi ? 128 - __clzll((long long)hi) : (lo ? 64 - __clzll((long long)lo) : 0);
@@ -183,19 +174,20 @@ __device__ __forceinline__ void projectedExtent(const Bounds& b, const remo_mat4
 
 struct AnalysisArgs {
     RollupSums* rollup;
-    uint32_t* arrived;      // arrivals per inner node; +8 every frame, never reset
     uint8_t* closed;
     remo_node_analysis* results;
     uint32_t* queue;
     uint32_t* queueCount;
-    const float* learned;   // [n * 2] predicted (surface variation, planarity), or null
-    uint32_t useLearned;
+    const uint16_t* mlpWeights;  // BF16 parameters of the learned score, or null
+    float arrival;               // fraction of the cloud's batches ingested
     uint32_t margin;
 };
 
 // Closure, covariance, eigen-solve and score of one node from its rolled-up sums.
+// w: the learned score's parameters expanded to FP32 (LDS), or null for the
+// analytic score.
 __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
-                             uint32_t node, const RollupSums& S, bool closed, uint32_t D) {
+                             uint32_t node, const RollupSums& S, bool closed, uint32_t D, const float* w) {
     remo_node_analysis r;
     memset(&r, 0, sizeof(r));
     r.n = S.n;
@@ -204,11 +196,14 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
     r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
     r.state = closed ? 2u : 0u;
     bool scored = false;
-    if (aa.useLearned && !closed && S.n >= 3) {
+    if (w && !closed && S.n >= 3) {
         // learned score for open nodes; the analytic path is the fallback when the
         // prediction says the node is not planar (surface variation > 0.05)
-        const float sv = aa.learned[2 * node], pl = aa.learned[2 * node + 1];
-        if (sv <= 0.05f) {
+        float f[kMlpIn], y[kMlpOut];
+        nodeFeatures(t, node, S, aa.arrival, f);
+        mlpForward(w, f, y);
+        const float sv = y[0], pl = y[1];
+        if (sv <= kLearnedFallbackSv) {
             r.surface_variation = fmaxf(sv, 0.0f);
             r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
             r.linearity = 0.0f;
@@ -315,7 +310,30 @@ __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa, 
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
-        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
+        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, nullptr);
+}
+
+// The learned variant: the MLP's BF16 parameters are expanded to FP32 in LDS once
+// per workgroup; open nodes run the MLP, closed ones (and fallbacks) the eigen-solve.
+__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, remo_shared_uniforms u) {
+    __shared__ float w[kMlpParams];
+    for (uint32_t i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = bf16ToF32(aa.mlpWeights[i]);
+    __syncthreads();
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    const uint32_t D = g->maxLevel;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
+        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, w);
+}
+
+// The learned score's inputs of every node (training pairs are logged from these).
+__global__ __launch_bounds__(kBlock) void kFeatures(DevTree t, AnalysisArgs aa, float* out, uint32_t cap) {
+    const uint32_t n = min(t.g->numNodes, cap);
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        float f[kMlpIn];
+        nodeFeatures(t, node, aa.rollup[node], aa.arrival, f);
+        for (uint32_t k = 0; k < kMlpIn; ++k) out[uint64_t(node) * kMlpIn + k] = f[k];
+    }
 }
 
 // Work queue: every node with a positive score, rebuilt each frame (one atomic per wave).
@@ -528,8 +546,6 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.results = static_cast<remo_node_analysis*>(p);
     A(&p, sizeof(RollupSums) * maxNodes, "roll-up sums");
     a.rollup = static_cast<RollupSums*>(p);
-    A(&p, 4 * maxNodes, "roll-up arrival counters");
-    a.arrived = static_cast<uint32_t*>(p);
     A(&p, maxNodes, "closure flags");
     a.closed = static_cast<uint8_t*>(p);
     A(&p, 4 * maxNodes, "work queue");
@@ -544,9 +560,7 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.drawList = static_cast<DrawItem*>(p);
     A(&p, 16, "draw counters");
     a.drawCounters = static_cast<uint32_t*>(p);
-    A(&p, 8 * maxNodes, "learned predictions");
-    a.learned = static_cast<float*>(p);
-    if (rc == REMO_OK) rc = mlpAllocate(a, maxNodes, alloc);
+    if (rc == REMO_OK) rc = mlpAllocate(a, alloc);
     return rc;
 }
 
@@ -557,19 +571,29 @@ void analysisReset(AnalysisState& a) {
 }
 
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-                const LaunchShape& ls, hipStream_t s, std::string* err) {
+                float arrival, const LaunchShape& ls, hipStream_t s, std::string* err) {
     const bool learned = mode == REMO_ANALYSIS_LEARNED;
-    if (learned) {
-        const int rc = mlpPredict(a, t, ls, s, err);
-        if (rc != REMO_OK) return rc;
+    if (learned && !a.mlpLoaded) {
+        if (err) *err = "no trained score model loaded (remo_set_score_model)";
+        return REMO_ERR_STATE;
     }
-    AnalysisArgs aa{a.rollup, a.arrived, a.closed, a.results, a.queue, a.queueCount,
-                    a.learned, learned ? 1u : 0u, a.closureMargin};
+    a.arrival = arrival;
+    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount,
+                    learned ? a.mlpWeights : nullptr, arrival, a.closureMargin};
     hipLaunchKernelGGL(kAnalysisLeaves, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa);
     for (int level = int(maxLevel) - 1; level >= 0; --level)
         hipLaunchKernelGGL(kAnalysisLevel, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, uint32_t(level));
-    hipLaunchKernelGGL(kFinalize, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, u);
+    if (learned)
+        hipLaunchKernelGGL(kFinalizeLearned, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, u);
+    else
+        hipLaunchKernelGGL(kFinalize, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, u);
     hipLaunchKernelGGL(kQueue, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa);
+    return REMO_OK;
+}
+
+int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s) {
+    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, nullptr, a.arrival, a.closureMargin};
+    hipLaunchKernelGGL(kFeatures, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, out, cap);
     return REMO_OK;
 }
 

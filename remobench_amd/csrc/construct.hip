@@ -341,7 +341,6 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
             t.firstChild[c] = -1;
             t.walk[c] = make_uint2(kNone, 0u);
             t.parent[c] = node;
-            t.arrivals[c] = 0;
             t.level[c] = lvl;
             t.nx[c] = 2 * X + ((i >> 2) & 1u);
             t.ny[c] = 2 * Y + ((i >> 1) & 1u);

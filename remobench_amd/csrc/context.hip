@@ -18,6 +18,7 @@
 
 #include "remolod_internal.h"
 #include "remolod_analysis.h"
+#include "remolod_mlp.h"
 
 using namespace remo_hip;
 
@@ -112,6 +113,7 @@ struct remo_ctx {
     bool dead = false;
     bool needsReset = true;
     uint32_t batchletIndex = 0;
+    uint32_t batchesTotal = 0;        // num_batches_total of the last build_step's ring
     std::vector<void*> allocations;
     AnalysisState analysis;
     AccumWork accWork;
@@ -360,7 +362,6 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         *p = q;
         return r;
     });
-    t.arrivals = c->analysis.arrived;
     if (rc != REMO_OK) {
         std::string e = c->err;
         releaseAll(c);
@@ -395,7 +396,6 @@ int remo_reset(remo_ctx* c) {
     DevTree& t = c->t;
     CK(hipMemsetAsync(t.g, 0, sizeof(Globals), c->stream), "reset");
     CK(hipMemsetAsync(t.accums, 0, sizeof(remo_node_accum) * uint64_t(t.maxNodes), c->stream), "reset");
-    CK(hipMemsetAsync(t.arrivals, 0, sizeof(uint32_t) * uint64_t(t.maxNodes), c->stream), "reset");
     CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
     launchResetTree(t, c->stream);
     CK(hipGetLastError(), "reset kernel");
@@ -424,6 +424,7 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
                  std::to_string(R) + " x " + std::to_string(c->cfg.slot_capacity) + ")";
         return REMO_ERR_RING;
     }
+    c->batchesTotal = v->num_batches_total;
     // The producer cursor only grows, so a value read earlier from the same ring is
     // a valid lower bound; it is re-read only when it would hold this step back.
     if (v->num_batches_uploaded != c->uploadedPtr ||
@@ -511,10 +512,36 @@ int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {
     int rc;
     {
         Scope sc(c->prof, kScopeAnalysis, c->stream);
-        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, c->shape, c->stream, &c->err);
+        const float arrival =
+            c->batchesTotal ? std::min(1.0f, float(c->batchletIndex) / float(c->batchesTotal)) : 1.0f;
+        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrival, c->shape, c->stream, &c->err);
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "analysis kernels");
+    return REMO_OK;
+}
+
+int remo_set_score_model(remo_ctx* c, const remo_mlp* mlp) {
+    if (!c || !mlp) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    CK(hipStreamSynchronize(c->stream), "device work");
+    const int rc = mlpCopyWeights(mlp, c->analysis.mlpWeights);
+    if (rc != REMO_OK) return rc;
+    c->analysis.mlpLoaded = true;
+    return REMO_OK;
+}
+
+int remo_mlp_features(remo_ctx* c, float* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    *num = c->hostG->numNodes;
+    if (!out) return REMO_OK;
+    rc = runFeatures(c->analysis, c->t, out, cap, c->shape, c->stream);
+    if (rc != REMO_OK) return rc;
+    CK(hipGetLastError(), "feature kernel");
+    CK(hipStreamSynchronize(c->stream), "feature kernel");
     return REMO_OK;
 }
 

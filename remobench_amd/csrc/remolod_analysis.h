@@ -21,6 +21,16 @@ struct RollupSums {
     uint32_t pad;
 };
 
+// Nearest double to an __int128 (two halves, one rounding each).
+__host__ __device__ inline double i128ToDouble(__int128 x) {
+    const bool neg = x < 0;
+    const unsigned __int128 u = neg ? (unsigned __int128)(-x) : (unsigned __int128)x;
+    const double hi = double(uint64_t(u >> 64));
+    const double lo = double(uint64_t(u));
+    const double d = hi * 18446744073709551616.0 + lo;
+    return neg ? -d : d;
+}
+
 // One item of the frontier draw list (kernels/shared/remo_draw.cuh:13-22).
 struct DrawItem {
     uint32_t node;
@@ -29,15 +39,12 @@ struct DrawItem {
     uint32_t level;
 };
 
-struct MlpState;
-
 struct AnalysisState {
     remo_node_analysis* results = nullptr;
     RollupSums* rollup = nullptr;
-    uint32_t* arrived = nullptr;      // children arrived, per node (bottom-up roll-up)
     uint8_t* closed = nullptr;        // closure flag per node
     uint32_t closureMargin = 8;       // batches untouched before a leaf is closed
-    float* learned = nullptr;         // [n * 2] MLP predictions
+    float arrival = 0.0f;             // fraction of the cloud's batches ingested (last Analysis)
     uint32_t* queue = nullptr;        // work queue of nodes (score > 0)
     uint32_t* queueCount = nullptr;
     uint8_t* visible = nullptr;
@@ -49,16 +56,20 @@ struct AnalysisState {
     // host mirrors of the last render's diagnostics
     uint32_t drawItems = 0, drawListOverflow = 0;
     uint64_t drawSamples = 0;
-    MlpState* mlp = nullptr;
+    uint16_t* mlpWeights = nullptr;    // BF16 parameters of the learned score
+    bool mlpLoaded = false;
 };
 
 using AllocFn = std::function<int(void**, uint64_t, const char*)>;
 
 int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc);
 void analysisReset(AnalysisState& a);
-// maxLevel: the deepest node level (Globals::maxLevel as last read back).
+// maxLevel: the deepest node level (Globals::maxLevel as last read back);
+// arrival: the fraction of the cloud's batches ingested (a learned-score input).
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-                const LaunchShape& ls, hipStream_t s, std::string* err);
+                float arrival, const LaunchShape& ls, hipStream_t s, std::string* err);
+// The learned score's 13 inputs of every node from the last Analysis roll-up.
+int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);
 int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb,
               const LaunchShape& ls, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);

@@ -76,7 +76,6 @@ struct DevTree {
     int32_t* firstChild;
     uint2* walk;            // per node {first child or kNone, grid offset >> 4 or 0}: one load per level
     uint32_t* parent;       // kNone for the root
-    uint32_t* arrivals;     // Analysis roll-up arrival counters (zero for a new node)
     uint32_t* level;
     uint32_t* nx;
     uint32_t* ny;

@@ -21,6 +21,7 @@ ERRORS = {
 REMO_FLAG_NO_ACCUM = 1
 REMO_ANALYSIS_ANALYTIC = 0
 REMO_ANALYSIS_LEARNED = 1
+MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_PARAMS = 13, 64, 2, 5186
 
 POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("color", "<u4")])
 
@@ -187,6 +188,17 @@ def load() -> C.CDLL:
         "remo_profile_enable": (C.c_int, [C.c_void_p, C.c_uint32]),
         "remo_profile_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_profile_clear": (C.c_int, [C.c_void_p]),
+        "remo_mlp_create": (C.c_int, [C.c_uint64, C.c_uint32, P(C.c_void_p)]),
+        "remo_mlp_destroy": (None, [C.c_void_p]),
+        "remo_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+        "remo_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+        "remo_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+        "remo_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_float),
+                                        C.c_void_p]),
+        "remo_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                          P(C.c_float)]),
+        "remo_set_score_model": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),
         "remo_scene_generate": (C.c_int, [P(SceneSpec), C.c_uint64, C.c_uint64, C.c_void_p,
                                           P(CloudMeta)]),
@@ -237,6 +249,48 @@ def generate_scene(num_points: int, seed: int, kind: str = "lidar", threads: int
     check(load().remo_scene_generate(C.byref(spec), seed, num_points,
                                      pts.ctypes.data_as(C.c_void_p), C.byref(meta)), "scene")
     return pts, meta
+
+
+class DeviceBuffer:
+    """A device allocation from remo_dev_alloc with numpy upload / download."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().remo_dev_alloc(max(16, self.nbytes), C.byref(p)), "device alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def of(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if a.nbytes:
+            check(load().remo_memcpy_h2d(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes), "upload")
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        if out.nbytes:
+            check(load().remo_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes),
+                  "download")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().remo_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def device_count() -> int:

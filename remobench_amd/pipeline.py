@@ -122,7 +122,7 @@ class Tree:
 
     def __init__(self, meta: L.CloudMeta, *, max_nodes=0, persistent_bytes=0, batches_per_step=0,
                  step_budget_ms=-1.0, slot_capacity=0, ring_slots=0, spill_capacity=0,
-                 backlog_capacity=0, accumulate=True):
+                 backlog_capacity=0, accumulate=True, closure_batches=0):
         self.lib = L.load()
         cfg = L.Config()
         cfg.persistent_bytes = persistent_bytes
@@ -134,6 +134,7 @@ class Tree:
         cfg.spill_capacity = spill_capacity
         cfg.backlog_capacity = backlog_capacity
         cfg.flags = 0 if accumulate else L.REMO_FLAG_NO_ACCUM
+        cfg.closure_batches = closure_batches
         self.cfg = cfg
         self.meta = meta
         self.h = C.c_void_p()
@@ -225,6 +226,24 @@ class Tree:
         out = np.zeros(n.value, dtype=L.ACCUM_DTYPE)
         self._ck(self.lib.remo_dump_accums(self.h, out.ctypes.data, len(out), C.byref(n)), "accums")
         return out
+
+    def features(self) -> np.ndarray:
+        """The learned score's 13 inputs of every node from the last analysis()."""
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_mlp_features(self.h, None, 0, C.byref(n)), "features")
+        if n.value == 0:
+            return np.zeros((0, L.MLP_INPUTS), dtype=np.float32)
+        buf = L.DeviceBuffer(4 * L.MLP_INPUTS * n.value)
+        try:
+            got = C.c_uint32(0)
+            self._ck(self.lib.remo_mlp_features(self.h, C.c_void_p(buf.ptr), n.value, C.byref(got)), "features")
+            return buf.download(np.float32, (n.value, L.MLP_INPUTS))
+        finally:
+            buf.free()
+
+    def set_score_model(self, mlp) -> None:
+        """Loads a trained score model (remobench_amd.score_model.ScoreMLP)."""
+        self._ck(self.lib.remo_set_score_model(self.h, mlp.h), "set_score_model")
 
     def analysis_results(self) -> np.ndarray:
         n = C.c_uint32(0)
