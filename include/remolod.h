@@ -227,6 +227,13 @@ int remo_build_step(remo_ctx* ctx, const remo_batch_view* view, uint32_t* batche
  * (src/pipelines/RemolodPipeline.cpp:242-294, kernels/remolod/remolod_accum.cu:47-233). */
 int remo_accumulate(remo_ctx* ctx, uint32_t batch_index);
 
+/* Precision comparator (no reference counterpart): recomputes every leaf's sums
+ * from its stored points with naive FP32 atomics -- out[node * 10 + k], k = 0 the
+ * count, 1..3 sum x y z, 4..9 sum xx xy xz yy yz zz of the node-local coordinates
+ * in [0,1] (device rows; inner nodes stay zero) -- so the int64 fixed-point sums
+ * can be compared against FP32 atomics and an FP64 reference. */
+int remo_accum_fp32_atomics(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* num);
+
 /* Analysis (README.md:80, wiki/02_KernelPipeline.md:205-224; no reference code):
  * roll-up of the exact sums, 3x3 eigen-solve, score = geometric x coverage,
  * work queue. mode: REMO_ANALYSIS_ANALYTIC or REMO_ANALYSIS_LEARNED. */
@@ -237,6 +244,31 @@ int remo_analysis(remo_ctx* ctx, const remo_shared_uniforms* u, int mode);
  * kernels/shared/remo_framebuffer.cuh:214-248). fb: device u64[width*height],
  * cleared and filled with (float_bits(depth) << 32) | rgba. */
 int remo_render(remo_ctx* ctx, const remo_shared_uniforms* u, uint64_t* fb);
+
+/* Analysis + Rasterize of one frame as one captured hipGraph (no reference
+ * counterpart: the reference launches kernel_render alone, eagerly). The graph is
+ * captured on the first call and re-captured when the tree depth, the mode, the
+ * framebuffer, its size or the launch shape change; the uniforms and the arrival
+ * fraction reach its kernels through device memory, so a replay serves a new
+ * camera. Results equal remo_analysis followed by remo_render. */
+int remo_frame(remo_ctx* ctx, const remo_shared_uniforms* u, int mode, uint64_t* fb);
+
+/* One launch shape of the per-frame kernels and its measured time. */
+typedef struct remo_tune_entry {
+    uint32_t node_blocks;    /* workgroups of the node-parallel Analysis / selection kernels */
+    uint32_t raster_blocks;  /* workgroups of the raster kernel                              */
+    uint32_t raster_tile;    /* samples per raster work unit                                 */
+    float ms;                /* best of 3 timed runs of Analysis + Rasterize                 */
+} remo_tune_entry;
+
+/* Autotuner: times Analysis + Rasterize on the current tree and camera over a
+ * grid of launch shapes and keeps the fastest for remo_analysis, remo_render and
+ * remo_frame. Writes up to cap entries (NULL: count only) and *num. Results do
+ * not depend on the shape. */
+int remo_autotune_frame(remo_ctx* ctx, const remo_shared_uniforms* u, int mode, uint64_t* fb,
+                        remo_tune_entry* out, uint32_t cap, uint32_t* num);
+/* The launch shape in use (ms = 0). */
+int remo_get_frame_shape(remo_ctx* ctx, remo_tune_entry* out);
 
 /* ILodPipeline::stats / diagnostics (src/pipelines/RemolodPipeline.cpp:440-470, :603-726). */
 int remo_get_stats(remo_ctx* ctx, remo_stats* out);
@@ -271,7 +303,7 @@ int remo_mlp_features(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* num);
  * context's stream; remo_profile_read harvests finished pairs into per-scope
  * totals. Scope names follow the reference's (src/pipelines/RemolodPipeline.cpp:474-479):
  * remolod.construct.{count,split,voxelSampling,allocChunks,insertPoints,insertVoxels},
- * remolod.accumulate, remolod.analysis, remolod.render. */
+ * remolod.accumulate, remolod.analysis, remolod.render, and remolod.frame (remo_frame). */
 typedef struct remo_profile_entry {
     char name[48];
     double total_ms;
@@ -279,7 +311,7 @@ typedef struct remo_profile_entry {
 } remo_profile_entry;
 
 /* scope_mask: bit i times scope i in the order listed above (count = bit 0 ...
- * render = bit 8); 0 disables, 0xFFFFFFFF times all. */
+ * render = bit 8, frame = bit 9); 0 disables, 0xFFFFFFFF times all. */
 int remo_profile_enable(remo_ctx* ctx, uint32_t scope_mask);
 int remo_profile_read(remo_ctx* ctx, remo_profile_entry* out, uint32_t cap, uint32_t* num);
 int remo_profile_clear(remo_ctx* ctx);

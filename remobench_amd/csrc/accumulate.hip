@@ -261,7 +261,52 @@ __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, u
     }
 }
 
+// The FP32-atomics comparator: every stored point of every leaf adds its
+// node-local coordinate (FP32 arithmetic, clamped to [0,1]) into per-node FP32
+// sums with one atomicAdd per sum, the naive alternative to the int64 fixed point.
+// out[node * 10 + k]: k = 0 count, 1..3 sum x y z, 4..9 sum xx xy xz yy yz zz.
+__global__ __launch_bounds__(kBlock) void kAccumFp32Atomics(DevTree t, float* out, uint32_t cap) {
+    const uint32_t n = min(t.g->numNodes, cap);
+    for (uint32_t node = xcdBlock(); node < n; node += gridDim.x) {
+        if (t.firstChild[node] >= 0) continue;
+        const uint32_t np = t.numPoints[node];
+        const float scale = float(1u << t.level[node]) / t.octreeSize;
+        const float NX = float(t.nx[node]), NY = float(t.ny[node]), NZ = float(t.nz[node]);
+        const uint32_t* dir = t.pointDir + uint64_t(node) * kPointDir;
+        float* o = out + uint64_t(node) * 10;
+        for (uint32_t s = threadIdx.x; s < np; s += kBlock) {
+            const uint32_t k = s / kPointsPerChunk;
+            const uint32_t ref = k < kPointDir ? dir[k] : 0u;
+            if (ref == 0) continue;
+            const remo_point p = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
+            const float x = fminf(fmaxf(p.x * scale - NX, 0.0f), 1.0f);
+            const float y = fminf(fmaxf(p.y * scale - NY, 0.0f), 1.0f);
+            const float z = fminf(fmaxf(p.z * scale - NZ, 0.0f), 1.0f);
+            atomicAdd(o + 0, 1.0f);
+            atomicAdd(o + 1, x);
+            atomicAdd(o + 2, y);
+            atomicAdd(o + 3, z);
+            atomicAdd(o + 4, x * x);
+            atomicAdd(o + 5, x * y);
+            atomicAdd(o + 6, x * z);
+            atomicAdd(o + 7, y * y);
+            atomicAdd(o + 8, y * z);
+            atomicAdd(o + 9, z * z);
+        }
+    }
+}
+
+__global__ void kZeroF32(float* p, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = 0.0f;
+}
+
 }  // namespace
+
+void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kZeroF32, dim3(256), dim3(kBlock), 0, s, out, uint64_t(cap) * 10);
+    hipLaunchKernelGGL(kAccumFp32Atomics, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, out, cap);
+}
 
 void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
                       hipStream_t s) {

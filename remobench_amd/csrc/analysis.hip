@@ -179,7 +179,7 @@ struct AnalysisArgs {
     uint32_t* queue;
     uint32_t* queueCount;
     const uint16_t* mlpWeights;  // BF16 parameters of the learned score, or null
-    float arrival;               // fraction of the cloud's batches ingested
+    const FrameParams* frame;    // uniforms and arrival of this frame (device)
     uint32_t margin;
 };
 
@@ -200,7 +200,7 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
         // learned score for open nodes; the analytic path is the fallback when the
         // prediction says the node is not planar (surface variation > 0.05)
         float f[kMlpIn], y[kMlpOut];
-        nodeFeatures(t, node, S, aa.arrival, f);
+        nodeFeatures(t, node, S, aa.frame->arrival, f);
         mlpForward(w, f, y);
         const float sv = y[0], pl = y[1];
         if (sv <= kLearnedFallbackSv) {
@@ -305,18 +305,20 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs
     }
 }
 
-__global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa, remo_shared_uniforms u) {
+__global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
+    const remo_shared_uniforms& u = aa.frame->u;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
         finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, nullptr);
 }
 
 // The learned variant: the MLP's BF16 parameters are expanded to FP32 in LDS once
 // per workgroup; open nodes run the MLP, closed ones (and fallbacks) the eigen-solve.
-__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, remo_shared_uniforms u) {
+__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa) {
     __shared__ float w[kMlpParams];
+    const remo_shared_uniforms& u = aa.frame->u;
     for (uint32_t i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = bf16ToF32(aa.mlpWeights[i]);
     __syncthreads();
     Globals* g = t.g;
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void kFeatures(DevTree t, AnalysisArgs aa, 
     const uint32_t n = min(t.g->numNodes, cap);
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
         float f[kMlpIn];
-        nodeFeatures(t, node, aa.rollup[node], aa.arrival, f);
+        nodeFeatures(t, node, aa.rollup[node], aa.frame->arrival, f);
         for (uint32_t k = 0; k < kMlpIn; ++k) out[uint64_t(node) * kMlpIn + k] = f[k];
     }
 }
@@ -392,19 +394,30 @@ struct RenderArgs {
     uint8_t* visible;
     uint8_t* large;
     DrawItem* items;
-    uint32_t* counters;  // [0] items, [1] overflow, [2..3] samples
+    uint32_t* counters;  // [0] items, [1] overflow, [2..3] samples, [4] units, [5] unit overflow
     uint32_t capacity;
+    RasterUnit* units;
+    uint32_t unitCapacity;
+    uint32_t tile;       // samples per raster unit
+    const FrameParams* frame;
 };
+
+// The frame's parameters into device memory (a kernel argument, so the copy is
+// stream-ordered and needs no host buffer that outlives the call).
+__global__ void kSetFrame(FrameParams* dst, FrameParams value) {
+    if (threadIdx.x == 0) *dst = value;
+}
 
 __global__ void kFbClear(uint64_t* fb, uint64_t n) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         fb[i] = kFbClearValue;
 }
 
-__global__ __launch_bounds__(kBlock) void kRenderFlags(DevTree t, RenderArgs ra, remo_shared_uniforms u) {
+__global__ __launch_bounds__(kBlock) void kRenderFlags(DevTree t, RenderArgs ra) {
     __shared__ Plane P[6];
+    const remo_shared_uniforms& u = ra.frame->u;
     if (threadIdx.x == 0) frustumPlanes(u.transform_frozen, P);
-    if (blockIdx.x == 0 && threadIdx.x < 4) ra.counters[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 8) ra.counters[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t n = min(t.g->numNodes, t.maxNodes);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -427,6 +440,20 @@ __device__ __forceinline__ void emit(const DevTree& t, const RenderArgs& ra, uin
     }
     ra.items[idx] = DrawItem{node, np, nv, t.level[node]};
     atomicAdd(reinterpret_cast<unsigned long long*>(ra.counters + 2), (unsigned long long)(np + nv));
+    // raster units of at most `tile` samples, so every workgroup gets similar work
+    // (every slot below min(counter, capacity) is written, so the raster never reads
+    // an unwritten unit; units past the capacity are dropped and flagged)
+    const uint32_t up = (np + ra.tile - 1) / ra.tile, uv = (nv + ra.tile - 1) / ra.tile;
+    const uint32_t base = atomicAdd(&ra.counters[4], up + uv);
+    for (uint32_t k = 0; k < up + uv; ++k) {
+        if (base + k >= ra.unitCapacity) {
+            ra.counters[5] = 1;
+            break;
+        }
+        const bool vox = k >= up;
+        const uint32_t kk = vox ? k - up : k, total = vox ? nv : np;
+        ra.units[base + k] = RasterUnit{node, vox ? 1u : 0u, kk * ra.tile, min(ra.tile, total - kk * ra.tile)};
+    }
 }
 
 // Frontier selection (remolod_render.cu:134-179).
@@ -495,40 +522,38 @@ __device__ __forceinline__ void drawPoint(uint64_t* fb, const remo_shared_unifor
     }
 }
 
-__global__ __launch_bounds__(kBlock) void kRenderRaster(DevTree t, RenderArgs ra, remo_shared_uniforms u,
-                                                        uint64_t* fb) {
+__global__ __launch_bounds__(kBlock) void kRenderRaster(DevTree t, RenderArgs ra, uint64_t* fb) {
+    const remo_shared_uniforms& u = ra.frame->u;
     if (u.show_points == 0) return;
-    const uint32_t numItems = min(ra.counters[0], ra.capacity);
+    const uint32_t numUnits = min(ra.counters[4], ra.unitCapacity);
     const int32_t w = int32_t(u.width), h = int32_t(u.height);
-    for (uint32_t it = xcdBlock(); it < numItems; it += gridDim.x) {
-        const DrawItem item = ra.items[it];
-        uint32_t key = item.node;
-        if (u.color_mode == 2) key = uint32_t(item.level * 2654435761ull);
-        for (uint32_t pass = 0; pass < 2; ++pass) {
-            const uint32_t count = pass == 0 ? item.numPoints : item.numVoxels;
-            if (count == 0) continue;
-            const uint32_t* dir;
-            uint32_t dirCap;
-            if (pass == 0) {
-                dir = t.pointDir + uint64_t(item.node) * kPointDir;
-                dirCap = kPointDir;
-            } else {
-                const uint32_t gi = t.gridNo[item.node];
-                if (gi == kNone) continue;
-                dir = t.voxelDir + uint64_t(gi) * kVoxelDir;
-                dirCap = kVoxelDir;
-            }
-            for (uint32_t s = threadIdx.x; s < count; s += blockDim.x) {
-                const uint32_t k = s / kPointsPerChunk;
-                if (k >= dirCap) break;
-                const uint32_t ref = dir[k];
-                if (ref == 0) continue;
-                const remo_point p = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
-                uint32_t color = p.color;
-                if (u.color_mode == 3) color = 0xFFFFFFFFu;
-                else if (u.color_mode == 1 || u.color_mode == 2) color = hashColor(key);
-                drawPoint(fb, u, w, h, p, color);
-            }
+    const int32_t colorMode = u.color_mode;
+    for (uint32_t ui = xcdBlock(); ui < numUnits; ui += gridDim.x) {
+        const RasterUnit unit = ra.units[ui];
+        uint32_t key = unit.node;
+        if (colorMode == 2) key = uint32_t(t.level[unit.node] * 2654435761ull);
+        const uint32_t* dir;
+        uint32_t dirCap;
+        if (unit.pass == 0) {
+            dir = t.pointDir + uint64_t(unit.node) * kPointDir;
+            dirCap = kPointDir;
+        } else {
+            const uint32_t gi = t.gridNo[unit.node];
+            if (gi == kNone) continue;
+            dir = t.voxelDir + uint64_t(gi) * kVoxelDir;
+            dirCap = kVoxelDir;
+        }
+        const uint32_t end = unit.first + unit.count;
+        for (uint32_t s = unit.first + threadIdx.x; s < end; s += blockDim.x) {
+            const uint32_t k = s / kPointsPerChunk;
+            if (k >= dirCap) break;
+            const uint32_t ref = dir[k];
+            if (ref == 0) continue;
+            const remo_point p = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
+            uint32_t color = p.color;
+            if (colorMode == 3) color = 0xFFFFFFFFu;
+            else if (colorMode == 1 || colorMode == 2) color = hashColor(key);
+            drawPoint(fb, u, w, h, p, color);
         }
     }
 }
@@ -558,8 +583,12 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.large = static_cast<uint8_t*>(p);
     A(&p, sizeof(DrawItem) * uint64_t(a.drawCapacity), "draw list");
     a.drawList = static_cast<DrawItem*>(p);
-    A(&p, 16, "draw counters");
+    A(&p, 32, "draw counters");
     a.drawCounters = static_cast<uint32_t*>(p);
+    A(&p, sizeof(RasterUnit) * uint64_t(a.unitCapacity), "raster units");
+    a.units = static_cast<RasterUnit*>(p);
+    A(&p, sizeof(FrameParams), "frame parameters");
+    a.frame = static_cast<FrameParams*>(p);
     if (rc == REMO_OK) rc = mlpAllocate(a, alloc);
     return rc;
 }
@@ -570,51 +599,122 @@ void analysisReset(AnalysisState& a) {
     a.drawSamples = 0;
 }
 
+namespace {
+
+void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, hipStream_t s) {
+    FrameParams f{};
+    f.u = u;
+    f.arrival = arrival;
+    hipLaunchKernelGGL(kSetFrame, dim3(1), dim3(64), 0, s, a.frame, f);
+}
+
+// The Analysis kernels alone (no host work, so they can be captured).
+void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t maxLevel, hipStream_t s) {
+    const uint32_t nb = a.shape.nodeBlocks;
+    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount,
+                    learned ? a.mlpWeights : nullptr, a.frame, a.closureMargin};
+    hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    for (int level = int(maxLevel) - 1; level >= 0; --level)
+        hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level));
+    if (learned)
+        hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    else
+        hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    hipLaunchKernelGGL(kQueue, dim3(nb), dim3(kBlock), 0, s, t, aa);
+}
+
+// The Rasterize kernels alone.
+void launchRender(AnalysisState& a, const DevTree& t, uint64_t* fb, uint64_t pixels, hipStream_t s) {
+    hipLaunchKernelGGL(kFbClear, dim3(1024), dim3(kBlock), 0, s, fb, pixels);
+    RenderArgs ra{a.visible, a.large, a.drawList, a.drawCounters, a.drawCapacity,
+                  a.units, a.unitCapacity, a.shape.rasterTile, a.frame};
+    hipLaunchKernelGGL(kRenderFlags, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, ra);
+    hipLaunchKernelGGL(kRenderEmit, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, ra);
+    hipLaunchKernelGGL(kRenderRaster, dim3(a.shape.rasterBlocks), dim3(kBlock), 0, s, t, ra, fb);
+}
+
+}  // namespace
+
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-                float arrival, const LaunchShape& ls, hipStream_t s, std::string* err) {
+                float arrival, hipStream_t s, std::string* err) {
     const bool learned = mode == REMO_ANALYSIS_LEARNED;
     if (learned && !a.mlpLoaded) {
         if (err) *err = "no trained score model loaded (remo_set_score_model)";
         return REMO_ERR_STATE;
     }
     a.arrival = arrival;
-    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount,
-                    learned ? a.mlpWeights : nullptr, arrival, a.closureMargin};
-    hipLaunchKernelGGL(kAnalysisLeaves, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa);
-    for (int level = int(maxLevel) - 1; level >= 0; --level)
-        hipLaunchKernelGGL(kAnalysisLevel, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, uint32_t(level));
-    if (learned)
-        hipLaunchKernelGGL(kFinalizeLearned, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, u);
-    else
-        hipLaunchKernelGGL(kFinalize, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, u);
-    hipLaunchKernelGGL(kQueue, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa);
+    setFrame(a, u, arrival, s);
+    launchAnalysis(a, t, learned, maxLevel, s);
     return REMO_OK;
 }
 
-int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s) {
-    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, nullptr, a.arrival, a.closureMargin};
-    hipLaunchKernelGGL(kFeatures, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, aa, out, cap);
+int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s) {
+    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, nullptr, a.frame, a.closureMargin};
+    hipLaunchKernelGGL(kFeatures, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, aa, out, cap);
     return REMO_OK;
 }
 
-int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb,
-              const LaunchShape& ls, hipStream_t s, std::string* err) {
+int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb, hipStream_t s,
+              std::string* err) {
     (void)err;
-    const uint64_t pixels = uint64_t(u.width) * uint64_t(u.height);
-    hipLaunchKernelGGL(kFbClear, dim3(1024), dim3(kBlock), 0, s, fb, pixels);
-    RenderArgs ra{a.visible, a.large, a.drawList, a.drawCounters, a.drawCapacity};
-    hipLaunchKernelGGL(kRenderFlags, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, ra, u);
-    hipLaunchKernelGGL(kRenderEmit, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, ra);
-    hipLaunchKernelGGL(kRenderRaster, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, ra, u, fb);
+    setFrame(a, u, a.arrival, s);
+    launchRender(a, t, fb, uint64_t(u.width) * uint64_t(u.height), s);
     return REMO_OK;
+}
+
+int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
+             float arrival, uint64_t* fb, hipStream_t s, std::string* err) {
+    const bool learned = mode == REMO_ANALYSIS_LEARNED;
+    if (learned && !a.mlpLoaded) {
+        if (err) *err = "no trained score model loaded (remo_set_score_model)";
+        return REMO_ERR_STATE;
+    }
+    const uint64_t pixels = uint64_t(u.width) * uint64_t(u.height);
+    const uint64_t key[6] = {maxLevel, uint64_t(mode), reinterpret_cast<uint64_t>(fb), pixels,
+                             (uint64_t(a.shape.nodeBlocks) << 32) | a.shape.rasterBlocks, a.shape.rasterTile};
+    a.arrival = arrival;
+    setFrame(a, u, arrival, s);  // outside the graph: the graph's kernels read it
+    if (!a.frameExec || std::memcmp(key, a.frameKey, sizeof(key)) != 0) {
+        if (a.frameExec) (void)hipGraphExecDestroy(a.frameExec);
+        a.frameExec = nullptr;
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            if (err) *err = "hipStreamBeginCapture failed";
+            return REMO_ERR_DEVICE;
+        }
+        launchAnalysis(a, t, learned, maxLevel, s);
+        launchRender(a, t, fb, pixels, s);
+        if (hipStreamEndCapture(s, &g) != hipSuccess || !g) {
+            if (err) *err = "hipStreamEndCapture failed";
+            return REMO_ERR_DEVICE;
+        }
+        const hipError_t e = hipGraphInstantiate(&a.frameExec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            a.frameExec = nullptr;
+            if (err) *err = std::string("hipGraphInstantiate failed: ") + hipGetErrorString(e);
+            return REMO_ERR_DEVICE;
+        }
+        std::memcpy(a.frameKey, key, sizeof(key));
+    }
+    if (hipGraphLaunch(a.frameExec, s) != hipSuccess) {
+        if (err) *err = "hipGraphLaunch failed";
+        return REMO_ERR_DEVICE;
+    }
+    return REMO_OK;
+}
+
+void analysisRelease(AnalysisState& a) {
+    if (a.frameExec) (void)hipGraphExecDestroy(a.frameExec);
+    a.frameExec = nullptr;
 }
 
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s) {
-    uint32_t c[4] = {0, 0, 0, 0};
+    uint32_t c[6] = {0, 0, 0, 0, 0, 0};
     if (hipMemcpyAsync(c, a.drawCounters, sizeof(c), hipMemcpyDeviceToHost, s) != hipSuccess) return;
     if (hipStreamSynchronize(s) != hipSuccess) return;
     a.drawItems = c[0];
-    a.drawListOverflow = c[1];
+    a.drawListOverflow = c[1] | c[5];
     uint64_t samples;
     std::memcpy(&samples, c + 2, 8);
     a.drawSamples = samples;

@@ -26,12 +26,13 @@ namespace {
 
 enum ScopeId : int {
     kScopeCount, kScopeSplit, kScopeVoxel, kScopeAlloc, kScopeInsertPoints, kScopeInsertVoxels,
-    kScopeAccumulate, kScopeAnalysis, kScopeRender, kNumScopes
+    kScopeAccumulate, kScopeAnalysis, kScopeRender, kScopeFrame, kNumScopes
 };
 const char* const kScopeNames[kNumScopes] = {
     "remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
     "remolod.construct.allocChunks", "remolod.construct.insertPoints",
     "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
+    "remolod.frame",
 };
 
 // HIP-event pairs per launch, harvested after the stream has passed them.
@@ -188,6 +189,11 @@ int readGlobals(remo_ctx* c) {
        "reading device counters");
     CK(hipStreamSynchronize(c->stream), "device work");
     return REMO_OK;
+}
+
+// The fraction of the cloud's batches ingested (a learned-score input).
+float arrivalOf(const remo_ctx* c) {
+    return c->batchesTotal ? std::min(1.0f, float(c->batchletIndex) / float(c->batchesTotal)) : 1.0f;
 }
 
 void fillStats(const remo_ctx* c, remo_stats* s) {
@@ -356,6 +362,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         A(&w.numTiles, 1, "accumulate tile count");
     }
     c->analysis.closureMargin = cfg.closure_batches;
+    c->analysis.shape = FrameShape{c->shape.nodeBlocks, c->shape.pointBlocks, 1000u};
     if (rc == REMO_OK) rc = analysisAllocate(c->analysis, N, [&](void** p, uint64_t bytes, const char* what) {
         uint8_t* q = nullptr;
         const int r = devAlloc(c, &q, bytes, what);
@@ -377,6 +384,7 @@ void remo_destroy(remo_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->prof.destroy();
+    analysisRelease(c->analysis);
     releaseAll(c);
     delete c;
 }
@@ -512,12 +520,23 @@ int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {
     int rc;
     {
         Scope sc(c->prof, kScopeAnalysis, c->stream);
-        const float arrival =
-            c->batchesTotal ? std::min(1.0f, float(c->batchletIndex) / float(c->batchesTotal)) : 1.0f;
-        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrival, c->shape, c->stream, &c->err);
+        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrivalOf(c), c->stream, &c->err);
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "analysis kernels");
+    return REMO_OK;
+}
+
+int remo_accum_fp32_atomics(remo_ctx* c, float* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    *num = c->hostG->numNodes;
+    if (!out) return REMO_OK;
+    launchAccumFp32Atomics(c->t, out, std::min(cap, c->hostG->numNodes), c->shape, c->stream);
+    CK(hipGetLastError(), "fp32 atomics kernel");
+    CK(hipStreamSynchronize(c->stream), "fp32 atomics kernel");
     return REMO_OK;
 }
 
@@ -538,7 +557,7 @@ int remo_mlp_features(remo_ctx* c, float* out, uint32_t cap, uint32_t* num) {
     if (rc != REMO_OK) return rc;
     *num = c->hostG->numNodes;
     if (!out) return REMO_OK;
-    rc = runFeatures(c->analysis, c->t, out, cap, c->shape, c->stream);
+    rc = runFeatures(c->analysis, c->t, out, cap, c->stream);
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "feature kernel");
     CK(hipStreamSynchronize(c->stream), "feature kernel");
@@ -552,10 +571,93 @@ int remo_render(remo_ctx* c, const remo_shared_uniforms* u, uint64_t* fb) {
     int rc;
     {
         Scope sc(c->prof, kScopeRender, c->stream);
-        rc = runRender(c->analysis, c->t, *u, fb, c->shape, c->stream, &c->err);
+        rc = runRender(c->analysis, c->t, *u, fb, c->stream, &c->err);
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "render kernels");
+    return REMO_OK;
+}
+
+int remo_frame(remo_ctx* c, const remo_shared_uniforms* u, int mode, uint64_t* fb) {
+    if (!c || !u || !fb) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (mode != REMO_ANALYSIS_ANALYTIC && mode != REMO_ANALYSIS_LEARNED) return REMO_ERR_INVALID;
+    if (!(u->width >= 1.0f) || !(u->height >= 1.0f)) return REMO_ERR_INVALID;
+    // the tree depth (read back by every build_step) fixes the number of roll-up
+    // launches in the graph
+    int rc;
+    {
+        Scope sc(c->prof, kScopeFrame, c->stream);
+        rc = runFrame(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrivalOf(c), fb, c->stream, &c->err);
+    }
+    if (rc != REMO_OK) return rc;
+    CK(hipGetLastError(), "frame graph");
+    return REMO_OK;
+}
+
+int remo_autotune_frame(remo_ctx* c, const remo_shared_uniforms* u, int mode, uint64_t* fb, remo_tune_entry* out,
+                        uint32_t cap, uint32_t* num) {
+    if (!c || !u || !fb || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (mode != REMO_ANALYSIS_ANALYTIC && mode != REMO_ANALYSIS_LEARNED) return REMO_ERR_INVALID;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    CK(hipGetDevice(&dev), "device");
+    CK(hipGetDeviceProperties(&prop, dev), "device properties");
+    const uint32_t cus = std::max(1, prop.multiProcessorCount);
+    const uint32_t nbs[3] = {cus, 2 * cus, 4 * cus};
+    const uint32_t rbs[3] = {4 * cus, 8 * cus, 16 * cus};
+    const uint32_t tiles[3] = {1000u, 2000u, 4000u};
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0), "event");
+    CK(hipEventCreate(&e1), "event");
+    const float arrival = arrivalOf(c);
+    std::vector<remo_tune_entry> res;
+    FrameShape best = c->analysis.shape;
+    float bestMs = 1e30f;
+    for (uint32_t nb : nbs)
+        for (uint32_t rb : rbs)
+            for (uint32_t tile : tiles) {
+                c->analysis.shape = FrameShape{nb, rb, tile};
+                float ms = 1e30f;
+                for (int rep = 0; rep < 4; ++rep) {  // the first run warms up
+                    (void)hipEventRecord(e0, c->stream);
+                    rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrival, c->stream, &c->err);
+                    if (rc == REMO_OK) rc = runRender(c->analysis, c->t, *u, fb, c->stream, &c->err);
+                    (void)hipEventRecord(e1, c->stream);
+                    if (rc != REMO_OK) break;
+                    if (hipEventSynchronize(e1) != hipSuccess) {
+                        rc = REMO_ERR_DEVICE;
+                        break;
+                    }
+                    float t = 0.0f;
+                    (void)hipEventElapsedTime(&t, e0, e1);
+                    if (rep > 0) ms = std::min(ms, t);
+                }
+                if (rc != REMO_OK) break;
+                res.push_back(remo_tune_entry{nb, rb, tile, ms});
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    best = c->analysis.shape;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->analysis.shape = best;
+    if (rc != REMO_OK) return rc;
+    CK(hipGetLastError(), "autotune");
+    *num = uint32_t(res.size());
+    if (out)
+        for (uint32_t i = 0; i < std::min<uint32_t>(cap, uint32_t(res.size())); ++i) out[i] = res[i];
+    return REMO_OK;
+}
+
+int remo_get_frame_shape(remo_ctx* c, remo_tune_entry* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    const FrameShape& f = c->analysis.shape;
+    *out = remo_tune_entry{f.nodeBlocks, f.rasterBlocks, f.rasterTile, 0.0f};
     return REMO_OK;
 }
 

@@ -31,6 +31,27 @@ __host__ __device__ inline double i128ToDouble(__int128 x) {
     return neg ? -d : d;
 }
 
+// Per-frame parameters, read by the Analysis and Rasterize kernels from device
+// memory so that a captured graph of those kernels stays valid from frame to frame.
+struct FrameParams {
+    remo_shared_uniforms u;
+    float arrival;        // fraction of the cloud's batches ingested
+    uint32_t pad[3];
+};
+
+// One unit of raster work: `count` samples of a draw item's points (pass 0) or
+// voxels (pass 1), starting at sample `first`.
+struct RasterUnit {
+    uint32_t node, pass, first, count;
+};
+
+// Launch shape of the per-frame kernels (the autotuner's knobs).
+struct FrameShape {
+    uint32_t nodeBlocks;    // workgroups of the node-parallel Analysis / selection kernels
+    uint32_t rasterBlocks;  // workgroups of the raster kernel
+    uint32_t rasterTile;    // samples per raster unit
+};
+
 // One item of the frontier draw list (kernels/shared/remo_draw.cuh:13-22).
 struct DrawItem {
     uint32_t node;
@@ -50,8 +71,15 @@ struct AnalysisState {
     uint8_t* visible = nullptr;
     uint8_t* large = nullptr;
     DrawItem* drawList = nullptr;
-    uint32_t* drawCounters = nullptr;  // [0] items, [1] overflow, [2..3] samples (u64)
+    uint32_t* drawCounters = nullptr;  // [0] items, [1] overflow, [2..3] samples (u64), [4] units, [5] unit overflow
     uint32_t drawCapacity = 131072;    // REMOLOD_DRAWLIST_CAPACITY, remolod_render.cu:22
+    RasterUnit* units = nullptr;
+    uint32_t unitCapacity = 1u << 20;
+    FrameParams* frame = nullptr;      // device copy of the current frame's parameters
+    FrameShape shape{0, 0, 1000};
+    // the captured Analysis + Rasterize graph and what it was captured for
+    hipGraphExec_t frameExec = nullptr;
+    uint64_t frameKey[6] = {0, 0, 0, 0, 0, 0};
     uint32_t maxNodes = 0;
     // host mirrors of the last render's diagnostics
     uint32_t drawItems = 0, drawListOverflow = 0;
@@ -64,14 +92,19 @@ using AllocFn = std::function<int(void**, uint64_t, const char*)>;
 
 int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc);
 void analysisReset(AnalysisState& a);
+void analysisRelease(AnalysisState& a);
 // maxLevel: the deepest node level (Globals::maxLevel as last read back);
 // arrival: the fraction of the cloud's batches ingested (a learned-score input).
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-                float arrival, const LaunchShape& ls, hipStream_t s, std::string* err);
+                float arrival, hipStream_t s, std::string* err);
 // The learned score's 13 inputs of every node from the last Analysis roll-up.
-int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);
-int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb,
-              const LaunchShape& ls, hipStream_t s, std::string* err);
+int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s);
+int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb, hipStream_t s,
+              std::string* err);
+// Analysis + Rasterize as one captured graph, re-captured when the tree depth,
+// the mode, the framebuffer, its size or the launch shape change.
+int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
+             float arrival, uint64_t* fb, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);
 
 }  // namespace remo_hip

@@ -218,6 +218,7 @@ void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);
 void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
                       hipStream_t s);
 

@@ -129,12 +129,18 @@ ANALYSIS_DTYPE = np.dtype([
 
 SCOPES = ["remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
           "remolod.construct.allocChunks", "remolod.construct.insertPoints",
-          "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render"]
+          "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
+          "remolod.frame"]
 PROFILE_ALL = 0xFFFFFFFF
 
 
 class ProfileEntry(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("total_ms", C.c_double), ("launches", C.c_uint64)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("node_blocks", C.c_uint32), ("raster_blocks", C.c_uint32), ("raster_tile", C.c_uint32),
+                ("ms", C.c_float)]
 
 
 class SceneSpec(C.Structure):
@@ -198,6 +204,11 @@ def load() -> C.CDLL:
         "remo_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                           P(C.c_float)]),
         "remo_set_score_model": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "remo_accum_fp32_atomics": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
+        "remo_frame": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int, C.c_void_p]),
+        "remo_autotune_frame": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_uint32, P(C.c_uint32)]),
+        "remo_get_frame_shape": (C.c_int, [C.c_void_p, P(TuneEntry)]),
         "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),
         "remo_scene_generate": (C.c_int, [P(SceneSpec), C.c_uint64, C.c_uint64, C.c_void_p,

@@ -168,6 +168,24 @@ class Tree:
     def render(self, uniforms: L.SharedUniforms, fb_ptr: int):
         self._ck(self.lib.remo_render(self.h, C.byref(uniforms), C.c_void_p(fb_ptr)), "render")
 
+    def frame(self, uniforms: L.SharedUniforms, fb_ptr: int, mode: int = L.REMO_ANALYSIS_ANALYTIC):
+        """Analysis + Rasterize as one captured hipGraph (remo_frame)."""
+        self._ck(self.lib.remo_frame(self.h, C.byref(uniforms), mode, C.c_void_p(fb_ptr)), "frame")
+
+    def autotune_frame(self, uniforms: L.SharedUniforms, fb_ptr: int, mode: int = L.REMO_ANALYSIS_ANALYTIC):
+        """Times the per-frame kernels over a grid of launch shapes, keeps the fastest;
+        returns [(node_blocks, raster_blocks, raster_tile, ms)]."""
+        n = C.c_uint32(0)
+        arr = (L.TuneEntry * 64)()
+        self._ck(self.lib.remo_autotune_frame(self.h, C.byref(uniforms), mode, C.c_void_p(fb_ptr), arr, 64,
+                                              C.byref(n)), "autotune")
+        return [(e.node_blocks, e.raster_blocks, e.raster_tile, e.ms) for e in arr[: n.value]]
+
+    def frame_shape(self):
+        e = L.TuneEntry()
+        self._ck(self.lib.remo_get_frame_shape(self.h, C.byref(e)), "frame shape")
+        return e.node_blocks, e.raster_blocks, e.raster_tile
+
     def sync(self):
         self._ck(self.lib.remo_sync(self.h), "sync")
 
@@ -238,6 +256,19 @@ class Tree:
             got = C.c_uint32(0)
             self._ck(self.lib.remo_mlp_features(self.h, C.c_void_p(buf.ptr), n.value, C.byref(got)), "features")
             return buf.download(np.float32, (n.value, L.MLP_INPUTS))
+        finally:
+            buf.free()
+
+    def fp32_atomic_sums(self) -> np.ndarray:
+        """[n][10] FP32-atomics sums of every leaf (count, sum x y z, xx xy xz yy yz zz)."""
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_accum_fp32_atomics(self.h, None, 0, C.byref(n)), "fp32 sums")
+        buf = L.DeviceBuffer(40 * max(1, n.value))
+        try:
+            got = C.c_uint32(0)
+            self._ck(self.lib.remo_accum_fp32_atomics(self.h, C.c_void_p(buf.ptr), n.value, C.byref(got)),
+                     "fp32 sums")
+            return buf.download(np.float32, (n.value, 10))
         finally:
             buf.free()
 
