@@ -100,31 +100,34 @@ __device__ __forceinline__ void leafSums(const DevTree& t, uint32_t node, uint32
     S.n = a.count;
 }
 
-// The child offsets are +-h with h = 2^k, so every product is a shift:
-// S1 += C1 + n o, S2 += C2 + o_a C1_b + C1_a o_b + n o_a o_b.
-__device__ __forceinline__ void parentSums(const DevTree& t, const RollupSums* R, uint32_t parent, uint32_t D,
-                                           RollupSums& S) {
-    const int k = int(18 + D - t.level[parent]);
-    const uint32_t c0 = uint32_t(t.firstChild[parent]);
-    for (int i = 0; i < 3; ++i) S.s1[i] = 0;
-    for (int i = 0; i < 6; ++i) S.s2[i] = 0;
-    S.n = 0;
+// The child offsets are +-h with h = 2^k, so every product is a shift: child c
+// contributes S1 += C1 + n o, S2 += C2 + o_a C1_b + C1_a o_b + n o_a o_b to its
+// parent, o the offset of c's centre from the parent's.
+__device__ __forceinline__ void childContribution(const RollupSums& C, uint32_t ci, int k, RollupSums& S) {
     const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-    for (int ci = 0; ci < 8; ++ci) {
-        const RollupSums& C = R[c0 + ci];
-        const bool pos[3] = {(ci & 4) != 0, (ci & 2) != 0, (ci & 1) != 0};
-        const i128 n = i128(C.n);
-        const i128 nh = shl(n, k);
-        for (int i = 0; i < 3; ++i) S.s1[i] += C.s1[i] + (pos[i] ? nh : -nh);
-        for (int q = 0; q < 6; ++q) {
-            const int a_ = pi[q], b_ = pj[q];
-            const i128 t1 = shl(C.s1[b_], k), t2 = shl(C.s1[a_], k);
-            const i128 cross = (pos[a_] ? t1 : -t1) + (pos[b_] ? t2 : -t2);
-            const i128 nn = shl(n, 2 * k);
-            S.s2[q] += C.s2[q] + cross + (pos[a_] == pos[b_] ? nn : -nn);
-        }
-        S.n += C.n;
+    const bool pos[3] = {(ci & 4) != 0, (ci & 2) != 0, (ci & 1) != 0};
+    const i128 n = i128(C.n);
+    const i128 nh = shl(n, k);
+    for (int i = 0; i < 3; ++i) S.s1[i] = C.s1[i] + (pos[i] ? nh : -nh);
+    const i128 nn = shl(n, 2 * k);
+    for (int q = 0; q < 6; ++q) {
+        const int a_ = pi[q], b_ = pj[q];
+        const i128 t1 = shl(C.s1[b_], k), t2 = shl(C.s1[a_], k);
+        const i128 cross = (pos[a_] ? t1 : -t1) + (pos[b_] ? t2 : -t2);
+        S.s2[q] = C.s2[q] + cross + (pos[a_] == pos[b_] ? nn : -nn);
     }
+    S.n = C.n;
+}
+
+// Sum over the 8 lanes of a group (xor butterfly within width 8).
+__device__ __forceinline__ i128 groupSum8(i128 v) {
+    for (int m = 1; m < 8; m <<= 1) {
+        const unsigned __int128 u = (unsigned __int128)v;
+        const unsigned long long lo = __shfl_xor((unsigned long long)uint64_t(u), m, 8);
+        const unsigned long long hi = __shfl_xor((unsigned long long)uint64_t(u >> 64), m, 8);
+        v += i128(((unsigned __int128)hi << 64) | lo);
+    }
+    return v;
 }
 
 // ---- finalise ------------------------------------------------------------------
@@ -178,16 +181,15 @@ struct AnalysisArgs {
     remo_node_analysis* results;
     uint32_t* queue;
     uint32_t* queueCount;
-    const uint16_t* mlpWeights;  // BF16 parameters of the learned score, or null
+    const float* learned;        // [n][2] predicted (surface variation, planarity), or null
     const FrameParams* frame;    // uniforms and arrival of this frame (device)
     uint32_t margin;
 };
 
 // Closure, covariance, eigen-solve and score of one node from its rolled-up sums.
-// w: the learned score's parameters expanded to FP32 (LDS), or null for the
-// analytic score.
+// With aa.learned, open nodes take the learned score's prediction.
 __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
-                             uint32_t node, const RollupSums& S, bool closed, uint32_t D, const float* w) {
+                             uint32_t node, const RollupSums& S, bool closed, uint32_t D) {
     remo_node_analysis r;
     memset(&r, 0, sizeof(r));
     r.n = S.n;
@@ -196,13 +198,10 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
     r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
     r.state = closed ? 2u : 0u;
     bool scored = false;
-    if (w && !closed && S.n >= 3) {
+    if (aa.learned && !closed && S.n >= 3) {
         // learned score for open nodes; the analytic path is the fallback when the
         // prediction says the node is not planar (surface variation > 0.05)
-        float f[kMlpIn], y[kMlpOut];
-        nodeFeatures(t, node, S, aa.frame->arrival, f);
-        mlpForward(w, f, y);
-        const float sv = y[0], pl = y[1];
+        const float sv = aa.learned[2 * node], pl = aa.learned[2 * node + 1];
         if (sv <= kLearnedFallbackSv) {
             r.surface_variation = fmaxf(sv, 0.0f);
             r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
@@ -284,24 +283,36 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLeaves(DevTree t, AnalysisArg
 }
 
 // Inner nodes of one level, bottom-up, one launch per level: the eight children
-// (one level deeper, finished by the previous launch) are added in child order.
-// Only the exact sums travel up the levels; the eigen-solve and score of every
-// node run afterwards in one node-parallel pass (kFinalize).
+// (one level deeper, finished by the previous launch) are added by 8 lanes, one
+// child each, and summed across the group (exact int128 addition, so the order
+// does not matter). Only the exact sums travel up the levels; the eigen-solve and
+// score of every node run afterwards in one node-parallel pass (kFinalize).
 __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs aa, uint32_t level) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
-    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+    const uint32_t sub = threadIdx.x & 7u;
+    const uint32_t groups = (gridDim.x * blockDim.x) >> 3;
+    // every lane of a group sees the same node, so the group's lanes stay together
+    for (uint32_t node = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; node < n; node += groups) {
         if (t.level[node] != level) continue;
         const int32_t fc = t.firstChild[node];
         if (fc < 0) continue;
+        const uint32_t child = uint32_t(fc) + sub;
+        RollupSums C;
+        childContribution(aa.rollup[child], sub, int(18 + D - level), C);
         RollupSums P;
-        parentSums(t, aa.rollup, node, D, P);
+        for (int i = 0; i < 3; ++i) P.s1[i] = groupSum8(C.s1[i]);
+        for (int q = 0; q < 6; ++q) P.s2[q] = groupSum8(C.s2[q]);
+        P.n = uint64_t(groupSum8(i128(C.n)));
         P.e = 20 + D - level;
-        uint8_t pc = 1;
-        for (int ci = 0; ci < 8; ++ci) pc &= aa.closed[uint32_t(fc) + ci];
-        aa.rollup[node] = P;
-        aa.closed[node] = pc;
+        P.pad = 0;
+        uint32_t cl = aa.closed[child];
+        for (int m = 1; m < 8; m <<= 1) cl &= __shfl_xor(cl, m, 8);
+        if (sub == 0) {
+            aa.rollup[node] = P;
+            aa.closed[node] = uint8_t(cl);
+        }
     }
 }
 
@@ -311,21 +322,7 @@ __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) 
     const uint32_t D = g->maxLevel;
     const remo_shared_uniforms& u = aa.frame->u;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
-        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, nullptr);
-}
-
-// The learned variant: the MLP's BF16 parameters are expanded to FP32 in LDS once
-// per workgroup; open nodes run the MLP, closed ones (and fallbacks) the eigen-solve.
-__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa) {
-    __shared__ float w[kMlpParams];
-    const remo_shared_uniforms& u = aa.frame->u;
-    for (uint32_t i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = bf16ToF32(aa.mlpWeights[i]);
-    __syncthreads();
-    Globals* g = t.g;
-    const uint32_t n = g->numNodes;
-    const uint32_t D = g->maxLevel;
-    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
-        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, w);
+        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
 }
 
 // The learned score's inputs of every node (training pairs are logged from these).
@@ -589,6 +586,10 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.units = static_cast<RasterUnit*>(p);
     A(&p, sizeof(FrameParams), "frame parameters");
     a.frame = static_cast<FrameParams*>(p);
+    A(&p, sizeof(float) * kMlpIn * maxNodes, "learned-score inputs");
+    a.feat = static_cast<float*>(p);
+    A(&p, sizeof(float) * kMlpOut * maxNodes, "learned-score predictions");
+    a.pred = static_cast<float*>(p);
     if (rc == REMO_OK) rc = mlpAllocate(a, alloc);
     return rc;
 }
@@ -609,17 +610,22 @@ void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, hi
 }
 
 // The Analysis kernels alone (no host work, so they can be captured).
-void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t maxLevel, hipStream_t s) {
+void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t maxLevel, uint32_t numNodes,
+                    hipStream_t s) {
     const uint32_t nb = a.shape.nodeBlocks;
     AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount,
-                    learned ? a.mlpWeights : nullptr, a.frame, a.closureMargin};
+                    learned ? a.pred : nullptr, a.frame, a.closureMargin};
     hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    // one launch per level: measured faster than one workgroup walking all levels
+    // with barriers (157 us against 101 us per Analysis on a 10,073-node tree)
     for (int level = int(maxLevel) - 1; level >= 0; --level)
         hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level));
-    if (learned)
-        hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa);
-    else
-        hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    if (learned) {
+        // every node's inputs, then the MLP on the matrix cores, then finalise
+        hipLaunchKernelGGL(kFeatures, dim3(nb), dim3(kBlock), 0, s, t, aa, a.feat, a.maxNodes);
+        launchMlpInfer(a.mlpWeights, a.feat, &t.g->numNodes, 0, a.pred, nb, s);
+    }
+    hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
     hipLaunchKernelGGL(kQueue, dim3(nb), dim3(kBlock), 0, s, t, aa);
 }
 
@@ -636,7 +642,7 @@ void launchRender(AnalysisState& a, const DevTree& t, uint64_t* fb, uint64_t pix
 }  // namespace
 
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-                float arrival, hipStream_t s, std::string* err) {
+                uint32_t numNodes, float arrival, hipStream_t s, std::string* err) {
     const bool learned = mode == REMO_ANALYSIS_LEARNED;
     if (learned && !a.mlpLoaded) {
         if (err) *err = "no trained score model loaded (remo_set_score_model)";
@@ -644,7 +650,7 @@ int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& 
     }
     a.arrival = arrival;
     setFrame(a, u, arrival, s);
-    launchAnalysis(a, t, learned, maxLevel, s);
+    launchAnalysis(a, t, learned, maxLevel, numNodes, s);
     return REMO_OK;
 }
 
@@ -663,14 +669,15 @@ int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u,
 }
 
 int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-             float arrival, uint64_t* fb, hipStream_t s, std::string* err) {
+             uint32_t numNodes, float arrival, uint64_t* fb, hipStream_t s, std::string* err) {
     const bool learned = mode == REMO_ANALYSIS_LEARNED;
     if (learned && !a.mlpLoaded) {
         if (err) *err = "no trained score model loaded (remo_set_score_model)";
         return REMO_ERR_STATE;
     }
     const uint64_t pixels = uint64_t(u.width) * uint64_t(u.height);
-    const uint64_t key[6] = {maxLevel, uint64_t(mode), reinterpret_cast<uint64_t>(fb), pixels,
+    const uint64_t key[6] = {maxLevel, uint64_t(mode),
+                             reinterpret_cast<uint64_t>(fb), pixels,
                              (uint64_t(a.shape.nodeBlocks) << 32) | a.shape.rasterBlocks, a.shape.rasterTile};
     a.arrival = arrival;
     setFrame(a, u, arrival, s);  // outside the graph: the graph's kernels read it
@@ -682,7 +689,7 @@ int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, 
             if (err) *err = "hipStreamBeginCapture failed";
             return REMO_ERR_DEVICE;
         }
-        launchAnalysis(a, t, learned, maxLevel, s);
+        launchAnalysis(a, t, learned, maxLevel, numNodes, s);
         launchRender(a, t, fb, pixels, s);
         if (hipStreamEndCapture(s, &g) != hipSuccess || !g) {
             if (err) *err = "hipStreamEndCapture failed";

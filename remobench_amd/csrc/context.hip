@@ -520,7 +520,8 @@ int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {
     int rc;
     {
         Scope sc(c->prof, kScopeAnalysis, c->stream);
-        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrivalOf(c), c->stream, &c->err);
+        rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, c->hostG->numNodes, arrivalOf(c), c->stream,
+                         &c->err);
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "analysis kernels");
@@ -588,7 +589,8 @@ int remo_frame(remo_ctx* c, const remo_shared_uniforms* u, int mode, uint64_t* f
     int rc;
     {
         Scope sc(c->prof, kScopeFrame, c->stream);
-        rc = runFrame(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrivalOf(c), fb, c->stream, &c->err);
+        rc = runFrame(c->analysis, c->t, *u, mode, c->hostG->maxLevel, c->hostG->numNodes, arrivalOf(c), fb,
+                      c->stream, &c->err);
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "frame graph");
@@ -624,7 +626,8 @@ int remo_autotune_frame(remo_ctx* c, const remo_shared_uniforms* u, int mode, ui
                 float ms = 1e30f;
                 for (int rep = 0; rep < 4; ++rep) {  // the first run warms up
                     (void)hipEventRecord(e0, c->stream);
-                    rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, arrival, c->stream, &c->err);
+                    rc = runAnalysis(c->analysis, c->t, *u, mode, c->hostG->maxLevel, c->hostG->numNodes, arrival,
+                                     c->stream, &c->err);
                     if (rc == REMO_OK) rc = runRender(c->analysis, c->t, *u, fb, c->stream, &c->err);
                     (void)hipEventRecord(e1, c->stream);
                     if (rc != REMO_OK) break;

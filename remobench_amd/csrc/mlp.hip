@@ -12,7 +12,7 @@
 //              partial gradient (5,186 values) in sample order;
 //   kMlpAdam   one thread per parameter: sums the groups' partials in group order
 //              (deterministic), Adam update of the master, BF16 copy;
-//   kMlpForwardBatch  inference, one sample per lane.
+//   kMlpInfer  inference on the matrix cores (FP32 MFMA), 16 samples per wave.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,16 +50,17 @@ __global__ void kMlpReset(const float* master, uint16_t* bf16, float* m, float* 
     }
 }
 
-__global__ __launch_bounds__(256) void kMlpForwardBatch(const uint16_t* wb, const float* x, uint32_t n, float* y) {
-    __shared__ float w[kMlpParams];
-    for (uint32_t i = threadIdx.x; i < kMlpParams; i += blockDim.x) w[i] = bf16ToF32(wb[i]);
+// Inference: rows of x -> rows of y on the matrix cores, 16 rows per wave. The
+// row count comes from nPtr (device, e.g. the node count) when given, else n.
+__global__ __launch_bounds__(64 * kMlpWavesPerBlock) void kMlpInfer(const uint16_t* wb, const float* x,
+                                                                   const uint32_t* nPtr, uint32_t n, float* y) {
+    __shared__ MlpInferShared sh;
+    for (uint32_t i = threadIdx.x; i < kMlpParams; i += blockDim.x) sh.w[i] = bf16ToF32(wb[i]);
     __syncthreads();
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        float xs[kMlpIn], ys[kMlpOut];
-        for (uint32_t k = 0; k < kMlpIn; ++k) xs[k] = x[uint64_t(s) * kMlpIn + k];
-        mlpForward(w, xs, ys);
-        for (uint32_t o = 0; o < kMlpOut; ++o) y[uint64_t(s) * kMlpOut + o] = ys[o];
-    }
+    const uint32_t rows = nPtr ? *nPtr : n;
+    const uint32_t wave = threadIdx.x / 64;
+    for (uint32_t tile = blockIdx.x * kMlpWavesPerBlock + wave; tile * 16 < rows; tile += gridDim.x * kMlpWavesPerBlock)
+        mlpInferTile(sh.w, sh.act[wave], x, rows, tile * 16, y);
 }
 
 // Loss L = 1 / (n * outputs) * sum (y - t)^2 over the batch.
@@ -181,6 +182,12 @@ uint64_t splitmix(uint64_t& s) {
 }
 
 }  // namespace
+
+void launchMlpInfer(const uint16_t* wb, const float* x, const uint32_t* nPtr, uint32_t n, float* y, uint32_t blocks,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(kMlpInfer, dim3(std::max(1u, blocks)), dim3(64 * kMlpWavesPerBlock), 0, s, wb, x, nPtr, n, y);
+}
+
 }  // namespace remo_hip
 
 using namespace remo_hip;
@@ -321,8 +328,7 @@ int remo_mlp_forward(remo_mlp* p, const float* x, uint32_t n, float* y) {
     if (!p || !x || !y) return REMO_ERR_INVALID;
     if (p->dead) return REMO_ERR_DEVICE;
     if (n == 0) return REMO_OK;
-    const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(kMlpForwardBatch, dim3(blocks), dim3(256), 0, p->stream, p->bf16, x, n, y);
+    launchMlpInfer(p->bf16, x, nullptr, n, y, std::min<uint32_t>((n + 63) / 64, 2048), p->stream);
     return mlpSync(p);
 }
 

@@ -85,6 +85,8 @@ struct AnalysisState {
     uint32_t drawItems = 0, drawListOverflow = 0;
     uint64_t drawSamples = 0;
     uint16_t* mlpWeights = nullptr;    // BF16 parameters of the learned score
+    float* feat = nullptr;             // [maxNodes][13] learned-score inputs
+    float* pred = nullptr;             // [maxNodes][2] its predictions
     bool mlpLoaded = false;
 };
 
@@ -95,8 +97,9 @@ void analysisReset(AnalysisState& a);
 void analysisRelease(AnalysisState& a);
 // maxLevel: the deepest node level (Globals::maxLevel as last read back);
 // arrival: the fraction of the cloud's batches ingested (a learned-score input).
+// numNodes: Globals::numNodes as last read back (picks the roll-up's launch shape).
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-                float arrival, hipStream_t s, std::string* err);
+                uint32_t numNodes, float arrival, hipStream_t s, std::string* err);
 // The learned score's 13 inputs of every node from the last Analysis roll-up.
 int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s);
 int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb, hipStream_t s,
@@ -104,7 +107,7 @@ int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u,
 // Analysis + Rasterize as one captured graph, re-captured when the tree depth,
 // the mode, the framebuffer, its size or the launch shape change.
 int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
-             float arrival, uint64_t* fb, hipStream_t s, std::string* err);
+             uint32_t numNodes, float arrival, uint64_t* fb, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);
 
 }  // namespace remo_hip

@@ -92,6 +92,71 @@ __device__ __forceinline__ void mlpForward(const float* w, const float x[kMlpIn]
 }
 static_assert(kMlpOut == 2, "mlpForward writes two outputs");
 
+// ---- batched inference on the matrix cores ------------------------------------------
+//
+// One wave computes 16 rows with v_mfma_f32_16x16x4_f32 (FP32 operands: the BF16
+// weights expanded, FP32 activations and accumulation). Operand layout of that
+// instruction for lane l: A[row l % 16][k l / 16], B[k l / 16][col l % 16], and
+// the 16x16 result D[row 4 (l / 16) + r][col l % 16] in element r. Layer outputs
+// go through LDS to become the next layer's A operand.
+typedef float mlpF4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kMlpWavesPerBlock = 4;
+constexpr uint32_t kMlpActRow = kMlpHid + 1;  // padded LDS row
+
+struct MlpInferShared {
+    float w[kMlpParams];
+    float act[kMlpWavesPerBlock][16][kMlpActRow];
+};
+
+// Rows [row0, row0 + 16) of x (n rows of kMlpIn) -> y (n rows of kMlpOut).
+__device__ inline void mlpInferTile(const float* w, float (*act)[kMlpActRow], const float* x, uint32_t n,
+                                    uint32_t row0, float* y) {
+    const uint32_t lane = __lane_id();
+    const uint32_t r16 = lane % 16, k4 = lane / 16;
+    const uint32_t row = row0 + r16;
+    const bool valid = row < n;
+    // layer 1: [16 x 16 (13 used)] x [16 x 64]
+    mlpF4 acc[4];
+    for (uint32_t nt = 0; nt < 4; ++nt) {
+        acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t ks = 0; ks < 4; ++ks) {
+            const uint32_t k = ks * 4 + k4;
+            const float a = (valid && k < kMlpIn) ? x[uint64_t(row) * kMlpIn + k] : 0.0f;
+            const float b = k < kMlpIn ? w[kW1 + (nt * 16 + r16) * kMlpIn + k] : 0.0f;
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+    for (uint32_t nt = 0; nt < 4; ++nt)
+        for (uint32_t r = 0; r < 4; ++r) act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kB1 + nt * 16 + r16]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 2: [16 x 64] x [64 x 64]
+    for (uint32_t nt = 0; nt < 4; ++nt) {
+        acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t ks = 0; ks < 16; ++ks) {
+            const uint32_t k = ks * 4 + k4;
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(act[r16][k], w[kW2 + (nt * 16 + r16) * kMlpHid + k],
+                                                            acc[nt], 0, 0, 0);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read layer 1
+    for (uint32_t nt = 0; nt < 4; ++nt)
+        for (uint32_t r = 0; r < 4; ++r) act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kB2 + nt * 16 + r16]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 3: [16 x 64] x [64 x 16 (2 used)]
+    mlpF4 out = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t ks = 0; ks < 16; ++ks) {
+        const uint32_t k = ks * 4 + k4;
+        const float b = r16 < kMlpOut ? w[kW3 + r16 * kMlpHid + k] : 0.0f;
+        out = __builtin_amdgcn_mfma_f32_16x16x4f32(act[r16][k], b, out, 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // act is reused by the next tile
+    if (r16 < kMlpOut)
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint32_t orow = row0 + 4 * k4 + r;
+            if (orow < n) y[uint64_t(orow) * kMlpOut + r16] = out[r] + w[kB3 + r16];
+        }
+}
+
 // ---- features --------------------------------------------------------------------
 
 // The 13 inputs of a node, from its rolled-up exact sums (units 2^-(20+e) of the
@@ -131,5 +196,9 @@ __device__ inline void nodeFeatures(const DevTree& t, uint32_t node, const Rollu
 int mlpAllocate(AnalysisState& a, const AllocFn& alloc);
 // Copies a training object's BF16 parameters to `dst` (device) and waits.
 int mlpCopyWeights(const remo_mlp* p, uint16_t* dst);
+// Batched inference (kMlpInfer): rows of x -> rows of y; the row count is *nPtr
+// (device) when nPtr is given, else n.
+void launchMlpInfer(const uint16_t* wb, const float* x, const uint32_t* nPtr, uint32_t n, float* y, uint32_t blocks,
+                    hipStream_t s);
 
 }  // namespace remo_hip
