@@ -133,6 +133,7 @@ def main() -> int:
     ap.add_argument("--scene", default="lidar")
     ap.add_argument("--cpu-sample-batches", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--raster", default="1920x1080", help="framebuffer size of the Rasterize timing, WxH")
     ap.add_argument("--analysis", choices=["analytic", "learned", "off"], default="analytic")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run remo_copy_kernel over this many MiB 5 times (the known-size "
@@ -290,27 +291,36 @@ def main() -> int:
     path_bytes = sum(s["algorithmic_bytes"] for s in scopes.values())
     path_gbs = path_bytes / dt / 1e9
 
-    # Rasterize, 1920x1080, three poses, on the full tree
+    # Rasterize, and Analysis + Rasterize as one captured graph (remo_frame), three
+    # poses, on the full tree
     from remobench_amd import camera
     import ctypes as C
-    raster = {}
+    rw, rh = (int(v) for v in args.raster.lower().split("x"))
+    raster = {"resolution": f"{rw}x{rh}"}
     fbp = C.c_void_p()
-    L.check(L.load().remo_dev_alloc(1920 * 1080 * 8, C.byref(fbp)), "framebuffer")
+    L.check(L.load().remo_dev_alloc(rw * rh * 8, C.byref(fbp)), "framebuffer")
     sample_pts = points[:: max(1, len(points) // 2_000_000)]
-    tree.profile_enable(True, ["remolod.render"])
-    for pose, u in camera.poses(sample_pts, meta.box_size).items():
+    tree.profile_enable(True, ["remolod.render", "remolod.frame"])
+    for pose, u in camera.poses(sample_pts, meta.box_size, rw, rh).items():
         tree.render(u, fbp.value)  # warm
+        tree.frame(u, fbp.value)
         tree.sync()
         tree.profile_clear()
         reps = 10
         for _ in range(reps):
             tree.render(u, fbp.value)
+        for _ in range(reps):
+            tree.frame(u, fbp.value)
         tree.sync()
-        ms, launches = tree.profile()["remolod.render"]
+        prof_r = tree.profile()
+        ms, launches = prof_r["remolod.render"]
+        fms, flaunches = prof_r["remolod.frame"]
         rs = tree.stats()
-        raster[pose] = {"ms": round(ms / max(1, launches), 4), "draw_items": rs["draw_items"],
-                        "samples": rs["draw_samples"]}
+        raster[pose] = {"ms": round(ms / max(1, launches), 4),
+                        "frame_graph_ms": round(fms / max(1, flaunches), 4),
+                        "draw_items": rs["draw_items"], "samples": rs["draw_samples"]}
         log(f"render {pose}: {raster[pose]}")
+    tree.profile_enable(False)
     L.load().remo_dev_free(fbp)
 
     cpu = None
@@ -345,9 +355,11 @@ def main() -> int:
         "path_hbm_gbs": round(path_gbs, 2),
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 5),
         "scopes": scopes,
-        "rasterize_1080p": raster,
+        "rasterize": raster,
         "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points",
-                                    "num_voxels", "max_depth", "allocated_bytes_persistent")},
+                                    "num_voxels", "max_depth", "allocated_bytes_persistent",
+                                    "mem_capacity_reached", "node_pool_overflow", "spill_overflow",
+                                    "backlog_overflow", "dir_overflow")},
     }
     print(json.dumps(out), flush=True)
     tree.close()
