@@ -333,12 +333,29 @@ void remo_ring_destroy(remo_ring* ring);
  * advances the uploaded cursor to batch_index + 1 if it was below. */
 int remo_ring_upload(remo_ring* ring, uint32_t batch_index, const remo_point* host, uint32_t n);
 int remo_ring_view(const remo_ring* ring, uint32_t num_batches_total, remo_batch_view* out);
-/* PointSource::rewind: uploaded cursor back to 0. */
+/* PointSource::rewind: uploaded cursor back to 0 (waits for pushes in flight). */
 int remo_ring_rewind(remo_ring* ring);
+/* Asynchronous upload on the ring's own copy stream, overlapping the consumer's
+ * kernels (the reference's ring has no second stream, README.md:185-191). `host`
+ * should be pinned (remo_host_alloc) for the copy to be truly asynchronous and must
+ * stay valid until remo_ring_sync. When the slot still holds an earlier batch, the
+ * copy first waits for `consumer`'s last build_step (may be NULL if the caller
+ * orders it otherwise). The producer keeps batch_index < consumed + num_slots
+ * (PointSource's backpressure contract, include/remo/PointSource.h:95-103). */
+int remo_ring_push(remo_ring* ring, uint32_t batch_index, const remo_point* host, uint32_t n,
+                   remo_ctx* consumer);
+/* Makes `consumer`'s stream wait for the pushes of batches
+ * [first_batch, first_batch + num_batches): call before remo_build_step. */
+int remo_ring_order_before(remo_ring* ring, remo_ctx* consumer, uint32_t first_batch, uint32_t num_batches);
+/* Waits for every push issued so far. */
+int remo_ring_sync(remo_ring* ring);
 
 /* ---- device memory plumbing -------------------------------------------- */
 int remo_dev_alloc(size_t bytes, void** out);
 int remo_dev_free(void* p);
+/* Pinned (page-locked) host memory, so ring pushes from it are asynchronous DMA. */
+int remo_host_alloc(size_t bytes, void** out);
+int remo_host_free(void* p);
 int remo_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int remo_memcpy_d2h(void* dst, const void* src, size_t bytes);
 /* Device-to-device copy by a plain streaming kernel (remo_copy_kernel): the

@@ -115,11 +115,20 @@ struct remo_ctx {
     bool needsReset = true;
     uint32_t batchletIndex = 0;
     uint32_t batchesTotal = 0;        // num_batches_total of the last build_step's ring
+    hipEvent_t built = nullptr;       // recorded after each build_step's last kernel
     std::vector<void*> allocations;
     AnalysisState analysis;
     AccumWork accWork;
     Profiler prof;
 };
+
+// A slot's size and the producer cursor, written on the copy stream after the
+// slot's data (so a consumer that sees the cursor sees the data).
+__global__ void kRingMark(uint32_t* sizes, uint32_t slot, uint32_t n, uint32_t* uploaded, uint32_t cursor) {
+    sizes[slot] = n;
+    if (*uploaded < cursor) *uploaded = cursor;
+}
+constexpr uint32_t kNoBatch = 0xFFFFFFFFu;
 
 struct remo_ring {
     remo_point* slots = nullptr;
@@ -127,6 +136,10 @@ struct remo_ring {
     uint32_t* uploaded = nullptr;
     uint32_t slotCapacity = 0, numSlots = 0, hostUploaded = 0;
     std::vector<uint32_t> hostSizes;
+    // asynchronous pushes: a copy stream, and per slot the event after its last push
+    hipStream_t copy = nullptr;
+    std::vector<hipEvent_t> slotEvent;
+    std::vector<uint32_t> slotBatch;  // batch index last pushed into each slot (kNoBatch: none)
 };
 
 namespace {
@@ -180,6 +193,8 @@ void releaseAll(remo_ctx* c) {
     c->allocations.clear();
     if (c->hostG) (void)hipHostFree(c->hostG);
     c->hostG = nullptr;
+    if (c->built) (void)hipEventDestroy(c->built);
+    c->built = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -294,6 +309,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     c->shape.nodeBlocks = std::max(8u, (cus * 2u) & ~7u);
 
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->built, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&c->hostG), sizeof(Globals) + 64, hipHostMallocDefault) !=
             hipSuccess) {
         releaseAll(c);
@@ -497,6 +513,8 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
     // read (remo_get_stats): nothing on the path consumes it. The consumed-batch
     // cursor is already known here: every batch either ran or hit `stop`, which
     // the round readback reported.
+    // a producer pushing into a slot this step read waits for this event
+    CK(hipEventRecord(c->built, c->stream), "build event");
     if (consumed) *consumed = c->batchletIndex;
     return (v->num_batches_total > 0 && c->batchletIndex >= v->num_batches_total) ? REMO_DONE : REMO_OK;
 }
@@ -849,6 +867,12 @@ int remo_ring_create(uint32_t slotCapacity, uint32_t numSlots, remo_ring** out) 
 
 void remo_ring_destroy(remo_ring* r) {
     if (!r) return;
+    if (r->copy) {
+        (void)hipStreamSynchronize(r->copy);
+        for (hipEvent_t e : r->slotEvent)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(r->copy);
+    }
     if (r->slots) (void)hipFree(r->slots);
     if (r->sizes) (void)hipFree(r->sizes);
     if (r->uploaded) (void)hipFree(r->uploaded);
@@ -883,8 +907,56 @@ int remo_ring_view(const remo_ring* r, uint32_t numBatchesTotal, remo_batch_view
     return REMO_OK;
 }
 
+int remo_ring_push(remo_ring* r, uint32_t batchIndex, const remo_point* host, uint32_t n, remo_ctx* consumer) {
+    if (!r || (!host && n) || n > r->slotCapacity) return REMO_ERR_INVALID;
+    if (!r->copy) {
+        if (hipStreamCreateWithFlags(&r->copy, hipStreamNonBlocking) != hipSuccess) return REMO_ERR_DEVICE;
+        r->slotEvent.assign(r->numSlots, nullptr);
+        r->slotBatch.assign(r->numSlots, kNoBatch);
+        for (auto& e : r->slotEvent)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return REMO_ERR_DEVICE;
+    }
+    const uint32_t slot = batchIndex % r->numSlots;
+    // the slot's previous batch may still be read by the consumer's last build_step
+    if (r->slotBatch[slot] != kNoBatch && consumer && consumer->built &&
+        hipStreamWaitEvent(r->copy, consumer->built, 0) != hipSuccess)
+        return REMO_ERR_DEVICE;
+    if (n && hipMemcpyAsync(r->slots + uint64_t(slot) * r->slotCapacity, host, sizeof(remo_point) * uint64_t(n),
+                            hipMemcpyHostToDevice, r->copy) != hipSuccess)
+        return REMO_ERR_DEVICE;
+    r->hostSizes[slot] = n;
+    r->hostUploaded = std::max(r->hostUploaded, batchIndex + 1);
+    hipLaunchKernelGGL(kRingMark, dim3(1), dim3(1), 0, r->copy, r->sizes, slot, n, r->uploaded, batchIndex + 1);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(r->slotEvent[slot], r->copy) != hipSuccess)
+        return REMO_ERR_DEVICE;
+    r->slotBatch[slot] = batchIndex;
+    return REMO_OK;
+}
+
+int remo_ring_order_before(remo_ring* r, remo_ctx* consumer, uint32_t firstBatch, uint32_t numBatches) {
+    if (!r || !consumer) return REMO_ERR_INVALID;
+    if (!r->copy) return REMO_OK;  // nothing was pushed asynchronously
+    for (uint32_t k = 0; k < std::min(numBatches, r->numSlots); ++k) {
+        const uint32_t b = firstBatch + k;
+        const uint32_t slot = b % r->numSlots;
+        if (r->slotBatch[slot] != b) continue;  // not pushed (yet)
+        if (hipStreamWaitEvent(consumer->stream, r->slotEvent[slot], 0) != hipSuccess) return REMO_ERR_DEVICE;
+    }
+    return REMO_OK;
+}
+
+int remo_ring_sync(remo_ring* r) {
+    if (!r) return REMO_ERR_INVALID;
+    if (r->copy && hipStreamSynchronize(r->copy) != hipSuccess) return REMO_ERR_DEVICE;
+    return REMO_OK;
+}
+
 int remo_ring_rewind(remo_ring* r) {
     if (!r) return REMO_ERR_INVALID;
+    if (r->copy) {
+        if (hipStreamSynchronize(r->copy) != hipSuccess) return REMO_ERR_DEVICE;
+        r->slotBatch.assign(r->numSlots, kNoBatch);
+    }
     r->hostUploaded = 0;
     return hipMemset(r->uploaded, 0, 4) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
 }
@@ -897,6 +969,14 @@ int remo_dev_alloc(size_t bytes, void** out) {
 }
 
 int remo_dev_free(void* p) { return hipFree(p) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE; }
+
+int remo_host_alloc(size_t bytes, void** out) {
+    if (!out) return REMO_ERR_INVALID;
+    return hipHostMalloc(out, std::max<size_t>(bytes, 16), hipHostMallocDefault) == hipSuccess ? REMO_OK
+                                                                                               : REMO_ERR_OOM;
+}
+
+int remo_host_free(void* p) { return hipHostFree(p) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE; }
 
 int remo_memcpy_h2d(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;

@@ -184,6 +184,11 @@ def load() -> C.CDLL:
         "remo_ring_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
         "remo_ring_view": (C.c_int, [C.c_void_p, C.c_uint32, P(BatchView)]),
         "remo_ring_rewind": (C.c_int, [C.c_void_p]),
+        "remo_ring_push": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
+        "remo_ring_order_before": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+        "remo_ring_sync": (C.c_int, [C.c_void_p]),
+        "remo_host_alloc": (C.c_int, [C.c_size_t, P(C.c_void_p)]),
+        "remo_host_free": (C.c_int, [C.c_void_p]),
         "remo_dev_alloc": (C.c_int, [C.c_size_t, P(C.c_void_p)]),
         "remo_dev_free": (C.c_int, [C.c_void_p]),
         "remo_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

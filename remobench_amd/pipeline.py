@@ -72,6 +72,7 @@ class PointRing:
 
     def close(self):
         if self.h:
+            self._free_pinned()
             self.lib.remo_ring_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -85,6 +86,7 @@ class PointRing:
         self._points = np.ascontiguousarray(points, dtype=L.POINT_DTYPE)
         n = len(self._points)
         self._sizes = [min(batch_size, n - s) for s in range(0, n, batch_size)]
+        self._batch = batch_size
         self.num_batches_total = len(self._sizes)
         self.rewind()
 
@@ -110,6 +112,46 @@ class PointRing:
 
     def set_batches_consumed(self, n: int):
         self.consumed = n
+
+    # ---- asynchronous streaming from pinned host memory ----------------------
+
+    def set_cloud_pinned(self, points: np.ndarray, batch_size: int):
+        """Like set_cloud, but copies the cloud once into pinned host memory, from
+        which push_async() streams batches on the ring's copy stream."""
+        self.set_cloud(points, batch_size)
+        self._free_pinned()
+        nbytes = self._points.nbytes
+        ptr = C.c_void_p()
+        L.check(self.lib.remo_host_alloc(nbytes, C.byref(ptr)), "pinned host memory")
+        self._pinned = ptr
+        buf = (C.c_uint8 * nbytes).from_address(ptr.value)
+        view = np.frombuffer(buf, dtype=L.POINT_DTYPE, count=len(self._points))
+        view[:] = self._points
+        self._points = view
+
+    def _free_pinned(self):
+        if getattr(self, "_pinned", None):
+            self.lib.remo_ring_sync(self.h)
+            self.lib.remo_host_free(self._pinned)
+            self._pinned = None
+
+    def push_async(self, consumer=None):
+        """Queues every batch the consumer window allows on the copy stream
+        (PointSource::pump without waiting); consumer: the Tree reading the ring."""
+        limit = min(self.consumed + self.num_slots, self.num_batches_total)
+        h = consumer.h if consumer is not None else None
+        while self.produced < limit:
+            b = self.produced
+            n = self._sizes[b]
+            ptr = self._points.ctypes.data + 16 * b * self._batch
+            L.check(self.lib.remo_ring_push(self.h, b, C.c_void_p(ptr), n, h), "push")
+            self.produced += 1
+
+    def order_before(self, consumer, first_batch: int, count: int):
+        L.check(self.lib.remo_ring_order_before(self.h, consumer.h, first_batch, count), "order")
+
+    def sync(self):
+        L.check(self.lib.remo_ring_sync(self.h), "ring sync")
 
     def view(self) -> L.BatchView:
         v = L.BatchView()

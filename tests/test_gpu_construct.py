@@ -120,3 +120,32 @@ def test_step_budget_cuts_steps_short(gpu):
     o = build_oracle(points, meta, 50_000)
     compare_trees(tree, o, size, points)
     assert stats_subset(tree.stats()) == stats_subset(o.stats())
+
+
+@pytest.mark.gpu
+def test_async_streaming_through_a_wrapping_ring(gpu):
+    """Batches pushed from pinned host memory on the ring's copy stream, 4 slots
+    for 12 batches, each step ordered after its batch's copy: the tree of a
+    synchronous single pass."""
+    points, meta = lidar_scene(600_000, seed=9)
+    size = max(meta.box_size)
+    batch, slots = 50_000, 4
+    nb = (len(points) + batch - 1) // batch
+    ring = PointRing(batch, slots)
+    ring.set_cloud_pinned(points, batch)
+    tree = Tree(meta, slot_capacity=batch, ring_slots=slots, batches_per_step=1, step_budget_ms=-1.0)
+    consumed, steps = 0, 0
+    while consumed < nb and steps < 100:
+        ring.push_async(tree)
+        ring.order_before(tree, consumed, 1)
+        rc, consumed = tree.build_step(ring.view())
+        ring.set_batches_consumed(consumed)
+        tree.accumulate(consumed)
+        steps += 1
+    ring.sync()
+    assert consumed == nb
+    o = build_oracle(points, meta, batch, batches_per_step=1)
+    compare_trees(tree, o, size, points)
+    assert stats_subset(tree.stats()) == stats_subset(o.stats())
+    compare_accums(tree, o)
+    ring.close()
