@@ -137,6 +137,9 @@ def main() -> int:
     ap.add_argument("--no-streaming", action="store_true",
                     help="skip the PCIe-inclusive run (cloud in pinned host memory, 50-slot ring)")
     ap.add_argument("--analysis", choices=["analytic", "learned", "off"], default="analytic")
+    ap.add_argument("--refine-budget-ms", type=float, default=2.0,
+                    help="Refinement budget per step of the interleaved refinement pass")
+    ap.add_argument("--no-refine", action="store_true", help="skip the Refinement passes")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run remo_copy_kernel over this many MiB 5 times (the known-size "
                          "reference for HBM byte counters) and exit")
@@ -325,6 +328,52 @@ def main() -> int:
     tree.profile_enable(False)
     L.load().remo_dev_free(fbp)
 
+    # Refinement, colour filtering: one complete pass over the finished tree, an
+    # idle call (nothing dirty), then the same steps again with a budgeted
+    # refinement after each one (the per-frame budget of the 1B-point config)
+    refinement = None
+    if not args.no_refine:
+        tree.sync()
+        t0 = time.perf_counter()
+        full = tree.refine()
+        full_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        idle = tree.refine()
+        idle_ms = (time.perf_counter() - t0) * 1e3
+        refinement = {"full_pass_ms": round(full_ms, 3), "full_pass_device_ms": round(full["device_ms"], 3),
+                      "nodes_refined": full["nodes_refined"], "points_folded": full["points_folded"],
+                      "voxels_written": full["voxels_written"], "levels": full["levels_processed"],
+                      "complete": full["complete"], "skipped": full["nodes_skipped"],
+                      "estimate_ms": round(full["estimate_ms"], 3),
+                      "points_plus_voxels_per_s_g": round((full["points_folded"] + full["voxels_written"]) /
+                                                          max(1e-9, full["device_ms"]) / 1e6, 3),
+                      "idle_call_ms": round(idle_ms, 3), "idle_dirty": idle["nodes_dirty"]}
+        log(f"refinement, full pass: {refinement}")
+        budget = args.refine_budget_ms
+        tree.reset()
+        tree.sync()
+        calls = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+            c0 = time.perf_counter()
+            rs = tree.refine(budget)
+            calls.append(((time.perf_counter() - c0) * 1e3, rs))
+        tree.sync()
+        dt_ref = time.perf_counter() - t0
+        ms = sorted(c[0] for c in calls)
+        drain = 0
+        while not tree.refine(budget)["complete"] and drain < 10_000:
+            drain += 1
+        refinement["interleaved"] = {
+            "budget_ms": budget, "calls": len(calls), "avg_ms": round(sum(ms) / len(ms), 3),
+            "p95_ms": round(ms[int(0.95 * (len(ms) - 1))], 3), "max_ms": round(ms[-1], 3),
+            "avg_device_ms": round(sum(c[1]["device_ms"] for c in calls) / len(calls), 3),
+            "nodes_refined": sum(c[1]["nodes_refined"] for c in calls),
+            "ingest_with_refinement_mpts": round(tree.stats()["num_points_processed"] / dt_ref / 1e6, 2),
+            "calls_to_finish_after_ingest": drain + 1}
+        log(f"refinement, interleaved: {refinement['interleaved']}")
+
     # PCIe-inclusive streaming: the cloud in pinned host memory (one untimed copy,
     # as a reader would leave it), the reference's 50-slot ring refilled on a copy
     # stream while the same steps run; each step waits only for its own batch
@@ -389,6 +438,7 @@ def main() -> int:
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 5),
         "scopes": scopes,
         "rasterize": raster,
+        "refinement": refinement,
         "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points",
                                     "num_voxels", "max_depth", "allocated_bytes_persistent",
                                     "mem_capacity_reached", "node_pool_overflow", "spill_overflow",

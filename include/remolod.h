@@ -298,12 +298,56 @@ int remo_set_score_model(remo_ctx* ctx, const remo_mlp* mlp);
  * rows out[num][13] (training pairs are logged from these). */
 int remo_mlp_features(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* num);
 
+/* ---- Refinement: colour filtering ----------------------------------------
+ * The reference specifies Refinement without code (README.md:81,
+ * wiki/02_KernelPipeline.md:228-242, constraints CLAUDE.md:181-190); this is its
+ * colour-filtering step. Every gridded node's voxel list is rewritten in cell
+ * order (x fastest, then y, then z), one voxel per occupied cell, at the cell
+ * centre voxel sampling uses, coloured with the rounded mean of each RGBA channel
+ * ((sum + n/2) / n) over every point of the node's subtree in that cell; the
+ * node's voxel counts become the occupied-cell count. Results depend only on the
+ * points in the tree, never on arrival order. The cell sums live in a sparse
+ * accumulator per grid, one entry per occupied cell, indexed by the cell's rank
+ * among the grid's set bits; a node's sums are its leaf children's points plus
+ * its inner children's sums, so each pass reads every point once and every
+ * child sum once.
+ *
+ * Incremental: a gridded node is dirty when a leaf below it gained points or its
+ * voxel count changed since it was last refined; a call refines the dirty nodes
+ * level by level from the deepest, up to `max_levels` levels (0: all) and while
+ * the cost model's estimate stays within budget_ms (< 0: no limit, 0: nothing);
+ * the last level may be cut to the nodes that fit (at least one node runs when
+ * budget_ms > 0). What is left stays dirty for the next call. A node is left alone (nodes_skipped) when its voxel list is
+ * incomplete, an inner child could not be refined, or the accumulator store is
+ * full; it stays dirty. Cell sums are exact up to 16,843,009 points in one cell
+ * (cell_overflow reports more). Synchronous: returns after the work ran. */
+typedef struct remo_refine_stats {
+    uint32_t levels_pending;    /* levels holding dirty gridded nodes at the start    */
+    uint32_t levels_processed;  /* of those, refined by this call (deepest first)     */
+    uint32_t nodes_dirty;       /* dirty gridded nodes at the start                   */
+    uint32_t nodes_refined;     /* voxel lists rewritten                              */
+    uint32_t nodes_skipped;     /* processed but left alone (see above)               */
+    uint32_t complete;          /* 1: no dirty node was left for a later call         */
+    uint64_t points_folded;     /* leaf points added into cell sums                   */
+    uint64_t child_sums;        /* inner children whose cell sums were folded         */
+    uint64_t voxels_written;
+    uint32_t missing_cells;     /* points whose cell bit was clear (inconsistent tree) */
+    uint32_t empty_cells;       /* set cells without points (written with colour 0)   */
+    uint32_t arena_full;        /* 1: the accumulator store ran out (reset next call) */
+    uint32_t cell_overflow;     /* 1: a cell held more than 16,843,009 points         */
+    float estimate_ms;          /* the cost model's estimate for the levels processed */
+    float device_ms;            /* measured: first to last kernel of the levels       */
+} remo_refine_stats;
+
+int remo_refine(remo_ctx* ctx, float budget_ms, uint32_t max_levels, remo_refine_stats* out);
+
 /* ---- timing scopes (GpuProfiler / GpuScope, include/remo/GpuProfiler.h:62-158) ----
  * When enabled, every phase launch is bracketed by a pair of HIP events on the
  * context's stream; remo_profile_read harvests finished pairs into per-scope
  * totals. Scope names follow the reference's (src/pipelines/RemolodPipeline.cpp:474-479):
  * remolod.construct.{count,split,voxelSampling,allocChunks,insertPoints,insertVoxels},
- * remolod.accumulate, remolod.analysis, remolod.render, and remolod.frame (remo_frame). */
+ * remolod.accumulate, remolod.analysis, remolod.render, remolod.frame (remo_frame)
+ * and remolod.refine (remo_refine). */
 typedef struct remo_profile_entry {
     char name[48];
     double total_ms;
@@ -311,7 +355,7 @@ typedef struct remo_profile_entry {
 } remo_profile_entry;
 
 /* scope_mask: bit i times scope i in the order listed above (count = bit 0 ...
- * render = bit 8, frame = bit 9); 0 disables, 0xFFFFFFFF times all. */
+ * render = bit 8, frame = bit 9, refine = bit 10); 0 disables, 0xFFFFFFFF times all. */
 int remo_profile_enable(remo_ctx* ctx, uint32_t scope_mask);
 int remo_profile_read(remo_ctx* ctx, remo_profile_entry* out, uint32_t cap, uint32_t* num);
 int remo_profile_clear(remo_ctx* ctx);

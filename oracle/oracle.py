@@ -59,6 +59,7 @@ def load():
         "oracle_render": (None, [C.c_void_p, P(SharedUniforms), C.c_void_p, P(C.c_uint32), P(C.c_uint64)]),
         "oracle_jacobi": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "oracle_morton": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "oracle_refine": (C.c_uint32, [C.c_void_p, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -141,7 +142,17 @@ def _render(self, uniforms):
     return fb, items.value, samples.value
 
 
+def _refine(self, min_level: int = 0):
+    """Colour filtering of every gridded node at level >= min_level; returns
+    (rewritten, skipped, empty cells)."""
+    sk = C.c_uint32(0)
+    em = C.c_uint32(0)
+    n = self.lib.oracle_refine(self.h, int(min_level), C.byref(sk), C.byref(em))
+    return n, sk.value, em.value
+
+
 OracleTree.analyse = _analyse
+OracleTree.refine = _refine
 OracleTree.render = _render
 
 

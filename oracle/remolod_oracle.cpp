@@ -18,7 +18,9 @@
 //   * quantisation                         -- exact rational arithmetic (fractions);
 //   * accumulate sums, Morton watermark    -- numpy textbook sums and a bit-by-bit
 //     interleave; fixed point against float128-free integer re-derivation;
-//   * persistent-byte accounting           -- the closed-form chunk/grid count.
+//   * persistent-byte accounting           -- the closed-form chunk/grid count;
+//   * refinement (colour filtering)        -- a numpy box filter of each node's
+//     region by cell (tests/test_oracle_refine.py).
 // Divergences from the reference, applied identically in the HIP path:
 //   * a split refused by the node pool does not spill its points (the reference
 //     spills them and re-inserts duplicates, remolod_octree.cu:250-321);
@@ -32,6 +34,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "../include/remolod.h"
@@ -276,6 +279,24 @@ struct oracle_tree {
         totSpilledPoints += spilled.size();
     }
 
+    // The voxel position of cell (cx, cy, cz) of node nd (remolod_octree.cu:118-131).
+    remo_point cellCentre(const ONode& nd, uint32_t cx, uint32_t cy, uint32_t cz) const {
+        // nodeSize = octreeSize / pow(2, level) is exact for integer levels
+        volatile float nodeSize = octreeSize / std::ldexp(1.0f, int(nd.level));
+        volatile float minx = (float(nd.X) + 0.0f) * nodeSize + 0.0f;
+        volatile float miny = (float(nd.Y) + 0.0f) * nodeSize + 0.0f;
+        volatile float minz = (float(nd.Z) + 0.0f) * nodeSize + 0.0f;
+        remo_point v;
+        volatile float tx = nodeSize * (float(cx) + 0.5f);
+        volatile float ty = nodeSize * (float(cy) + 0.5f);
+        volatile float tz = nodeSize * (float(cz) + 0.5f);
+        v.x = minx + tx / float(kGridSize);
+        v.y = miny + ty / float(kGridSize);
+        v.z = minz + tz / float(kGridSize);
+        v.color = 0;
+        return v;
+    }
+
     // sampleVoxel (remolod_octree.cu:95-140)
     void sampleVoxel(uint32_t id, const uint32_t q28[3], const remo_point& p) {
         ONode& nd = nodes[id];
@@ -290,18 +311,7 @@ struct oracle_tree {
         if (word & mask) return;
         word |= mask;
         nd.numVoxels += 1;
-        // nodeSize = octreeSize / pow(2, level) is exact for integer levels
-        volatile float nodeSize = octreeSize / std::ldexp(1.0f, int(nd.level));
-        volatile float minx = (float(nd.X) + 0.0f) * nodeSize + 0.0f;
-        volatile float miny = (float(nd.Y) + 0.0f) * nodeSize + 0.0f;
-        volatile float minz = (float(nd.Z) + 0.0f) * nodeSize + 0.0f;
-        remo_point v;
-        volatile float tx = nodeSize * (float(cx) + 0.5f);
-        volatile float ty = nodeSize * (float(cy) + 0.5f);
-        volatile float tz = nodeSize * (float(cz) + 0.5f);
-        v.x = minx + tx / float(kGridSize);
-        v.y = miny + ty / float(kGridSize);
-        v.z = minz + tz / float(kGridSize);
+        remo_point v = cellCentre(nd, cx, cy, cz);
         v.color = p.color;
         backlogVoxels.push_back(v);
         backlogTargets.push_back(id);
@@ -417,6 +427,107 @@ struct oracle_tree {
 
     void analyse(const remo_shared_uniforms& u, uint32_t closureMargin,
                  std::vector<struct oracle_analysis_out>& out);
+
+    // Refinement, colour filtering. The reference specifies it without code
+    // (README.md:81, wiki/02_KernelPipeline.md:228-242); this restates the HIP
+    // path's definition directly, without its hierarchy: for every gridded node at
+    // level >= minLevel, walk every stored point of every leaf and add its colour
+    // into each gridded node on its root path at that node's cell of the point;
+    // then rewrite the node's voxel list in cell order (x fastest, then y, then
+    // z), one voxel per set grid bit at the sampling cell centre, coloured
+    // (sum + n/2) / n per RGBA channel, and set numVoxels = numVoxelsStored = the
+    // set-bit count. A node is left alone when numVoxelsStored != numVoxels, when
+    // numVoxels is below the set-bit count, when a point's cell bit is clear, or
+    // when an inner child (refined in the same pass, deeper first) was left alone
+    // or has no grid. Returns the number of nodes rewritten.
+    uint32_t refine(uint32_t minLevel, uint32_t* skippedOut, uint32_t* emptyOut) {
+        struct Sum {
+            uint64_t c[4] = {0, 0, 0, 0};
+            uint64_t n = 0;
+        };
+        const size_t N = nodes.size();
+        std::vector<int64_t> parent(N, -1);
+        for (size_t i = 0; i < N; ++i)
+            for (int c : nodes[i].children)
+                if (c >= 0) parent[size_t(c)] = int64_t(i);
+        // cell sums per gridded node, keyed by the 21-bit cell index
+        std::vector<std::unordered_map<uint32_t, Sum>> sums(N);
+        std::vector<char> missing(N, 0);
+        auto cellOf = [&](const uint32_t q28[3], uint32_t level) {
+            const uint32_t div = 1u << ((kMaxDepth + 1) - level);
+            return ((q28[0] / div) % kGridSize) + ((q28[1] / div) % kGridSize) * kGridSize +
+                   ((q28[2] / div) % kGridSize) * kGridSize * kGridSize;
+        };
+        for (size_t i = 0; i < N; ++i) {
+            const ONode& leaf = nodes[i];
+            if (!leaf.isLeaf()) continue;
+            for (const remo_point& p : leaf.points) {
+                uint32_t q20[3], q28[3];
+                quantise(p, octreeSize, q20, q28);
+                for (int64_t x = int64_t(i); x >= 0; x = parent[size_t(x)]) {
+                    const ONode& a = nodes[size_t(x)];
+                    if (a.grid < 0 || a.level < minLevel) continue;
+                    const uint32_t vi = cellOf(q28, a.level);
+                    if (!(grids[a.grid][vi / 32] & (1u << (vi % 32)))) {
+                        missing[size_t(x)] = 1;
+                        continue;
+                    }
+                    Sum& s = sums[size_t(x)][vi];
+                    for (int ch = 0; ch < 4; ++ch) s.c[ch] += (p.color >> (8 * ch)) & 0xFFu;
+                    s.n += 1;
+                }
+            }
+        }
+        uint32_t maxLevel = 0;
+        for (const ONode& nd : nodes) maxLevel = std::max(maxLevel, nd.level);
+        std::vector<char> done(N, 0), skipped(N, 0);
+        uint32_t refined = 0, nSkipped = 0, empty = 0;
+        for (int L = int(maxLevel); L >= int(minLevel); --L) {
+            for (size_t i = 0; i < N; ++i) {
+                ONode& nd = nodes[i];
+                if (nd.grid < 0 || int(nd.level) != L) continue;
+                const std::vector<uint32_t>& g = grids[nd.grid];
+                uint32_t cells = 0;
+                for (uint32_t w : g) cells += uint32_t(__builtin_popcount(w));
+                bool ok = nd.numVoxelsStored == nd.numVoxels && nd.numVoxels >= cells && !missing[i];
+                for (int c : nd.children)
+                    if (c >= 0 && !nodes[size_t(c)].isLeaf() &&
+                        (nodes[size_t(c)].grid < 0 || skipped[size_t(c)] || !done[size_t(c)]))
+                        ok = false;
+                if (!ok) {
+                    skipped[i] = 1;
+                    ++nSkipped;
+                    continue;
+                }
+                std::vector<remo_point> out;
+                out.reserve(cells);
+                for (uint32_t vi = 0; vi < kGridWords * 32; ++vi) {
+                    if (!(g[vi / 32] & (1u << (vi % 32)))) continue;
+                    remo_point v = cellCentre(nd, vi % kGridSize, (vi / kGridSize) % kGridSize,
+                                              vi / (kGridSize * kGridSize));
+                    const auto it = sums[i].find(vi);
+                    const Sum* s = it == sums[i].end() ? nullptr : &it->second;
+                    if (!s || s->n == 0) {
+                        ++empty;
+                    } else {
+                        uint32_t color = 0;
+                        for (int ch = 0; ch < 4; ++ch)
+                            color |= uint32_t(((s->c[ch] + s->n / 2) / s->n) & 0xFFu) << (8 * ch);
+                        v.color = color;
+                    }
+                    out.push_back(v);
+                }
+                nd.voxels.swap(out);
+                nd.numVoxels = nd.numVoxelsStored = cells;
+                done[i] = 1;
+                ++refined;
+            }
+        }
+        sweepStats();
+        if (skippedOut) *skippedOut = nSkipped;
+        if (emptyOut) *emptyOut = empty;
+        return refined;
+    }
 
     void render(const remo_shared_uniforms& u, uint64_t* fb, uint32_t* numItems, uint64_t* numSamples,
                 uint32_t capacity);
@@ -1009,6 +1120,10 @@ void oracle_analyse(oracle_tree* t, const remo_shared_uniforms* u, uint32_t clos
     std::vector<oracle_analysis_out> v;
     t->analyse(*u, closureMargin, v);
     std::memcpy(out, v.data(), v.size() * sizeof(oracle_analysis_out));
+}
+
+uint32_t oracle_refine(oracle_tree* t, uint32_t minLevel, uint32_t* skipped, uint32_t* empty) {
+    return t->refine(minLevel, skipped, empty);
 }
 
 void oracle_render(oracle_tree* t, const remo_shared_uniforms* u, uint64_t* fb, uint32_t* numItems,

@@ -18,7 +18,7 @@ LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libremolod.so"
 
 HIP_SOURCES = ["construct.hip", "accumulate.hip", "analysis.hip", "mlp.hip",
-               "context.hip"]
+               "refine.hip", "context.hip"]
 CPP_SOURCES = ["scene.cpp"]
 
 # gfx950 only; IEEE division and no FMA contraction (the quantisation and the

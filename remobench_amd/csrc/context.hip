@@ -19,6 +19,7 @@
 #include "remolod_internal.h"
 #include "remolod_analysis.h"
 #include "remolod_mlp.h"
+#include "remolod_refine.h"
 
 using namespace remo_hip;
 
@@ -26,13 +27,13 @@ namespace {
 
 enum ScopeId : int {
     kScopeCount, kScopeSplit, kScopeVoxel, kScopeAlloc, kScopeInsertPoints, kScopeInsertVoxels,
-    kScopeAccumulate, kScopeAnalysis, kScopeRender, kScopeFrame, kNumScopes
+    kScopeAccumulate, kScopeAnalysis, kScopeRender, kScopeFrame, kScopeRefine, kNumScopes
 };
 const char* const kScopeNames[kNumScopes] = {
     "remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
     "remolod.construct.allocChunks", "remolod.construct.insertPoints",
     "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
-    "remolod.frame",
+    "remolod.frame", "remolod.refine",
 };
 
 // HIP-event pairs per launch, harvested after the stream has passed them.
@@ -118,6 +119,7 @@ struct remo_ctx {
     hipEvent_t built = nullptr;       // recorded after each build_step's last kernel
     std::vector<void*> allocations;
     AnalysisState analysis;
+    RefineState refine;
     AccumWork accWork;
     Profiler prof;
 };
@@ -401,6 +403,7 @@ void remo_destroy(remo_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->prof.destroy();
     analysisRelease(c->analysis);
+    refineRelease(c->refine);
     releaseAll(c);
     delete c;
 }
@@ -422,6 +425,7 @@ int remo_reset(remo_ctx* c) {
     CK(hipMemsetAsync(t.accums, 0, sizeof(remo_node_accum) * uint64_t(t.maxNodes), c->stream), "reset");
     CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
     launchResetTree(t, c->stream);
+    refineReset(c->refine, t, c->stream);
     CK(hipGetLastError(), "reset kernel");
     const int rc = readGlobals(c);
     if (rc != REMO_OK) return rc;
@@ -543,6 +547,69 @@ int remo_analysis(remo_ctx* c, const remo_shared_uniforms* u, int mode) {
     }
     if (rc != REMO_OK) return rc;
     CK(hipGetLastError(), "analysis kernels");
+    return REMO_OK;
+}
+
+int remo_refine(remo_ctx* c, float budgetMs, uint32_t maxLevels, remo_refine_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    std::memset(out, 0, sizeof(*out));
+    RefineState& r = c->refine;
+    // The sums store: two entries per voxel created so far (a node takes 1.125
+    // entries per occupied cell, and new ones when it outgrows them). When sums
+    // were handed out past three quarters of it, or it ran out, it is replaced and
+    // every node is recomputed.
+    const bool fresh = !r.allocated;
+    const bool compact = !fresh && (r.host->arenaFull || r.host->arenaUsed > r.accEntries / 4 * 3);
+    if (fresh || compact) {
+        int rc = readGlobals(c);
+        if (rc != REMO_OK) return rc;
+        const uint64_t v = c->hostG->totNewVoxels;
+        const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(1ull << 22, 2 * v + (1ull << 20)),
+                                                 fresh ? 0ull : r.accEntries);
+        if (fresh) {
+            rc = refineAllocate(r, c->t, want, [&](void** p, uint64_t bytes, const char* what) {
+                uint8_t* q = nullptr;
+                const int e = devAlloc(c, &q, bytes, what);
+                *p = q;
+                return e;
+            }, c->stream);
+            if (rc != 0) {
+                if (c->err.empty()) c->err = "refinement: allocation failed";
+                return rc == -3 ? REMO_ERR_DEVICE : REMO_ERR_OOM;
+            }
+        } else {
+            rc = refineGrow(r, c->t, want, c->stream, &c->err);
+            if (rc != 0) return rc == -3 ? REMO_ERR_DEVICE : REMO_ERR_OOM;
+        }
+    }
+    RefineStatsOut so;
+    int rc;
+    {
+        Scope sc(c->prof, kScopeRefine, c->stream);
+        rc = runRefine(r, c->t, c->hostG->numNodes, budgetMs, maxLevels, c->shape, c->stream, &so, &c->err);
+    }
+    if (rc != 0) {
+        CK(hipGetLastError(), "refinement kernels");
+        CK(hipStreamSynchronize(c->stream), "refinement kernels");
+        return REMO_ERR_DEVICE;
+    }
+    CK(hipGetLastError(), "refinement kernels");
+    out->levels_pending = so.levelsPending;
+    out->levels_processed = so.levelsProcessed;
+    out->nodes_dirty = so.nodesDirty;
+    out->nodes_refined = so.nodesRefined;
+    out->nodes_skipped = so.nodesSkipped;
+    out->complete = so.complete;
+    out->points_folded = so.pointsFolded;
+    out->child_sums = so.childSums;
+    out->voxels_written = so.voxelsWritten;
+    out->missing_cells = so.missingCells;
+    out->empty_cells = so.emptyCells;
+    out->arena_full = so.arenaFull;
+    out->cell_overflow = so.cellOverflow;
+    out->estimate_ms = so.estimateMs;
+    out->device_ms = so.deviceMs;
     return REMO_OK;
 }
 

@@ -176,6 +176,17 @@ bool RemolodHipPipeline::analyse(const FrameContext& frame, int mode, std::strin
     return true;
 }
 
+bool RemolodHipPipeline::refine(float budgetMs, remo_refine_stats* out, std::string* err) {
+    if (!m_ctx || m_needsReset || !out) return false;
+    const int rc = remo_refine(m_ctx, budgetMs, 0, out);
+    if (rc != REMO_OK) {
+        fail("refine", rc);
+        if (err) *err = m_lastError;
+        return false;
+    }
+    return true;
+}
+
 void RemolodHipPipeline::render(const FrameContext& frame) {
     // src/pipelines/RemolodPipeline.cpp:559-601.
     if (!m_ctx || m_needsReset || !frame.targets.framebuffer) return;

@@ -46,6 +46,9 @@ public:
 
     // The per-frame Analysis (no reference code; README.md:80): analytic or learned.
     bool analyse(const FrameContext& frame, int mode, std::string* err);
+    // Refinement's colour filtering within budgetMs (no reference code; README.md:81):
+    // runs only when budget remains, may change nothing.
+    bool refine(float budgetMs, remo_refine_stats* out, std::string* err);
 
     remo_ctx* context() const { return m_ctx; }
     const remo_stats& rawStats() const { return m_raw; }

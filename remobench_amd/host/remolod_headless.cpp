@@ -137,7 +137,9 @@ void usage() {
                  "usage: remolod_headless [--points N] [--seed S] [--slot N] [--batches-per-step N]\n"
                  "                        [--budget-ms MS] [--analysis none|analytic] [--width W --height H]\n"
                  "                        [--white] [--no-accum] [--profile] [--device-budget-mb MB]\n"
-                 "                        [--dump-uniforms FILE] [--dump-fb FILE]\n");
+                 "                        [--dump-uniforms FILE] [--dump-fb FILE]\n"
+                 "                        [--refine-budget-ms MS] (Refinement each frame; then one\n"
+                 "                        unbudgeted pass before the final frame)\n");
 }
 
 }  // namespace
@@ -149,6 +151,7 @@ int main(int argc, char** argv) {
     int width = 1920, height = 1080;
     bool white = false, accumulate = true, profile = false;
     std::string analysis = "analytic", dumpUniforms, dumpFb;
+    float refineBudgetMs = -2.0f;  // below -1: no Refinement
     uint64_t deviceBudgetMb = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -173,6 +176,7 @@ int main(int argc, char** argv) {
         else if (a == "--device-budget-mb") deviceBudgetMb = std::strtoull(next(), nullptr, 10);
         else if (a == "--dump-uniforms") dumpUniforms = next();
         else if (a == "--dump-fb") dumpFb = next();
+        else if (a == "--refine-budget-ms") refineBudgetMs = std::strtof(next(), nullptr);
         else if (a == "--help" || a == "-h") {
             usage();
             return 0;
@@ -237,13 +241,23 @@ int main(int argc, char** argv) {
     frame.profile = profile;
 
     const auto t1 = clock::now();
-    uint32_t frames = 0;
+    uint32_t frames = 0, refineCalls = 0;
+    uint64_t refineNodes = 0;
+    double refineDeviceMs = 0.0;
     for (;;) {
         frame.uniforms.frame_counter = frames;
         pipe->render(frame);
         source->pump();
         const bool more = pipe->build(*source, frame);
         if (more && analysis == "analytic" && pipe->batchesConsumed() > 0) pipe->analyse(frame, REMO_ANALYSIS_ANALYTIC, &err);
+        if (more && refineBudgetMs >= 0.0f && pipe->batchesConsumed() > 0) {
+            remo_refine_stats rs{};
+            if (pipe->refine(refineBudgetMs, &rs, &err)) {
+                ++refineCalls;
+                refineNodes += rs.nodes_refined;
+                refineDeviceMs += rs.device_ms;
+            }
+        }
         ++frames;
         if (!more) break;
         if (frames > 1'000'000) break;
@@ -253,6 +267,11 @@ int main(int argc, char** argv) {
         return 1;
     }
     if (analysis == "analytic") pipe->analyse(frame, REMO_ANALYSIS_ANALYTIC, &err);
+    remo_refine_stats finalRefine{};
+    if (refineBudgetMs >= 0.0f && !pipe->refine(-1.0f, &finalRefine, &err)) {
+        std::fprintf(stderr, "remolod_headless: %s\n", err.c_str());
+        return 1;
+    }
     pipe->render(frame);
     remo_sync(pipe->context());
     const double loopS = std::chrono::duration<double>(clock::now() - t1).count();
@@ -289,6 +308,12 @@ int main(int argc, char** argv) {
                 (unsigned long long)ps.numPointsIngested, ps.maxDepth, s.batchlet_index,
                 (unsigned long long)s.sum_leaf_counts, s.inner_with_sums, ps.numVisibleNodes,
                 (unsigned long long)ps.numVisiblePoints, int(ps.memCapacityReached));
+    if (refineBudgetMs >= 0.0f)
+        std::printf(", \"refine\": {\"budget_ms\": %.3f, \"calls\": %u, \"nodes_refined\": %llu, "
+                    "\"device_ms\": %.3f, \"final_nodes_refined\": %u, \"final_complete\": %u, "
+                    "\"final_skipped\": %u}",
+                    refineBudgetMs, refineCalls, (unsigned long long)refineNodes, refineDeviceMs,
+                    finalRefine.nodes_refined, finalRefine.complete, finalRefine.nodes_skipped);
     std::printf(", \"diagnostics\": [");
     const auto diag = pipe->diagnostics();
     for (size_t i = 0; i < diag.size(); ++i) {

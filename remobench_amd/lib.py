@@ -130,7 +130,7 @@ ANALYSIS_DTYPE = np.dtype([
 SCOPES = ["remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
           "remolod.construct.allocChunks", "remolod.construct.insertPoints",
           "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
-          "remolod.frame"]
+          "remolod.frame", "remolod.refine"]
 PROFILE_ALL = 0xFFFFFFFF
 
 
@@ -141,6 +141,19 @@ class ProfileEntry(C.Structure):
 class TuneEntry(C.Structure):
     _fields_ = [("node_blocks", C.c_uint32), ("raster_blocks", C.c_uint32), ("raster_tile", C.c_uint32),
                 ("ms", C.c_float)]
+
+
+class RefineStats(C.Structure):
+    """remo_refine_stats (Refinement, colour filtering)."""
+    _fields_ = [("levels_pending", C.c_uint32), ("levels_processed", C.c_uint32),
+                ("nodes_dirty", C.c_uint32), ("nodes_refined", C.c_uint32), ("nodes_skipped", C.c_uint32),
+                ("complete", C.c_uint32), ("points_folded", C.c_uint64), ("child_sums", C.c_uint64),
+                ("voxels_written", C.c_uint64), ("missing_cells", C.c_uint32), ("empty_cells", C.c_uint32),
+                ("arena_full", C.c_uint32), ("cell_overflow", C.c_uint32), ("estimate_ms", C.c_float),
+                ("device_ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class SceneSpec(C.Structure):
@@ -214,6 +227,7 @@ def load() -> C.CDLL:
         "remo_autotune_frame": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int, C.c_void_p, C.c_void_p,
                                           C.c_uint32, P(C.c_uint32)]),
         "remo_get_frame_shape": (C.c_int, [C.c_void_p, P(TuneEntry)]),
+        "remo_refine": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, P(RefineStats)]),
         "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),
         "remo_scene_generate": (C.c_int, [P(SceneSpec), C.c_uint64, C.c_uint64, C.c_void_p,

@@ -223,6 +223,13 @@ class Tree:
                                               C.byref(n)), "autotune")
         return [(e.node_blocks, e.raster_blocks, e.raster_tile, e.ms) for e in arr[: n.value]]
 
+    def refine(self, budget_ms: float = -1.0, max_levels: int = 0) -> dict:
+        """Refinement, colour filtering: dirty gridded nodes rewritten deepest
+        level first (budget_ms < 0: no limit; max_levels 0: all levels)."""
+        st = L.RefineStats()
+        self._ck(self.lib.remo_refine(self.h, float(budget_ms), int(max_levels), C.byref(st)), "refine")
+        return st.as_dict()
+
     def frame_shape(self):
         e = L.TuneEntry()
         self._ck(self.lib.remo_get_frame_shape(self.h, C.byref(e)), "frame shape")

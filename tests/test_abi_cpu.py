@@ -5,6 +5,7 @@ from __future__ import annotations
 
 import ctypes as C
 import re
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -37,6 +38,24 @@ def test_struct_sizes_match_the_header():
     assert L.NODE_DTYPE.itemsize == 48
     assert L.ACCUM_DTYPE.itemsize == 72 + 72 + 24 + 16 + 8
     assert L.ANALYSIS_DTYPE.itemsize == 48
+
+
+@pytest.mark.parametrize("cname,mirror", [("remo_refine_stats", "RefineStats"), ("remo_stats", "Stats"),
+                                           ("remo_tune_entry", "TuneEntry"), ("remo_config", "Config")])
+def test_ctypes_mirrors_match_the_c_layout(cname, mirror, tmp_path):
+    """Every field offset and the size of each ctypes mirror, against what a C
+    compiler lays out from include/remolod.h."""
+    cls = getattr(L, mirror)
+    lines = [f'printf("%zu\\n", sizeof({cname}));']
+    lines += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f, _ in cls._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "remolod.h"\nint main(void) {\n' +
+                   "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == C.sizeof(cls)
+    assert got[1:] == [getattr(cls, f).offset for f, _ in cls._fields_]
 
 
 def test_scene_is_deterministic_across_thread_counts():

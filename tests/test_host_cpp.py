@@ -78,3 +78,33 @@ def test_headless_frame_loop_matches_the_oracle(gpu, tmp_path):
     assert out["visible_nodes"] == items and out["visible_samples"] == samples
     assert np.array_equal(got, want)
     o.close()
+
+
+@pytest.mark.gpu
+def test_headless_with_refinement_renders_colour_bit_exact(gpu, tmp_path):
+    """The same loop with Refinement each frame (0.5 ms budget) and one full pass
+    before the final frame: voxel colours are then box-filtered and deterministic,
+    so the coloured framebuffer equals the oracle's after its refinement."""
+    n, slot, seed = 2_097_152, 262_144, 3
+    uf, fbf = tmp_path / "u.bin", tmp_path / "fb.bin"
+    r = _run("--points", n, "--seed", seed, "--slot", slot, "--budget-ms", -1, "--batches-per-step", 2,
+             "--refine-budget-ms", 0.5, "--dump-uniforms", uf, "--dump-fb", fbf)
+    assert r.returncode == 0, r.stderr.decode()
+    out = json.loads(r.stdout.decode().strip().splitlines()[-1])
+    ref = out["refine"]
+    assert ref["calls"] >= 3 and ref["nodes_refined"] > 0
+    assert ref["final_complete"] == 1 and ref["final_skipped"] == 0
+
+    points, meta = L.generate_scene(n, seed, "lidar")
+    slots, sizes, nb = batch_arrays(points, slot)
+    o = OracleTree(meta.box_size)
+    o.build_step(slots, sizes, nb, slot, nb)
+    refined, skipped, empty = o.refine(0)
+    assert skipped == 0 and empty == 0
+    assert out["num_voxels"] == o.stats()["num_voxels"]
+    u = L.SharedUniforms.from_buffer_copy(uf.read_bytes())
+    want, items, samples = o.render(u)
+    got = np.fromfile(fbf, dtype=np.uint64)
+    assert out["visible_samples"] == samples
+    assert np.array_equal(got, want)
+    o.close()
