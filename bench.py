@@ -140,6 +140,8 @@ def main() -> int:
     ap.add_argument("--refine-budget-ms", type=float, default=2.0,
                     help="Refinement budget per step of the interleaved refinement pass")
     ap.add_argument("--no-refine", action="store_true", help="skip the Refinement passes")
+    ap.add_argument("--fixed-depth", type=int, default=0,
+                    help="the analytic-bucketing expand arm at this depth (0: the iterative expand)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run remo_copy_kernel over this many MiB 5 times (the known-size "
                          "reference for HBM byte counters) and exit")
@@ -190,7 +192,7 @@ def main() -> int:
     upload_s = time.perf_counter() - t_up
     view = ring.view()
     tree = Tree(meta, slot_capacity=args.batch, ring_slots=nb, batches_per_step=1,
-                step_budget_ms=-1.0)
+                step_budget_ms=-1.0, fixed_depth=args.fixed_depth)
     log(f"uploaded; tree context created ({time.perf_counter() - t:.1f} s)")
 
     uniforms = L.SharedUniforms()
@@ -277,7 +279,8 @@ def main() -> int:
     workload = (f"{args.scene}-{len(points) // 1_000_000}M: {len(points):,} points in {nb} "
                 f"batches of {args.batch:,}; Update+Accumulate"
                 + ("+Analysis (analytic)" if analysis_ok and args.analysis == "analytic" else
-                   "+Analysis (learned)" if analysis_ok else ""))
+                   "+Analysis (learned)" if analysis_ok else "")
+                + (f"; fixed-depth expand D={args.fixed_depth}" if args.fixed_depth else ""))
     roofline = None
     if dominant:
         d = scopes[dominant]
@@ -382,7 +385,8 @@ def main() -> int:
         depth = min(50, nb)
         sring = PointRing(args.batch, depth)
         sring.set_cloud_pinned(points, args.batch)
-        stree = Tree(meta, slot_capacity=args.batch, ring_slots=depth, batches_per_step=1, step_budget_ms=-1.0)
+        stree = Tree(meta, slot_capacity=args.batch, ring_slots=depth, batches_per_step=1, step_budget_ms=-1.0,
+                     fixed_depth=args.fixed_depth)
         stree.sync()
         t0 = time.perf_counter()
         consumed = 0
@@ -426,6 +430,7 @@ def main() -> int:
         "config": {
             "workload": workload,
             "points": len(points), "batch": args.batch, "batches_timed": steps,
+            "expand": f"fixed depth {args.fixed_depth}" if args.fixed_depth else "iterative",
             "analysis": args.analysis if analysis_ok else "unavailable",
         },
         "roofline": roofline,
@@ -439,7 +444,8 @@ def main() -> int:
         "scopes": scopes,
         "rasterize": raster,
         "refinement": refinement,
-        "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points",
+        "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "max_points_per_node",
+                                    "total_expand_iters", "total_spilled_points",
                                     "num_voxels", "max_depth", "allocated_bytes_persistent",
                                     "mem_capacity_reached", "node_pool_overflow", "spill_overflow",
                                     "backlog_overflow", "dir_overflow")},

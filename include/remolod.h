@@ -114,7 +114,12 @@ typedef struct remo_config {
     uint32_t closure_batches;       /* Analysis: a leaf untouched for this many batches is
                                        closed (8); the reference's named alternative to a
                                        Morton watermark, plans/03_AccumulatorHook.md:507-509 */
+    uint32_t fixed_depth;           /* 0: the iterative expand; 1..8: the analytic-bucketing
+                                       arm that pre-splits to this depth
+                                       (--remolod-fixed-depth, remolod_octree.cu:464-675) */
 } remo_config;
+
+#define REMO_MAX_FIXED_DEPTH 8u
 
 #define REMO_FLAG_NO_ACCUM   1u     /* --remolod-no-accum                       */
 

@@ -41,7 +41,7 @@ def load():
     lib = C.CDLL(str(LIB_PATH))
     P = C.POINTER
     sig = {
-        "oracle_create": (C.c_void_p, [P(C.c_float), C.c_uint32, C.c_uint64, C.c_uint32]),
+        "oracle_create": (C.c_void_p, [P(C.c_float), C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
         "oracle_destroy": (None, [C.c_void_p]),
         "oracle_reset": (None, [C.c_void_p]),
         "oracle_build_step": (C.c_uint32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -73,10 +73,10 @@ def load():
 class OracleTree:
     """The reference's construct/accumulate path, restated on the CPU."""
 
-    def __init__(self, box_size, max_nodes=200_000, persistent_bytes=1 << 62, batches_per_step=20):
+    def __init__(self, box_size, max_nodes=200_000, persistent_bytes=1 << 62, batches_per_step=20, fixed_depth=0):
         self.lib = load()
         bs = (C.c_float * 3)(*[float(v) for v in box_size])
-        self.h = self.lib.oracle_create(bs, max_nodes, persistent_bytes, batches_per_step)
+        self.h = self.lib.oracle_create(bs, max_nodes, persistent_bytes, batches_per_step, fixed_depth)
 
     def close(self):
         if self.h:

@@ -116,6 +116,7 @@ struct oracle_tree {
     uint32_t maxNodes;
     uint64_t persistentCapacity;
     uint32_t batchesPerStep;
+    uint32_t fixedDepth = 0;  // 0: iterative expand; else the analytic-bucketing arm
 
     std::vector<ONode> nodes;
     std::vector<std::vector<uint32_t>> grids;
@@ -297,6 +298,72 @@ struct oracle_tree {
         return v;
     }
 
+    // fixedMorton (remolod_octree.cu:469-481)
+    static uint32_t fixedMorton(uint32_t x, uint32_t y, uint32_t z, uint32_t level) {
+        uint32_t code = 0;
+        for (uint32_t l = 0; l < level; ++l) {
+            const uint32_t bit = level - 1 - l;
+            code = (code << 3) | (((x >> bit) & 1u) << 2) | (((y >> bit) & 1u) << 1) | ((z >> bit) & 1u);
+        }
+        return code;
+    }
+
+    // expandFixed (remolod_octree.cu:464-675): bucket the batch at the fixed
+    // depth D, sum the pyramid, split level by level every leaf above D whose
+    // cell holds points, seed the leaves' counters. One expand iteration, no
+    // spill. As in the HIP path, a leaf above D that holds points (its split was
+    // refused earlier) is not split: the reference's arm would drop its points.
+    void expandFixed(const remo_point* pts, uint32_t n) {
+        const uint32_t D = fixedDepth;
+        std::vector<std::vector<uint32_t>> cells(D + 1);
+        for (uint32_t L = 0; L <= D; ++L) cells[L].assign(size_t(1) << (3 * L), 0u);
+        const uint32_t shift = uint32_t(kMaxDepth) - D, maxCoord = (1u << D) - 1u;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t q20[3], q28[3];
+            quantise(pts[i], octreeSize, q20, q28);
+            cells[D][fixedMorton((q20[0] >> shift) & maxCoord, (q20[1] >> shift) & maxCoord,
+                                 (q20[2] >> shift) & maxCoord, D)] += 1;
+        }
+        for (int L = int(D) - 1; L >= 0; --L)
+            for (size_t c = 0; c < cells[L].size(); ++c) {
+                uint32_t sum = 0;
+                for (int k = 0; k < 8; ++k) sum += cells[L + 1][8 * c + k];
+                cells[L][c] = sum;
+            }
+        for (uint32_t L = 0; L < D; ++L) {
+            spilling.clear();
+            for (uint32_t id = 0; id < nodes.size(); ++id) {
+                const ONode& nd = nodes[id];
+                if (!nd.isLeaf() || nd.level != L || nd.numPoints != 0) continue;
+                if (cells[L][fixedMorton(nd.X, nd.Y, nd.Z, L)] == 0) continue;
+                spilling.push_back(id);
+            }
+            std::sort(spilling.begin(), spilling.end(), [&](uint32_t a, uint32_t b) {
+                const ONode &na = nodes[a], &nb = nodes[b];
+                return Key{na.level, na.X, na.Y, na.Z} < Key{nb.level, nb.X, nb.Y, nb.Z};
+            });
+            std::vector<uint32_t> accepted;
+            uint64_t room = nodes.size();
+            for (uint32_t id : spilling) {
+                if (room + 8 > maxNodes) {
+                    nodePoolOverflow = true;
+                    continue;
+                }
+                room += 8;
+                accepted.push_back(id);
+            }
+            spilling.swap(accepted);
+            if (!spilling.empty()) doSplitting();
+        }
+        spilling.clear();
+        for (ONode& nd : nodes) {
+            if (!nd.isLeaf() || nd.level > D) continue;
+            nd.counter += cells[nd.level][fixedMorton(nd.X, nd.Y, nd.Z, nd.level)];
+        }
+        ++expandIters;
+        ++totExpandIters;
+    }
+
     // sampleVoxel (remolod_octree.cu:95-140)
     void sampleVoxel(uint32_t id, const uint32_t q28[3], const remo_point& p) {
         ONode& nd = nodes[id];
@@ -393,7 +460,10 @@ struct oracle_tree {
         spilled.clear();
         backlogVoxels.clear();
         backlogTargets.clear();
-        expand(pts, n, batchIndex);
+        if (fixedDepth)
+            expandFixed(pts, n);
+        else
+            expand(pts, n, batchIndex);
         voxelSampling(pts, n);
         allocatePointChunks();
         allocateVoxelChunks();
@@ -1050,8 +1120,9 @@ void oracle_tree::render(const remo_shared_uniforms& u, uint64_t* fb, uint32_t* 
 extern "C" {
 
 oracle_tree* oracle_create(const float boxSize[3], uint32_t maxNodes, uint64_t persistentCapacity,
-                           uint32_t batchesPerStep) {
+                           uint32_t batchesPerStep, uint32_t fixedDepth) {
     oracle_tree* t = new oracle_tree();
+    t->fixedDepth = fixedDepth;
     for (int a = 0; a < 3; ++a) t->boxSize[a] = boxSize[a];
     t->octreeSize = std::max(std::max(boxSize[0], boxSize[1]), boxSize[2]);
     t->maxNodes = maxNodes ? maxNodes : REMO_DEFAULT_MAX_NODES;

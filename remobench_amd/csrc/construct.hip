@@ -19,6 +19,8 @@
 // Each phase is its own kernel on one stream; kernel boundaries are the
 // reference's grid.sync()s.
 
+#include <algorithm>
+
 #include "remolod_internal.h"
 
 namespace remo_hip {
@@ -357,8 +359,6 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
             so[0] = so[1] = make_uint4(0, 0, 0, 0);
             uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * c);
             bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
-            uint4* pd = reinterpret_cast<uint4*>(t.pointDir + uint64_t(c) * kPointDir);
-            for (uint32_t k = 0; k < kPointDir / 4; ++k) pd[k] = make_uint4(0, 0, 0, 0);
             remo_node_accum* a = t.accums + c;
             *a = remo_node_accum{};
         }
@@ -418,14 +418,16 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
 // doCounting, remolod_octree.cu:250-285) and zero its grid
 // (remolod_octree.cu:372-387), including a grid the node already had (the
 // root's, from reset). Blocks [0, numAccepted) copy; all blocks zero.
-__global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t) {
+__global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t, uint32_t countRound) {
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t na = g->numAccepted;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         g->numNodes += 8u * na;
-        g->expandIters += 1;
-        g->totExpandIters += 1;
+        if (countRound) {
+            g->expandIters += 1;
+            g->totExpandIters += 1;
+        }
     }
     // one (accepted node, chunk) pair per block step, so a large node's copy
     // spreads over many workgroups
@@ -448,6 +450,124 @@ __global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t) {
         const uint32_t gi = t.gridNo[node];
         if (gi == kNone) continue;
         reinterpret_cast<uint4*>(t.arena + t.gridOffset[gi])[i % perGrid] = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The analytic-bucketing expand arm (expandFixed, remolod_octree.cu:464-675).
+// With the depth D fixed, a point's destination cell is the top D bits of its
+// 20-bit coordinate (masked, so a point on boxMax wraps to cell 0 as the descent
+// does), so one bucketing pass counts the batch into a dense Morton-ordered
+// array at level D; a pyramid sums it up to the root (the Morton order of
+// childIndex makes a cell's parent cell >> 3); D rounds split every leaf at level
+// L < D whose cell holds points, through the same split kernels as the iterative
+// arm; and every leaf at level <= D adds its cell's count to its counter.
+// Leaves at level D never split, whatever their count. A leaf above D that holds
+// points (one whose split the node pool refused) is not split either: the
+// reference's arm would drop its points.
+
+__device__ __forceinline__ uint32_t fixedLevelOffset(uint32_t level) { return ((1u << (3 * level)) - 1u) / 7u; }
+
+// fixedMorton (remolod_octree.cu:469-481): childIndex order, most significant level first.
+__device__ __forceinline__ uint32_t fixedMorton(uint32_t x, uint32_t y, uint32_t z, uint32_t level) {
+    uint32_t code = 0;
+    for (uint32_t l = 0; l < level; ++l) {
+        const uint32_t bit = level - 1 - l;
+        code = (code << 3) | (((x >> bit) & 1u) << 2) | (((y >> bit) & 1u) << 1) | ((z >> bit) & 1u);
+    }
+    return code;
+}
+
+__global__ __launch_bounds__(kBlock) void kFixedClear(DevTree t, uint32_t entries) {
+    if (t.g->stop) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < entries; i += gridDim.x * blockDim.x)
+        t.cellCount[i] = 0;
+}
+
+// 1. bucket: the batch's points into level-D cells, aggregated per tile in LDS.
+__global__ __launch_bounds__(kBlock) void kFixedBucket(DevTree t, BatchRef b) {
+    __shared__ BlockAgg agg;
+    const uint32_t batchSize = batchSizeOf(t, b);
+    if (t.g->stop) return;
+    const uint32_t D = t.fixedDepth;
+    const uint32_t shift = uint32_t(kMaxDepth) - D, maxCoord = (1u << D) - 1u;
+    const uint32_t base = fixedLevelOffset(D);
+    const TileRange tr = myTiles(batchSize);
+    const float size = t.octreeSize;
+    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        aggClear(agg);
+        __syncthreads();
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            if (i >= batchSize) continue;
+            const remo_point p = streamPoint(t, b, batchSize, i);
+            const uint32_t cx = ((quant28(p.x, size) >> 8) >> shift) & maxCoord;
+            const uint32_t cy = ((quant28(p.y, size) >> 8) >> shift) & maxCoord;
+            const uint32_t cz = ((quant28(p.z, size) >> 8) >> shift) & maxCoord;
+            const uint32_t key = base + fixedMorton(cx, cy, cz, D);
+            uint32_t rank;
+            if (aggInsert(agg, key, &rank) == kEmpty32) atomicAdd(&t.cellCount[key], 1u);
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
+            const uint32_t key = agg.key[k];
+            if (key != kEmpty32) atomicAdd(&t.cellCount[key], agg.cnt[k]);
+        }
+        __syncthreads();
+    }
+}
+
+// 2. pyramid: level `level` from level + 1.
+__global__ __launch_bounds__(kBlock) void kFixedPyramid(DevTree t, uint32_t level) {
+    if (t.g->stop) return;
+    const uint32_t cells = 1u << (3 * level);
+    uint32_t* dst = t.cellCount + fixedLevelOffset(level);
+    const uint32_t* src = t.cellCount + fixedLevelOffset(level + 1);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+        uint32_t sum = 0;
+        for (uint32_t i = 0; i < 8; ++i) sum += src[8u * c + i];
+        dst[c] = sum;
+    }
+}
+
+__global__ void kFixedRound(DevTree t) {
+    t.g->numCandidates = 0;
+    t.g->numAccepted = 0;
+    t.g->numPending = 0;
+}
+
+// 3. materialise, one round: every empty leaf at `level` whose cell holds points.
+__global__ __launch_bounds__(kBlock) void kFixedDecide(DevTree t, uint32_t level) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t n = g->numNodes;
+    const uint32_t base = fixedLevelOffset(level);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (t.firstChild[i] >= 0 || t.level[i] != level || t.numPoints[i] != 0) continue;
+        if (t.cellCount[base + fixedMorton(t.nx[i], t.ny[i], t.nz[i], level)] == 0) continue;
+        const uint32_t slot = atomicAdd(&g->numCandidates, 1u);
+        if (slot < t.candidateCap)
+            t.candidates[slot] = i;
+        else
+            g->candidateOverflow = 1;
+    }
+}
+
+// 4. seed: every leaf at level <= D adds its cell's count (allocatePointChunks'
+// sizing input); one expand iteration per batch.
+__global__ __launch_bounds__(kBlock) void kFixedSeed(DevTree t) {
+    Globals* g = t.g;
+    if (g->stop) return;
+    const uint32_t n = g->numNodes, D = t.fixedDepth;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g->expandIters += 1;
+        g->totExpandIters += 1;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t L = t.level[i];
+        if (t.firstChild[i] >= 0 || L > D) continue;
+        const uint32_t cnt = t.cellCount[fixedLevelOffset(L) + fixedMorton(t.nx[i], t.ny[i], t.nz[i], L)];
+        if (cnt) t.counter[i] += cnt;
     }
 }
 
@@ -920,7 +1040,27 @@ void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
 
 void launchSpillAndZero(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
     (void)ls;
-    hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t, 1u);
+}
+
+void launchFixedExpand(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
+    const uint32_t D = t.fixedDepth;
+    const uint32_t entries = uint32_t(fixedPyramidEntries(D));
+    hipLaunchKernelGGL(kFixedClear, dim3(std::min<uint32_t>(4096, (entries + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, t, entries);
+    hipLaunchKernelGGL(kFixedBucket, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
+    for (int level = int(D) - 1; level >= 0; --level) {
+        const uint32_t cells = 1u << (3 * level);
+        hipLaunchKernelGGL(kFixedPyramid, dim3(std::min<uint32_t>(4096, (cells + kBlock - 1) / kBlock)), dim3(kBlock),
+                           0, s, t, uint32_t(level));
+    }
+    for (uint32_t level = 0; level < D; ++level) {
+        hipLaunchKernelGGL(kFixedRound, dim3(1), dim3(1), 0, s, t);
+        hipLaunchKernelGGL(kFixedDecide, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, level);
+        hipLaunchKernelGGL(kSplitApply, dim3(1), dim3(1024), 0, s, t);
+        hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t, 0u);
+    }
+    hipLaunchKernelGGL(kFixedSeed, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
 }
 
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {

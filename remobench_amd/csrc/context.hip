@@ -286,9 +286,9 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     if (!cfg.persistent_bytes) {
         cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20);
     }
-    if (meta->num_points == 0) {
+    if (meta->num_points == 0 || cfg.fixed_depth > REMO_MAX_FIXED_DEPTH) {
         delete c;
-        return REMO_ERR_INVALID;  // "cloud is empty"
+        return REMO_ERR_INVALID;  // "cloud is empty", or no such fixed depth
     }
     const float size = std::max(std::max(meta->box_size[0], meta->box_size[1]), meta->box_size[2]);
     if (!(size > 0.0f)) {
@@ -336,6 +336,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     t.spillCap = cfg.spill_capacity;
     t.backlogCap = cfg.backlog_capacity;
     t.candidateCap = cfg.max_nodes;
+    t.fixedDepth = cfg.fixed_depth;
     int rc = REMO_OK;
     auto A = [&](auto** p, uint64_t n, const char* what) {
         if (rc == REMO_OK) rc = devAlloc(c, p, n, what);
@@ -368,6 +369,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 3, "accepted splits");
     A(&t.g, 1, "device counters");
     A(&t.accums, N, "the accumulator side array");
+    if (t.fixedDepth) A(&t.cellCount, fixedPyramidEntries(t.fixedDepth), "the fixed-depth cell counts");
     {
         AccumWork& w = c->accWork;
         w.tileCap = N + (uint64_t(cfg.slot_capacity) * cfg.max_batches_per_step + cfg.spill_capacity) /
@@ -423,6 +425,8 @@ int remo_reset(remo_ctx* c) {
     DevTree& t = c->t;
     CK(hipMemsetAsync(t.g, 0, sizeof(Globals), c->stream), "reset");
     CK(hipMemsetAsync(t.accums, 0, sizeof(remo_node_accum) * uint64_t(t.maxNodes), c->stream), "reset");
+    // node indices restart at 0: every chunk directory starts empty (splits rely on it)
+    CK(hipMemsetAsync(t.pointDir, 0, 4ull * kPointDir * uint64_t(t.maxNodes), c->stream), "reset");
     CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
     launchResetTree(t, c->stream);
     refineReset(c->refine, t, c->stream);
@@ -475,8 +479,18 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
         const uint32_t slot = bi % v->num_slots;
         BatchRef b{v->slots + uint64_t(slot) * v->slot_capacity, v->batch_sizes + slot};
         launchBeginBatch(t, b, bi != first, c->budgetTicks, s);
+        if (t.fixedDepth) {
+            // the analytic-bucketing arm: one pass over the batch, no re-counting
+            {
+                Scope sc(c->prof, kScopeSplit, s);
+                launchFixedExpand(t, b, c->shape, s);
+            }
+            CK(hipGetLastError(), "fixed-depth expand kernels");
+            int rc = readGlobals(c);
+            if (rc != REMO_OK) return rc;
+        }
         uint32_t round = 0;
-        for (;;) {
+        for (; !t.fixedDepth;) {
             {
                 Scope sc(c->prof, kScopeCount, s);
                 launchCount(t, b, round, c->shape, s);

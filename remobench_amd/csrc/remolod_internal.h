@@ -30,7 +30,7 @@ constexpr uint64_t kChunkAllocBytes = 16032;   // 16 * ((16,016 + 16) / 16)
 constexpr uint64_t kGridAllocBytes = 262160;   // 16 * ((262,144 + 16) / 16)
 constexpr uint64_t kArenaFirstOffset = 16;
 constexpr uint64_t kSafetyMargin = 200000000ull;  // remolod_octree.cu:1195
-constexpr uint32_t kPointDir = 64;     // chunk slots per leaf: 64,000 points
+constexpr uint32_t kPointDir = 256;    // chunk slots per leaf: 256,000 points (fixed-depth leaves exceed 50,000)
 constexpr uint32_t kVoxelDir = 2112;   // >= ceil(2^21 / 1000) chunk slots per grid
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRounds = 20;    // the expand loop bound, remolod_octree.cu:411
@@ -110,6 +110,8 @@ struct DevTree {
     uint32_t maxNodes;
     uint32_t maxGrids;
     float octreeSize;
+    uint32_t fixedDepth;    // 0: iterative expand; else the analytic-bucketing arm's depth
+    uint32_t* cellCount;    // fixed-depth cell-count pyramid, levels 0..fixedDepth
     Globals* g;
     remo_node_accum* accums;
 };
@@ -213,6 +215,11 @@ void launchCount(const DevTree& t, BatchRef b, uint32_t round, const LaunchShape
 void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, hipStream_t s);
 void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchSpillAndZero(const DevTree& t, const LaunchShape& ls, hipStream_t s);
+// The analytic-bucketing expand of one batch (fixed depth): bucket, pyramid,
+// fixedDepth split rounds, counter seeding.
+void launchFixedExpand(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
+// Entries of the fixed-depth cell-count pyramid for depth d (levels 0..d).
+inline uint64_t fixedPyramidEntries(uint32_t d) { return ((1ull << (3 * (d + 1))) - 1ull) / 7ull; }
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);

@@ -54,7 +54,10 @@ bool RemolodHipPipeline::allocate(const CloudMeta& meta, const DeviceBudget& bud
                    std::to_string(available >> 20) + " MB available";
         return false;
     }
-    m_persistentBytes = std::min(want, available - fixed);
+    // The fixed-depth arm pre-splits, and every inner node keeps a 256 KB grid, so
+    // it is not sized by the per-point coefficient: it takes what the budget
+    // leaves (src/pipelines/RemolodPipeline.cpp:137-147).
+    m_persistentBytes = m_opt.fixedDepth ? available - fixed : std::min(want, available - fixed);
 
     remo_config cfg{};
     cfg.persistent_bytes = m_persistentBytes;
@@ -65,6 +68,7 @@ bool RemolodHipPipeline::allocate(const CloudMeta& meta, const DeviceBudget& bud
     cfg.ring_slots = REMO_DEFAULT_RING_SLOTS;
     cfg.flags = m_opt.accumulate ? 0u : REMO_FLAG_NO_ACCUM;
     cfg.closure_batches = m_opt.closureBatches;
+    cfg.fixed_depth = m_opt.fixedDepth;
     const remo_cloud_meta m = meta.toAbi();
     const int rc = remo_create(&cfg, &m, &m_ctx);
     if (rc != REMO_OK) {

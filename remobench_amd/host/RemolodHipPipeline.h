@@ -26,6 +26,7 @@ public:
         int analysisMode = -1;              // -1: no Analysis in build(); else REMO_ANALYSIS_*
         uint32_t closureBatches = 8;
         uint32_t slotCapacity = kSlotCapacity;  // ring stride; the source must use the same
+        uint32_t fixedDepth = 0;            // --remolod-fixed-depth: the analytic-bucketing arm
     };
 
     RemolodHipPipeline() = default;

@@ -138,6 +138,7 @@ void usage() {
                  "                        [--budget-ms MS] [--analysis none|analytic] [--width W --height H]\n"
                  "                        [--white] [--no-accum] [--profile] [--device-budget-mb MB]\n"
                  "                        [--dump-uniforms FILE] [--dump-fb FILE]\n"
+                 "                        [--remolod-fixed-depth D]\n"
                  "                        [--refine-budget-ms MS] (Refinement each frame; then one\n"
                  "                        unbudgeted pass before the final frame)\n");
 }
@@ -152,6 +153,7 @@ int main(int argc, char** argv) {
     bool white = false, accumulate = true, profile = false;
     std::string analysis = "analytic", dumpUniforms, dumpFb;
     float refineBudgetMs = -2.0f;  // below -1: no Refinement
+    uint32_t fixedDepth = 0;
     uint64_t deviceBudgetMb = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -177,6 +179,7 @@ int main(int argc, char** argv) {
         else if (a == "--dump-uniforms") dumpUniforms = next();
         else if (a == "--dump-fb") dumpFb = next();
         else if (a == "--refine-budget-ms") refineBudgetMs = std::strtof(next(), nullptr);
+        else if (a == "--remolod-fixed-depth") fixedDepth = uint32_t(std::strtoul(next(), nullptr, 10));
         else if (a == "--help" || a == "-h") {
             usage();
             return 0;
@@ -220,6 +223,7 @@ int main(int argc, char** argv) {
     opt.batchesPerStep = perStep;
     opt.stepBudgetMs = budgetMs;
     opt.slotCapacity = slot;
+    opt.fixedDepth = fixedDepth;
     remo::PipelineRegistry registry;
     registry.add([opt] { return std::make_unique<remo::RemolodHipPipeline>(opt); });
     std::string err;

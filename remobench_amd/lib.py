@@ -75,7 +75,7 @@ class Config(C.Structure):
                 ("max_batches_per_step", C.c_uint32), ("step_budget_ms", C.c_float),
                 ("slot_capacity", C.c_uint32), ("ring_slots", C.c_uint32),
                 ("spill_capacity", C.c_uint32), ("backlog_capacity", C.c_uint32),
-                ("flags", C.c_uint32), ("closure_batches", C.c_uint32)]
+                ("flags", C.c_uint32), ("closure_batches", C.c_uint32), ("fixed_depth", C.c_uint32)]
 
 
 class Stats(C.Structure):

@@ -164,7 +164,7 @@ class Tree:
 
     def __init__(self, meta: L.CloudMeta, *, max_nodes=0, persistent_bytes=0, batches_per_step=0,
                  step_budget_ms=-1.0, slot_capacity=0, ring_slots=0, spill_capacity=0,
-                 backlog_capacity=0, accumulate=True, closure_batches=0):
+                 backlog_capacity=0, accumulate=True, closure_batches=0, fixed_depth=0):
         self.lib = L.load()
         cfg = L.Config()
         cfg.persistent_bytes = persistent_bytes
@@ -177,6 +177,7 @@ class Tree:
         cfg.backlog_capacity = backlog_capacity
         cfg.flags = 0 if accumulate else L.REMO_FLAG_NO_ACCUM
         cfg.closure_batches = closure_batches
+        cfg.fixed_depth = fixed_depth
         self.cfg = cfg
         self.meta = meta
         self.h = C.c_void_p()
