@@ -335,11 +335,38 @@ def main() -> int:
     # idle call (nothing dirty), then the same steps again with a budgeted
     # refinement after each one (the per-frame budget of the 1B-point config)
     refinement = None
+    image_quality = None
     if not args.no_refine:
+        # image ground truth: the flat (LOD-free) frame of the resident cloud,
+        # against the LOD frame before and after the complete pass below
+        from remobench_amd import image_quality as IQ
+        iq_frames = IQ.Frames(rw, rh)
+        iq_poses = camera.poses(sample_pts, meta.box_size, rw, rh)
+        flat_fb, before_fb = {}, {}
+        tree.profile_enable(True, ["flat.render"])
+        tree.profile_clear()
+        for pose, u in iq_poses.items():
+            flat_fb[pose] = IQ.flat_frame(tree, int(view.slots), len(points), u, iq_frames)
+            before_fb[pose] = IQ.lod_frame(tree, u, iq_frames)
+        flat_ms, flat_n = tree.profile()["flat.render"]
+        tree.profile_enable(False)
         tree.sync()
         t0 = time.perf_counter()
         full = tree.refine()
         full_ms = (time.perf_counter() - t0) * 1e3
+        image_quality = {"flat_render_ms": round(flat_ms / max(1, flat_n), 3), "resolution": f"{rw}x{rh}"}
+        for pose, u in iq_poses.items():
+            after = IQ.lod_frame(tree, u, iq_frames)
+            a = IQ.compare(before_fb[pose], flat_fb[pose], rw, rh)
+            b = IQ.compare(after, flat_fb[pose], rw, rh)
+            image_quality[pose] = {"coverage": round(a["coverage"], 4), "extra": round(a["extra"], 4),
+                                   "depth_rel_p50": a.get("depth_rel_p50"),
+                                   "psnr_rgb_first_come": round(a.get("psnr_rgb", 0.0), 3),
+                                   "psnr_rgb_refined": round(b.get("psnr_rgb", 0.0), 3),
+                                   "mean_abs_rgb_first_come": round(a.get("mean_abs_rgb", 0.0), 3),
+                                   "mean_abs_rgb_refined": round(b.get("mean_abs_rgb", 0.0), 3)}
+        iq_frames.free()
+        log(f"image quality against the flat frame: {image_quality}")
         t0 = time.perf_counter()
         idle = tree.refine()
         idle_ms = (time.perf_counter() - t0) * 1e3
@@ -444,6 +471,7 @@ def main() -> int:
         "scopes": scopes,
         "rasterize": raster,
         "refinement": refinement,
+        "image_quality": image_quality,
         "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "max_points_per_node",
                                     "total_expand_iters", "total_spilled_points",
                                     "num_voxels", "max_depth", "allocated_bytes_persistent",

@@ -250,6 +250,16 @@ int remo_analysis(remo_ctx* ctx, const remo_shared_uniforms* u, int mode);
  * cleared and filled with (float_bits(depth) << 32) | rgba. */
 int remo_render(remo_ctx* ctx, const remo_shared_uniforms* u, uint64_t* fb);
 
+/* The `flat` pipeline's Rasterize (kernels/flat/flat_render.cu:9-76): every one
+ * of n device points (e.g. a ring that holds the whole cloud) splatted into fb
+ * with the same projection and packing as remo_render, no LOD -- the image
+ * ground truth a LOD frame is compared against. Ordered on ctx's stream. */
+int remo_render_flat(remo_ctx* ctx, const remo_point* points, uint64_t n, const remo_shared_uniforms* u,
+                     uint64_t* fb);
+/* Eye-dome lighting of a rendered fb in place when u->enable_edl
+ * (remoApplyEDL, kernels/shared/remo_framebuffer.cuh:78-122); depth is kept. */
+int remo_apply_edl(remo_ctx* ctx, const remo_shared_uniforms* u, uint64_t* fb);
+
 /* Analysis + Rasterize of one frame as one captured hipGraph (no reference
  * counterpart: the reference launches kernel_render alone, eagerly). The graph is
  * captured on the first call and re-captured when the tree depth, the mode, the
@@ -352,7 +362,8 @@ int remo_refine(remo_ctx* ctx, float budget_ms, uint32_t max_levels, remo_refine
  * totals. Scope names follow the reference's (src/pipelines/RemolodPipeline.cpp:474-479):
  * remolod.construct.{count,split,voxelSampling,allocChunks,insertPoints,insertVoxels},
  * remolod.accumulate, remolod.analysis, remolod.render, remolod.frame (remo_frame)
- * and remolod.refine (remo_refine). */
+ * remolod.refine (remo_refine) and flat.render (remo_render_flat,
+ * src/pipelines/FlatPipeline.cpp:89-92). */
 typedef struct remo_profile_entry {
     char name[48];
     double total_ms;
@@ -360,7 +371,8 @@ typedef struct remo_profile_entry {
 } remo_profile_entry;
 
 /* scope_mask: bit i times scope i in the order listed above (count = bit 0 ...
- * render = bit 8, frame = bit 9, refine = bit 10); 0 disables, 0xFFFFFFFF times all. */
+ * render = bit 8, frame = bit 9, refine = bit 10, flat.render = bit 11); 0 disables,
+ * 0xFFFFFFFF times all. */
 int remo_profile_enable(remo_ctx* ctx, uint32_t scope_mask);
 int remo_profile_read(remo_ctx* ctx, remo_profile_entry* out, uint32_t cap, uint32_t* num);
 int remo_profile_clear(remo_ctx* ctx);

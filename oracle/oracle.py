@@ -60,6 +60,8 @@ def load():
         "oracle_jacobi": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "oracle_morton": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
         "oracle_refine": (C.c_uint32, [C.c_void_p, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
+        "oracle_render_flat": (None, [C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
+        "oracle_apply_edl": (None, [P(SharedUniforms), C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -175,3 +177,17 @@ def quantise(point, size: float):
 
 def morton(x: int, y: int, z: int) -> int:
     return int(load().oracle_morton(x, y, z))
+
+
+def render_flat(points: np.ndarray, uniforms) -> np.ndarray:
+    """Every point splatted, no LOD (the flat pipeline's frame)."""
+    pts = np.ascontiguousarray(points, dtype=POINT_DTYPE)
+    fb = np.zeros(int(uniforms.width) * int(uniforms.height), dtype=np.uint64)
+    load().oracle_render_flat(pts.ctypes.data, len(pts), C.byref(uniforms), fb.ctypes.data)
+    return fb
+
+
+def apply_edl(fb: np.ndarray, uniforms) -> np.ndarray:
+    out = np.ascontiguousarray(fb, dtype=np.uint64).copy()
+    load().oracle_apply_edl(C.byref(uniforms), out.ctypes.data)
+    return out

@@ -1193,6 +1193,55 @@ void oracle_analyse(oracle_tree* t, const remo_shared_uniforms* u, uint32_t clos
     std::memcpy(out, v.data(), v.size() * sizeof(oracle_analysis_out));
 }
 
+// The flat pipeline's Rasterize (kernels/flat/flat_render.cu:9-76): every point,
+// no LOD, the shared splat. COLOR_RGB and COLOR_WHITE.
+void oracle_render_flat(const remo_point* pts, uint64_t n, const remo_shared_uniforms* u, uint64_t* fb) {
+    const int32_t w = int32_t(u->width), h = int32_t(u->height);
+    for (int64_t i = 0; i < int64_t(w) * h; ++i) fb[i] = 0x7f80000000332211ull;
+    if (u->show_points == 0) return;
+    for (uint64_t i = 0; i < n; ++i) rast::drawPoint(fb, *u, w, h, pts[i]);
+}
+
+// remoApplyEDL (kernels/shared/remo_framebuffer.cuh:78-122), with the C library's
+// log2f / expf where the reference uses the fast intrinsics.
+void oracle_apply_edl(const remo_shared_uniforms* u, uint64_t* fb) {
+    if (u->enable_edl == 0) return;
+    const int32_t w = int32_t(u->width), h = int32_t(u->height);
+    auto depthOf = [](uint64_t px) {
+        const uint32_t bits = uint32_t(px >> 32);
+        float d;
+        std::memcpy(&d, &bits, 4);
+        return d;
+    };
+    for (int32_t y = 0; y < h; ++y)
+        for (int32_t x = 0; x < w; ++x) {
+            const uint64_t pixel = fb[int64_t(y) * w + x];
+            const float depth = depthOf(pixel);
+            if (!(depth < 3.0e38f)) continue;
+            const float logDepth = std::log2(depth);
+            float response = 0.0f;
+            int taps = 0;
+            const int32_t off[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+            for (int k = 0; k < 4; ++k) {
+                const int32_t nx = x + off[k][0], ny = y + off[k][1];
+                if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+                const float nd = depthOf(fb[int64_t(ny) * w + nx]);
+                if (!(nd < 3.0e38f)) continue;
+                response += std::max(0.0f, logDepth - std::log2(nd));
+                ++taps;
+            }
+            if (taps == 0) continue;
+            response /= float(taps);
+            const float shade = std::exp(-response * 24.0f * u->edl_strength);
+            const uint32_t c = uint32_t(pixel);
+            const uint32_t r = uint32_t(float(c & 0xFFu) * shade);
+            const uint32_t g = uint32_t(float((c >> 8) & 0xFFu) * shade);
+            const uint32_t b = uint32_t(float((c >> 16) & 0xFFu) * shade);
+            fb[int64_t(y) * w + x] =
+                (pixel & 0xFFFFFFFF00000000ull) | (r & 0xFFu) | ((g & 0xFFu) << 8) | ((b & 0xFFu) << 16) | (255u << 24);
+        }
+}
+
 uint32_t oracle_refine(oracle_tree* t, uint32_t minLevel, uint32_t* skipped, uint32_t* empty) {
     return t->refine(minLevel, skipped, empty);
 }

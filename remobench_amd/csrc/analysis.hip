@@ -555,7 +555,74 @@ __global__ __launch_bounds__(kBlock) void kRenderRaster(DevTree t, RenderArgs ra
     }
 }
 
+// The `flat` pipeline's Rasterize (kernels/flat/flat_render.cu:9-76): every
+// point of a resident cloud, no LOD, the same projection and 64-bit splat. The
+// reference hands out 65,536-point slices as draw items; a slice is the key of
+// the per-item colour modes here too (level 0 for COLOR_LEVEL).
+constexpr uint64_t kFlatSlice = 65536;
+
+__global__ __launch_bounds__(kBlock) void kRenderFlat(const remo_point* pts, uint64_t n, remo_shared_uniforms u,
+                                                      uint64_t* fb) {
+    if (u.show_points == 0) return;
+    const int32_t w = int32_t(u.width), h = int32_t(u.height);
+    const int32_t colorMode = u.color_mode;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const remo_point p = pts[i];
+        uint32_t color = p.color;
+        if (colorMode == 3) color = 0xFFFFFFFFu;
+        else if (colorMode == 1) color = hashColor(i / kFlatSlice);
+        else if (colorMode == 2) color = hashColor(0);
+        drawPoint(fb, u, w, h, p, color);
+    }
+}
+
+// Eye-dome lighting on the u64 buffer (remoApplyEDL, remo_framebuffer.cuh:78-122):
+// each covered pixel is darkened by exp(-24 * strength * mean over its covered
+// 4-neighbours of max(0, log2 d - log2 d_n)), alpha 255; depth is kept, so the
+// neighbours' depth reads do not depend on the order pixels are shaded in.
+__global__ __launch_bounds__(kBlock) void kApplyEdl(uint64_t* fb, remo_shared_uniforms u) {
+    const int32_t w = int32_t(u.width), h = int32_t(u.height);
+    const uint64_t n = uint64_t(w) * uint64_t(h);
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const int32_t x = int32_t(i % uint64_t(w)), y = int32_t(i / uint64_t(w));
+        const uint64_t pixel = fb[i];
+        const float depth = __uint_as_float(uint32_t(pixel >> 32));
+        if (!(depth < 3.0e38f)) continue;
+        const float logDepth = log2f(depth);
+        float response = 0.0f;
+        int taps = 0;
+        const int32_t off[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        for (int k = 0; k < 4; ++k) {
+            const int32_t nx = x + off[k][0], ny = y + off[k][1];
+            if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+            const float nd = __uint_as_float(uint32_t(fb[uint64_t(ny) * uint64_t(w) + uint64_t(nx)] >> 32));
+            if (!(nd < 3.0e38f)) continue;
+            response += fmaxf(0.0f, logDepth - log2f(nd));
+            ++taps;
+        }
+        if (taps == 0) continue;
+        response /= float(taps);
+        const float shade = expf(-response * 24.0f * u.edl_strength);
+        const uint32_t c = uint32_t(pixel);
+        const uint32_t r = uint32_t(float(c & 0xFFu) * shade);
+        const uint32_t g = uint32_t(float((c >> 8) & 0xFFu) * shade);
+        const uint32_t b = uint32_t(float((c >> 16) & 0xFFu) * shade);
+        fb[i] = (pixel & 0xFFFFFFFF00000000ull) | (r & 0xFFu) | ((g & 0xFFu) << 8) | ((b & 0xFFu) << 16) | (255u << 24);
+    }
+}
+
 }  // namespace
+
+void launchRenderFlat(const remo_point* pts, uint64_t n, const remo_shared_uniforms& u, uint64_t* fb, uint32_t blocks,
+                      hipStream_t s) {
+    const uint64_t pixels = uint64_t(u.width) * uint64_t(u.height);
+    hipLaunchKernelGGL(kFbClear, dim3(1024), dim3(kBlock), 0, s, fb, pixels);
+    if (n) hipLaunchKernelGGL(kRenderFlat, dim3(blocks), dim3(kBlock), 0, s, pts, n, u, fb);
+}
+
+void launchApplyEdl(uint64_t* fb, const remo_shared_uniforms& u, hipStream_t s) {
+    hipLaunchKernelGGL(kApplyEdl, dim3(1024), dim3(kBlock), 0, s, fb, u);
+}
 
 int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) {
     a.maxNodes = uint32_t(maxNodes);

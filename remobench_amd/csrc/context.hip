@@ -27,13 +27,13 @@ namespace {
 
 enum ScopeId : int {
     kScopeCount, kScopeSplit, kScopeVoxel, kScopeAlloc, kScopeInsertPoints, kScopeInsertVoxels,
-    kScopeAccumulate, kScopeAnalysis, kScopeRender, kScopeFrame, kScopeRefine, kNumScopes
+    kScopeAccumulate, kScopeAnalysis, kScopeRender, kScopeFrame, kScopeRefine, kScopeFlat, kNumScopes
 };
 const char* const kScopeNames[kNumScopes] = {
     "remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
     "remolod.construct.allocChunks", "remolod.construct.insertPoints",
     "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
-    "remolod.frame", "remolod.refine",
+    "remolod.frame", "remolod.refine", "flat.render",
 };
 
 // HIP-event pairs per launch, harvested after the stream has passed them.
@@ -624,6 +624,27 @@ int remo_refine(remo_ctx* c, float budgetMs, uint32_t maxLevels, remo_refine_sta
     out->cell_overflow = so.cellOverflow;
     out->estimate_ms = so.estimateMs;
     out->device_ms = so.deviceMs;
+    return REMO_OK;
+}
+
+int remo_render_flat(remo_ctx* c, const remo_point* points, uint64_t n, const remo_shared_uniforms* u, uint64_t* fb) {
+    if (!c || !u || !fb || (n && !points)) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (!(u->width >= 1.0f && u->height >= 1.0f)) return REMO_ERR_INVALID;
+    {
+        Scope sc(c->prof, kScopeFlat, c->stream);
+        launchRenderFlat(points, n, *u, fb, c->shape.pointBlocks, c->stream);
+    }
+    CK(hipGetLastError(), "flat render kernels");
+    return REMO_OK;
+}
+
+int remo_apply_edl(remo_ctx* c, const remo_shared_uniforms* u, uint64_t* fb) {
+    if (!c || !u || !fb) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (u->enable_edl == 0) return REMO_OK;
+    launchApplyEdl(fb, *u, c->stream);
+    CK(hipGetLastError(), "EDL kernel");
     return REMO_OK;
 }
 

@@ -109,5 +109,9 @@ int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u,
 int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
              uint32_t numNodes, float arrival, uint64_t* fb, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);
+// The flat (LOD-free) Rasterize of n resident points, and eye-dome lighting.
+void launchRenderFlat(const remo_point* pts, uint64_t n, const remo_shared_uniforms& u, uint64_t* fb, uint32_t blocks,
+                      hipStream_t s);
+void launchApplyEdl(uint64_t* fb, const remo_shared_uniforms& u, hipStream_t s);
 
 }  // namespace remo_hip

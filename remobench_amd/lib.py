@@ -130,7 +130,7 @@ ANALYSIS_DTYPE = np.dtype([
 SCOPES = ["remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
           "remolod.construct.allocChunks", "remolod.construct.insertPoints",
           "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
-          "remolod.frame", "remolod.refine"]
+          "remolod.frame", "remolod.refine", "flat.render"]
 PROFILE_ALL = 0xFFFFFFFF
 
 
@@ -228,6 +228,8 @@ def load() -> C.CDLL:
                                           C.c_uint32, P(C.c_uint32)]),
         "remo_get_frame_shape": (C.c_int, [C.c_void_p, P(TuneEntry)]),
         "remo_refine": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, P(RefineStats)]),
+        "remo_render_flat": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
+        "remo_apply_edl": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_void_p]),
         "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),
         "remo_scene_generate": (C.c_int, [P(SceneSpec), C.c_uint64, C.c_uint64, C.c_void_p,

@@ -231,6 +231,14 @@ class Tree:
         self._ck(self.lib.remo_refine(self.h, float(budget_ms), int(max_levels), C.byref(st)), "refine")
         return st.as_dict()
 
+    def render_flat(self, points_ptr: int, n: int, uniforms: L.SharedUniforms, fb_ptr: int):
+        """Every one of n device points splatted into fb (no LOD): the image ground truth."""
+        self._ck(self.lib.remo_render_flat(self.h, C.c_void_p(points_ptr), n, C.byref(uniforms),
+                                           C.c_void_p(fb_ptr)), "render_flat")
+
+    def apply_edl(self, uniforms: L.SharedUniforms, fb_ptr: int):
+        self._ck(self.lib.remo_apply_edl(self.h, C.byref(uniforms), C.c_void_p(fb_ptr)), "apply_edl")
+
     def frame_shape(self):
         e = L.TuneEntry()
         self._ck(self.lib.remo_get_frame_shape(self.h, C.byref(e)), "frame shape")
