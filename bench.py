@@ -98,7 +98,7 @@ SCOPE_KERNELS = {
     "remolod.construct.insertPoints": ["kInsertPoints"],
     "remolod.construct.insertVoxels": ["kInsertVoxels"],
     "remolod.accumulate": ["kAccumPlan", "kAccumTiles", "kAccumFinish"],
-    "remolod.analysis": ["kAnalysisLeaves", "kAnalysisLevel", "kFinalize", "kQueue"],
+    "remolod.analysis": ["kAnalysisLeaves", "kAnalysisLevel", "kFinalize", "kSetFrame"],
 }
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_lidar100M.txt"
 

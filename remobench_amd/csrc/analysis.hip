@@ -264,21 +264,32 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
 
 // Leaves: sums from the fixed-point accumulators, closure from the batch age,
 // then their analysis.
+// A leaf's rolled-up sums and closure (written for kFinalize, returned for the
+// parent's roll-up).
+__device__ __forceinline__ uint8_t leafRecord(const DevTree& t, const AnalysisArgs& aa, uint32_t node, uint32_t D,
+                                              uint32_t current, RollupSums& S) {
+    leafSums(t, node, D, S);
+    S.e = 20 + D - t.level[node];
+    const remo_node_accum& a = t.accums[node];
+    const uint8_t cl = (a.count == 0 || current >= a.last_touched_batch + aa.margin) ? 1 : 0;
+    aa.rollup[node] = S;
+    aa.closed[node] = cl;
+    return cl;
+}
+
+// Leaves: sums from the fixed-point accumulators, closure from the batch age.
+// (Forming leaf children's sums inside the level launches instead was slower:
+// 61 against 53 us per Analysis on the 100M-point tree.)
 __global__ __launch_bounds__(kBlock) void kAnalysisLeaves(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
     const uint32_t current = g->batchletIndex;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *aa.queueCount = 0;  // kQueue runs after the levels
+    if (blockIdx.x == 0 && threadIdx.x == 0) *aa.queueCount = 0;  // kFinalize fills the queue afterwards
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
         if (t.firstChild[node] >= 0) continue;
         RollupSums S;
-        leafSums(t, node, D, S);
-        S.e = 20 + D - t.level[node];
-        const remo_node_accum& a = t.accums[node];
-        const uint8_t cl = (a.count == 0 || current >= a.last_touched_batch + aa.margin) ? 1 : 0;
-        aa.rollup[node] = S;
-        aa.closed[node] = cl;
+        leafRecord(t, aa, node, D, current, S);
     }
 }
 
@@ -316,13 +327,28 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs
     }
 }
 
+// Every node's eigen-solve and score, and the work queue of scored nodes (one
+// atomic per wave).
 __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
     const remo_shared_uniforms& u = aa.frame->u;
-    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x)
-        finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t node = base + threadIdx.x;
+        bool q = false;
+        if (node < n) {
+            finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
+            q = aa.results[node].score > 0.0f;
+        }
+        const uint64_t m = __ballot(q);
+        if (m == 0) continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t at = 0;
+        if (laneId() == uint32_t(leader)) at = atomicAdd(aa.queueCount, uint32_t(__popcll(m)));
+        at = __shfl(at, leader);
+        if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
+    }
 }
 
 // The learned score's inputs of every node (training pairs are logged from these).
@@ -336,21 +362,6 @@ __global__ __launch_bounds__(kBlock) void kFeatures(DevTree t, AnalysisArgs aa, 
 }
 
 // Work queue: every node with a positive score, rebuilt each frame (one atomic per wave).
-__global__ __launch_bounds__(kBlock) void kQueue(DevTree t, AnalysisArgs aa) {
-    const uint32_t n = t.g->numNodes;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t node = base + threadIdx.x;
-        const bool q = node < n && aa.results[node].score > 0.0f;
-        const uint64_t m = __ballot(q);
-        if (m == 0) continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t at = 0;
-        if (laneId() == uint32_t(leader)) at = atomicAdd(aa.queueCount, uint32_t(__popcll(m)));
-        at = __shfl(at, leader);
-        if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
-    }
-}
-
 // ---- rasterize -----------------------------------------------------------------
 
 struct Plane {
@@ -669,11 +680,16 @@ void analysisReset(AnalysisState& a) {
 
 namespace {
 
+// The device copy of the frame's parameters; skipped when it already holds them
+// (an earlier write on the same stream).
 void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, hipStream_t s) {
     FrameParams f{};
     f.u = u;
     f.arrival = arrival;
+    if (a.frameSet && std::memcmp(&f, &a.lastFrame, sizeof(f)) == 0) return;
     hipLaunchKernelGGL(kSetFrame, dim3(1), dim3(64), 0, s, a.frame, f);
+    a.lastFrame = f;
+    a.frameSet = true;
 }
 
 // The Analysis kernels alone (no host work, so they can be captured).
@@ -682,9 +698,9 @@ void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t m
     const uint32_t nb = a.shape.nodeBlocks;
     AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount,
                     learned ? a.pred : nullptr, a.frame, a.closureMargin};
-    hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
     // one launch per level: measured faster than one workgroup walking all levels
     // with barriers (157 us against 101 us per Analysis on a 10,073-node tree)
+    hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
     for (int level = int(maxLevel) - 1; level >= 0; --level)
         hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level));
     if (learned) {
@@ -693,7 +709,6 @@ void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t m
         launchMlpInfer(a.mlpWeights, a.feat, &t.g->numNodes, 0, a.pred, nb, s);
     }
     hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
-    hipLaunchKernelGGL(kQueue, dim3(nb), dim3(kBlock), 0, s, t, aa);
 }
 
 // The Rasterize kernels alone.

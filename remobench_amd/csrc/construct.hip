@@ -173,8 +173,15 @@ __global__ void kBeginStep(DevTree t) {
 }
 
 __global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t checkBudget,
-                            uint64_t budgetTicks) {
+                            uint64_t budgetTicks, uint32_t beginStep) {
     Globals* g = t.g;
+    if (beginStep) {
+        // the step's first batch also opens the step (kBeginStep, one launch fewer)
+        g->stepStart = wall_clock64();
+        g->batches = g->expandIters = g->nodesSplit = g->spilledPoints = g->newVoxels = 0;
+        g->stop = 0;
+        g->budgetStop = 0;
+    }
     // remolod_octree.cu:1230-1240: after a batch, stop once the step has used its
     // device-time budget. Checked here, before the next batch touches the tree.
     if (checkBudget && budgetTicks && wall_clock64() - g->stepStart > budgetTicks) {
@@ -1020,9 +1027,10 @@ void launchBeginStep(const DevTree& t, hipStream_t s) {
     hipLaunchKernelGGL(kBeginStep, dim3(1), dim3(1), 0, s, t);
 }
 
-void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, hipStream_t s) {
+void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, bool beginStep,
+                      hipStream_t s) {
     hipLaunchKernelGGL(kBeginBatch, dim3(1), dim3(1), 0, s, t, b, t.arenaCapacity, checkBudget ? 1u : 0u,
-                       budgetTicks);
+                       budgetTicks, beginStep ? 1u : 0u);
 }
 
 void launchCount(const DevTree& t, BatchRef b, uint32_t round, const LaunchShape& ls, hipStream_t s) {

@@ -470,15 +470,15 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
     const uint32_t uploaded = c->uploadedSeen;
     DevTree& t = c->t;
     hipStream_t s = c->stream;
-    // step start time and per-step counters (DeviceTimeline counter block: per launch)
-    launchBeginStep(t, s);
-
     const uint32_t first = c->batchletIndex;
     const uint32_t numBatches = uploaded > first ? std::min(uploaded - first, c->cfg.max_batches_per_step) : 0u;
+    // step start time and per-step counters (DeviceTimeline counter block: per
+    // launch); the first batch's begin kernel does it when there is one
+    if (numBatches == 0) launchBeginStep(t, s);
     for (uint32_t bi = first; bi < first + numBatches; ++bi) {
         const uint32_t slot = bi % v->num_slots;
         BatchRef b{v->slots + uint64_t(slot) * v->slot_capacity, v->batch_sizes + slot};
-        launchBeginBatch(t, b, bi != first, c->budgetTicks, s);
+        launchBeginBatch(t, b, bi != first, c->budgetTicks, bi == first, s);
         if (t.fixedDepth) {
             // the analytic-bucketing arm: one pass over the batch, no re-counting
             {

@@ -76,6 +76,8 @@ struct AnalysisState {
     RasterUnit* units = nullptr;
     uint32_t unitCapacity = 1u << 20;
     FrameParams* frame = nullptr;      // device copy of the current frame's parameters
+    FrameParams lastFrame{};           // what it holds (host mirror), when frameSet
+    bool frameSet = false;
     FrameShape shape{0, 0, 1000};
     // the captured Analysis + Rasterize graph and what it was captured for
     hipGraphExec_t frameExec = nullptr;

@@ -210,7 +210,9 @@ struct LaunchShape {
 
 void launchResetTree(const DevTree& t, hipStream_t s);
 void launchBeginStep(const DevTree& t, hipStream_t s);
-void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, hipStream_t s);
+// beginStep: this batch is the step's first, so it also opens the step (kBeginStep's work).
+void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, bool beginStep,
+                      hipStream_t s);
 void launchCount(const DevTree& t, BatchRef b, uint32_t round, const LaunchShape& ls, hipStream_t s);
 void launchSplitDecide(const DevTree& t, uint32_t round, const LaunchShape& ls, hipStream_t s);
 void launchSplitApply(const DevTree& t, const LaunchShape& ls, hipStream_t s);

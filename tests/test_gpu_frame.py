@@ -123,3 +123,27 @@ def test_render_point_size_3(built):
         L.load().remo_dev_free(fb)
     want, _, _ = o.render(u)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["single_point", "at_capacity", "one_over_one_octant"])
+def test_analysis_parity_small_trees(gpu, name):
+    """A root that is still a leaf takes the leaves-only launch; one split level
+    folds the leaves into the level launch."""
+    from .cases import tiny_cases
+    points, meta, batch = tiny_cases()[name]
+    tree, _ = build_hip(points, meta, batch)
+    o = build_oracle(points, meta, batch)
+    u = camera.poses(points, meta.box_size)["overview"]
+    tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
+    got = tree.analysis_results()
+    want = o.analyse(u)
+    kg, ko = keyed(tree.nodes()), keyed(o.nodes())
+    for k, ig in kg.items():
+        g, w = got[ig], want[ko[k]]
+        r = w["r"]
+        assert g["n"] == r["n"] == w["n"], k
+        assert g["state"] == r["state"], k
+        if w["n"] >= 3:
+            trace = float(w["lambda64"].sum())
+            np.testing.assert_allclose(g["lambda"], r["lambda"], rtol=0, atol=1e-6 * trace + 1e-30)
