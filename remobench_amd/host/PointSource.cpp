@@ -31,10 +31,29 @@ remo_batch_view BatchView::toAbi() const {
 HostCloudSource::HostCloudSource(std::vector<remo_point> points, const CloudMeta& meta, uint32_t slotCapacity)
     : m_points(std::move(points)), m_meta(meta), m_slotCapacity(slotCapacity) {
     m_meta.numPoints = m_points.size();
+    m_hostCount = m_points.size();
     m_numBatches = uint32_t((m_points.size() + slotCapacity - 1) / slotCapacity);
 }
 
-HostCloudSource::~HostCloudSource() { stop(); }
+HostCloudSource::~HostCloudSource() {
+    stop();
+    if (m_pinned) remo_host_free(m_pinned);
+    m_pinned = nullptr;
+}
+
+bool HostCloudSource::enableAsync(remo_ctx* consumer, uint32_t batchesPerStep) {
+    if (!consumer) return false;
+    if (!m_pinned && !m_points.empty()) {
+        void* p = nullptr;
+        if (remo_host_alloc(m_points.size() * sizeof(remo_point), &p) != REMO_OK) return false;
+        m_pinned = static_cast<remo_point*>(p);
+        std::copy(m_points.begin(), m_points.end(), m_pinned);
+        std::vector<remo_point>().swap(m_points);  // one host copy of the cloud
+    }
+    m_consumer = consumer;
+    m_perStep = std::max(1u, batchesPerStep);
+    return true;
+}
 
 bool HostCloudSource::start(Mode mode, uint32_t ringSlots, size_t maxDeviceBytes, std::string* err) {
     stop();
@@ -57,14 +76,17 @@ bool HostCloudSource::start(Mode mode, uint32_t ringSlots, size_t maxDeviceBytes
 }
 
 void HostCloudSource::rewind() {
-    if (m_ring) remo_ring_rewind(m_ring);
+    if (m_ring) remo_ring_rewind(m_ring);  // waits for queued copies first
     m_produced = 0;
     m_consumed = 0;
     m_pointsUploaded = 0;
 }
 
 void HostCloudSource::stop() {
-    if (m_ring) remo_ring_destroy(m_ring);
+    if (m_ring) {
+        remo_ring_sync(m_ring);
+        remo_ring_destroy(m_ring);
+    }
     m_ring = nullptr;
 }
 
@@ -74,14 +96,19 @@ void HostCloudSource::pump() {
     const uint32_t limit = std::min(m_numBatches, m_consumed + m_numSlots);
     while (m_produced < limit) {
         const uint64_t first = uint64_t(m_produced) * m_slotCapacity;
-        const uint32_t n = uint32_t(std::min<uint64_t>(m_slotCapacity, m_points.size() - first));
-        if (remo_ring_upload(m_ring, m_produced, m_points.data() + first, n) != REMO_OK) {
+        const uint32_t n = uint32_t(std::min<uint64_t>(m_slotCapacity, m_hostCount - first));
+        const int rc = m_consumer ? remo_ring_push(m_ring, m_produced, hostPoints() + first, n, m_consumer)
+                                  : remo_ring_upload(m_ring, m_produced, hostPoints() + first, n);
+        if (rc != REMO_OK) {
             std::fprintf(stderr, "remobench: ring upload of batch %u failed\n", m_produced);
             return;
         }
         m_pointsUploaded += n;
         ++m_produced;
     }
+    // the consumer's next step reads at most m_perStep batches from m_consumed on
+    if (m_consumer && m_produced > m_consumed)
+        remo_ring_order_before(m_ring, m_consumer, m_consumed, std::min(m_perStep, m_produced - m_consumed));
 }
 
 BatchView HostCloudSource::view() const {

@@ -86,8 +86,22 @@ public:
     uint32_t numBatchesTotal() const { return m_numBatches; }
     uint32_t batchesProduced() const { return m_produced; }
 
+    // Asynchronous ingest (no reference counterpart: its pump copies
+    // synchronously): the cloud moves to pinned host memory, pump() queues each
+    // batch on the ring's copy stream (remo_ring_push; a slot is refilled only
+    // after `consumer` has finished the build step that read it), and orders
+    // `consumer`'s next build step after the copies of the next `batchesPerStep`
+    // batches only. Call before start(); false when pinned memory is refused.
+    bool enableAsync(remo_ctx* consumer, uint32_t batchesPerStep);
+
 private:
+    const remo_point* hostPoints() const { return m_pinned ? m_pinned : m_points.data(); }
+
     std::vector<remo_point> m_points;
+    remo_point* m_pinned = nullptr;  // the cloud in pinned host memory (asynchronous ingest)
+    uint64_t m_hostCount = 0;
+    remo_ctx* m_consumer = nullptr;
+    uint32_t m_perStep = 1;
     CloudMeta m_meta;
     uint32_t m_slotCapacity;
     uint32_t m_numBatches = 0;

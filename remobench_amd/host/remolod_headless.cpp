@@ -138,7 +138,7 @@ void usage() {
                  "                        [--budget-ms MS] [--analysis none|analytic] [--width W --height H]\n"
                  "                        [--white] [--no-accum] [--profile] [--device-budget-mb MB]\n"
                  "                        [--dump-uniforms FILE] [--dump-fb FILE]\n"
-                 "                        [--remolod-fixed-depth D]\n"
+                 "                        [--remolod-fixed-depth D] [--async-upload]\n"
                  "                        [--refine-budget-ms MS] (Refinement each frame; then one\n"
                  "                        unbudgeted pass before the final frame)\n");
 }
@@ -154,6 +154,7 @@ int main(int argc, char** argv) {
     std::string analysis = "analytic", dumpUniforms, dumpFb;
     float refineBudgetMs = -2.0f;  // below -1: no Refinement
     uint32_t fixedDepth = 0;
+    bool asyncUpload = false;
     uint64_t deviceBudgetMb = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -180,6 +181,7 @@ int main(int argc, char** argv) {
         else if (a == "--dump-fb") dumpFb = next();
         else if (a == "--refine-budget-ms") refineBudgetMs = std::strtof(next(), nullptr);
         else if (a == "--remolod-fixed-depth") fixedDepth = uint32_t(std::strtoul(next(), nullptr, 10));
+        else if (a == "--async-upload") asyncUpload = true;
         else if (a == "--help" || a == "-h") {
             usage();
             return 0;
@@ -232,6 +234,10 @@ int main(int argc, char** argv) {
         return 1;
     }
     auto* pipe = static_cast<remo::RemolodHipPipeline*>(registry.active());
+    if (asyncUpload && !source->enableAsync(pipe->context(), perStep)) {
+        std::fprintf(stderr, "remolod_headless: pinned host memory for the cloud was refused\n");
+        return 1;
+    }
 
     uint64_t* fb = nullptr;
     if (remo_dev_alloc(size_t(width) * height * 8, reinterpret_cast<void**>(&fb)) != REMO_OK) {

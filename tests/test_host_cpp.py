@@ -108,3 +108,24 @@ def test_headless_with_refinement_renders_colour_bit_exact(gpu, tmp_path):
     assert out["visible_samples"] == samples
     assert np.array_equal(got, want)
     o.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [("--async-upload",), ("--async-upload", "--remolod-fixed-depth", "5")])
+def test_headless_async_upload_and_fixed_depth(gpu, extra, tmp_path):
+    """The same loop with the cloud in pinned host memory pushed on the ring's copy
+    stream (and with the fixed-depth arm): the same tree as the oracle's."""
+    n, slot, seed = 2_097_152, 131_072, 4
+    fixed = int(extra[2]) if len(extra) > 2 else 0
+    r = _run("--points", n, "--seed", seed, "--slot", slot, "--budget-ms", -1, "--batches-per-step", 3, *extra)
+    assert r.returncode == 0, r.stderr.decode()
+    out = json.loads(r.stdout.decode().strip().splitlines()[-1])
+    points, meta = L.generate_scene(n, seed, "lidar")
+    slots, sizes, nb = batch_arrays(points, slot)
+    o = OracleTree(meta.box_size, fixed_depth=fixed, persistent_bytes=out.get("persistent_bytes", 1 << 62))
+    o.build_step(slots, sizes, nb, slot, nb)
+    s = o.stats()
+    for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "num_voxels", "max_depth"):
+        assert out[k] == s[k], (k, out[k], s[k])
+    assert out["num_points_ingested"] == n and out["sum_leaf_counts"] == n
+    o.close()
