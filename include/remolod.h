@@ -335,7 +335,10 @@ int remo_mlp_features(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* num);
  * budget_ms > 0). What is left stays dirty for the next call. A node is left alone (nodes_skipped) when its voxel list is
  * incomplete, an inner child could not be refined, or the accumulator store is
  * full; it stays dirty. Cell sums are exact up to 16,843,009 points in one cell
- * (cell_overflow reports more). Synchronous: returns after the work ran. */
+ * (cell_overflow reports more). The store (24 B per entry) starts at about two
+ * entries per voxel; when a pass would not fit beside the live sums it is
+ * replaced first and every node is recomputed (environment variable
+ * REMO_REFINE_STORE_ENTRIES caps its size). Synchronous: returns after the work ran. */
 typedef struct remo_refine_stats {
     uint32_t levels_pending;    /* levels holding dirty gridded nodes at the start    */
     uint32_t levels_processed;  /* of those, refined by this call (deepest first)     */

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -579,8 +580,14 @@ int remo_refine(remo_ctx* c, float budgetMs, uint32_t maxLevels, remo_refine_sta
         int rc = readGlobals(c);
         if (rc != REMO_OK) return rc;
         const uint64_t v = c->hostG->totNewVoxels;
-        const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(1ull << 22, 2 * v + (1ull << 20)),
-                                                 fresh ? 0ull : r.accEntries);
+        uint64_t want = std::max<uint64_t>(std::max<uint64_t>(1ull << 22, 2 * v + (1ull << 20)),
+                                           fresh ? 0ull : r.accEntries);
+        // REMO_REFINE_STORE_ENTRIES caps the store (memory-constrained callers;
+        // tests use it to exercise the full-store path)
+        if (const char* cap = std::getenv("REMO_REFINE_STORE_ENTRIES")) {
+            const unsigned long long e = std::strtoull(cap, nullptr, 10);
+            if (e > 0) want = e;
+        }
         if (fresh) {
             rc = refineAllocate(r, c->t, want, [&](void** p, uint64_t bytes, const char* what) {
                 uint8_t* q = nullptr;
@@ -602,6 +609,19 @@ int remo_refine(remo_ctx* c, float budgetMs, uint32_t maxLevels, remo_refine_sta
     {
         Scope sc(c->prof, kScopeRefine, c->stream);
         rc = runRefine(r, c->t, c->hostG->numNodes, budgetMs, maxLevels, c->shape, c->stream, &so, &c->err);
+        if (rc == kRefineReplaceStore) {
+            // a fresh store (at least as large), every node dirty, then the pass
+            int g = readGlobals(c);
+            if (g != REMO_OK) return g;
+            uint64_t want = std::max<uint64_t>(r.accEntries, 2 * c->hostG->totNewVoxels + (1ull << 20));
+            if (const char* cap = std::getenv("REMO_REFINE_STORE_ENTRIES")) {
+                const unsigned long long e = std::strtoull(cap, nullptr, 10);
+                if (e > 0) want = e;
+            }
+            g = refineGrow(r, c->t, want, c->stream, &c->err);
+            if (g != 0) return g == -3 ? REMO_ERR_DEVICE : REMO_ERR_OOM;
+            rc = runRefine(r, c->t, c->hostG->numNodes, budgetMs, maxLevels, c->shape, c->stream, &so, &c->err);
+        }
     }
     if (rc != 0) {
         CK(hipGetLastError(), "refinement kernels");

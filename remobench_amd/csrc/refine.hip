@@ -696,6 +696,15 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
         out->complete = deepest < 0 ? 1u : 0u;
         return 0;
     }
+    // entries the dirty nodes may take (prep reserves cells + cells/8 + 64): when
+    // they would not fit beside what is handed out, the caller replaces the store
+    // first (reclaiming what grown nodes left behind) and calls again
+    {
+        uint64_t need = 0;
+        for (uint32_t L = 0; L < kRefineLevels; ++L)
+            need += c.levelVoxels[L] + c.levelVoxels[L] / 8 + 64ull * c.levelCount[L];
+        if (c.arenaUsed > 0 && c.arenaUsed + need > r.accEntries) return kRefineReplaceStore;
+    }
     // Levels from the deepest while the estimate fits. The last level taken may be
     // cut to the nodes that fit at its average node cost (nodes of one level do
     // not depend on each other; the rest stay dirty); at least one node runs.

@@ -72,6 +72,9 @@ void refineReset(RefineState& r, const DevTree& t, hipStream_t s);
 // Replaces the store by one of `entries` entries and forgets every refinement
 // (all sums invalid, so the next call recomputes every gridded node).
 int refineGrow(RefineState& r, const DevTree& t, uint64_t entries, hipStream_t s, std::string* err);
+// runRefine's answer when the dirty nodes' sums would not fit beside the live
+// ones: replace the store (refineGrow) and call again. Nothing was changed.
+constexpr int kRefineReplaceStore = 1;
 // One Refinement call (see remo_refine). numNodes: Globals::numNodes as last read.
 int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetMs, uint32_t maxLevels,
               const LaunchShape& ls, hipStream_t s, RefineStatsOut* out, std::string* err);

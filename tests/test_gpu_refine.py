@@ -183,3 +183,51 @@ def test_refine_tiny_budget_runs_one_node_and_converges(gpu):
         assert calls < 10_000
     o.refine(0)
     compare_refined(tree, o)
+
+
+@pytest.mark.gpu
+def test_refine_store_replacement_and_recovery(gpu, monkeypatch):
+    """A sums store that fills up: nodes are skipped and reported, the tree stays
+    consistent, and once the store may grow again the next calls finish the pass
+    with the oracle's result. A store that only overflows through re-allocation
+    is replaced and recomputed transparently."""
+    from .helpers import compare_trees
+    points, meta = lidar_scene(2_097_152, seed=4)
+    size = max(meta.box_size)
+    o = build_oracle(points, meta, 262_144)
+    o.refine(0)
+    # too small for the tree (1.36M voxels need ~1.53M entries)
+    monkeypatch.setenv("REMO_REFINE_STORE_ENTRIES", "500000")
+    tree, _ = build_hip(points, meta, 262_144)
+    st = tree.refine()
+    assert st["arena_full"] == 1 and st["nodes_skipped"] > 0 and st["complete"] == 0, st
+    assert st["missing_cells"] == 0 and st["empty_cells"] == 0
+    compare_trees(tree, o, size, points, check_points=True, check_colours=False)
+    monkeypatch.delenv("REMO_REFINE_STORE_ENTRIES")
+    for _ in range(4):
+        st = tree.refine()
+        if st["complete"]:
+            break
+    assert clean(st) and st["complete"] == 1, st
+    compare_refined(tree, o)
+    tree.close()
+
+    # room for one pass, not for the leaks of incremental passes: replaced on the way
+    monkeypatch.setenv("REMO_REFINE_STORE_ENTRIES", "2200000")
+    nb = 8
+    ring = PointRing(262_144, nb)
+    ring.set_cloud(points, 262_144)
+    ring.pump()
+    tree = Tree(meta, slot_capacity=262_144, ring_slots=nb, batches_per_step=2, step_budget_ms=-1.0)
+    consumed, dirty = 0, []
+    while consumed < nb:
+        _, consumed = tree.build_step(ring.view())
+        tree.accumulate(consumed)
+        st = tree.refine()
+        assert clean(st) and st["complete"] == 1, st
+        dirty.append(st["nodes_dirty"])
+    compare_refined(tree, o)
+    gridded = int((tree.nodes()["grid"] >= 0).sum())
+    print("dirty per call", dirty, "gridded", gridded, flush=True)
+    tree.close()
+    ring.close()
