@@ -353,7 +353,8 @@ typedef struct remo_refine_stats {
     uint32_t empty_cells;       /* set cells without points (written with colour 0)   */
     uint32_t arena_full;        /* 1: the accumulator store ran out (reset next call) */
     uint32_t cell_overflow;     /* 1: a cell held more than 16,843,009 points         */
-    float estimate_ms;          /* the cost model's estimate for the levels processed */
+    float estimate_ms;          /* the cost model's estimate for the work processed,
+                                   corrected by recent calls' measured/estimated ratio */
     float device_ms;            /* measured: first to last kernel of the levels       */
 } remo_refine_stats;
 

@@ -714,7 +714,8 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
     for (int L = deepest; L >= 0; --L) {
         if (!c.levelCount[L]) continue;
         if (maxLevels && taken >= maxLevels) break;
-        const double fixed = kLevelUs * 1e-3, nodes = nodesCostUs(c, uint32_t(L)) * 1e-3;
+        // the fitted model, corrected by what recent calls measured against it
+        const double fixed = kLevelUs * 1e-3 * r.modelScale, nodes = nodesCostUs(c, uint32_t(L)) * 1e-3 * r.modelScale;
         uint32_t k = c.levelCount[L];
         if (budgetMs > 0.0f && est + fixed + nodes > double(budgetMs)) {
             const double room = double(budgetMs) - est - fixed;
@@ -781,6 +782,11 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
     out->arenaFull = d.arenaFull;
     out->cellOverflow = d.cellOverflow;
     out->estimateMs = float(est);
+    // a running correction of the model: the measured time over the estimate
+    if (est > 0.05 && ms > 0.0f) {
+        const double ratio = double(ms) / (est / r.modelScale);
+        r.modelScale = std::min(8.0, std::max(0.25, 0.7 * r.modelScale + 0.3 * ratio));
+    }
     out->deviceMs = ms;
     return 0;
 }

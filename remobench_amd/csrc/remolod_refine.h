@@ -52,6 +52,7 @@ struct RefineState {
     RefineCounters* ctr = nullptr;
     RefineCounters* host = nullptr;  // pinned mirror
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    double modelScale = 1.0;         // measured / estimated device time of recent calls (running)
 };
 
 using RefineAllocFn = std::function<int(void**, uint64_t, const char*)>;
