@@ -378,6 +378,18 @@ def main() -> int:
                       "points_plus_voxels_per_s_g": round((full["points_folded"] + full["voxels_written"]) /
                                                           max(1e-9, full["device_ms"]) / 1e6, 3),
                       "idle_call_ms": round(idle_ms, 3), "idle_dirty": idle["nodes_dirty"]}
+        # local deepening of the 64 densest leaves, then Refinement of what it dirtied
+        tree.sync()
+        t0 = time.perf_counter()
+        dp = tree.deepen(64, 25_000)
+        deepen_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        after = tree.refine()
+        refinement["deepening"] = {"max_splits": 64, "min_points": 25_000, "splits": dp["splits"],
+                                   "candidates": dp["candidates"], "points_moved": dp["points_moved"],
+                                   "new_voxels": dp["new_voxels"], "ms": round(deepen_ms, 3),
+                                   "refine_after_ms": round((time.perf_counter() - t0) * 1e3, 3),
+                                   "refine_after_nodes": after["nodes_refined"]}
         log(f"refinement, full pass: {refinement}")
         budget = args.refine_budget_ms
         tree.reset()

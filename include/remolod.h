@@ -360,6 +360,27 @@ typedef struct remo_refine_stats {
 
 int remo_refine(remo_ctx* ctx, float budget_ms, uint32_t max_levels, remo_refine_stats* out);
 
+/* Refinement's local deepening (README.md:81, wiki/02_KernelPipeline.md:232-236:
+ * "local split and redistribute"; no reference code). Up to max_splits leaves
+ * holding at least min_points stored points (and no points counted but not yet
+ * stored) are split before they reach 50,000, the densest first (ties by
+ * level, x, y, z), and their points are redistributed to the children exactly
+ * as a split during Update does: the leaf gets a zeroed grid and voxels from its
+ * own points, its chunks are recycled, the children take the points octant by
+ * octant. The node pool's limit is respected and reported like Update's
+ * (node_pool_overflow). Refused when the persistent store is within Update's
+ * safety margin. Synchronous. */
+typedef struct remo_deepen_stats {
+    uint32_t candidates;        /* leaves that qualified                    */
+    uint32_t chosen;            /* of those, offered to the split (<= max_splits) */
+    uint32_t splits;            /* accepted by the node pool                */
+    uint32_t refused;           /* 1: store near capacity, nothing done     */
+    uint64_t points_moved;      /* stored points redistributed              */
+    uint64_t new_voxels;        /* voxels the split leaves gained           */
+} remo_deepen_stats;
+
+int remo_deepen(remo_ctx* ctx, uint32_t max_splits, uint32_t min_points, remo_deepen_stats* out);
+
 /* ---- timing scopes (GpuProfiler / GpuScope, include/remo/GpuProfiler.h:62-158) ----
  * When enabled, every phase launch is bracketed by a pair of HIP events on the
  * context's stream; remo_profile_read harvests finished pairs into per-scope

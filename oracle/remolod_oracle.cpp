@@ -455,6 +455,81 @@ struct oracle_tree {
         }
     }
 
+    // Refinement's local deepening (README.md:81, wiki/02_KernelPipeline.md:232-236;
+    // no reference code), as the HIP path does it: the leaves holding at least
+    // minPoints stored points (all counted points stored, level below
+    // MAX_DEPTH), the densest first with ties by (level, X, Y, Z), up to
+    // maxSplits; the node pool accepts them in key order as in a split round;
+    // each accepted leaf's stored points go through the spill list to its
+    // children (counters = the points per octant), with voxel sampling, chunk
+    // acquisition and insertion as for a batch of no new points.
+    void deepen(uint32_t maxSplits, uint32_t minPoints, uint32_t* candidatesOut, uint32_t* splitsOut,
+                uint64_t* movedOut, uint64_t* voxelsOut) {
+        struct Cand {
+            uint32_t np, level, X, Y, Z, id;
+        };
+        std::vector<Cand> cands;
+        for (uint32_t id = 0; id < nodes.size(); ++id) {
+            const ONode& nd = nodes[id];
+            if (!nd.isLeaf() || nd.level >= uint32_t(kMaxDepth)) continue;
+            if (nd.numPoints < std::max(1u, minPoints) || nd.counter != nd.numPoints) continue;
+            cands.push_back({nd.numPoints, nd.level, nd.X, nd.Y, nd.Z, id});
+        }
+        *candidatesOut = uint32_t(cands.size());
+        std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
+            if (a.np != b.np) return a.np > b.np;
+            return Key{a.level, a.X, a.Y, a.Z} < Key{b.level, b.X, b.Y, b.Z};
+        });
+        if (cands.size() > maxSplits) cands.resize(maxSplits);
+        spilling.clear();
+        for (const Cand& c : cands) spilling.push_back(c.id);
+        std::sort(spilling.begin(), spilling.end(), [&](uint32_t a, uint32_t b) {
+            const ONode &na = nodes[a], &nb = nodes[b];
+            return Key{na.level, na.X, na.Y, na.Z} < Key{nb.level, nb.X, nb.Y, nb.Z};
+        });
+        std::vector<uint32_t> accepted;
+        uint64_t room = nodes.size();
+        for (uint32_t id : spilling) {
+            if (room + 8 > maxNodes) {
+                nodePoolOverflow = true;
+                continue;
+            }
+            room += 8;
+            accepted.push_back(id);
+        }
+        spilling.swap(accepted);
+        spilled.clear();
+        backlogVoxels.clear();
+        backlogTargets.clear();
+        std::vector<std::vector<remo_point>> moved;
+        for (uint32_t id : spilling) {
+            moved.push_back(nodes[id].points);
+            spilled.insert(spilled.end(), nodes[id].points.begin(), nodes[id].points.end());
+        }
+        doSplitting();
+        for (size_t k = 0; k < spilling.size(); ++k) {
+            const ONode& nd = nodes[spilling[k]];
+            for (const remo_point& p : moved[k]) {
+                uint32_t q20[3], q28[3];
+                quantise(p, octreeSize, q20, q28);
+                const uint32_t sh = uint32_t(kMaxDepth) - nd.level - 1;
+                const int o = int((((q20[0] >> sh) & 1u) << 2) | (((q20[1] >> sh) & 1u) << 1) | ((q20[2] >> sh) & 1u));
+                nodes[size_t(nd.children[o])].counter += 1;
+            }
+        }
+        voxelSampling(nullptr, 0);
+        allocatePointChunks();
+        allocateVoxelChunks();
+        *voxelsOut = backlogVoxels.size();
+        insertPoints(nullptr, 0);
+        insertVoxels();
+        *splitsOut = uint32_t(spilling.size());
+        *movedOut = spilled.size();
+        spilling.clear();
+        spilled.clear();
+        sweepStats();
+    }
+
     // addBatch (remolod_octree.cu:940-1060)
     void addBatch(const remo_point* pts, uint32_t n, uint32_t batchIndex) {
         spilled.clear();
@@ -1240,6 +1315,11 @@ void oracle_apply_edl(const remo_shared_uniforms* u, uint64_t* fb) {
             fb[int64_t(y) * w + x] =
                 (pixel & 0xFFFFFFFF00000000ull) | (r & 0xFFu) | ((g & 0xFFu) << 8) | ((b & 0xFFu) << 16) | (255u << 24);
         }
+}
+
+void oracle_deepen(oracle_tree* t, uint32_t maxSplits, uint32_t minPoints, uint32_t* candidates, uint32_t* splits,
+                   uint64_t* moved, uint64_t* voxels) {
+    t->deepen(maxSplits, minPoints, candidates, splits, moved, voxels);
 }
 
 uint32_t oracle_refine(oracle_tree* t, uint32_t minLevel, uint32_t* skipped, uint32_t* empty) {

@@ -901,22 +901,24 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
 
 // insertVoxels (remolod_octree.cu:918-938), ranked per tile in LDS by node like
 // insertPoints.
-__global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b) {
+__global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, uint32_t closeBatch) {
     __shared__ BlockAgg agg;
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t total = min(g->numBacklog, t.backlogCap);
     __syncthreads();  // every thread has read the counters before block 0 ends the batch
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // the batch loop's closing update (remolod_octree.cu:1223-1226)
-        const uint64_t spilled = min(g->numSpilled, t.spillCap);
-        g->batchletIndex += 1;
-        g->numPointsProcessed += *b.sizePtr;
-        g->batches += 1;
-        g->spilledPoints += spilled;
+        if (closeBatch) {
+            // the batch loop's closing update (remolod_octree.cu:1223-1226)
+            const uint64_t spilled = min(g->numSpilled, t.spillCap);
+            g->batchletIndex += 1;
+            g->numPointsProcessed += *b.sizePtr;
+            g->batches += 1;
+            g->spilledPoints += spilled;
+            g->totBatches += 1;
+            g->totSpilledPoints += spilled;
+        }
         g->newVoxels += total;
-        g->totBatches += 1;
-        g->totSpilledPoints += spilled;
         g->totNewVoxels += total;
     }
     const TileRange tr = myTiles(total);
@@ -1083,8 +1085,56 @@ void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hip
     hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
 }
 
-void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
+void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool closeBatch) {
+    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b, closeBatch ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// Refinement's local deepening: leaves split outside Update, their stored points
+// redistributed to the children through the spill list, exactly as a split
+// during a batch would (kSplitApply, kSpillAndZero, then voxel sampling, chunk
+// acquisition and insertion of the spilled points with an empty batch).
+
+struct DeepenCandidate {
+    uint32_t numPoints, level, x, y, z, node;
+};
+
+__global__ __launch_bounds__(kBlock) void kDeepenList(DevTree t, DeepenCandidate* out, uint32_t* count,
+                                                      uint32_t minPoints) {
+    const uint32_t n = t.g->numNodes;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (t.firstChild[i] >= 0 || t.level[i] >= uint32_t(kMaxDepth)) continue;
+        const uint32_t np = t.numPoints[i];
+        if (np < minPoints || t.counter[i] != np) continue;
+        out[atomicAdd(count, 1u)] = DeepenCandidate{np, t.level[i], t.nx[i], t.ny[i], t.nz[i], i};
+    }
+}
+
+// The chosen splits become this round's candidates; a fresh round of counters.
+__global__ void kDeepenBegin(DevTree t, uint32_t numChosen) {
+    Globals* g = t.g;
+    g->numCandidates = numChosen;
+    g->numAccepted = 0;
+    g->numPending = 0;
+    g->numSpilled = 0;
+    g->numBacklog = 0;
+    g->stop = 0;
+}
+
+void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t minPoints, const LaunchShape& ls,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(kDeepenList, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, static_cast<DeepenCandidate*>(out),
+                       count, minPoints);
+}
+
+void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kDeepenBegin, dim3(1), dim3(1), 0, s, t, numChosen);
+    hipLaunchKernelGGL(kSplitApply, dim3(1), dim3(1024), 0, s, t);
+    hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t, 0u);
+    hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
+    hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
+    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);
 }
 
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s) {

@@ -121,6 +121,8 @@ struct remo_ctx {
     std::vector<void*> allocations;
     AnalysisState analysis;
     RefineState refine;
+    void* deepenRecords = nullptr;    // [maxNodes] candidate records (first remo_deepen)
+    uint32_t* deepenCount = nullptr;  // device counter; deepenCount[1] stays 0: an empty batch's size
     AccumWork accWork;
     Profiler prof;
 };
@@ -644,6 +646,64 @@ int remo_refine(remo_ctx* c, float budgetMs, uint32_t maxLevels, remo_refine_sta
     out->cell_overflow = so.cellOverflow;
     out->estimate_ms = so.estimateMs;
     out->device_ms = so.deviceMs;
+    return REMO_OK;
+}
+
+int remo_deepen(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    std::memset(out, 0, sizeof(*out));
+    DevTree& t = c->t;
+    hipStream_t s = c->stream;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    if (c->hostG->memCapacityReached || c->hostG->arenaOffset + kSafetyMargin >= t.arenaCapacity) {
+        out->refused = 1;
+        return REMO_OK;
+    }
+    if (maxSplits == 0) return REMO_OK;
+    if (!c->deepenRecords) {
+        rc = devAlloc(c, reinterpret_cast<uint8_t**>(&c->deepenRecords), uint64_t(t.maxNodes) * kDeepenRecordBytes,
+                      "deepening candidates");
+        if (rc == REMO_OK) rc = devAlloc(c, &c->deepenCount, 2, "deepening counter");
+        if (rc != REMO_OK) return rc;
+        CK(hipMemsetAsync(c->deepenCount, 0, 8, s), "deepening counter");
+    }
+    CK(hipMemsetAsync(c->deepenCount, 0, 4, s), "deepening counter");
+    launchDeepenList(t, c->deepenRecords, c->deepenCount, std::max(1u, minPoints), c->shape, s);
+    CK(hipGetLastError(), "deepening list kernel");
+    uint32_t n = 0;
+    CK(hipMemcpyAsync(c->hostScratch, c->deepenCount, 4, hipMemcpyDeviceToHost, s), "deepening count");
+    CK(hipStreamSynchronize(s), "deepening list");
+    n = std::min(c->hostScratch[0], t.maxNodes);
+    out->candidates = n;
+    if (n == 0) return REMO_OK;
+    struct Rec {
+        uint32_t numPoints, level, x, y, z, node;
+    };
+    static_assert(sizeof(Rec) == kDeepenRecordBytes, "record layout");
+    std::vector<Rec> recs(n);
+    CK(hipMemcpy(recs.data(), c->deepenRecords, sizeof(Rec) * n, hipMemcpyDeviceToHost), "deepening records");
+    // the densest first; ties by canonical key, so the choice is deterministic
+    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+        if (a.numPoints != b.numPoints) return a.numPoints > b.numPoints;
+        if (a.level != b.level) return a.level < b.level;
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        return a.z < b.z;
+    });
+    const uint32_t chosen = std::min(std::min(n, maxSplits), t.candidateCap);
+    std::vector<uint32_t> ids(chosen);
+    for (uint32_t k = 0; k < chosen; ++k) ids[k] = recs[k].node;
+    CK(hipMemcpy(t.candidates, ids.data(), 4ull * chosen, hipMemcpyHostToDevice), "deepening candidates");
+    out->chosen = chosen;
+    launchDeepenSplit(t, chosen, BatchRef{t.spill, c->deepenCount + 1}, c->shape, s);
+    CK(hipGetLastError(), "deepening kernels");
+    rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    out->splits = c->hostG->numAccepted;
+    out->points_moved = std::min<uint64_t>(c->hostG->numSpilled, t.spillCap);
+    out->new_voxels = std::min<uint64_t>(c->hostG->numBacklog, t.backlogCap);
     return REMO_OK;
 }
 

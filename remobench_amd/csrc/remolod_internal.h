@@ -225,7 +225,15 @@ inline uint64_t fixedPyramidEntries(uint32_t d) { return ((1ull << (3 * (d + 1))
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
-void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
+// closeBatch: also the batch loop's closing update (batch index, processed points)
+void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool closeBatch = true);
+// Refinement's local deepening: list the leaves that may split (records of 24 B:
+// numPoints, level, x, y, z, node), then split the chosen ones (already in
+// t.candidates) and redistribute their points; `empty` is a batch of size 0.
+constexpr uint32_t kDeepenRecordBytes = 24;
+void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t minPoints, const LaunchShape& ls,
+                      hipStream_t s);
+void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, const LaunchShape& ls, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);
 void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,

@@ -191,6 +191,18 @@ bool RemolodHipPipeline::refine(float budgetMs, remo_refine_stats* out, std::str
     return true;
 }
 
+bool RemolodHipPipeline::deepen(uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out, std::string* err) {
+    if (!m_ctx || m_needsReset || !out) return false;
+    const int rc = remo_deepen(m_ctx, maxSplits, minPoints, out);
+    if (rc != REMO_OK) {
+        fail("deepen", rc);
+        if (err) *err = m_lastError;
+        return false;
+    }
+    readStats();
+    return true;
+}
+
 void RemolodHipPipeline::render(const FrameContext& frame) {
     // src/pipelines/RemolodPipeline.cpp:559-601.
     if (!m_ctx || m_needsReset || !frame.targets.framebuffer) return;

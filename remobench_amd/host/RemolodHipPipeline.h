@@ -50,6 +50,8 @@ public:
     // Refinement's colour filtering within budgetMs (no reference code; README.md:81):
     // runs only when budget remains, may change nothing.
     bool refine(float budgetMs, remo_refine_stats* out, std::string* err);
+    // Refinement's local deepening: split up to maxSplits of the densest leaves.
+    bool deepen(uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out, std::string* err);
 
     remo_ctx* context() const { return m_ctx; }
     const remo_stats& rawStats() const { return m_raw; }

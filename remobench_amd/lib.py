@@ -156,6 +156,15 @@ class RefineStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class DeepenStats(C.Structure):
+    """remo_deepen_stats (Refinement, local deepening)."""
+    _fields_ = [("candidates", C.c_uint32), ("chosen", C.c_uint32), ("splits", C.c_uint32),
+                ("refused", C.c_uint32), ("points_moved", C.c_uint64), ("new_voxels", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class SceneSpec(C.Structure):
     _fields_ = [("extent", C.c_double), ("num_boxes", C.c_uint32), ("num_cylinders", C.c_uint32),
                 ("cluster_fraction", C.c_double), ("cluster_volume_fraction", C.c_double),
@@ -229,6 +238,7 @@ def load() -> C.CDLL:
         "remo_get_frame_shape": (C.c_int, [C.c_void_p, P(TuneEntry)]),
         "remo_refine": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, P(RefineStats)]),
         "remo_render_flat": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
+        "remo_deepen": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(DeepenStats)]),
         "remo_apply_edl": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_void_p]),
         "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),

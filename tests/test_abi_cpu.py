@@ -40,7 +40,8 @@ def test_struct_sizes_match_the_header():
     assert L.ANALYSIS_DTYPE.itemsize == 48
 
 
-@pytest.mark.parametrize("cname,mirror", [("remo_refine_stats", "RefineStats"), ("remo_stats", "Stats"),
+@pytest.mark.parametrize("cname,mirror", [("remo_refine_stats", "RefineStats"), ("remo_deepen_stats", "DeepenStats"),
+                                           ("remo_stats", "Stats"),
                                            ("remo_tune_entry", "TuneEntry"), ("remo_config", "Config")])
 def test_ctypes_mirrors_match_the_c_layout(cname, mirror, tmp_path):
     """Every field offset and the size of each ctypes mirror, against what a C
