@@ -416,6 +416,54 @@ def main() -> int:
             "calls_to_finish_after_ingest": drain + 1}
         log(f"refinement, interleaved: {refinement['interleaved']}")
 
+    # The full per-frame loop of the 1B-point configuration at this workload's
+    # size: per step one batch of Update + Accumulate, Analysis + Rasterize as the
+    # captured frame graph at the Rasterize resolution (street pose), Refinement
+    # within the budget, and local deepening of the two densest leaves; analytic
+    # then learned score on the same steps. Score error: the learned path's scores
+    # against the analytic ones at half the steps (mean |difference| over the mean
+    # analytic score, nodes with a positive analytic score).
+    full_loop = None
+    if not args.no_refine and analysis_mode is not None:
+        import numpy as np
+        full_loop = {"budget_ms": args.refine_budget_ms, "deepen_per_frame": 2, "pose": "street",
+                     "resolution": f"{rw}x{rh}"}
+        u_loop = camera.poses(sample_pts, meta.box_size, rw, rh)["street"]
+        fbl = C.c_void_p()
+        L.check(L.load().remo_dev_alloc(rw * rh * 8, C.byref(fbl)), "framebuffer")
+        modes = [("analytic", L.REMO_ANALYSIS_ANALYTIC)] + (
+            [("learned", L.REMO_ANALYSIS_LEARNED)] if model is not None else [])
+        for name, mode in modes:
+            tree.reset()
+            tree.sync()
+            frames_ms = []
+            t0 = time.perf_counter()
+            for k in range(steps):
+                f0 = time.perf_counter()
+                _, consumed = tree.build_step(view)
+                tree.accumulate(consumed)
+                tree.frame(u_loop, fbl.value, mode)
+                tree.refine(args.refine_budget_ms)
+                tree.deepen(2, 25_000)
+                frames_ms.append((time.perf_counter() - f0) * 1e3)
+                if name == "learned" and k == steps // 2:
+                    tree.analysis(u_loop, L.REMO_ANALYSIS_ANALYTIC)
+                    sa = tree.analysis_results()["score"].astype(np.float64)
+                    tree.analysis(u_loop, L.REMO_ANALYSIS_LEARNED)
+                    sl = tree.analysis_results()["score"].astype(np.float64)
+                    m = sa > 0
+                    full_loop["score_error_learned"] = round(float(np.abs(sl[m] - sa[m]).mean() /
+                                                                   max(1e-12, sa[m].mean())), 5)
+            tree.sync()
+            dtl = time.perf_counter() - t0
+            fs = sorted(frames_ms)
+            full_loop[name] = {"ingest_mpts": round(tree.stats()["num_points_processed"] / dtl / 1e6, 2),
+                               "frame_ms_avg": round(sum(fs) / len(fs), 3),
+                               "frame_ms_p95": round(fs[int(0.95 * (len(fs) - 1))], 3),
+                               "nodes": tree.stats()["num_nodes"]}
+        L.load().remo_dev_free(fbl)
+        log(f"full loop: {full_loop}")
+
     # PCIe-inclusive streaming: the cloud in pinned host memory (one untimed copy,
     # as a reader would leave it), the reference's 50-slot ring refilled on a copy
     # stream while the same steps run; each step waits only for its own batch
@@ -484,6 +532,7 @@ def main() -> int:
         "rasterize": raster,
         "refinement": refinement,
         "image_quality": image_quality,
+        "full_loop": full_loop,
         "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "max_points_per_node",
                                     "total_expand_iters", "total_spilled_points",
                                     "num_voxels", "max_depth", "allocated_bytes_persistent",
