@@ -27,6 +27,8 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
+constexpr uint32_t kSlotsPerThread = AccumWork::kTile / kBlock;
+static_assert(AccumWork::kTile % kBlock == 0, "a tile is a whole number of slots per thread");
 
 __device__ __forceinline__ uint64_t mortonSpread(uint32_t v) {
     uint64_t x = uint64_t(v) & 0xFFFFFull;
@@ -36,6 +38,26 @@ __device__ __forceinline__ uint64_t mortonSpread(uint32_t v) {
     x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
     x = (x | (x << 2)) & 0x1249249249249249ull;
     return x;
+}
+
+// Morton order of (X, Y, Z) with X the most significant bit of each level,
+// without interleaving: the dimension whose coordinates differ in the highest
+// bit decides (Chan's most-significant-differing-bit comparison).
+__device__ __forceinline__ bool msbLess(uint32_t a, uint32_t b) { return a < b && a < (a ^ b); }
+__device__ __forceinline__ bool mortonLess(uint32_t aX, uint32_t aY, uint32_t aZ, uint32_t bX, uint32_t bY,
+                                           uint32_t bZ) {
+    const uint32_t dx = aX ^ bX, dy = aY ^ bY, dz = aZ ^ bZ;
+    uint32_t a = aX, b = bX, d = dx;
+    if (msbLess(d, dy)) {
+        a = aY;
+        b = bY;
+        d = dy;
+    }
+    if (msbLess(d, dz)) {
+        a = aZ;
+        b = bZ;
+    }
+    return a < b;
 }
 
 template <typename T>
@@ -90,7 +112,10 @@ __global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w) {
     }
 }
 
-// Persistent fold of all tiles into partial sums.
+// Persistent fold of all tiles into partial sums. Tile i goes to hardware block
+// i: a launch holds fewer tiles (~760 on the 100M-point scene) than blocks, and
+// hardware blocks go round-robin to the 8 XCDs, so every XCD gets an equal share
+// (an XCD-contiguous renumbering would hand all of them to the first three).
 __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
     __shared__ long long sq[9][kWaves];
     __shared__ double sg[9][kWaves];
@@ -101,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
     const double fGrid = double(1u << kMaxDepth);
     const uint32_t numTiles = min(*w.numTiles, w.tileCap);
 
-    for (uint32_t ti = xcdBlock(); ti < numTiles; ti += gridDim.x) {
+    for (uint32_t ti = blockIdx.x; ti < numTiles; ti += gridDim.x) {
         const uint4 tile = w.tiles[ti];
         const uint32_t node = tile.x;
         if (node == kNone) continue;  // planned past the capacity
@@ -113,12 +138,23 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
         long long q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         double lg[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         float lc[6] = {0, 0, 0, 0, 0, 0};
-        uint64_t lm = 0;
-        for (uint32_t s = tile.y + threadIdx.x; s < tile.z; s += kBlock) {
+        uint32_t bX = 0, bY = 0, bZ = 0;  // the lane's greatest point in Morton order so far
+        // the lane's loads first, then the sums in slot order (the order of one
+        // slot at a time, so the FP sums are the same)
+        uint32_t ref[kSlotsPerThread];
+        remo_point pts[kSlotsPerThread];
+        for (uint32_t j = 0; j < kSlotsPerThread; ++j) {
+            const uint32_t s = tile.y + threadIdx.x + j * kBlock;
             const uint32_t k = s / kPointsPerChunk;
-            const uint32_t ref = k < kPointDir ? dir[k] : 0u;
-            if (ref == 0) continue;
-            const remo_point p = chunkPtr(t.arena, ref)[s % kPointsPerChunk];
+            ref[j] = (s < tile.z && k < kPointDir) ? dir[k] : 0u;
+        }
+        for (uint32_t j = 0; j < kSlotsPerThread; ++j) {
+            const uint32_t s = tile.y + threadIdx.x + j * kBlock;
+            if (ref[j] != 0) pts[j] = chunkPtr(t.arena, ref[j])[s % kPointsPerChunk];
+        }
+        for (uint32_t j = 0; j < kSlotsPerThread; ++j) {
+            if (ref[j] == 0) continue;
+            const remo_point p = pts[j];
             // remolod_accum.cu:115-132, octreeMin = 0
             const double lx = double(p.x) * invSize;
             const double ly = double(p.y) * invSize;
@@ -127,8 +163,11 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
             const uint32_t X = min(uint32_t(fGrid * lx), maxCoord);
             const uint32_t Y = min(uint32_t(fGrid * ly), maxCoord);
             const uint32_t Z = min(uint32_t(fGrid * lz), maxCoord);
-            const uint64_t m = (mortonSpread(X) << 2) | (mortonSpread(Y) << 1) | mortonSpread(Z);
-            lm = m > lm ? m : lm;
+            if (mortonLess(bX, bY, bZ, X, Y, Z)) {
+                bX = X;
+                bY = Y;
+                bZ = Z;
+            }
             double x = lx * levelScale - NX;
             double y = ly * levelScale - NY;
             double z = lz * levelScale - NZ;
@@ -138,12 +177,14 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
             lg[0] += x; lg[1] += y; lg[2] += z;
             lg[3] += x * x; lg[4] += x * y; lg[5] += x * z;
             lg[6] += y * y; lg[7] += y * z; lg[8] += z * z;
-            const long long qx = (long long)floor(x * 1048576.0) - 524288;
-            const long long qy = (long long)floor(y * 1048576.0) - 524288;
-            const long long qz = (long long)floor(z * 1048576.0) - 524288;
+            // x in [0, 1]: floor(x * 2^20) is an exact int32, and so are the
+            // centred values; their products need 64 bits
+            const int32_t qx = int32_t(floor(x * 1048576.0)) - 524288;
+            const int32_t qy = int32_t(floor(y * 1048576.0)) - 524288;
+            const int32_t qz = int32_t(floor(z * 1048576.0)) - 524288;
             q[0] += qx; q[1] += qy; q[2] += qz;
-            q[3] += qx * qx; q[4] += qx * qy; q[5] += qx * qz;
-            q[6] += qy * qy; q[7] += qy * qz; q[8] += qz * qz;
+            q[3] += (long long)qx * qx; q[4] += (long long)qx * qy; q[5] += (long long)qx * qz;
+            q[6] += (long long)qy * qy; q[7] += (long long)qy * qz; q[8] += (long long)qz * qz;
             const float cr = float((p.color >> 0) & 0xFFu) * (1.0f / 255.0f);
             const float cg = float((p.color >> 8) & 0xFFu) * (1.0f / 255.0f);
             const float cb = float((p.color >> 16) & 0xFFu) * (1.0f / 255.0f);
@@ -163,6 +204,7 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
             if (lane == 0) sc[i][wave] = v;
         }
         {
+            const uint64_t lm = (mortonSpread(bX) << 2) | (mortonSpread(bY) << 1) | mortonSpread(bZ);
             const uint64_t v = waveMax(lm);
             if (lane == 0) sm[wave] = v;
         }

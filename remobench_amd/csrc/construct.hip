@@ -224,18 +224,26 @@ __global__ __launch_bounds__(kBlock) void kCount(DevTree t, BatchRef b, uint32_t
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
         aggClear(agg);
         __syncthreads();
+        uint32_t X[kPerThread], Y[kPerThread], Z[kPerThread], leaf[kPerThread], level[kPerThread];
+        bool valid[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            if (i >= total) continue;
+            valid[j] = i < total;
+            X[j] = Y[j] = Z[j] = 0;
+            if (!valid[j]) continue;
             const remo_point p = streamPoint(t, b, batchSize, i);
-            const uint32_t X = quant28(p.x, size) >> 8;
-            const uint32_t Y = quant28(p.y, size) >> 8;
-            const uint32_t Z = quant28(p.z, size) >> 8;
-            uint32_t level;
-            const uint32_t leaf = descend(t.firstChild, X, Y, Z, level);
-            if (round != 0 && t.pending[leaf] == 0) continue;
+            X[j] = quant28(p.x, size) >> 8;
+            Y[j] = quant28(p.y, size) >> 8;
+            Z[j] = quant28(p.z, size) >> 8;
+        }
+        descendN(t.firstChild, X, Y, Z, valid, leaf, level);
+        bool counted[kPerThread];
+        for (uint32_t j = 0; j < kPerThread; ++j) counted[j] = valid[j] && (round == 0 || t.pending[leaf[j]] != 0);
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            if (!counted[j]) continue;
             // a leaf at MAX_DEPTH never splits; its points count in octant 0
-            const uint32_t key = leaf * 8u + (level < uint32_t(kMaxDepth) ? octantAt(X, Y, Z, level) : 0u);
+            const uint32_t key =
+                leaf[j] * 8u + (level[j] < uint32_t(kMaxDepth) ? octantAt(X[j], Y[j], Z[j], level[j]) : 0u);
             uint32_t rank;
             if (aggInsert(agg, key, &rank) == kEmpty32) atomicAdd(&t.batchOct[key], 1u);
         }
@@ -857,18 +865,27 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
         __syncthreads();
         remo_point p[kPerThread];
         uint32_t leaf[kPerThread], slotOf[kPerThread], rank[kPerThread];
+        uint32_t X[kPerThread], Y[kPerThread], Z[kPerThread], level[kPerThread];
+        bool valid[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            slotOf[j] = kEmpty32;
-            leaf[j] = kNone;
-            if (i >= total) continue;
+            valid[j] = i < total;
+            X[j] = Y[j] = Z[j] = 0;
+            if (!valid[j]) continue;
             p[j] = streamPoint(t, b, batchSize, i);
-            const uint32_t X = quant28(p[j].x, size) >> 8;
-            const uint32_t Y = quant28(p[j].y, size) >> 8;
-            const uint32_t Z = quant28(p[j].z, size) >> 8;
-            uint32_t level;
-            leaf[j] = descend(t.firstChild, X, Y, Z, level);
-            const uint32_t key = leaf[j] * 8u + (level < uint32_t(kMaxDepth) ? octantAt(X, Y, Z, level) : 0u);
+            X[j] = quant28(p[j].x, size) >> 8;
+            Y[j] = quant28(p[j].y, size) >> 8;
+            Z[j] = quant28(p[j].z, size) >> 8;
+        }
+        descendN(t.firstChild, X, Y, Z, valid, leaf, level);
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            slotOf[j] = kEmpty32;
+            if (!valid[j]) {
+                leaf[j] = kNone;
+                continue;
+            }
+            const uint32_t key =
+                leaf[j] * 8u + (level[j] < uint32_t(kMaxDepth) ? octantAt(X[j], Y[j], Z[j], level[j]) : 0u);
             slotOf[j] = aggInsert(agg, key, &rank[j]);
             if (slotOf[j] == kEmpty32) {
                 // table full: take a slot directly

@@ -161,20 +161,37 @@ __device__ __forceinline__ uint32_t quant28(float c, float size) {
     return f2u_sat(__fdiv_rn(num, size));
 }
 
-// Root -> leaf descent on the 20-bit coordinate (remolod_octree.cu:182-199).
-__device__ __forceinline__ uint32_t descend(const int32_t* __restrict__ firstChild, uint32_t X,
-                                            uint32_t Y, uint32_t Z, uint32_t& levelOut) {
-    uint32_t cur = 0;
-    uint32_t level = 0;
-    for (; level < uint32_t(kMaxDepth); ++level) {
-        const int32_t fc = firstChild[cur];
-        if (fc < 0) break;
-        const uint32_t sh = kMaxDepth - level - 1;
-        const uint32_t ci = (((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) | ((Z >> sh) & 1u);
-        cur = uint32_t(fc) + ci;
+// Root -> leaf descent on the 20-bit coordinate (remolod_octree.cu:182-199) for
+// N points of one lane in lock-step: the N chains of dependent firstChild loads
+// are in flight together instead of one after another.
+template <uint32_t N>
+__device__ __forceinline__ void descendN(const int32_t* __restrict__ firstChild, const uint32_t (&X)[N],
+                                         const uint32_t (&Y)[N], const uint32_t (&Z)[N],
+                                         const bool (&valid)[N], uint32_t (&leaf)[N],
+                                         uint32_t (&levelOut)[N]) {
+    bool live[N];
+    for (uint32_t j = 0; j < N; ++j) {
+        leaf[j] = 0;
+        levelOut[j] = 0;
+        live[j] = valid[j];
     }
-    levelOut = level;
-    return cur;
+    for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
+        int32_t fc[N];
+        for (uint32_t j = 0; j < N; ++j) fc[j] = live[j] ? firstChild[leaf[j]] : -1;
+        bool any = false;
+        const uint32_t sh = kMaxDepth - level - 1;
+        for (uint32_t j = 0; j < N; ++j) {
+            if (fc[j] < 0) {
+                live[j] = false;
+                continue;
+            }
+            const uint32_t ci = (((X[j] >> sh) & 1u) << 2) | (((Y[j] >> sh) & 1u) << 1) | ((Z[j] >> sh) & 1u);
+            leaf[j] = uint32_t(fc[j]) + ci;
+            levelOut[j] = level + 1;
+            any = true;
+        }
+        if (!any) break;
+    }
 }
 
 __device__ __forceinline__ uint32_t octantAt(uint32_t X, uint32_t Y, uint32_t Z, uint32_t level) {
