@@ -109,7 +109,9 @@ typedef struct remo_config {
     uint32_t slot_capacity;         /* ring slot stride in points (1,000,000)             */
     uint32_t ring_slots;            /* ring depth the batch index wraps at (50)           */
     uint32_t spill_capacity;        /* spilled points per batch (10,000,000)              */
-    uint32_t backlog_capacity;      /* new voxels per batch (10,000,000)                  */
+    uint32_t backlog_capacity;      /* new voxels per batch (10,000,000), rounded up to a
+                                       multiple of 64: 64 equal segments, one per group of
+                                       workgroups; a full segment sets backlog_overflow    */
     uint32_t flags;                 /* REMO_FLAG_*                                        */
     uint32_t closure_batches;       /* Analysis: a leaf untouched for this many batches is
                                        closed (8); the reference's named alternative to a

@@ -46,37 +46,6 @@ __device__ __forceinline__ uint32_t batchSizeOf(const DevTree& t, BatchRef b) {
     return t.g->stop ? 0u : *b.sizePtr;
 }
 
-// Wave-aggregated atomicAdd(base[key], 1) over the lanes with `valid`; returns
-// this lane's pre-increment value (base + rank in its group). Must be reached
-// by every lane of the wave.
-__device__ __forceinline__ uint32_t waveAggInc(uint32_t* base, uint32_t key, bool valid) {
-    uint64_t pend = __ballot(valid);
-    uint32_t out = 0;
-    const uint32_t lane = laneId();
-    while (pend) {
-        const int leader = __ffsll((unsigned long long)pend) - 1;
-        const uint32_t lk = __shfl(key, leader);
-        const uint64_t grp = __ballot(valid && key == lk) & pend;
-        uint32_t old = 0;
-        if (lane == uint32_t(leader)) old = atomicAdd(&base[lk], uint32_t(__popcll(grp)));
-        old = __shfl(old, leader);
-        if ((grp >> lane) & 1ull) out = old + uint32_t(__popcll(grp & lanesBelow()));
-        pend &= ~grp;
-    }
-    return out;
-}
-
-// Wave-aggregated atomicAdd(&ctr, 1) on one global counter; returns the slot.
-__device__ __forceinline__ uint32_t waveAppend(uint32_t* ctr, bool valid) {
-    const uint64_t m = __ballot(valid);
-    if (m == 0) return 0;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (laneId() == uint32_t(leader)) base = atomicAdd(ctr, uint32_t(__popcll(m)));
-    base = __shfl(base, leader);
-    return base + uint32_t(__popcll(m & lanesBelow()));
-}
-
 // Blocks cover contiguous tile ranges, XCD-aware, so a wave sees 64
 // consecutive (scan-line coherent) points.
 struct TileRange {
@@ -96,7 +65,6 @@ __device__ __forceinline__ TileRange myTiles(uint32_t total) {
 // key instead of one per point (or per wave group).
 constexpr uint32_t kAggSlots = 2048;
 constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
-constexpr unsigned long long kEmpty64 = ~0ull;
 
 struct BlockAgg {
     uint32_t key[kAggSlots];
@@ -172,9 +140,9 @@ __global__ void kBeginStep(DevTree t) {
     g->budgetStop = 0;
 }
 
-__global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t checkBudget,
-                            uint64_t budgetTicks, uint32_t beginStep) {
-    Globals* g = t.g;
+// The batch's opening update by one thread; 0 when the step's budget stops it.
+__device__ uint32_t beginBatchThread0(Globals* g, uint64_t capacity, uint32_t checkBudget, uint64_t budgetTicks,
+                                      uint32_t beginStep) {
     if (beginStep) {
         // the step's first batch also opens the step (kBeginStep, one launch fewer)
         g->stepStart = wall_clock64();
@@ -187,7 +155,7 @@ __global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t c
     if (checkBudget && budgetTicks && wall_clock64() - g->stepStart > budgetTicks) {
         g->stop = 1;
         g->budgetStop = 1;
-        return;
+        return 0;
     }
     g->numSpilled = 0;
     g->numBacklog = 0;
@@ -199,6 +167,18 @@ __global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t c
     const bool reached = g->arenaOffset + kSafetyMargin >= capacity;
     g->memCapacityReached = reached ? 1u : 0u;
     g->stop = reached ? 1u : 0u;
+    return 1;
+}
+
+// Launched with kBacklogShards threads: thread 0 opens the batch, every thread
+// clears one backlog segment counter once the batch is going ahead.
+__global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t checkBudget,
+                            uint64_t budgetTicks, uint32_t beginStep) {
+    Globals* g = t.g;
+    __shared__ uint32_t opened;
+    if (threadIdx.x == 0) opened = beginBatchThread0(g, capacity, checkBudget, budgetTicks, beginStep);
+    __syncthreads();
+    if (opened) t.backlogFill[threadIdx.x * kShardStride] = 0;
     (void)b;
 }
 
@@ -600,22 +580,11 @@ __global__ __launch_bounds__(kBlock) void kFixedSeed(DevTree t) {
 // At steady state most points stop after one peek.
 //
 // New voxels are staged per wave in LDS and reach the global backlog with one
-// atomicAdd per flush; numVoxels is bumped once per (wave, node) group without a
-// return value.
+// atomicAdd per flush, on the counter of the block's backlog segment. numVoxels
+// is counted afterwards from the backlog (kVoxelCount): bumping the per-node
+// counters here sent every wave's atomics to the few lines that hold the
+// counters of the nodes a batch touches (~33 us per 1M-point batch).
 constexpr uint32_t kStage = 128;  // staged voxels per wave
-
-// Wave-aggregated atomicAdd(base[key], 1) for the lanes with `valid`, without a
-// return value (the atomics are fire-and-forget).
-__device__ __forceinline__ void waveAggAdd(uint32_t* base, uint32_t key, bool valid) {
-    uint64_t pend = __ballot(valid);
-    while (pend) {
-        const int leader = __ffsll((unsigned long long)pend) - 1;
-        const uint32_t lk = __shfl(key, leader);
-        const uint64_t grp = __ballot(valid && key == lk) & pend;
-        if (laneId() == uint32_t(leader)) atomicAdd(&base[lk], uint32_t(__popcll(grp)));
-        pend &= ~grp;
-    }
-}
 
 struct VoxelStage {
     remo_point pts[kStage];
@@ -627,16 +596,20 @@ __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uin
     if (count == 0) return;
     // the wave's own LDS writes must land before other lanes read them
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t shard = blockIdx.x & (kBacklogShards - 1u);
     uint32_t base = 0;
-    if (laneId() == 0) base = atomicAdd(&t.g->numBacklog, count);
+    if (laneId() == 0) base = atomicAdd(&t.backlogFill[shard * kShardStride], count);
     base = __shfl(base, 0);
     for (uint32_t k = laneId(); k < count; k += 64) {
         const uint32_t at = base + k;
-        if (at < t.backlogCap) {
-            t.backlogPts[at] = st.pts[k];
-            t.backlogNode[at] = st.node[k];
+        if (at < t.backlogSegCap) {
+            const uint64_t slot = uint64_t(shard) * t.backlogSegCap + at;
+            t.backlogPts[slot] = st.pts[k];
+            t.backlogNode[slot] = st.node[k];
         } else {
+            // a dropped voxel still counts (the backlog pass never sees it)
             t.g->backlogOverflow = 1;
+            atomicAdd(&t.numVoxels[st.node[k]], 1u);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -734,7 +707,6 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                 isNew[j] = (peek[j] & mask[j]) == 0 && (atomicOr(word[j], mask[j]) & mask[j]) == 0;
             for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
                 const uint32_t nd = node[j];
-                waveAggAdd(t.numVoxels, nd, isNew[j]);
                 const uint64_t nm = __ballot(isNew[j]);
                 const uint32_t cnt = uint32_t(__popcll(nm));
                 if (cnt) {
@@ -769,6 +741,66 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
         }
     }
     flushStage(t, st, staged);
+}
+
+// Exclusive prefix over the backlog segments' fills, clipped to the segment size,
+// in LDS; prefix[kBacklogShards] is the number of entries held. Every thread of
+// the block calls it.
+__device__ __forceinline__ void backlogPrefix(const DevTree& t, uint32_t* prefix) {
+    if (threadIdx.x < 64) {
+        static_assert(kBacklogShards == 64, "one lane per segment");
+        const uint32_t lane = threadIdx.x;
+        const uint32_t own = min(t.backlogFill[lane * kShardStride], t.backlogSegCap);
+        uint32_t v = own;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(v, off);
+            if (lane >= off) v += o;
+        }
+        prefix[lane + 1] = v;
+        if (lane == 0) prefix[0] = 0;
+    }
+    __syncthreads();
+}
+
+// Backlog entry i (< prefix[kBacklogShards]) -> its slot in the segmented arrays.
+__device__ __forceinline__ uint64_t backlogSlot(const DevTree& t, const uint32_t* prefix, uint32_t i) {
+    uint32_t lo = 0, hi = kBacklogShards;  // prefix[lo] <= i < prefix[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return uint64_t(lo) * t.backlogSegCap + (i - prefix[lo]);
+}
+
+// numVoxels += the batch's backlog entries per node (the voxel sampling kernel's
+// newVoxels count, remolod_octree.cu:642-668), aggregated per tile in LDS. Also
+// publishes the number of entries held as numBacklog.
+__global__ __launch_bounds__(kBlock) void kVoxelCount(DevTree t) {
+    __shared__ BlockAgg agg;
+    __shared__ uint32_t prefix[kBacklogShards + 1];
+    if (t.g->stop) return;  // the batch did not run: the counters are last batch's
+    backlogPrefix(t, prefix);
+    const uint32_t total = prefix[kBacklogShards];
+    if (blockIdx.x == 0 && threadIdx.x == 0) t.g->numBacklog = total;
+    const TileRange tr = myTiles(total);
+    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        aggClear(agg);
+        __syncthreads();
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            if (i >= total) continue;
+            const uint32_t node = t.backlogNode[backlogSlot(t, prefix, i)];
+            uint32_t rank;
+            if (aggInsert(agg, node, &rank) == kEmpty32) atomicAdd(&t.numVoxels[node], 1u);
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
+            const uint32_t key = agg.key[k];
+            if (key != kEmpty32) atomicAdd(&t.numVoxels[key], agg.cnt[k]);
+        }
+        __syncthreads();
+    }
 }
 
 // allocatePointChunks + allocateVoxelChunks (remolod_octree.cu:742-792, :885-916).
@@ -920,9 +952,11 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
 // insertPoints.
 __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, uint32_t closeBatch) {
     __shared__ BlockAgg agg;
+    __shared__ uint32_t prefix[kBacklogShards + 1];
     Globals* g = t.g;
     if (g->stop) return;
-    const uint32_t total = min(g->numBacklog, t.backlogCap);
+    backlogPrefix(t, prefix);
+    const uint32_t total = prefix[kBacklogShards];
     __syncthreads();  // every thread has read the counters before block 0 ends the batch
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (closeBatch) {
@@ -948,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
             node[j] = kNone;
             slotOf[j] = kEmpty32;
             if (i >= total) continue;
-            node[j] = t.backlogNode[i];
+            node[j] = t.backlogNode[backlogSlot(t, prefix, i)];
             slotOf[j] = aggInsert(agg, node[j], &rank[j]);
             if (slotOf[j] == kEmpty32) rank[j] = atomicAdd(&t.numVoxelsStored[node[j]], 1u);
         }
@@ -967,7 +1001,7 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
             const uint32_t ref =
                 (gi != kNone && k < kVoxelDir) ? t.voxelDir[uint64_t(gi) * kVoxelDir + k] : 0u;
             if (ref != 0)
-                chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[i];
+                chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[backlogSlot(t, prefix, i)];
             else
                 g->dirOverflow = 1;
         }
@@ -1048,7 +1082,7 @@ void launchBeginStep(const DevTree& t, hipStream_t s) {
 
 void launchBeginBatch(const DevTree& t, BatchRef b, bool checkBudget, uint64_t budgetTicks, bool beginStep,
                       hipStream_t s) {
-    hipLaunchKernelGGL(kBeginBatch, dim3(1), dim3(1), 0, s, t, b, t.arenaCapacity, checkBudget ? 1u : 0u,
+    hipLaunchKernelGGL(kBeginBatch, dim3(1), dim3(kBacklogShards), 0, s, t, b, t.arenaCapacity, checkBudget ? 1u : 0u,
                        budgetTicks, beginStep ? 1u : 0u);
 }
 
@@ -1092,6 +1126,7 @@ void launchFixedExpand(const DevTree& t, BatchRef b, const LaunchShape& ls, hipS
 
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
     hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
+    hipLaunchKernelGGL(kVoxelCount, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t);
 }
 
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
@@ -1130,6 +1165,8 @@ __global__ __launch_bounds__(kBlock) void kDeepenList(DevTree t, DeepenCandidate
 // The chosen splits become this round's candidates; a fresh round of counters.
 __global__ void kDeepenBegin(DevTree t, uint32_t numChosen) {
     Globals* g = t.g;
+    t.backlogFill[threadIdx.x * kShardStride] = 0;  // kBacklogShards threads
+    if (threadIdx.x != 0) return;
     g->numCandidates = numChosen;
     g->numAccepted = 0;
     g->numPending = 0;
@@ -1145,10 +1182,11 @@ void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t min
 }
 
 void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(kDeepenBegin, dim3(1), dim3(1), 0, s, t, numChosen);
+    hipLaunchKernelGGL(kDeepenBegin, dim3(1), dim3(kBacklogShards), 0, s, t, numChosen);
     hipLaunchKernelGGL(kSplitApply, dim3(1), dim3(1024), 0, s, t);
     hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t, 0u);
     hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
+    hipLaunchKernelGGL(kVoxelCount, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t);
     hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
     hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
     hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);

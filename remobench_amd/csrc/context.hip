@@ -285,6 +285,9 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     if (!cfg.ring_slots) cfg.ring_slots = REMO_DEFAULT_RING_SLOTS;
     if (!cfg.spill_capacity) cfg.spill_capacity = 10000000u;
     if (!cfg.backlog_capacity) cfg.backlog_capacity = 10000000u;
+    // whole segments (the backlog is cut into kBacklogShards equal segments)
+    cfg.backlog_capacity = uint32_t(std::min<uint64_t>(
+        (uint64_t(cfg.backlog_capacity) + kBacklogShards - 1) / kBacklogShards * kBacklogShards, 0xFFFFFFC0ull));
     if (!cfg.closure_batches) cfg.closure_batches = 8u;
     if (!cfg.persistent_bytes) {
         cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20);
@@ -338,6 +341,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     t.chunkQueueCap = cfg.persistent_bytes / kChunkAllocBytes + 1;
     t.spillCap = cfg.spill_capacity;
     t.backlogCap = cfg.backlog_capacity;
+    t.backlogSegCap = cfg.backlog_capacity / kBacklogShards;
     t.candidateCap = cfg.max_nodes;
     t.fixedDepth = cfg.fixed_depth;
     int rc = REMO_OK;
@@ -368,6 +372,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.spill, t.spillCap, "spilled points");
     A(&t.backlogPts, t.backlogCap, "the voxel backlog");
     A(&t.backlogNode, t.backlogCap, "the voxel backlog targets");
+    A(&t.backlogFill, uint64_t(kBacklogShards) * kShardStride, "the voxel backlog segment counters");
     A(&t.candidates, t.candidateCap, "split candidates");
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 3, "accepted splits");
     A(&t.g, 1, "device counters");
@@ -431,6 +436,7 @@ int remo_reset(remo_ctx* c) {
     // node indices restart at 0: every chunk directory starts empty (splits rely on it)
     CK(hipMemsetAsync(t.pointDir, 0, 4ull * kPointDir * uint64_t(t.maxNodes), c->stream), "reset");
     CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
+    CK(hipMemsetAsync(t.backlogFill, 0, 4ull * kBacklogShards * kShardStride, c->stream), "reset");
     launchResetTree(t, c->stream);
     refineReset(c->refine, t, c->stream);
     CK(hipGetLastError(), "reset kernel");

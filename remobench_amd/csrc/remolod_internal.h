@@ -34,6 +34,11 @@ constexpr uint32_t kPointDir = 256;    // chunk slots per leaf: 256,000 points (
 constexpr uint32_t kVoxelDir = 2112;   // >= ceil(2^21 / 1000) chunk slots per grid
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMaxRounds = 20;    // the expand loop bound, remolod_octree.cu:411
+// The per-batch voxel backlog is cut into segments, each with its own reservation
+// counter on its own 64-byte line: one shared counter took every wave's
+// returning atomic on one address (~20 us per 1M-point batch).
+constexpr uint32_t kBacklogShards = 64;
+constexpr uint32_t kShardStride = 16;  // u32 words between two segment counters
 
 // Device-resident counters; one instance per context.
 struct Globals {
@@ -102,8 +107,10 @@ struct DevTree {
     remo_point* spill;
     uint32_t spillCap;
     uint32_t backlogCap;
-    remo_point* backlogPts;
+    remo_point* backlogPts;  // [backlogCap]: segment s holds [s * backlogSegCap, (s + 1) * backlogSegCap)
     uint32_t* backlogNode;
+    uint32_t* backlogFill;   // [kBacklogShards * kShardStride] entries reserved per segment
+    uint32_t backlogSegCap;  // backlogCap / kBacklogShards
     uint32_t* candidates;   // split candidates of a round
     uint32_t* acceptedSpill;// [candidateCap * 3] (node, spill offset, stored points) of accepted splits
     uint32_t candidateCap;
