@@ -581,9 +581,9 @@ __global__ __launch_bounds__(kBlock) void kFixedSeed(DevTree t) {
 //
 // New voxels are staged per wave in LDS and reach the global backlog with one
 // atomicAdd per flush, on the counter of the block's backlog segment. numVoxels
-// is counted afterwards from the backlog (kVoxelCount): bumping the per-node
-// counters here sent every wave's atomics to the few lines that hold the
-// counters of the nodes a batch touches (~33 us per 1M-point batch).
+// is counted afterwards from the backlog (kVoxelCount, kAllocChunks): bumping
+// the per-node counters here sent every wave's atomics to the few lines that
+// hold the counters of the nodes a batch touches (~33 us per 1M-point batch).
 constexpr uint32_t kStage = 128;  // staged voxels per wave
 
 struct VoxelStage {
@@ -773,8 +773,11 @@ __device__ __forceinline__ uint64_t backlogSlot(const DevTree& t, const uint32_t
     return uint64_t(lo) * t.backlogSegCap + (i - prefix[lo]);
 }
 
-// numVoxels += the batch's backlog entries per node (the voxel sampling kernel's
-// newVoxels count, remolod_octree.cu:642-668), aggregated per tile in LDS. Also
+// Counts the batch's backlog entries per grid (the voxel sampling kernel's
+// newVoxels count, remolod_octree.cu:642-668) and ranks each entry among its
+// grid's entries of its shard (tile index mod kVoxelShards), aggregated per tile in
+// LDS: one returning atomic per (tile, grid) on the shard's counter. kAllocChunks
+// turns the counts into slot ranges; kInsertVoxels then needs no atomics. Also
 // publishes the number of entries held as numBacklog.
 __global__ __launch_bounds__(kBlock) void kVoxelCount(DevTree t) {
     __shared__ BlockAgg agg;
@@ -785,19 +788,30 @@ __global__ __launch_bounds__(kBlock) void kVoxelCount(DevTree t) {
     if (blockIdx.x == 0 && threadIdx.x == 0) t.g->numBacklog = total;
     const TileRange tr = myTiles(total);
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
+        uint32_t* counts = t.voxelShardCount + uint64_t(tile & (kVoxelShards - 1u)) * t.maxGrids;
         aggClear(agg);
         __syncthreads();
+        uint32_t gi[kPerThread], slotOf[kPerThread], rank[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) {
             const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            gi[j] = kNone;
+            slotOf[j] = kEmpty32;
             if (i >= total) continue;
-            const uint32_t node = t.backlogNode[backlogSlot(t, prefix, i)];
-            uint32_t rank;
-            if (aggInsert(agg, node, &rank) == kEmpty32) atomicAdd(&t.numVoxels[node], 1u);
+            gi[j] = t.gridNo[t.backlogNode[backlogSlot(t, prefix, i)]];
+            if (gi[j] == kNone) continue;  // not reached: sampled nodes have grids
+            slotOf[j] = aggInsert(agg, gi[j], &rank[j]);
+            if (slotOf[j] == kEmpty32) rank[j] = atomicAdd(&counts[gi[j]], 1u);
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
             const uint32_t key = agg.key[k];
-            if (key != kEmpty32) atomicAdd(&t.numVoxels[key], agg.cnt[k]);
+            if (key != kEmpty32) agg.base[k] = atomicAdd(&counts[key], agg.cnt[k]);
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < kPerThread; ++j) {
+            if (gi[j] == kNone) continue;
+            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
+            t.backlogRank[i] = slotOf[j] == kEmpty32 ? rank[j] : agg.base[slotOf[j]] + rank[j];
         }
         __syncthreads();
     }
@@ -853,6 +867,22 @@ __global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
         }
         const uint32_t gi = t.gridNo[i];
         if (gi != kNone) {
+            // this batch's backlog entries of the grid take the slots after the
+            // stored voxels, shard after shard (kVoxelCount ranked them per shard)
+            uint32_t added = 0;
+            const uint32_t stored = t.numVoxelsStored[i];
+            for (uint32_t sh = 0; sh < kVoxelShards; ++sh) {
+                const uint64_t at = uint64_t(sh) * t.maxGrids + gi;
+                const uint32_t c = t.voxelShardCount[at];
+                if (c == 0) continue;
+                t.voxelShardStart[at] = stored + added;
+                t.voxelShardCount[at] = 0;
+                added += c;
+            }
+            if (added) {
+                t.numVoxels[i] += added;
+                t.numVoxelsStored[i] = stored + added;
+            }
             const uint32_t need = (t.numVoxels[i] + kPointsPerChunk - 1) / kPointsPerChunk;
             const uint32_t have = t.voxelChunks[gi];
             if (need > have) {
@@ -948,10 +978,9 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
     }
 }
 
-// insertVoxels (remolod_octree.cu:918-938), ranked per tile in LDS by node like
-// insertPoints.
+// insertVoxels (remolod_octree.cu:918-938): every backlog entry goes to slot
+// start(shard, grid) + rank, both taken before (kVoxelCount, kAllocChunks).
 __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, uint32_t closeBatch) {
-    __shared__ BlockAgg agg;
     __shared__ uint32_t prefix[kBacklogShards + 1];
     Globals* g = t.g;
     if (g->stop) return;
@@ -972,40 +1001,18 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
         g->newVoxels += total;
         g->totNewVoxels += total;
     }
-    const TileRange tr = myTiles(total);
-    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
-        aggClear(agg);
-        __syncthreads();
-        uint32_t node[kPerThread], slotOf[kPerThread], rank[kPerThread];
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            node[j] = kNone;
-            slotOf[j] = kEmpty32;
-            if (i >= total) continue;
-            node[j] = t.backlogNode[backlogSlot(t, prefix, i)];
-            slotOf[j] = aggInsert(agg, node[j], &rank[j]);
-            if (slotOf[j] == kEmpty32) rank[j] = atomicAdd(&t.numVoxelsStored[node[j]], 1u);
-        }
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
-            const uint32_t key = agg.key[k];
-            if (key != kEmpty32) agg.base[k] = atomicAdd(&t.numVoxelsStored[key], agg.cnt[k]);
-        }
-        __syncthreads();
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            if (node[j] == kNone) continue;
-            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            const uint32_t slot = slotOf[j] == kEmpty32 ? rank[j] : agg.base[slotOf[j]] + rank[j];
-            const uint32_t gi = t.gridNo[node[j]];
-            const uint32_t k = slot / kPointsPerChunk;
-            const uint32_t ref =
-                (gi != kNone && k < kVoxelDir) ? t.voxelDir[uint64_t(gi) * kVoxelDir + k] : 0u;
-            if (ref != 0)
-                chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[backlogSlot(t, prefix, i)];
-            else
-                g->dirOverflow = 1;
-        }
-        __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const uint64_t at = backlogSlot(t, prefix, i);
+        const uint32_t gi = t.gridNo[t.backlogNode[at]];
+        if (gi == kNone) continue;
+        const uint32_t sh = (i / kTile) & (kVoxelShards - 1u);
+        const uint32_t slot = t.voxelShardStart[uint64_t(sh) * t.maxGrids + gi] + t.backlogRank[i];
+        const uint32_t k = slot / kPointsPerChunk;
+        const uint32_t ref = k < kVoxelDir ? t.voxelDir[uint64_t(gi) * kVoxelDir + k] : 0u;
+        if (ref != 0)
+            chunkPtr(t.arena, ref)[slot % kPointsPerChunk] = t.backlogPts[at];
+        else
+            g->dirOverflow = 1;
     }
 }
 

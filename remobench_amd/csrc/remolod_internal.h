@@ -39,6 +39,10 @@ constexpr uint32_t kMaxRounds = 20;    // the expand loop bound, remolod_octree.
 // returning atomic on one address (~20 us per 1M-point batch).
 constexpr uint32_t kBacklogShards = 64;
 constexpr uint32_t kShardStride = 16;  // u32 words between two segment counters
+// A batch's new voxels of one grid are counted and ranked in kVoxelShards
+// separate ranges (backlog tile index mod kVoxelShards), so the hot grids' counters
+// take an eighth of the same-address atomics each.
+constexpr uint32_t kVoxelShards = 8;
 
 // Device-resident counters; one instance per context.
 struct Globals {
@@ -111,6 +115,9 @@ struct DevTree {
     uint32_t* backlogNode;
     uint32_t* backlogFill;   // [kBacklogShards * kShardStride] entries reserved per segment
     uint32_t backlogSegCap;  // backlogCap / kBacklogShards
+    uint32_t* backlogRank;   // [backlogCap] entry i's rank among its grid's entries of its voxel shard
+    uint32_t* voxelShardCount;  // [kVoxelShards * maxGrids] this batch's entries per (shard, grid)
+    uint32_t* voxelShardStart;  // [kVoxelShards * maxGrids] first voxel slot of each (shard, grid) range
     uint32_t* candidates;   // split candidates of a round
     uint32_t* acceptedSpill;// [candidateCap * 3] (node, spill offset, stored points) of accepted splits
     uint32_t candidateCap;
