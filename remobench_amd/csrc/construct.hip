@@ -620,8 +620,11 @@ __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uin
 // returning atomics, so more waves in flight beat more points per lane
 // (measured on the 100M-point scene: 232 us per 1M-point batch at one point per
 // lane, 256 us at four, 258 us at eight).
-constexpr uint32_t kVoxelPerThread = 1;
-constexpr uint32_t kVoxelTile = kBlock * kVoxelPerThread;
+constexpr uint32_t kVoxelTile = kBlock;
+// The descent keeps its last kPathRegs ancestors (node, grid) in registers, so a
+// climb step is a peek and an atomic, not also a parent and a grid load; a climb
+// above them (trees deeper than kPathRegs levels below its start) loads them.
+constexpr uint32_t kPathRegs = 4;
 
 __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
     __shared__ VoxelStage stage[kBlock / 64];
@@ -634,110 +637,106 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
     VoxelStage& st = stage[threadIdx.x >> 6];
     uint32_t staged = 0;  // wave-uniform
     for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
-        uint32_t pX[kVoxelPerThread], pY[kVoxelPerThread], pZ[kVoxelPerThread], col[kVoxelPerThread];
-        uint32_t node[kVoxelPerThread], lvl[kVoxelPerThread];
-        bool active[kVoxelPerThread];
-        for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
-            const uint32_t i = tile * kVoxelTile + j * kBlock + threadIdx.x;
-            active[j] = i < total;
-            pX[j] = pY[j] = pZ[j] = col[j] = 0;
-            if (active[j]) {
-                const remo_point p = streamPoint(t, b, batchSize, i);
-                pX[j] = quant28(p.x, size);
-                pY[j] = quant28(p.y, size);
-                pZ[j] = quant28(p.z, size);
-                col[j] = p.color;
-            }
+        const uint32_t i = tile * kVoxelTile + threadIdx.x;
+        bool active = i < total;
+        uint32_t pX = 0, pY = 0, pZ = 0, col = 0;
+        if (active) {
+            const remo_point p = streamPoint(t, b, batchSize, i);
+            pX = quant28(p.x, size);
+            pY = quant28(p.y, size);
+            pZ = quant28(p.z, size);
+            col = p.color;
         }
-        // descend to the leaf; the climb starts at the deepest node with a grid
-        // (the leaf's parent, or the root while it is still a leaf)
-        uint32_t parentOf[kVoxelPerThread];
-        for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
-            node[j] = 0;
-            lvl[j] = 0;
-            parentOf[j] = kNone;
+        // descend to the leaf; pathNode[0] / pathGrid[0] is the leaf's parent
+        uint32_t pathNode[kPathRegs], pathGrid[kPathRegs];
+        for (uint32_t k = 0; k < kPathRegs; ++k) {
+            pathNode[k] = kNone;
+            pathGrid[k] = 0;
         }
-        for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
-            bool any = false;
-            uint2 rec[kVoxelPerThread];
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j) rec[j] = active[j] ? t.walk[node[j]] : make_uint2(kNone, 0u);
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
-                if (!active[j] || rec[j].x == kNone || lvl[j] != level) continue;
-                const uint32_t s2 = kMaxDepth - level - 1;
-                const uint32_t X = pX[j] >> 8, Y = pY[j] >> 8, Z = pZ[j] >> 8;
-                parentOf[j] = node[j];
-                node[j] = rec[j].x + ((((X >> s2) & 1u) << 2) | (((Y >> s2) & 1u) << 1) | ((Z >> s2) & 1u));
-                lvl[j] = level + 1;
-                any = true;
-            }
-            if (__ballot(any) == 0) break;
-        }
-        uint32_t gref[kVoxelPerThread];
-        for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
-            gref[j] = active[j] ? t.walk[node[j]].y : 0u;
-            if (active[j] && gref[j] == 0) {
-                if (parentOf[j] == kNone) {
-                    active[j] = false;  // a root without a grid: nothing to sample
-                } else {
-                    node[j] = parentOf[j];
-                    lvl[j] -= 1;
+        uint32_t node = 0, lvl = 0, gref = 0;
+        if (active) {
+            const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
+            for (uint32_t level = 0;; ++level) {
+                const uint2 rec = t.walk[node];
+                if (rec.x == kNone || level == uint32_t(kMaxDepth)) {
+                    gref = rec.y;
+                    break;
                 }
+                for (uint32_t k = kPathRegs - 1; k > 0; --k) {
+                    pathNode[k] = pathNode[k - 1];
+                    pathGrid[k] = pathGrid[k - 1];
+                }
+                pathNode[0] = node;
+                pathGrid[0] = rec.y;
+                const uint32_t s2 = kMaxDepth - level - 1;
+                node = rec.x + ((((X >> s2) & 1u) << 2) | (((Y >> s2) & 1u) << 1) | ((Z >> s2) & 1u));
+                lvl = level + 1;
             }
+        }
+        // the climb starts at the deepest node with a grid: the root while it is
+        // still a leaf, else the leaf's parent
+        auto up = [&]() {
+            lvl -= 1;
+            if (pathNode[0] != kNone) {
+                node = pathNode[0];
+                gref = pathGrid[0];
+                for (uint32_t k = 0; k + 1 < kPathRegs; ++k) {
+                    pathNode[k] = pathNode[k + 1];
+                    pathGrid[k] = pathGrid[k + 1];
+                }
+                pathNode[kPathRegs - 1] = kNone;
+            } else {
+                node = t.parent[node];
+                gref = t.walk[node].y;
+            }
+        };
+        if (active && gref == 0) {
+            if (lvl == 0)
+                active = false;  // a root without a grid: nothing to sample
+            else
+                up();
         }
         // climb while new cells are set
         for (uint32_t step = 0; step <= uint32_t(kMaxDepth); ++step) {
-            bool any = false;
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j) any |= active[j];
-            if (__ballot(any) == 0) break;
-            uint32_t vi[kVoxelPerThread], mask[kVoxelPerThread], peek[kVoxelPerThread];
-            uint32_t* word[kVoxelPerThread];
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
-                gref[j] = active[j] ? t.walk[node[j]].y : 0u;
-                if (gref[j] == 0) active[j] = false;  // no grid here (allocation overflow)
-                const uint32_t sh = (kMaxDepth + 1) - lvl[j];
-                vi[j] = ((pX[j] >> sh) & (kGridSize - 1)) + ((pY[j] >> sh) & (kGridSize - 1)) * kGridSize +
-                        ((pZ[j] >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
-                mask[j] = 1u << (vi[j] & 31u);
-                word[j] = reinterpret_cast<uint32_t*>(t.arena + (uint64_t(gref[j]) << 4)) + (vi[j] >> 5);
-            }
-            // independent peeks, then independent atomics; a stale peek only costs an atomicOr
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j) peek[j] = active[j] ? *word[j] : 0xFFFFFFFFu;
-            bool isNew[kVoxelPerThread];
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j)
-                isNew[j] = (peek[j] & mask[j]) == 0 && (atomicOr(word[j], mask[j]) & mask[j]) == 0;
-            for (uint32_t j = 0; j < kVoxelPerThread; ++j) {
-                const uint32_t nd = node[j];
-                const uint64_t nm = __ballot(isNew[j]);
-                const uint32_t cnt = uint32_t(__popcll(nm));
-                if (cnt) {
-                    if (staged + cnt > kStage) flushStage(t, st, staged);
-                    if (isNew[j]) {
-                        const uint32_t cx = vi[j] & (kGridSize - 1);
-                        const uint32_t cy = (vi[j] >> 7) & (kGridSize - 1);
-                        const uint32_t cz = vi[j] >> 14;
-                        const float nodeSize = size / float(1u << lvl[j]);
-                        const float minx = float(t.nx[nd]) * nodeSize;
-                        const float miny = float(t.ny[nd]) * nodeSize;
-                        const float minz = float(t.nz[nd]) * nodeSize;
-                        remo_point v;
-                        v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
-                        v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
-                        v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
-                        v.color = col[j];
-                        const uint32_t slot = staged + uint32_t(__popcll(nm & lanesBelow()));
-                        st.pts[slot] = v;
-                        st.node[slot] = nd;
-                    }
-                    staged += cnt;
+            if (__ballot(active) == 0) break;
+            if (gref == 0) active = false;  // no grid here (allocation overflow)
+            const uint32_t sh = (kMaxDepth + 1) - lvl;
+            const uint32_t vi = ((pX >> sh) & (kGridSize - 1)) + ((pY >> sh) & (kGridSize - 1)) * kGridSize +
+                                ((pZ >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
+            const uint32_t mask = 1u << (vi & 31u);
+            uint32_t* word = reinterpret_cast<uint32_t*>(t.arena + (uint64_t(gref) << 4)) + (vi >> 5);
+            // a stale peek only costs an atomicOr
+            const uint32_t peek = active ? *word : 0xFFFFFFFFu;
+            const bool isNew = (peek & mask) == 0 && (atomicOr(word, mask) & mask) == 0;
+            const uint32_t nd = node;
+            const uint64_t nm = __ballot(isNew);
+            const uint32_t cnt = uint32_t(__popcll(nm));
+            if (cnt) {
+                if (staged + cnt > kStage) flushStage(t, st, staged);
+                if (isNew) {
+                    const uint32_t cx = vi & (kGridSize - 1);
+                    const uint32_t cy = (vi >> 7) & (kGridSize - 1);
+                    const uint32_t cz = vi >> 14;
+                    const float nodeSize = size / float(1u << lvl);
+                    const float minx = float(t.nx[nd]) * nodeSize;
+                    const float miny = float(t.ny[nd]) * nodeSize;
+                    const float minz = float(t.nz[nd]) * nodeSize;
+                    remo_point v;
+                    v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
+                    v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
+                    v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
+                    v.color = col;
+                    const uint32_t slot = staged + uint32_t(__popcll(nm & lanesBelow()));
+                    st.pts[slot] = v;
+                    st.node[slot] = nd;
                 }
-                // a set bit means every ancestor cell is set: stop; a new one climbs
-                if (!isNew[j] || lvl[j] == 0) {
-                    active[j] = false;
-                } else {
-                    node[j] = t.parent[nd];
-                    lvl[j] -= 1;
-                }
+                staged += cnt;
             }
+            // a set bit means every ancestor cell is set: stop; a new one climbs
+            if (!isNew || lvl == 0)
+                active = false;
+            else
+                up();
         }
     }
     flushStage(t, st, staged);
