@@ -330,6 +330,20 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         }
         const uint32_t c0 = uint32_t(childOffset);
         const uint32_t lvl = t.level[node] + 1;
+        // the split's counter reservations, issued together: each is a returning
+        // atomic on one global counter, and waiting for one before issuing the
+        // next made a round's splits a chain of five memory round trips
+        const uint32_t np = t.numPoints[node];
+        const uint32_t nChunks = (np + kPointsPerChunk - 1) / kPointsPerChunk;
+        const bool needGrid = t.gridNo[node] == kNone;
+        const uint32_t spillAt = np > 0 ? atomicAdd(&g->numSpilled, np) : 0u;
+        const int64_t top = np > 0 ? int64_t(atomicAdd((unsigned long long*)&g->numAllocatedChunks,
+                                                       (unsigned long long)(-int64_t(nChunks))))
+                                   : 0;
+        const uint32_t slot = atomicAdd(&g->numAccepted, 1u);
+        const uint32_t gi = needGrid ? atomicAdd(&g->numGrids, 1u) : 0u;
+        const unsigned long long gridAt =
+            needGrid ? atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes) : 0ull;
         atomicMax(&g->maxLevel, lvl);
         const uint32_t X = t.nx[node], Y = t.ny[node], Z = t.nz[node];
         for (uint32_t i = 0; i < 8; ++i) {
@@ -362,38 +376,31 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * node);
         so[0] = so[1] = make_uint4(0, 0, 0, 0);
 
-        // stored points -> spill list (copied by kSpillCopy); chunks -> queue
-        const uint32_t np = t.numPoints[node];
+        // stored points -> spill list (copied by kSpillAndZero); chunks -> queue
         uint32_t spillOff = kNone;
         if (np > 0) {
-            const uint32_t off = atomicAdd(&g->numSpilled, np);
-            if (uint64_t(off) + np <= t.spillCap)
-                spillOff = off;
+            if (uint64_t(spillAt) + np <= t.spillCap)
+                spillOff = spillAt;
             else
                 g->spillOverflow = 1;
-            const uint32_t nChunks = (np + kPointsPerChunk - 1) / kPointsPerChunk;
-            const int64_t top = atomicAdd((unsigned long long*)&g->numAllocatedChunks,
-                                          (unsigned long long)(-int64_t(nChunks)));
             for (uint32_t k = 0; k < nChunks; ++k) {
                 const int64_t qi = top - 1 - int64_t(k);
                 if (qi >= 0 && uint64_t(qi) < t.chunkQueueCap)
                     t.chunkQueue[qi] = t.pointDir[uint64_t(node) * kPointDir + k];
             }
         }
-        const uint32_t slot = atomicAdd(&g->numAccepted, 1u);
         t.acceptedSpill[3 * slot] = node;
         t.acceptedSpill[3 * slot + 1] = spillOff;
         t.acceptedSpill[3 * slot + 2] = np;
         t.numPoints[node] = 0;  // remolod_octree.cu:362; kSpillAndZero copies from the recorded count
 
-        if (t.gridNo[node] == kNone) {
-            const uint32_t gi = atomicAdd(&g->numGrids, 1u);
-            const unsigned long long off =
-                atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes);
-            if (gi < t.maxGrids && off + kGridAllocBytes <= t.arenaCapacity) {
-                t.gridOffset[gi] = off;
+        uint32_t gno = t.gridNo[node];
+        if (needGrid) {
+            if (gi < t.maxGrids && gridAt + kGridAllocBytes <= t.arenaCapacity) {
+                t.gridOffset[gi] = gridAt;
                 t.voxelChunks[gi] = 0;
                 t.gridNo[node] = gi;
+                gno = gi;
             } else {
                 g->allocOverflow = 1;
             }
@@ -402,7 +409,6 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         atomicAdd((unsigned long long*)&g->totNodesSplit, 1ull);
         // publish the children last: later kernels descend through them
         t.firstChild[node] = int32_t(c0);
-        const uint32_t gno = t.gridNo[node];
         t.walk[node] = make_uint2(c0, gno == kNone ? 0u : uint32_t(t.gridOffset[gno] >> 4));
     }
 }
@@ -816,58 +822,50 @@ __global__ __launch_bounds__(kBlock) void kVoxelCount(DevTree t) {
     }
 }
 
+// Exclusive prefix sum over the wave's lanes; `total` gets the wave's sum.
+template <typename T>
+__device__ __forceinline__ T waveExclusiveSum(T v, T& total) {
+    const uint32_t lane = laneId();
+    T inc = v;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const T o = __shfl_up(inc, off);
+        if (lane >= off) inc += o;
+    }
+    total = __shfl(inc, 63);
+    return inc - v;
+}
+
 // allocatePointChunks + allocateVoxelChunks (remolod_octree.cu:742-792, :885-916).
 // Point chunks come from the recycled queue while the in-use count is below the
-// pool's high-water mark, from the persistent store beyond it.
+// pool's high-water mark, from the persistent store beyond it. A wave reserves
+// its lanes' chunks with one atomic on the chunk counter and one on the store's
+// offset (one per node took a returning atomic on each of these two addresses).
 __global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
     Globals* g = t.g;
     if (g->stop) return;
     const uint32_t n = g->numNodes;
     const int64_t poolSize = g->chunkPoolSize;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (t.firstChild[i] < 0) {
+    const uint32_t lane = laneId();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        // point chunks a leaf lacks for its counter
+        uint32_t have = 0, add = 0;
+        if (valid && t.firstChild[i] < 0) {
             const uint32_t np = t.numPoints[i], cnt = t.counter[i];
             if (np < cnt) {
-                const uint32_t have = (np + kPointsPerChunk - 1) / kPointsPerChunk;
+                have = (np + kPointsPerChunk - 1) / kPointsPerChunk;
                 const uint32_t need = (cnt + kPointsPerChunk - 1) / kPointsPerChunk;
-                if (need > have) {
-                    const uint32_t add = need - have;
-                    const int64_t first = int64_t(atomicAdd(
-                        (unsigned long long*)&g->numAllocatedChunks, (unsigned long long)add));
-                    // indices at or above the pool size need fresh memory
-                    const int64_t freshFrom = first > poolSize ? first : poolSize;
-                    const int64_t last = first + add;
-                    const uint32_t fresh = last > freshFrom ? uint32_t(last - freshFrom) : 0u;
-                    uint64_t off = 0;
-                    if (fresh)
-                        off = atomicAdd((unsigned long long*)&g->arenaOffset,
-                                        (unsigned long long)(fresh * kChunkAllocBytes));
-                    uint32_t f = 0;
-                    for (uint32_t k = 0; k < add; ++k) {
-                        const int64_t qi = first + k;
-                        uint32_t ref = 0;
-                        if (qi >= poolSize) {
-                            const uint64_t o = off + uint64_t(f++) * kChunkAllocBytes;
-                            if (o + kChunkAllocBytes <= t.arenaCapacity)
-                                ref = uint32_t(o >> 4);
-                            else
-                                g->allocOverflow = 1;
-                        } else if (uint64_t(qi) < t.chunkQueueCap) {
-                            ref = t.chunkQueue[qi];
-                        }
-                        const uint32_t slot = have + k;
-                        if (slot < kPointDir)
-                            t.pointDir[uint64_t(i) * kPointDir + slot] = ref;
-                        else
-                            g->dirOverflow = 1;
-                    }
-                }
+                if (need > have) add = need - have;
             }
         }
-        const uint32_t gi = t.gridNo[i];
+        // voxel chunks a grid lacks; this batch's backlog entries of the grid take
+        // the slots after the stored voxels, shard after shard (kVoxelCount ranked
+        // them per shard)
+        const uint32_t gi = valid ? t.gridNo[i] : kNone;
+        uint32_t vHave = 0, vAdd = 0;
         if (gi != kNone) {
-            // this batch's backlog entries of the grid take the slots after the
-            // stored voxels, shard after shard (kVoxelCount ranked them per shard)
             uint32_t added = 0;
             const uint32_t stored = t.numVoxelsStored[i];
             for (uint32_t sh = 0; sh < kVoxelShards; ++sh) {
@@ -878,29 +876,73 @@ __global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
                 t.voxelShardCount[at] = 0;
                 added += c;
             }
+            uint32_t nv = t.numVoxels[i];
             if (added) {
-                t.numVoxels[i] += added;
+                nv += added;
+                t.numVoxels[i] = nv;
                 t.numVoxelsStored[i] = stored + added;
             }
-            const uint32_t need = (t.numVoxels[i] + kPointsPerChunk - 1) / kPointsPerChunk;
-            const uint32_t have = t.voxelChunks[gi];
-            if (need > have) {
-                const uint32_t add = need - have;
-                const uint64_t off = atomicAdd((unsigned long long*)&g->arenaOffset,
-                                               (unsigned long long)(add * kChunkAllocBytes));
-                for (uint32_t k = 0; k < add; ++k) {
-                    const uint64_t o = off + uint64_t(k) * kChunkAllocBytes;
-                    const uint32_t slot = have + k;
-                    if (o + kChunkAllocBytes > t.arenaCapacity) {
-                        g->allocOverflow = 1;
-                    } else if (slot < kVoxelDir) {
-                        t.voxelDir[uint64_t(gi) * kVoxelDir + slot] = uint32_t(o >> 4);
-                    } else {
-                        g->dirOverflow = 1;
-                    }
-                }
-                t.voxelChunks[gi] = need;
+            const uint32_t need = (nv + kPointsPerChunk - 1) / kPointsPerChunk;
+            vHave = t.voxelChunks[gi];
+            if (need > vHave) vAdd = need - vHave;
+        }
+        // the wave's chunk indices, then its fresh memory (indices at or above the
+        // pool size, and every voxel chunk)
+        uint32_t addTotal;
+        const uint32_t addBefore = waveExclusiveSum(add, addTotal);
+        int64_t waveFirst = 0;
+        if (addTotal) {
+            if (lane == 0)
+                waveFirst = int64_t(atomicAdd((unsigned long long*)&g->numAllocatedChunks,
+                                              (unsigned long long)addTotal));
+            waveFirst = __shfl(waveFirst, 0);
+        }
+        const int64_t first = waveFirst + addBefore;
+        const int64_t freshFrom = first > poolSize ? first : poolSize;
+        const int64_t last = first + add;
+        const uint32_t fresh = (add && last > freshFrom) ? uint32_t(last - freshFrom) : 0u;
+        unsigned long long bytesTotal;
+        const unsigned long long bytesBefore =
+            waveExclusiveSum((unsigned long long)(fresh + vAdd) * kChunkAllocBytes, bytesTotal);
+        unsigned long long waveOff = 0;
+        if (bytesTotal) {
+            if (lane == 0) waveOff = atomicAdd((unsigned long long*)&g->arenaOffset, bytesTotal);
+            waveOff = __shfl(waveOff, 0);
+        }
+        uint64_t off = waveOff + bytesBefore;
+        uint32_t f = 0;
+        for (uint32_t k = 0; k < add; ++k) {
+            const int64_t qi = first + k;
+            uint32_t ref = 0;
+            if (qi >= poolSize) {
+                const uint64_t o = off + uint64_t(f++) * kChunkAllocBytes;
+                if (o + kChunkAllocBytes <= t.arenaCapacity)
+                    ref = uint32_t(o >> 4);
+                else
+                    g->allocOverflow = 1;
+            } else if (uint64_t(qi) < t.chunkQueueCap) {
+                ref = t.chunkQueue[qi];
             }
+            const uint32_t slot = have + k;
+            if (slot < kPointDir)
+                t.pointDir[uint64_t(i) * kPointDir + slot] = ref;
+            else
+                g->dirOverflow = 1;
+        }
+        off += uint64_t(fresh) * kChunkAllocBytes;
+        if (vAdd) {
+            for (uint32_t k = 0; k < vAdd; ++k) {
+                const uint64_t o = off + uint64_t(k) * kChunkAllocBytes;
+                const uint32_t slot = vHave + k;
+                if (o + kChunkAllocBytes > t.arenaCapacity) {
+                    g->allocOverflow = 1;
+                } else if (slot < kVoxelDir) {
+                    t.voxelDir[uint64_t(gi) * kVoxelDir + slot] = uint32_t(o >> 4);
+                } else {
+                    g->dirOverflow = 1;
+                }
+            }
+            t.voxelChunks[gi] = vHave + vAdd;
         }
     }
 }
