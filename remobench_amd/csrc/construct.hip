@@ -130,6 +130,9 @@ __global__ void kZeroRootGrid(DevTree t) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kGridWords / 4;
          i += gridDim.x * blockDim.x)
         w[i] = make_uint4(0, 0, 0, 0);
+    // every top-index cell starts at the root (node 0, level 0)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kTopCells; i += gridDim.x * blockDim.x)
+        t.topCell[i] = 0u;
 }
 
 __global__ void kBeginStep(DevTree t) {
@@ -216,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void kCount(DevTree t, BatchRef b, uint32_t
             Y[j] = quant28(p.y, size) >> 8;
             Z[j] = quant28(p.z, size) >> 8;
         }
-        descendN(t.firstChild, X, Y, Z, valid, leaf, level);
+        descendN(t.firstChild, t.topCell, X, Y, Z, valid, leaf, level);
         bool counted[kPerThread];
         for (uint32_t j = 0; j < kPerThread; ++j) counted[j] = valid[j] && (round == 0 || t.pending[leaf[j]] != 0);
         for (uint32_t j = 0; j < kPerThread; ++j) {
@@ -297,10 +300,16 @@ __device__ __forceinline__ bool keyLess4(const uint4& a, const uint4& b) {
     return a.w < b.w;
 }
 
+// Splits whose children are at most kTopLevel deep: one per node above that level.
+constexpr uint32_t kTopSplitSlots = (kTopCells - 1u) / 7u;  // 1 + 8 + ... + 8^(kTopLevel-1)
+
 __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
     __shared__ uint4 keys[kRankSlots];  // (level, X, Y, Z) of each candidate
+    __shared__ uint4 topSplits[kTopSplitSlots];  // (X, Y, Z, first child | child level << 27)
+    __shared__ uint32_t numTopSplits;
     Globals* g = t.g;
     if (g->stop) return;
+    if (threadIdx.x == 0) numTopSplits = 0;
     const uint32_t nc = min(g->numCandidates, t.candidateCap);
     const uint32_t base = g->numNodes;
     const bool staged = nc <= kRankSlots;
@@ -410,6 +419,23 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         // publish the children last: later kernels descend through them
         t.firstChild[node] = int32_t(c0);
         t.walk[node] = make_uint2(c0, gno == kNone ? 0u : uint32_t(t.gridOffset[gno] >> 4));
+        if (lvl <= kTopLevel) topSplits[atomicAdd(&numTopSplits, 1u)] = make_uint4(X, Y, Z, c0 | (lvl << kTopLevelShift));
+    }
+    // the split nodes' top-index cells now start at the children holding them
+    // (the whole block writes them: a split near the root covers thousands)
+    __syncthreads();
+    for (uint32_t q = 0; q < numTopSplits; ++q) {
+        const uint4 sp = topSplits[q];
+        const uint32_t c0 = sp.w & kTopNodeMask, lvl = sp.w >> kTopLevelShift;
+        const uint32_t sb = kTopLevel - lvl;  // a child spans 2^sb top cells per axis
+        const uint32_t m = (1u << sb) - 1u;
+        for (uint32_t k = threadIdx.x; k < (8u << (3 * sb)); k += blockDim.x) {
+            const uint32_t ci = k >> (3 * sb);
+            const uint32_t cx = ((2 * sp.x + ((ci >> 2) & 1u)) << sb) | ((k >> (2 * sb)) & m);
+            const uint32_t cy = ((2 * sp.y + ((ci >> 1) & 1u)) << sb) | ((k >> sb) & m);
+            const uint32_t cz = ((2 * sp.z + (ci & 1u)) << sb) | (k & m);
+            t.topCell[(cx << (2 * kTopLevel)) | (cy << kTopLevel) | cz] = (c0 + ci) | (lvl << kTopLevelShift);
+        }
     }
 }
 
@@ -629,7 +655,8 @@ __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uin
 constexpr uint32_t kVoxelTile = kBlock;
 // The descent keeps its last kPathRegs ancestors (node, grid) in registers, so a
 // climb step is a peek and an atomic, not also a parent and a grid load; a climb
-// above them (trees deeper than kPathRegs levels below its start) loads them.
+// above them loads them. (It starts at the root, not at the top index: entered
+// there, the climb lost the ancestors above it and cost 8 us more per batch.)
 constexpr uint32_t kPathRegs = 4;
 
 __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
@@ -662,9 +689,9 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
         uint32_t node = 0, lvl = 0, gref = 0;
         if (active) {
             const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
-            for (uint32_t level = 0;; ++level) {
+            for (;;) {
                 const uint2 rec = t.walk[node];
-                if (rec.x == kNone || level == uint32_t(kMaxDepth)) {
+                if (rec.x == kNone || lvl == uint32_t(kMaxDepth)) {
                     gref = rec.y;
                     break;
                 }
@@ -674,9 +701,9 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                 }
                 pathNode[0] = node;
                 pathGrid[0] = rec.y;
-                const uint32_t s2 = kMaxDepth - level - 1;
+                const uint32_t s2 = kMaxDepth - lvl - 1;
                 node = rec.x + ((((X >> s2) & 1u) << 2) | (((Y >> s2) & 1u) << 1) | ((Z >> s2) & 1u));
-                lvl = level + 1;
+                lvl += 1;
             }
         }
         // the climb starts at the deepest node with a grid: the root while it is
@@ -987,7 +1014,7 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
             Y[j] = quant28(p[j].y, size) >> 8;
             Z[j] = quant28(p[j].z, size) >> 8;
         }
-        descendN(t.firstChild, X, Y, Z, valid, leaf, level);
+        descendN(t.firstChild, t.topCell, X, Y, Z, valid, leaf, level);
         for (uint32_t j = 0; j < kInsertPerThread; ++j) {
             slotOf[j] = kEmpty32;
             if (!valid[j]) {

@@ -292,9 +292,9 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     if (!cfg.persistent_bytes) {
         cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20);
     }
-    if (meta->num_points == 0 || cfg.fixed_depth > REMO_MAX_FIXED_DEPTH) {
+    if (meta->num_points == 0 || cfg.fixed_depth > REMO_MAX_FIXED_DEPTH || cfg.max_nodes > kTopNodeMask) {
         delete c;
-        return REMO_ERR_INVALID;  // "cloud is empty", or no such fixed depth
+        return REMO_ERR_INVALID;  // "cloud is empty", no such fixed depth, or a node pool past 2^27
     }
     const float size = std::max(std::max(meta->box_size[0], meta->box_size[1]), meta->box_size[2]);
     if (!(size > 0.0f)) {
@@ -364,6 +364,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.batchOct, N * 8, "batch octant histograms");
     A(&t.pending, N, "pending flags");
     A(&t.pointDir, N * kPointDir, "point chunk directories");
+    A(&t.topCell, kTopCells, "the top-level descent index");
     A(&t.gridOffset, t.maxGrids, "grid offsets");
     A(&t.voxelDir, uint64_t(t.maxGrids) * kVoxelDir, "voxel chunk directories");
     A(&t.voxelChunks, t.maxGrids, "voxel chunk counts");

@@ -43,6 +43,13 @@ constexpr uint32_t kShardStride = 16;  // u32 words between two segment counters
 // separate ranges (backlog tile index mod kVoxelShards), so the hot grids' counters
 // take an eighth of the same-address atomics each.
 constexpr uint32_t kVoxelShards = 8;
+// Descents start from a dense index of the tree's top levels: for every cell of
+// level kTopLevel, the deepest node of level <= kTopLevel that contains it (its
+// index | its level << kTopLevelShift). The split kernel keeps it current.
+constexpr uint32_t kTopLevel = 4;
+constexpr uint32_t kTopCells = 1u << (3 * kTopLevel);
+constexpr uint32_t kTopLevelShift = 27;  // node indices below 2^27
+constexpr uint32_t kTopNodeMask = (1u << kTopLevelShift) - 1u;
 
 // Device-resident counters; one instance per context.
 struct Globals {
@@ -98,6 +105,7 @@ struct DevTree {
     uint32_t* batchOct;     // [n * 8] octant histogram counted this round
     uint32_t* pending;      // 1: created this batch with > 50,000 points
     uint32_t* pointDir;     // [n * kPointDir] chunk refs (arena offset >> 4), 0 = none
+    uint32_t* topCell;      // [kTopCells] the descent's starting node per level-kTopLevel cell
     // grids, capacity maxGrids
     uint64_t* gridOffset;   // arena offset of grid g
     uint32_t* voxelDir;     // [g * kVoxelDir] chunk refs
@@ -175,33 +183,43 @@ __device__ __forceinline__ uint32_t quant28(float c, float size) {
     return f2u_sat(__fdiv_rn(num, size));
 }
 
+// The top-index cell of a 20-bit coordinate. A coordinate at the box maximum
+// quantises to 2^20, whose bit 20 the descent never reads (it wraps to cell 0
+// there): the mask keeps that wrap, and the index in bounds.
+__device__ __forceinline__ uint32_t topCellOf(uint32_t X, uint32_t Y, uint32_t Z) {
+    constexpr uint32_t sh = kMaxDepth - kTopLevel;
+    constexpr uint32_t m = (1u << kTopLevel) - 1u;
+    return (((X >> sh) & m) << (2 * kTopLevel)) | (((Y >> sh) & m) << kTopLevel) | ((Z >> sh) & m);
+}
+
 // Root -> leaf descent on the 20-bit coordinate (remolod_octree.cu:182-199) for
-// N points of one lane in lock-step: the N chains of dependent firstChild loads
-// are in flight together instead of one after another.
+// N points of one lane in lock-step, entered at the top index: the N chains of
+// dependent firstChild loads are in flight together instead of one after
+// another, and the levels above the index cost one load.
 template <uint32_t N>
-__device__ __forceinline__ void descendN(const int32_t* __restrict__ firstChild, const uint32_t (&X)[N],
-                                         const uint32_t (&Y)[N], const uint32_t (&Z)[N],
-                                         const bool (&valid)[N], uint32_t (&leaf)[N],
-                                         uint32_t (&levelOut)[N]) {
+__device__ __forceinline__ void descendN(const int32_t* __restrict__ firstChild, const uint32_t* __restrict__ top,
+                                         const uint32_t (&X)[N], const uint32_t (&Y)[N], const uint32_t (&Z)[N],
+                                         const bool (&valid)[N], uint32_t (&leaf)[N], uint32_t (&levelOut)[N]) {
     bool live[N];
     for (uint32_t j = 0; j < N; ++j) {
-        leaf[j] = 0;
-        levelOut[j] = 0;
+        const uint32_t e = valid[j] ? top[topCellOf(X[j], Y[j], Z[j])] : 0u;
+        leaf[j] = e & kTopNodeMask;
+        levelOut[j] = e >> kTopLevelShift;
         live[j] = valid[j];
     }
-    for (uint32_t level = 0; level < uint32_t(kMaxDepth); ++level) {
+    for (uint32_t step = 0; step < uint32_t(kMaxDepth); ++step) {
         int32_t fc[N];
         for (uint32_t j = 0; j < N; ++j) fc[j] = live[j] ? firstChild[leaf[j]] : -1;
         bool any = false;
-        const uint32_t sh = kMaxDepth - level - 1;
         for (uint32_t j = 0; j < N; ++j) {
-            if (fc[j] < 0) {
+            if (fc[j] < 0 || levelOut[j] >= uint32_t(kMaxDepth)) {
                 live[j] = false;
                 continue;
             }
+            const uint32_t sh = kMaxDepth - levelOut[j] - 1;
             const uint32_t ci = (((X[j] >> sh) & 1u) << 2) | (((Y[j] >> sh) & 1u) << 1) | ((Z[j] >> sh) & 1u);
             leaf[j] = uint32_t(fc[j]) + ci;
-            levelOut[j] = level + 1;
+            levelOut[j] += 1;
             any = true;
         }
         if (!any) break;
