@@ -681,11 +681,13 @@ void analysisReset(AnalysisState& a) {
 namespace {
 
 // The device copy of the frame's parameters; skipped when it already holds them
-// (an earlier write on the same stream).
-void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, hipStream_t s) {
+// (an earlier write on the same stream). Only the learned score reads the
+// arrival fraction, which changes every step: without it, a frame of the same
+// camera needs no write (one launch less per step).
+void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, bool needArrival, hipStream_t s) {
     FrameParams f{};
     f.u = u;
-    f.arrival = arrival;
+    f.arrival = (needArrival || !a.frameSet) ? arrival : a.lastFrame.arrival;
     if (a.frameSet && std::memcmp(&f, &a.lastFrame, sizeof(f)) == 0) return;
     hipLaunchKernelGGL(kSetFrame, dim3(1), dim3(64), 0, s, a.frame, f);
     a.lastFrame = f;
@@ -731,12 +733,14 @@ int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& 
         return REMO_ERR_STATE;
     }
     a.arrival = arrival;
-    setFrame(a, u, arrival, s);
+    setFrame(a, u, arrival, learned, s);
     launchAnalysis(a, t, learned, maxLevel, numNodes, s);
     return REMO_OK;
 }
 
 int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s) {
+    // the inputs of the last Analysis: its arrival fraction on the device
+    if (a.frameSet) setFrame(a, a.lastFrame.u, a.arrival, true, s);
     AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, nullptr, a.frame, a.closureMargin};
     hipLaunchKernelGGL(kFeatures, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, aa, out, cap);
     return REMO_OK;
@@ -745,7 +749,7 @@ int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hi
 int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb, hipStream_t s,
               std::string* err) {
     (void)err;
-    setFrame(a, u, a.arrival, s);
+    setFrame(a, u, a.arrival, false, s);
     launchRender(a, t, fb, uint64_t(u.width) * uint64_t(u.height), s);
     return REMO_OK;
 }
@@ -762,7 +766,7 @@ int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, 
                              reinterpret_cast<uint64_t>(fb), pixels,
                              (uint64_t(a.shape.nodeBlocks) << 32) | a.shape.rasterBlocks, a.shape.rasterTile};
     a.arrival = arrival;
-    setFrame(a, u, arrival, s);  // outside the graph: the graph's kernels read it
+    setFrame(a, u, arrival, learned, s);  // outside the graph: the graph's kernels read it
     if (!a.frameExec || std::memcmp(key, a.frameKey, sizeof(key)) != 0) {
         if (a.frameExec) (void)hipGraphExecDestroy(a.frameExec);
         a.frameExec = nullptr;
