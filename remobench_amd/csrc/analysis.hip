@@ -327,6 +327,71 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs
     }
 }
 
+// Sum over the lanes 8 apart (one lane of each width-8 group), in every lane.
+__device__ __forceinline__ i128 strideSum8(i128 v) {
+    for (int m = 8; m < 64; m <<= 1) {
+        const unsigned __int128 u = (unsigned __int128)v;
+        const unsigned long long lo = __shfl_xor((unsigned long long)uint64_t(u), m);
+        const unsigned long long hi = __shfl_xor((unsigned long long)uint64_t(u >> 64), m);
+        v += i128(((unsigned __int128)hi << 64) | lo);
+    }
+    return v;
+}
+
+// Two levels in one launch: a wave per inner node of `level`; lane 8c + g adds
+// grandchild g of child c, the 8 lanes of a group sum child c (written when the
+// child is inner; a leaf child's sums are already final), and lanes 8 apart sum
+// the children into the node. Halves the launches of the per-level pass (each
+// ~5 us, mostly fixed cost); the integer sums are the same.
+__global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArgs aa, uint32_t level) {
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    const uint32_t D = g->maxLevel;
+    const uint32_t lane = laneId(), c = lane >> 3, gi = lane & 7u;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t node = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; node < n; node += waves) {
+        if (t.level[node] != level) continue;
+        const int32_t fc = t.firstChild[node];
+        if (fc < 0) continue;
+        const uint32_t child = uint32_t(fc) + c;
+        const int32_t gfc = t.firstChild[child];  // the same for the 8 lanes of a group
+        RollupSums C;
+        uint32_t cl;
+        if (gfc >= 0) {
+            const uint32_t gc = uint32_t(gfc) + gi;
+            RollupSums G;
+            childContribution(aa.rollup[gc], gi, int(18 + D - (level + 1)), G);
+            for (int i = 0; i < 3; ++i) C.s1[i] = groupSum8(G.s1[i]);
+            for (int q = 0; q < 6; ++q) C.s2[q] = groupSum8(G.s2[q]);
+            C.n = uint64_t(groupSum8(i128(G.n)));
+            C.e = 20 + D - (level + 1);
+            C.pad = 0;
+            cl = aa.closed[gc];
+            for (int m = 1; m < 8; m <<= 1) cl &= __shfl_xor(cl, m, 8);
+            if (gi == 0) {
+                aa.rollup[child] = C;
+                aa.closed[child] = uint8_t(cl);
+            }
+        } else {
+            C = aa.rollup[child];
+            cl = aa.closed[child];
+        }
+        RollupSums T;
+        childContribution(C, c, int(18 + D - level), T);
+        RollupSums P;
+        for (int i = 0; i < 3; ++i) P.s1[i] = strideSum8(T.s1[i]);
+        for (int q = 0; q < 6; ++q) P.s2[q] = strideSum8(T.s2[q]);
+        P.n = uint64_t(strideSum8(i128(T.n)));
+        P.e = 20 + D - level;
+        P.pad = 0;
+        for (int m = 8; m < 64; m <<= 1) cl &= __shfl_xor(cl, m);
+        if (lane == 0) {
+            aa.rollup[node] = P;
+            aa.closed[node] = uint8_t(cl);
+        }
+    }
+}
+
 // Every node's eigen-solve and score, and the work queue of scored nodes (one
 // atomic per wave).
 __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) {
@@ -703,8 +768,10 @@ void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t m
     // one launch per level: measured faster than one workgroup walking all levels
     // with barriers (157 us against 101 us per Analysis on a 10,073-node tree)
     hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
-    for (int level = int(maxLevel) - 1; level >= 0; --level)
-        hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level));
+    int level = int(maxLevel) - 1;  // the deepest level with inner nodes
+    for (; level >= 1; level -= 2)
+        hipLaunchKernelGGL(kAnalysisLevel2, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level - 1));
+    if (level == 0) hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, 0u);
     if (learned) {
         // every node's inputs, then the MLP on the matrix cores, then finalise
         hipLaunchKernelGGL(kFeatures, dim3(nb), dim3(kBlock), 0, s, t, aa, a.feat, a.maxNodes);
