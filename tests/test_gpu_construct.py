@@ -9,7 +9,7 @@ from remobench_amd import lib as L
 from remobench_amd.pipeline import PointRing, Tree
 
 from .cases import lidar_scene, tiny_cases
-from .helpers import batch_arrays, compare_trees, keyed, stats_subset
+from .helpers import batch_arrays, compare_trees, keyed, point_rows, stats_subset
 
 
 def build_hip(points, meta, batch, *, num_slots=None, batches_per_step=20, step_budget_ms=-1.0, **kw):
@@ -149,3 +149,45 @@ def test_async_streaming_through_a_wrapping_ring(gpu):
     assert stats_subset(tree.stats()) == stats_subset(o.stats())
     compare_accums(tree, o)
     ring.close()
+
+
+@pytest.mark.gpu
+def test_voxel_backlog_segment_overflow(gpu):
+    """A voxel backlog of 64 segments far too small for a batch's new voxels: the
+    voxels that find their segment full raise backlog_overflow and are still
+    counted, so every node's voxel count (and the whole tree) equals the
+    oracle's; the stored voxels are distinct cells of the oracle's, fewer than
+    counted where a segment overflowed."""
+    points, meta = lidar_scene()
+    batch = 262_144
+    print("backlog overflow: building on the GPU with 64 x 512 backlog entries", flush=True)
+    tree, _ = build_hip(points, meta, batch, backlog_capacity=64 * 512)
+    o = build_oracle(points, meta, batch)
+    st = tree.stats()
+    assert st["backlog_overflow"] == 1 and st["dir_overflow"] == 0, st
+    tn, on = tree.nodes(), o.nodes()
+    kt, ko = keyed(tn), keyed(on)
+    assert set(kt) == set(ko)
+    short = 0
+    for k, it in kt.items():
+        rt, ro = tn[it], on[ko[k]]
+        assert rt["num_points"] == ro["num_points"] and rt["num_voxels"] == ro["num_voxels"], k
+        stored = int(rt["num_voxels_stored"])
+        assert stored <= int(rt["num_voxels"]), k
+        if stored == 0:
+            continue
+        got = np.ascontiguousarray(tree.samples(it, True)).view(np.uint32).reshape(-1, 4)[:, :3]
+        want = np.ascontiguousarray(o.samples(ko[k], True)).view(np.uint32).reshape(-1, 4)[:, :3]
+        assert len(got) == stored
+        gs = {tuple(r) for r in got.tolist()}
+        assert len(gs) == stored, f"duplicate stored voxel at {k}"
+        assert gs <= {tuple(r) for r in want.tolist()}, f"stored voxel outside the oracle's cells at {k}"
+        short += stored < int(rt["num_voxels"])
+    assert short > 0
+    # the occupancy grids and the points are untouched by the voxel backlog
+    for k, it in kt.items():
+        rt = tn[it]
+        if rt["grid"] >= 0:
+            assert np.array_equal(tree.grid(it), o.grid(ko[k])), f"occupancy grid differs at {k}"
+        if rt["first_child"] < 0 and rt["num_points"] > 0:
+            assert np.array_equal(point_rows(tree.samples(it)), point_rows(o.samples(ko[k]))), k
