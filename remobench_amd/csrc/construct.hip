@@ -2,18 +2,21 @@
 //
 // Follows kernel_construct / addBatch (kernels/remolod/remolod_octree.cu:940-1320)
 // phase for phase -- count, split, voxel-sample, allocate chunks, insert points,
-// insert voxels -- with three changes that remove work without changing the
-// tree (node set, counters, occupied cells and per-leaf point multisets are
-// the reference's):
+// insert voxels -- with changes that remove work without changing the tree
+// (node set, counters, occupied cells and per-leaf point multisets are the
+// reference's):
 //
 //  * Every leaf keeps an octant histogram of its stored points (storedOct). A
 //    split therefore knows its children's counters at once (stored + this
 //    batch's octant counts), so the reference's re-count of the spilled points
 //    (expand iterations 1..N, 21.5% of construct, wiki/03_ExpandStageFindings.md)
 //    only happens for a child that itself holds > 50,000 points.
-//  * Counting and inserting aggregate a wave64 by (leaf, octant) with
-//    ballot/readlane, one atomic per group (the reference's labeled_partition,
-//    remolod_octree.cu:205-219, for 64-wide waves).
+//  * Counting and inserting aggregate a tile by (leaf, octant) in an LDS table,
+//    one global atomic per key (the reference's labeled_partition,
+//    remolod_octree.cu:205-219, at workgroup scope).
+//  * Descents enter at a dense index of the level-4 cells; voxel sampling climbs
+//    only while cells are new; new voxels go to a backlog cut into 64 segments
+//    and are counted, ranked and inserted without per-voxel atomics.
 //  * Chunks are found through a per-node directory, not a list walk.
 //
 // Each phase is its own kernel on one stream; kernel boundaries are the

@@ -59,7 +59,7 @@ struct Globals {
     int64_t numAllocatedChunks;    // point chunks in use == chunk-queue stack pointer
     int64_t chunkPoolSize;         // high-water mark of numAllocatedChunks
     uint32_t numSpilled;
-    uint32_t numBacklog;
+    uint32_t numBacklog;           // backlog entries held this batch (published by kVoxelCount)
     uint32_t numCandidates;        // split candidates of the current round
     uint32_t numAccepted;          // accepted splits of the current round
     uint32_t numPending;           // new children that must split again this batch
