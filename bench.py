@@ -131,6 +131,7 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--scene", default="lidar")
+    ap.add_argument("--simlod", default="", help="bench a .simlod point file instead of a generated scene")
     ap.add_argument("--cpu-sample-batches", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--raster", default="1920x1080", help="framebuffer size of the Rasterize timing, WxH")
@@ -175,7 +176,12 @@ def main() -> int:
 
     t = time.perf_counter()
     threads = min(16, os.cpu_count() or 1)
-    points, meta = L.generate_scene(args.points, args.seed, args.scene, threads=threads)
+    if args.simlod:
+        from remobench_amd import simlod
+        points, meta = simlod.open_cloud(args.simlod)
+        args.scene = f"simlod:{Path(args.simlod).name}"
+    else:
+        points, meta = L.generate_scene(args.points, args.seed, args.scene, threads=threads)
     nb = (len(points) + args.batch - 1) // args.batch
     log(f"scene: {len(points):,} points, {nb} batches, box {list(meta.box_size)} "
         f"({time.perf_counter() - t:.1f} s)")
@@ -513,7 +519,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+u32+int64",
-        "data": f"synthetic: seed-generated lidar-like scene (seed {args.seed}), resident in HBM",
+        "data": (f"file: {args.simlod}, resident in HBM" if args.simlod else
+                 f"synthetic: seed-generated {args.scene} scene (seed {args.seed}), resident in HBM"),
         "config": {
             "workload": workload,
             "points": len(points), "batch": args.batch, "batches_timed": steps,

@@ -77,7 +77,7 @@ def build(verbose: bool = False) -> Path:
 
 
 HOST = PKG / "host"
-HOST_SOURCES = ["PointSource.cpp", "RemolodHipPipeline.cpp", "PipelineRegistry.cpp"]
+HOST_SOURCES = ["PointSource.cpp", "SimlodReader.cpp", "RemolodHipPipeline.cpp", "PipelineRegistry.cpp"]
 HOST_LIB = LIBDIR / "libremolod_host.so"
 HEADLESS = LIBDIR / "remolod_headless"
 

@@ -138,7 +138,7 @@ void usage() {
                  "                        [--budget-ms MS] [--analysis none|analytic] [--width W --height H]\n"
                  "                        [--white] [--no-accum] [--profile] [--device-budget-mb MB]\n"
                  "                        [--dump-uniforms FILE] [--dump-fb FILE]\n"
-                 "                        [--remolod-fixed-depth D] [--async-upload]\n"
+                 "                        [--remolod-fixed-depth D] [--async-upload] [--simlod FILE]\n"
                  "                        [--refine-budget-ms MS] (Refinement each frame; then one\n"
                  "                        unbudgeted pass before the final frame)\n");
 }
@@ -151,7 +151,7 @@ int main(int argc, char** argv) {
     float budgetMs = 10.0f;
     int width = 1920, height = 1080;
     bool white = false, accumulate = true, profile = false;
-    std::string analysis = "analytic", dumpUniforms, dumpFb;
+    std::string analysis = "analytic", dumpUniforms, dumpFb, simlodPath;
     float refineBudgetMs = -2.0f;  // below -1: no Refinement
     uint32_t fixedDepth = 0;
     bool asyncUpload = false;
@@ -182,6 +182,7 @@ int main(int argc, char** argv) {
         else if (a == "--refine-budget-ms") refineBudgetMs = std::strtof(next(), nullptr);
         else if (a == "--remolod-fixed-depth") fixedDepth = uint32_t(std::strtoul(next(), nullptr, 10));
         else if (a == "--async-upload") asyncUpload = true;
+        else if (a == "--simlod") simlodPath = next();
         else if (a == "--help" || a == "-h") {
             usage();
             return 0;
@@ -203,11 +204,30 @@ int main(int argc, char** argv) {
     }
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
-    auto source = remo::makeSceneSource(points, seed, nullptr, slot);
-    if (!source) {
-        std::fprintf(stderr, "scene generation failed\n");
-        return 1;
+    // the cloud: a .simlod file read batch by batch, or the seed-generated scene
+    std::unique_ptr<remo::RingSource> source;
+    remo::HostCloudSource* hostCloud = nullptr;
+    if (!simlodPath.empty()) {
+        if (asyncUpload) {
+            std::fprintf(stderr, "--async-upload needs the cloud in host memory, not --simlod\n");
+            return 2;
+        }
+        std::string openErr;
+        source = remo::makeSimlodSource(simlodPath, slot, &openErr);
+        if (!source) {
+            std::fprintf(stderr, "remolod_headless: %s\n", openErr.c_str());
+            return 1;
+        }
+    } else {
+        auto scene = remo::makeSceneSource(points, seed, nullptr, slot);
+        if (!scene) {
+            std::fprintf(stderr, "scene generation failed\n");
+            return 1;
+        }
+        hostCloud = scene.get();
+        source = std::move(scene);
     }
+    points = source->meta().numPoints;
     const double genS = std::chrono::duration<double>(clock::now() - t0).count();
 
     // App::computeBudget (src/shell/App.cpp:97-114)
@@ -234,7 +254,7 @@ int main(int argc, char** argv) {
         return 1;
     }
     auto* pipe = static_cast<remo::RemolodHipPipeline*>(registry.active());
-    if (asyncUpload && !source->enableAsync(pipe->context(), perStep)) {
+    if (asyncUpload && !hostCloud->enableAsync(pipe->context(), perStep)) {
         std::fprintf(stderr, "remolod_headless: pinned host memory for the cloud was refused\n");
         return 1;
     }
@@ -258,6 +278,10 @@ int main(int argc, char** argv) {
         frame.uniforms.frame_counter = frames;
         pipe->render(frame);
         source->pump();
+        if (!source->lastError().empty()) {
+            std::fprintf(stderr, "remolod_headless: %s\n", source->lastError().c_str());
+            return 1;
+        }
         const bool more = pipe->build(*source, frame);
         if (more && analysis == "analytic" && pipe->batchesConsumed() > 0) pipe->analyse(frame, REMO_ANALYSIS_ANALYTIC, &err);
         if (more && refineBudgetMs >= 0.0f && pipe->batchesConsumed() > 0) {
@@ -311,13 +335,15 @@ int main(int argc, char** argv) {
                 "\"num_nodes\": %u, \"num_inner\": %u, \"num_leaves\": %u, \"num_points\": %llu, "
                 "\"num_voxels\": %llu, \"num_points_ingested\": %llu, \"max_depth\": %u, "
                 "\"batchlet_index\": %u, \"sum_leaf_counts\": %llu, \"inner_with_sums\": %u, "
-                "\"visible_nodes\": %u, \"visible_samples\": %llu, \"mem_capacity_reached\": %d",
+                "\"visible_nodes\": %u, \"visible_samples\": %llu, \"mem_capacity_reached\": %d, "
+                "\"box_size\": [%.9g, %.9g, %.9g]",
                 registry.activeId().c_str(), (unsigned long long)points, (unsigned long long)seed, frames, genS, loopS,
                 double(ps.numPointsIngested) / loopS / 1e6, ps.numNodes, ps.numInner, ps.numLeaves,
                 (unsigned long long)ps.numPoints, (unsigned long long)ps.numVoxels,
                 (unsigned long long)ps.numPointsIngested, ps.maxDepth, s.batchlet_index,
                 (unsigned long long)s.sum_leaf_counts, s.inner_with_sums, ps.numVisibleNodes,
-                (unsigned long long)ps.numVisiblePoints, int(ps.memCapacityReached));
+                (unsigned long long)ps.numVisiblePoints, int(ps.memCapacityReached), double(source->meta().boxSize[0]),
+                double(source->meta().boxSize[1]), double(source->meta().boxSize[2]));
     if (refineBudgetMs >= 0.0f)
         std::printf(", \"refine\": {\"budget_ms\": %.3f, \"calls\": %u, \"nodes_refined\": %llu, "
                     "\"device_ms\": %.3f, \"final_nodes_refined\": %u, \"final_complete\": %u, "
