@@ -388,22 +388,20 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * node);
         so[0] = so[1] = make_uint4(0, 0, 0, 0);
 
-        // stored points -> spill list (copied by kSpillAndZero); chunks -> queue
+        // stored points -> spill list, chunks -> queue slots [top - nChunks, top):
+        // both copied by kSpillAndZero, a workgroup per chunk (a loop over a
+        // leaf's ~50 chunks here was a chain of dependent loads and stores)
         uint32_t spillOff = kNone;
         if (np > 0) {
             if (uint64_t(spillAt) + np <= t.spillCap)
                 spillOff = spillAt;
             else
                 g->spillOverflow = 1;
-            for (uint32_t k = 0; k < nChunks; ++k) {
-                const int64_t qi = top - 1 - int64_t(k);
-                if (qi >= 0 && uint64_t(qi) < t.chunkQueueCap)
-                    t.chunkQueue[qi] = t.pointDir[uint64_t(node) * kPointDir + k];
-            }
         }
-        t.acceptedSpill[3 * slot] = node;
-        t.acceptedSpill[3 * slot + 1] = spillOff;
-        t.acceptedSpill[3 * slot + 2] = np;
+        t.acceptedSpill[4 * slot] = node;
+        t.acceptedSpill[4 * slot + 1] = spillOff;
+        t.acceptedSpill[4 * slot + 2] = np;
+        t.acceptedSpill[4 * slot + 3] = uint32_t(top);
         t.numPoints[node] = 0;  // remolod_octree.cu:362; kSpillAndZero copies from the recorded count
 
         uint32_t gno = t.gridNo[node];
@@ -463,11 +461,17 @@ __global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t, uint32_t coun
     // spreads over many workgroups
     for (uint32_t pair = blockIdx.x; pair < na * kPointDir; pair += gridDim.x) {
         const uint32_t a = pair / kPointDir, k = pair % kPointDir;
-        const uint32_t node = t.acceptedSpill[3 * a];
-        const uint32_t off = t.acceptedSpill[3 * a + 1];
-        const uint32_t np = t.acceptedSpill[3 * a + 2];
-        if (off == kNone || k * kPointsPerChunk >= np) continue;
+        const uint32_t node = t.acceptedSpill[4 * a];
+        const uint32_t off = t.acceptedSpill[4 * a + 1];
+        const uint32_t np = t.acceptedSpill[4 * a + 2];
+        if (k * kPointsPerChunk >= np) continue;
         const uint32_t ref = t.pointDir[uint64_t(node) * kPointDir + k];
+        if (threadIdx.x == 0) {
+            // the chunk goes back to the queue (doSplitting, remolod_octree.cu:349-360)
+            const int64_t qi = int64_t(t.acceptedSpill[4 * a + 3]) - 1 - int64_t(k);
+            if (qi >= 0 && uint64_t(qi) < t.chunkQueueCap) t.chunkQueue[qi] = ref;
+        }
+        if (off == kNone) continue;
         const uint32_t m = min(kPointsPerChunk, np - k * kPointsPerChunk);
         const remo_point* src = chunkPtr(t.arena, ref);
         for (uint32_t s = threadIdx.x; s < m; s += blockDim.x) t.spill[off + k * kPointsPerChunk + s] = src[s];
@@ -476,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t, uint32_t coun
     const uint64_t total = uint64_t(na) * perGrid;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t node = t.acceptedSpill[3 * (i / perGrid)];
+        const uint32_t node = t.acceptedSpill[4 * (i / perGrid)];
         const uint32_t gi = t.gridNo[node];
         if (gi == kNone) continue;
         reinterpret_cast<uint4*>(t.arena + t.gridOffset[gi])[i % perGrid] = make_uint4(0, 0, 0, 0);

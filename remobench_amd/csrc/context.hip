@@ -378,7 +378,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.voxelShardCount, uint64_t(kVoxelShards) * t.maxGrids, "voxel counts per shard and grid");
     A(&t.voxelShardStart, uint64_t(kVoxelShards) * t.maxGrids, "voxel slot starts per shard and grid");
     A(&t.candidates, t.candidateCap, "split candidates");
-    A(&t.acceptedSpill, uint64_t(t.candidateCap) * 3, "accepted splits");
+    A(&t.acceptedSpill, uint64_t(t.candidateCap) * 4, "accepted splits");
     A(&t.g, 1, "device counters");
     A(&t.accums, N, "the accumulator side array");
     if (t.fixedDepth) A(&t.cellCount, fixedPyramidEntries(t.fixedDepth), "the fixed-depth cell counts");

@@ -127,7 +127,7 @@ struct DevTree {
     uint32_t* voxelShardCount;  // [kVoxelShards * maxGrids] this batch's entries per (shard, grid)
     uint32_t* voxelShardStart;  // [kVoxelShards * maxGrids] first voxel slot of each (shard, grid) range
     uint32_t* candidates;   // split candidates of a round
-    uint32_t* acceptedSpill;// [candidateCap * 3] (node, spill offset, stored points) of accepted splits
+    uint32_t* acceptedSpill;// [candidateCap * 4] (node, spill offset, stored points, chunk-queue top) of accepted splits
     uint32_t candidateCap;
     uint32_t maxNodes;
     uint32_t maxGrids;
