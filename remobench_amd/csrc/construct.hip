@@ -341,13 +341,26 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
             continue;
         }
         const uint32_t c0 = uint32_t(childOffset);
-        const uint32_t lvl = t.level[node] + 1;
+        // everything the split reads, loaded before its first store (a load from
+        // one of these arrays cannot move past a store to it, so reading the
+        // octant counts child by child between the children's stores made eight
+        // round trips in a row)
+        const uint4* so4 = reinterpret_cast<const uint4*>(t.storedOct + 8ull * node);
+        const uint4* bo4 = reinterpret_cast<const uint4*>(t.batchOct + 8ull * node);
+        const uint4 sLo = so4[0], sHi = so4[1], bLo = bo4[0], bHi = bo4[1];
+        const uint32_t np = t.numPoints[node];
+        const uint32_t gno0 = t.gridNo[node];
+        const uint4 key = staged ? keys[ci] : make_uint4(t.level[node], t.nx[node], t.ny[node], t.nz[node]);
+        const uint32_t lvl = key.x + 1;
+        const uint32_t X = key.y, Y = key.z, Z = key.w;
+        const uint64_t gridOff0 = gno0 != kNone ? t.gridOffset[gno0] : 0ull;
+        const uint32_t octCount[8] = {sLo.x + bLo.x, sLo.y + bLo.y, sLo.z + bLo.z, sLo.w + bLo.w,
+                                      sHi.x + bHi.x, sHi.y + bHi.y, sHi.z + bHi.z, sHi.w + bHi.w};
         // the split's counter reservations, issued together: each is a returning
         // atomic on one global counter, and waiting for one before issuing the
         // next made a round's splits a chain of five memory round trips
-        const uint32_t np = t.numPoints[node];
         const uint32_t nChunks = (np + kPointsPerChunk - 1) / kPointsPerChunk;
-        const bool needGrid = t.gridNo[node] == kNone;
+        const bool needGrid = gno0 == kNone;
         const uint32_t spillAt = np > 0 ? atomicAdd(&g->numSpilled, np) : 0u;
         const int64_t top = np > 0 ? int64_t(atomicAdd((unsigned long long*)&g->numAllocatedChunks,
                                                        (unsigned long long)(-int64_t(nChunks))))
@@ -357,10 +370,9 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         const unsigned long long gridAt =
             needGrid ? atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes) : 0ull;
         atomicMax(&g->maxLevel, lvl);
-        const uint32_t X = t.nx[node], Y = t.ny[node], Z = t.nz[node];
         for (uint32_t i = 0; i < 8; ++i) {
             const uint32_t c = c0 + i;
-            const uint32_t cnt = t.storedOct[8ull * node + i] + t.batchOct[8ull * node + i];
+            const uint32_t cnt = octCount[i];
             t.firstChild[c] = -1;
             t.walk[c] = make_uint2(kNone, 0u);
             t.parent[c] = node;
@@ -404,13 +416,15 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         t.acceptedSpill[4 * slot + 3] = uint32_t(top);
         t.numPoints[node] = 0;  // remolod_octree.cu:362; kSpillAndZero copies from the recorded count
 
-        uint32_t gno = t.gridNo[node];
+        uint32_t gno = gno0;
+        uint64_t gridOff = gridOff0;
         if (needGrid) {
             if (gi < t.maxGrids && gridAt + kGridAllocBytes <= t.arenaCapacity) {
                 t.gridOffset[gi] = gridAt;
                 t.voxelChunks[gi] = 0;
                 t.gridNo[node] = gi;
                 gno = gi;
+                gridOff = gridAt;
             } else {
                 g->allocOverflow = 1;
             }
@@ -419,7 +433,7 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         atomicAdd((unsigned long long*)&g->totNodesSplit, 1ull);
         // publish the children last: later kernels descend through them
         t.firstChild[node] = int32_t(c0);
-        t.walk[node] = make_uint2(c0, gno == kNone ? 0u : uint32_t(t.gridOffset[gno] >> 4));
+        t.walk[node] = make_uint2(c0, gno == kNone ? 0u : uint32_t(gridOff >> 4));
         if (lvl <= kTopLevel) topSplits[atomicAdd(&numTopSplits, 1u)] = make_uint4(X, Y, Z, c0 | (lvl << kTopLevelShift));
     }
     // the split nodes' top-index cells now start at the children holding them
