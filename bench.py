@@ -93,7 +93,7 @@ def cpu_baseline(points, meta, batch: int, sample_batches: int) -> dict:
 SCOPE_KERNELS = {
     "remolod.construct.count": ["kCount"],
     "remolod.construct.split": ["kSplitDecide", "kSplitApply", "kSpillAndZero"],
-    "remolod.construct.voxelSampling": ["kVoxelSample", "kVoxelCount"],
+    "remolod.construct.voxelSampling": ["kVoxelSample"],
     "remolod.construct.allocChunks": ["kAllocChunks"],
     "remolod.construct.insertPoints": ["kInsertPoints"],
     "remolod.construct.insertVoxels": ["kInsertVoxels"],

@@ -633,15 +633,17 @@ __global__ __launch_bounds__(kBlock) void kFixedSeed(DevTree t) {
 // At steady state most points stop after one peek.
 //
 // New voxels are staged per wave in LDS and reach the global backlog with one
-// atomicAdd per flush, on the counter of the block's backlog segment. numVoxels
-// is counted afterwards from the backlog (kVoxelCount, kAllocChunks): bumping
-// the per-node counters here sent every wave's atomics to the few lines that
-// hold the counters of the nodes a batch touches (~33 us per 1M-point batch).
+// atomicAdd per flush, on the counter of the block's backlog segment, and are
+// ranked with one more per (flush, grid) on that grid's counter for the segment;
+// kAllocChunks sums those counters into numVoxels. Bumping per-node counters in
+// the climb sent every wave's atomics to the few lines that hold the counters of
+// the nodes a batch touches (~33 us per 1M-point batch).
 constexpr uint32_t kStage = 128;  // staged voxels per wave
 
 struct VoxelStage {
     remo_point pts[kStage];
     uint32_t node[kStage];
+    uint32_t grid[kStage];  // the node's grid number
 };
 
 // Moves a wave's staged voxels to the global backlog (one atomic).
@@ -650,17 +652,37 @@ __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uin
     // the wave's own LDS writes must land before other lanes read them
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint32_t shard = blockIdx.x & (kBacklogShards - 1u);
+    const uint32_t lane = laneId();
     uint32_t base = 0;
-    if (laneId() == 0) base = atomicAdd(&t.backlogFill[shard * kShardStride], count);
+    if (lane == 0) base = atomicAdd(&t.backlogFill[shard * kShardStride], count);
     base = __shfl(base, 0);
-    for (uint32_t k = laneId(); k < count; k += 64) {
+    for (uint32_t k0 = 0; k0 < count; k0 += 64) {
+        const uint32_t k = k0 + lane;
         const uint32_t at = base + k;
-        if (at < t.backlogSegCap) {
-            const uint64_t slot = uint64_t(shard) * t.backlogSegCap + at;
+        const bool held = k < count && at < t.backlogSegCap;
+        const uint64_t slot = uint64_t(shard) * t.backlogSegCap + at;
+        // rank among this batch's entries of the grid in this segment: one
+        // returning atomic per (flush, grid) on the (grid, segment) counter
+        const uint32_t key = held ? st.grid[k] : kNone;
+        uint32_t rank = 0;
+        uint64_t pend = __ballot(held);
+        while (pend) {
+            const int leader = __ffsll((unsigned long long)pend) - 1;
+            const uint32_t lk = __shfl(key, leader);
+            const uint64_t grp = __ballot(held && key == lk) & pend;
+            uint32_t gbase = 0;
+            if (lane == uint32_t(leader))
+                gbase = atomicAdd(&t.voxelShardCount[uint64_t(lk) * kBacklogShards + shard], uint32_t(__popcll(grp)));
+            gbase = __shfl(gbase, leader);
+            if ((grp >> lane) & 1ull) rank = gbase + uint32_t(__popcll(grp & lanesBelow()));
+            pend &= ~grp;
+        }
+        if (held) {
             t.backlogPts[slot] = st.pts[k];
             t.backlogNode[slot] = st.node[k];
-        } else {
-            // a dropped voxel still counts (the backlog pass never sees it)
+            t.backlogRank[slot] = rank;
+        } else if (k < count) {
+            // a dropped voxel still counts (no backlog entry will)
             t.g->backlogOverflow = 1;
             atomicAdd(&t.numVoxels[st.node[k]], 1u);
         }
@@ -783,6 +805,7 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                     const uint32_t slot = staged + uint32_t(__popcll(nm & lanesBelow()));
                     st.pts[slot] = v;
                     st.node[slot] = nd;
+                    st.grid[slot] = t.gridNo[nd];
                 }
                 staged += cnt;
             }
@@ -826,50 +849,6 @@ __device__ __forceinline__ uint64_t backlogSlot(const DevTree& t, const uint32_t
     return uint64_t(lo) * t.backlogSegCap + (i - prefix[lo]);
 }
 
-// Counts the batch's backlog entries per grid (the voxel sampling kernel's
-// newVoxels count, remolod_octree.cu:642-668) and ranks each entry among its
-// grid's entries of its shard (tile index mod kVoxelShards), aggregated per tile in
-// LDS: one returning atomic per (tile, grid) on the shard's counter. kAllocChunks
-// turns the counts into slot ranges; kInsertVoxels then needs no atomics. Also
-// publishes the number of entries held as numBacklog.
-__global__ __launch_bounds__(kBlock) void kVoxelCount(DevTree t) {
-    __shared__ BlockAgg agg;
-    __shared__ uint32_t prefix[kBacklogShards + 1];
-    if (t.g->stop) return;  // the batch did not run: the counters are last batch's
-    backlogPrefix(t, prefix);
-    const uint32_t total = prefix[kBacklogShards];
-    if (blockIdx.x == 0 && threadIdx.x == 0) t.g->numBacklog = total;
-    const TileRange tr = myTiles(total);
-    for (uint32_t tile = tr.begin; tile < tr.end; ++tile) {
-        uint32_t* counts = t.voxelShardCount + uint64_t(tile & (kVoxelShards - 1u)) * t.maxGrids;
-        aggClear(agg);
-        __syncthreads();
-        uint32_t gi[kPerThread], slotOf[kPerThread], rank[kPerThread];
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            gi[j] = kNone;
-            slotOf[j] = kEmpty32;
-            if (i >= total) continue;
-            gi[j] = t.gridNo[t.backlogNode[backlogSlot(t, prefix, i)]];
-            if (gi[j] == kNone) continue;  // not reached: sampled nodes have grids
-            slotOf[j] = aggInsert(agg, gi[j], &rank[j]);
-            if (slotOf[j] == kEmpty32) rank[j] = atomicAdd(&counts[gi[j]], 1u);
-        }
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < kAggSlots; k += blockDim.x) {
-            const uint32_t key = agg.key[k];
-            if (key != kEmpty32) agg.base[k] = atomicAdd(&counts[key], agg.cnt[k]);
-        }
-        __syncthreads();
-        for (uint32_t j = 0; j < kPerThread; ++j) {
-            if (gi[j] == kNone) continue;
-            const uint32_t i = tile * kTile + j * kBlock + threadIdx.x;
-            t.backlogRank[i] = slotOf[j] == kEmpty32 ? rank[j] : agg.base[slotOf[j]] + rank[j];
-        }
-        __syncthreads();
-    }
-}
-
 // Exclusive prefix sum over the wave's lanes; `total` gets the wave's sum.
 template <typename T>
 __device__ __forceinline__ T waveExclusiveSum(T v, T& total) {
@@ -909,20 +888,26 @@ __global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
             }
         }
         // voxel chunks a grid lacks; this batch's backlog entries of the grid take
-        // the slots after the stored voxels, shard after shard (kVoxelCount ranked
-        // them per shard)
+        // the slots after the stored voxels, segment after segment (the flushes
+        // ranked them per segment); the 64 counts are read as 16 vectors before
+        // any store
         const uint32_t gi = valid ? t.gridNo[i] : kNone;
         uint32_t vHave = 0, vAdd = 0;
         if (gi != kNone) {
-            uint32_t added = 0;
+            uint4* c4 = reinterpret_cast<uint4*>(t.voxelShardCount + uint64_t(gi) * kBacklogShards);
+            uint4 cnt[kBacklogShards / 4];
+            for (uint32_t q = 0; q < kBacklogShards / 4; ++q) cnt[q] = c4[q];
             const uint32_t stored = t.numVoxelsStored[i];
-            for (uint32_t sh = 0; sh < kVoxelShards; ++sh) {
-                const uint64_t at = uint64_t(sh) * t.maxGrids + gi;
-                const uint32_t c = t.voxelShardCount[at];
-                if (c == 0) continue;
-                t.voxelShardStart[at] = stored + added;
-                t.voxelShardCount[at] = 0;
-                added += c;
+            uint4* s4 = reinterpret_cast<uint4*>(t.voxelShardStart + uint64_t(gi) * kBacklogShards);
+            uint32_t added = 0;
+            for (uint32_t q = 0; q < kBacklogShards / 4; ++q) {
+                const uint4 c = cnt[q];
+                if ((c.x | c.y | c.z | c.w) == 0) continue;
+                const uint32_t s0 = stored + added;
+                const uint32_t s1 = s0 + c.x, s2 = s1 + c.y, s3 = s2 + c.z;
+                added += c.x + c.y + c.z + c.w;
+                s4[q] = make_uint4(s0, s1, s2, s3);
+                c4[q] = make_uint4(0, 0, 0, 0);
             }
             uint32_t nv = t.numVoxels[i];
             if (added) {
@@ -1075,7 +1060,7 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
 }
 
 // insertVoxels (remolod_octree.cu:918-938): every backlog entry goes to slot
-// start(shard, grid) + rank, both taken before (kVoxelCount, kAllocChunks).
+// start(grid, segment) + rank, both taken before (the voxel flush, kAllocChunks).
 __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, uint32_t closeBatch) {
     __shared__ uint32_t prefix[kBacklogShards + 1];
     Globals* g = t.g;
@@ -1094,15 +1079,16 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
             g->totBatches += 1;
             g->totSpilledPoints += spilled;
         }
+        g->numBacklog = total;
         g->newVoxels += total;
         g->totNewVoxels += total;
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         const uint64_t at = backlogSlot(t, prefix, i);
         const uint32_t gi = t.gridNo[t.backlogNode[at]];
-        if (gi == kNone) continue;
-        const uint32_t sh = (i / kTile) & (kVoxelShards - 1u);
-        const uint32_t slot = t.voxelShardStart[uint64_t(sh) * t.maxGrids + gi] + t.backlogRank[i];
+        if (gi == kNone) continue;  // not reached: sampled nodes have grids
+        const uint32_t seg = uint32_t(at / t.backlogSegCap);
+        const uint32_t slot = t.voxelShardStart[uint64_t(gi) * kBacklogShards + seg] + t.backlogRank[at];
         const uint32_t k = slot / kPointsPerChunk;
         const uint32_t ref = k < kVoxelDir ? t.voxelDir[uint64_t(gi) * kVoxelDir + k] : 0u;
         if (ref != 0)
@@ -1229,7 +1215,6 @@ void launchFixedExpand(const DevTree& t, BatchRef b, const LaunchShape& ls, hipS
 
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
     hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
-    hipLaunchKernelGGL(kVoxelCount, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t);
 }
 
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
@@ -1289,7 +1274,6 @@ void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, con
     hipLaunchKernelGGL(kSplitApply, dim3(1), dim3(1024), 0, s, t);
     hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t, 0u);
     hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
-    hipLaunchKernelGGL(kVoxelCount, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t);
     hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
     hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
     hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);

@@ -375,8 +375,8 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.backlogNode, t.backlogCap, "the voxel backlog targets");
     A(&t.backlogFill, uint64_t(kBacklogShards) * kShardStride, "the voxel backlog segment counters");
     A(&t.backlogRank, t.backlogCap, "the voxel backlog ranks");
-    A(&t.voxelShardCount, uint64_t(kVoxelShards) * t.maxGrids, "voxel counts per shard and grid");
-    A(&t.voxelShardStart, uint64_t(kVoxelShards) * t.maxGrids, "voxel slot starts per shard and grid");
+    A(&t.voxelShardCount, uint64_t(kBacklogShards) * t.maxGrids, "voxel counts per grid and backlog segment");
+    A(&t.voxelShardStart, uint64_t(kBacklogShards) * t.maxGrids, "voxel slot starts per grid and backlog segment");
     A(&t.candidates, t.candidateCap, "split candidates");
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 4, "accepted splits");
     A(&t.g, 1, "device counters");
@@ -441,7 +441,7 @@ int remo_reset(remo_ctx* c) {
     CK(hipMemsetAsync(t.pointDir, 0, 4ull * kPointDir * uint64_t(t.maxNodes), c->stream), "reset");
     CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
     CK(hipMemsetAsync(t.backlogFill, 0, 4ull * kBacklogShards * kShardStride, c->stream), "reset");
-    CK(hipMemsetAsync(t.voxelShardCount, 0, 4ull * kVoxelShards * t.maxGrids, c->stream), "reset");
+    CK(hipMemsetAsync(t.voxelShardCount, 0, 4ull * kBacklogShards * t.maxGrids, c->stream), "reset");
     launchResetTree(t, c->stream);
     refineReset(c->refine, t, c->stream);
     CK(hipGetLastError(), "reset kernel");

@@ -39,10 +39,6 @@ constexpr uint32_t kMaxRounds = 20;    // the expand loop bound, remolod_octree.
 // returning atomic on one address (~20 us per 1M-point batch).
 constexpr uint32_t kBacklogShards = 64;
 constexpr uint32_t kShardStride = 16;  // u32 words between two segment counters
-// A batch's new voxels of one grid are counted and ranked in kVoxelShards
-// separate ranges (backlog tile index mod kVoxelShards), so the hot grids' counters
-// take an eighth of the same-address atomics each.
-constexpr uint32_t kVoxelShards = 8;
 // Descents start from a dense index of the tree's top levels: for every cell of
 // level kTopLevel, the deepest node of level <= kTopLevel that contains it (its
 // index | its level << kTopLevelShift). The split kernel keeps it current.
@@ -59,7 +55,7 @@ struct Globals {
     int64_t numAllocatedChunks;    // point chunks in use == chunk-queue stack pointer
     int64_t chunkPoolSize;         // high-water mark of numAllocatedChunks
     uint32_t numSpilled;
-    uint32_t numBacklog;           // backlog entries held this batch (published by kVoxelCount)
+    uint32_t numBacklog;           // backlog entries held this batch (published by kInsertVoxels)
     uint32_t numCandidates;        // split candidates of the current round
     uint32_t numAccepted;          // accepted splits of the current round
     uint32_t numPending;           // new children that must split again this batch
@@ -123,9 +119,9 @@ struct DevTree {
     uint32_t* backlogNode;
     uint32_t* backlogFill;   // [kBacklogShards * kShardStride] entries reserved per segment
     uint32_t backlogSegCap;  // backlogCap / kBacklogShards
-    uint32_t* backlogRank;   // [backlogCap] entry i's rank among its grid's entries of its voxel shard
-    uint32_t* voxelShardCount;  // [kVoxelShards * maxGrids] this batch's entries per (shard, grid)
-    uint32_t* voxelShardStart;  // [kVoxelShards * maxGrids] first voxel slot of each (shard, grid) range
+    uint32_t* backlogRank;   // [backlogCap] an entry's rank among its segment's entries of its grid
+    uint32_t* voxelShardCount;  // [maxGrids * kBacklogShards] this batch's entries per (grid, segment)
+    uint32_t* voxelShardStart;  // [maxGrids * kBacklogShards] first voxel slot of each (grid, segment) range
     uint32_t* candidates;   // split candidates of a round
     uint32_t* acceptedSpill;// [candidateCap * 4] (node, spill offset, stored points, chunk-queue top) of accepted splits
     uint32_t candidateCap;
