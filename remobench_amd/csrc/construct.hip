@@ -69,13 +69,16 @@ __device__ __forceinline__ TileRange myTiles(uint32_t total) {
 constexpr uint32_t kAggSlots = 2048;
 constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
 
-struct BlockAgg {
+// Keys and counts; the insert kernels add each key's reserved base.
+struct KeyCounts {
     uint32_t key[kAggSlots];
     uint32_t cnt[kAggSlots];
+};
+struct BlockAgg : KeyCounts {
     uint32_t base[kAggSlots];
 };
 
-__device__ __forceinline__ void aggClear(BlockAgg& a) {
+__device__ __forceinline__ void aggClear(KeyCounts& a) {
     for (uint32_t i = threadIdx.x; i < kAggSlots; i += blockDim.x) {
         a.key[i] = kEmpty32;
         a.cnt[i] = 0;
@@ -86,7 +89,7 @@ __device__ __forceinline__ uint32_t aggHash(uint32_t key) { return (key * 265443
 
 // Finds or claims the slot of `key` and takes a rank in it; kEmpty32 when the
 // probe sequence is exhausted (the caller then goes to global memory directly).
-__device__ __forceinline__ uint32_t aggInsert(BlockAgg& a, uint32_t key, uint32_t* rank) {
+__device__ __forceinline__ uint32_t aggInsert(KeyCounts& a, uint32_t key, uint32_t* rank) {
     uint32_t h = aggHash(key);
     for (uint32_t probe = 0; probe < 64; ++probe) {
         const uint32_t prev = atomicCAS(&a.key[h], kEmpty32, key);
@@ -195,7 +198,7 @@ __global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t c
 // Counts go to the octant histogram batchOct[leaf][octant], aggregated per
 // tile in LDS.
 __global__ __launch_bounds__(kBlock) void kCount(DevTree t, BatchRef b, uint32_t round) {
-    __shared__ BlockAgg agg;
+    __shared__ KeyCounts agg;  // 16 KB: eight workgroups of a CU resident at once
     const uint32_t batchSize = batchSizeOf(t, b);
     if (t.g->stop) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void kFixedClear(DevTree t, uint32_t entrie
 
 // 1. bucket: the batch's points into level-D cells, aggregated per tile in LDS.
 __global__ __launch_bounds__(kBlock) void kFixedBucket(DevTree t, BatchRef b) {
-    __shared__ BlockAgg agg;
+    __shared__ KeyCounts agg;
     const uint32_t batchSize = batchSizeOf(t, b);
     if (t.g->stop) return;
     const uint32_t D = t.fixedDepth;
