@@ -181,15 +181,16 @@ struct AnalysisArgs {
     remo_node_analysis* results;
     uint32_t* queue;
     uint32_t* queueCount;
-    const float* learned;        // [n][2] predicted (surface variation, planarity), or null
     const FrameParams* frame;    // uniforms and arrival of this frame (device)
     uint32_t margin;
 };
 
 // Closure, covariance, eigen-solve and score of one node from its rolled-up sums.
-// With aa.learned, open nodes take the learned score's prediction.
+// With `pred` (the learned score's surface variation and planarity for this
+// node), an open node takes the prediction.
 __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
-                             uint32_t node, const RollupSums& S, bool closed, uint32_t D) {
+                             uint32_t node, const RollupSums& S, bool closed, uint32_t D,
+                             const float* pred = nullptr) {
     remo_node_analysis r;
     memset(&r, 0, sizeof(r));
     r.n = S.n;
@@ -198,10 +199,10 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
     r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
     r.state = closed ? 2u : 0u;
     bool scored = false;
-    if (aa.learned && !closed && S.n >= 3) {
+    if (pred && !closed && S.n >= 3) {
         // learned score for open nodes; the analytic path is the fallback when the
         // prediction says the node is not planar (surface variation > 0.05)
-        const float sv = aa.learned[2 * node], pl = aa.learned[2 * node + 1];
+        const float sv = pred[0], pl = pred[1];
         if (sv <= kLearnedFallbackSv) {
             r.surface_variation = fmaxf(sv, 0.0f);
             r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
@@ -392,8 +393,18 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
     }
 }
 
-// Every node's eigen-solve and score, and the work queue of scored nodes (one
-// atomic per wave).
+// Appends the wave's scored nodes to the work queue (one atomic per wave).
+__device__ inline void queueScored(const AnalysisArgs& aa, uint32_t node, bool q) {
+    const uint64_t m = __ballot(q);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t at = 0;
+    if (laneId() == uint32_t(leader)) at = atomicAdd(aa.queueCount, uint32_t(__popcll(m)));
+    at = __shfl(at, leader);
+    if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
+}
+
+// Every node's eigen-solve and score, and the work queue of scored nodes.
 __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
@@ -406,13 +417,51 @@ __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) 
             finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
             q = aa.results[node].score > 0.0f;
         }
-        const uint64_t m = __ballot(q);
-        if (m == 0) continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t at = 0;
-        if (laneId() == uint32_t(leader)) at = atomicAdd(aa.queueCount, uint32_t(__popcll(m)));
-        at = __shfl(at, leader);
-        if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
+        queueScored(aa, node, q);
+    }
+}
+
+// kFinalize with the learned score, 16 nodes per wave (the Analysis trees hold
+// some 10^4 nodes, so a wave per 16 keeps more of the chip busy than a wave per
+// 64 running four MLP tiles in a row): lanes 0..15 write their node's 13 inputs
+// into LDS, the wave runs one MLP tile on the matrix cores (the same tile as
+// kMlpInfer), and lanes 0..15 finalise their nodes, open nodes taking the
+// prediction.
+__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, const uint16_t* wb) {
+    static_assert(kBlock == kMlpInferThreads, "one MLP wave per finalize wave");
+    constexpr uint32_t kPerBlock = 16 * kMlpWavesPerBlock;
+    __shared__ MlpInferShared sh;
+    __shared__ float pred[kMlpWavesPerBlock][16 * kMlpOut];
+    Globals* g = t.g;
+    const uint32_t n = g->numNodes;
+    if (blockIdx.x * kPerBlock >= n) return;  // the whole block: before the barrier
+    const uint32_t D = g->maxLevel;
+    const remo_shared_uniforms& u = aa.frame->u;
+    const float arrival = aa.frame->arrival;
+    const uint32_t wave = threadIdx.x / 64, lane = laneId();
+    float(*act)[kMlpActRow] = sh.act[wave];
+    mlpStageWeights(wb, sh.w);
+    __syncthreads();
+    for (uint32_t base = blockIdx.x * kPerBlock; base < n; base += gridDim.x * kPerBlock) {
+        const uint32_t w0 = base + wave * 16;  // this wave's nodes
+        const uint32_t node = w0 + lane;
+        const bool mine = lane < 16 && node < n;
+        if (w0 < n) {
+            if (lane < 16) {
+                float f[kMlpIn];
+                if (node < n) nodeFeatures(t, node, aa.rollup[node], arrival, f);
+                for (uint32_t k = 0; k < 16; ++k) act[lane][k] = (node < n && k < kMlpIn) ? f[k] : 0.0f;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            mlpInferTile(sh.w, act, n, w0, pred[wave]);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the predictions are in LDS
+        }
+        bool q = false;
+        if (mine) {
+            finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, &pred[wave][lane * kMlpOut]);
+            q = aa.results[node].score > 0.0f;
+        }
+        queueScored(aa, node, q);
     }
 }
 
@@ -729,10 +778,6 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.units = static_cast<RasterUnit*>(p);
     A(&p, sizeof(FrameParams), "frame parameters");
     a.frame = static_cast<FrameParams*>(p);
-    A(&p, sizeof(float) * kMlpIn * maxNodes, "learned-score inputs");
-    a.feat = static_cast<float*>(p);
-    A(&p, sizeof(float) * kMlpOut * maxNodes, "learned-score predictions");
-    a.pred = static_cast<float*>(p);
     if (rc == REMO_OK) rc = mlpAllocate(a, alloc);
     return rc;
 }
@@ -763,8 +808,7 @@ void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, bo
 void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t maxLevel, uint32_t numNodes,
                     hipStream_t s) {
     const uint32_t nb = a.shape.nodeBlocks;
-    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount,
-                    learned ? a.pred : nullptr, a.frame, a.closureMargin};
+    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin};
     // one launch per level: measured faster than one workgroup walking all levels
     // with barriers (157 us against 101 us per Analysis on a 10,073-node tree)
     hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
@@ -772,12 +816,10 @@ void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t m
     for (; level >= 1; level -= 2)
         hipLaunchKernelGGL(kAnalysisLevel2, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level - 1));
     if (level == 0) hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, 0u);
-    if (learned) {
-        // every node's inputs, then the MLP on the matrix cores, then finalise
-        hipLaunchKernelGGL(kFeatures, dim3(nb), dim3(kBlock), 0, s, t, aa, a.feat, a.maxNodes);
-        launchMlpInfer(a.mlpWeights, a.feat, &t.g->numNodes, 0, a.pred, nb, s);
-    }
-    hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    if (learned)
+        hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa, a.mlpWeights);
+    else
+        hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
 }
 
 // The Rasterize kernels alone.
@@ -808,7 +850,7 @@ int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& 
 int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s) {
     // the inputs of the last Analysis: its arrival fraction on the device
     if (a.frameSet) setFrame(a, a.lastFrame.u, a.arrival, true, s);
-    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, nullptr, a.frame, a.closureMargin};
+    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin};
     hipLaunchKernelGGL(kFeatures, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, aa, out, cap);
     return REMO_OK;
 }

@@ -52,15 +52,18 @@ __global__ void kMlpReset(const float* master, uint16_t* bf16, float* m, float* 
 
 // Inference: rows of x -> rows of y on the matrix cores, 16 rows per wave. The
 // row count comes from nPtr (device, e.g. the node count) when given, else n.
-__global__ __launch_bounds__(64 * kMlpWavesPerBlock) void kMlpInfer(const uint16_t* wb, const float* x,
-                                                                   const uint32_t* nPtr, uint32_t n, float* y) {
+__global__ __launch_bounds__(kMlpInferThreads) void kMlpInfer(const uint16_t* wb, const float* x, const uint32_t* nPtr,
+                                                              uint32_t n, float* y) {
     __shared__ MlpInferShared sh;
-    for (uint32_t i = threadIdx.x; i < kMlpParams; i += blockDim.x) sh.w[i] = bf16ToF32(wb[i]);
+    mlpStageWeights(wb, sh.w);
     __syncthreads();
     const uint32_t rows = nPtr ? *nPtr : n;
     const uint32_t wave = threadIdx.x / 64;
-    for (uint32_t tile = blockIdx.x * kMlpWavesPerBlock + wave; tile * 16 < rows; tile += gridDim.x * kMlpWavesPerBlock)
-        mlpInferTile(sh.w, sh.act[wave], x, rows, tile * 16, y);
+    for (uint32_t tile = blockIdx.x * kMlpWavesPerBlock + wave; tile * 16 < rows;
+         tile += gridDim.x * kMlpWavesPerBlock) {
+        mlpLoadRows(x, rows, tile * 16, sh.act[wave]);
+        mlpInferTile(sh.w, sh.act[wave], rows, tile * 16, y + uint64_t(tile) * 16 * kMlpOut);
+    }
 }
 
 // Loss L = 1 / (n * outputs) * sum (y - t)^2 over the batch.
@@ -185,7 +188,7 @@ uint64_t splitmix(uint64_t& s) {
 
 void launchMlpInfer(const uint16_t* wb, const float* x, const uint32_t* nPtr, uint32_t n, float* y, uint32_t blocks,
                     hipStream_t s) {
-    hipLaunchKernelGGL(kMlpInfer, dim3(std::max(1u, blocks)), dim3(64 * kMlpWavesPerBlock), 0, s, wb, x, nPtr, n, y);
+    hipLaunchKernelGGL(kMlpInfer, dim3(std::max(1u, blocks)), dim3(kMlpInferThreads), 0, s, wb, x, nPtr, n, y);
 }
 
 }  // namespace remo_hip

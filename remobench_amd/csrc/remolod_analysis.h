@@ -87,8 +87,6 @@ struct AnalysisState {
     uint32_t drawItems = 0, drawListOverflow = 0;
     uint64_t drawSamples = 0;
     uint16_t* mlpWeights = nullptr;    // BF16 parameters of the learned score
-    float* feat = nullptr;             // [maxNodes][13] learned-score inputs
-    float* pred = nullptr;             // [maxNodes][2] its predictions
     bool mlpLoaded = false;
 };
 

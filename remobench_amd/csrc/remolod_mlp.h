@@ -101,59 +101,147 @@ static_assert(kMlpOut == 2, "mlpForward writes two outputs");
 // go through LDS to become the next layer's A operand.
 typedef float mlpF4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kMlpWavesPerBlock = 4;
-constexpr uint32_t kMlpActRow = kMlpHid + 1;  // padded LDS row
+constexpr uint32_t kMlpInferThreads = 64 * kMlpWavesPerBlock;
+// LDS rows padded to 68 floats: lane (r16, k4) of an operand read touches bank
+// 4 r16 + k4, and a result write (row 4 k4 + r, column r16) bank 16 k4 + r16, so
+// neither conflicts.
+constexpr uint32_t kMlpActRow = kMlpHid + 4;
+constexpr uint32_t kMlpW2Row = kMlpHid + 4;
+// the parameters in LDS: the global layout with W2's rows padded to kMlpW2Row
+constexpr uint32_t kMlpShW2 = kW2;
+constexpr uint32_t kMlpShB2 = kMlpShW2 + kMlpHid * kMlpW2Row;
+constexpr uint32_t kMlpShW3 = kMlpShB2 + kMlpHid;
+constexpr uint32_t kMlpShB3 = kMlpShW3 + kMlpOut * kMlpHid;
+constexpr uint32_t kMlpShParams = kMlpShB3 + kMlpOut;
 
 struct MlpInferShared {
-    float w[kMlpParams];
+    float w[kMlpShParams];
     float act[kMlpWavesPerBlock][16][kMlpActRow];
 };
 
-// Rows [row0, row0 + 16) of x (n rows of kMlpIn) -> y (n rows of kMlpOut).
-__device__ inline void mlpInferTile(const float* w, float (*act)[kMlpActRow], const float* x, uint32_t n,
-                                    uint32_t row0, float* y) {
+__device__ __forceinline__ uint32_t mlpShIndex(uint32_t p) {
+    if (p < kW2) return p;
+    if (p < kB2) return kMlpShW2 + (p - kW2) / kMlpHid * kMlpW2Row + (p - kW2) % kMlpHid;
+    return p + kMlpHid * (kMlpW2Row - kMlpHid);
+}
+
+// The BF16 parameters (16-byte aligned, as every device allocation is) expanded
+// into the LDS layout by the whole block of kMlpInferThreads: 16-byte loads, all
+// issued before the first store.
+__device__ inline void mlpStageWeights(const uint16_t* wb, float* w) {
+    constexpr uint32_t kGroups = kMlpParams / 8;
+    constexpr uint32_t kPer = (kGroups + kMlpInferThreads - 1) / kMlpInferThreads;
+    const uint4* v = reinterpret_cast<const uint4*>(wb);
+    uint4 r[kPer];
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t g = threadIdx.x + i * kMlpInferThreads;
+        if (g < kGroups) r[i] = v[g];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t g = threadIdx.x + i * kMlpInferThreads;
+        if (g >= kGroups) continue;
+        const uint32_t h[4] = {r[i].x, r[i].y, r[i].z, r[i].w};
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t p = g * 8 + 2 * q;
+            w[mlpShIndex(p)] = bf16ToF32(uint16_t(h[q] & 0xFFFFu));
+            w[mlpShIndex(p + 1)] = bf16ToF32(uint16_t(h[q] >> 16));
+        }
+    }
+    for (uint32_t p = kGroups * 8 + threadIdx.x; p < kMlpParams; p += kMlpInferThreads) w[mlpShIndex(p)] = bf16ToF32(wb[p]);
+}
+
+// Rows [row0, row0 + 16) of x (n rows of kMlpIn) into act[0..16)[0..16), zero
+// past the inputs and past n.
+__device__ inline void mlpLoadRows(const float* x, uint32_t n, uint32_t row0, float (*act)[kMlpActRow]) {
     const uint32_t lane = __lane_id();
     const uint32_t r16 = lane % 16, k4 = lane / 16;
     const uint32_t row = row0 + r16;
-    const bool valid = row < n;
-    // layer 1: [16 x 16 (13 used)] x [16 x 64]
+    for (uint32_t ks = 0; ks < 4; ++ks) {
+        const uint32_t k = ks * 4 + k4;
+        act[r16][k] = (row < n && k < kMlpIn) ? x[uint64_t(row) * kMlpIn + k] : 0.0f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// One wave: the 16 input rows in act[0..16)[0..16) (w in the LDS layout) -> the
+// outputs of rows row0 .. row0 + 15 that are below n, at y[(row - row0) kMlpOut + o]. Each layer reads all its operands
+// first and then issues its MFMAs k-step by k-step across the four independent
+// column tiles, so neither LDS latency nor the MFMA dependency is exposed per
+// instruction; every column tile still sums its k-steps in order.
+__device__ inline void mlpInferTile(const float* w, float (*act)[kMlpActRow], uint32_t n, uint32_t row0, float* y) {
+    const uint32_t lane = __lane_id();
+    const uint32_t r16 = lane % 16, k4 = lane / 16;
     mlpF4 acc[4];
-    for (uint32_t nt = 0; nt < 4; ++nt) {
-        acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+    // layer 1: [16 x 16 (13 used)] x [16 x 64]
+    {
+        float a[4], b[4][4];
+#pragma unroll
         for (uint32_t ks = 0; ks < 4; ++ks) {
             const uint32_t k = ks * 4 + k4;
-            const float a = (valid && k < kMlpIn) ? x[uint64_t(row) * kMlpIn + k] : 0.0f;
-            const float b = k < kMlpIn ? w[kW1 + (nt * 16 + r16) * kMlpIn + k] : 0.0f;
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[nt], 0, 0, 0);
+            a[ks] = act[r16][k];
+#pragma unroll
+            for (uint32_t nt = 0; nt < 4; ++nt) b[nt][ks] = k < kMlpIn ? w[kW1 + (nt * 16 + r16) * kMlpIn + k] : 0.0f;
         }
+#pragma unroll
+        for (uint32_t nt = 0; nt < 4; ++nt) acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (uint32_t nt = 0; nt < 4; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[nt][ks], acc[nt], 0, 0, 0);
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read the inputs
+#pragma unroll
     for (uint32_t nt = 0; nt < 4; ++nt)
+#pragma unroll
         for (uint32_t r = 0; r < 4; ++r) act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kB1 + nt * 16 + r16]);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     // layer 2: [16 x 64] x [64 x 64]
-    for (uint32_t nt = 0; nt < 4; ++nt) {
-        acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+    {
+        float a[16], b[4][16];
+#pragma unroll
         for (uint32_t ks = 0; ks < 16; ++ks) {
             const uint32_t k = ks * 4 + k4;
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(act[r16][k], w[kW2 + (nt * 16 + r16) * kMlpHid + k],
-                                                            acc[nt], 0, 0, 0);
+            a[ks] = act[r16][k];
+#pragma unroll
+            for (uint32_t nt = 0; nt < 4; ++nt) b[nt][ks] = w[kMlpShW2 + (nt * 16 + r16) * kMlpW2Row + k];
         }
+#pragma unroll
+        for (uint32_t nt = 0; nt < 4; ++nt) acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (uint32_t nt = 0; nt < 4; ++nt)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[nt][ks], acc[nt], 0, 0, 0);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read layer 1
+#pragma unroll
     for (uint32_t nt = 0; nt < 4; ++nt)
-        for (uint32_t r = 0; r < 4; ++r) act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kB2 + nt * 16 + r16]);
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r)
+            act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kMlpShB2 + nt * 16 + r16]);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     // layer 3: [16 x 64] x [64 x 16 (2 used)]
     mlpF4 out = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (uint32_t ks = 0; ks < 16; ++ks) {
-        const uint32_t k = ks * 4 + k4;
-        const float b = r16 < kMlpOut ? w[kW3 + r16 * kMlpHid + k] : 0.0f;
-        out = __builtin_amdgcn_mfma_f32_16x16x4f32(act[r16][k], b, out, 0, 0, 0);
+    {
+        float a[16], b[16];
+#pragma unroll
+        for (uint32_t ks = 0; ks < 16; ++ks) {
+            const uint32_t k = ks * 4 + k4;
+            a[ks] = act[r16][k];
+            b[ks] = r16 < kMlpOut ? w[kMlpShW3 + r16 * kMlpHid + k] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t ks = 0; ks < 16; ++ks) out = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[ks], out, 0, 0, 0);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // act is reused by the next tile
     if (r16 < kMlpOut)
         for (uint32_t r = 0; r < 4; ++r) {
             const uint32_t orow = row0 + 4 * k4 + r;
-            if (orow < n) y[uint64_t(orow) * kMlpOut + r16] = out[r] + w[kB3 + r16];
+            if (orow < n) y[(4 * k4 + r) * kMlpOut + r16] = out[r] + w[kMlpShB3 + r16];
         }
 }
 

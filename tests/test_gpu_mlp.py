@@ -150,6 +150,34 @@ def test_learned_analysis_uses_the_model_for_open_nodes_only(built):
 
 
 @pytest.mark.gpu
+def test_learned_analysis_predictions_equal_inference_on_the_features(built):
+    """Analysis computes each node's inputs and prediction in one kernel: the
+    predictions it applies are those of batched inference on the features
+    entry point's rows, and exactly the open nodes predicted <= 0.05 take them."""
+    points, meta, tree, ring, o = built
+    u = L.SharedUniforms()
+    tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
+    f = tree.features()
+    m = ScoreMLP(seed=1, max_batch=len(f))
+    theta = _theta(7, 0.1)
+    theta[-2:] = 0.0
+    m.set_params(theta)
+    p0 = m.forward(f)
+    theta[-2:] = [0.05 - np.median(p0[:, 0]), 0.5]  # about half the nodes under the threshold
+    m.set_params(theta)
+    pred = m.forward(f)
+    tree.set_score_model(m)
+    tree.analysis(u, L.REMO_ANALYSIS_LEARNED)
+    r = tree.analysis_results()
+    open_ = (r["state"] == 0) & (r["n"] >= 3)
+    take = open_ & (pred[:, 0] <= np.float32(0.05))
+    assert take.sum() > 10 and (open_ & ~take).sum() > 10
+    np.testing.assert_array_equal((r["flags"] & 2) != 0, take)
+    np.testing.assert_allclose(r["surface_variation"][take], np.maximum(pred[take, 0], 0), rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(r["planarity"][take], np.clip(pred[take, 1], 0, 1), rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.gpu
 def test_learned_mode_without_a_model_is_refused(gpu):
     points, meta = lidar_scene(300_000, seed=3)
     tree, ring = build_hip(points, meta, 100_000)
