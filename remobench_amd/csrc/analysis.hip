@@ -427,7 +427,10 @@ __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) 
 // into LDS, the wave runs one MLP tile on the matrix cores (the same tile as
 // kMlpInfer), and lanes 0..15 finalise their nodes, open nodes taking the
 // prediction.
-__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, const uint16_t* wb) {
+// `arrival` is the batch fraction of the inputs, or negative to read it from the
+// frame parameters (a captured graph, whose arguments are fixed at capture).
+__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, const uint16_t* wb,
+                                                           float arrival) {
     static_assert(kBlock == kMlpInferThreads, "one MLP wave per finalize wave");
     constexpr uint32_t kPerBlock = 16 * kMlpWavesPerBlock;
     __shared__ MlpInferShared sh;
@@ -437,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
     if (blockIdx.x * kPerBlock >= n) return;  // the whole block: before the barrier
     const uint32_t D = g->maxLevel;
     const remo_shared_uniforms& u = aa.frame->u;
-    const float arrival = aa.frame->arrival;
+    if (arrival < 0.0f) arrival = aa.frame->arrival;
     const uint32_t wave = threadIdx.x / 64, lane = laneId();
     float(*act)[kMlpActRow] = sh.act[wave];
     mlpStageWeights(wb, sh.w);
@@ -805,8 +808,10 @@ void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, bo
 }
 
 // The Analysis kernels alone (no host work, so they can be captured).
+// The learned score's arrival fraction goes in as an argument when arrival >= 0,
+// else it is read from the frame parameters.
 void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t maxLevel, uint32_t numNodes,
-                    hipStream_t s) {
+                    float arrival, hipStream_t s) {
     const uint32_t nb = a.shape.nodeBlocks;
     AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin};
     // one launch per level: measured faster than one workgroup walking all levels
@@ -817,7 +822,7 @@ void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t m
         hipLaunchKernelGGL(kAnalysisLevel2, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level - 1));
     if (level == 0) hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, 0u);
     if (learned)
-        hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa, a.mlpWeights);
+        hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa, a.mlpWeights, arrival);
     else
         hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
 }
@@ -842,8 +847,10 @@ int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& 
         return REMO_ERR_STATE;
     }
     a.arrival = arrival;
-    setFrame(a, u, arrival, learned, s);
-    launchAnalysis(a, t, learned, maxLevel, numNodes, s);
+    // the arrival fraction, which changes every step, goes to the kernel as an
+    // argument: the frame parameters are written only when the camera changes
+    setFrame(a, u, arrival, false, s);
+    launchAnalysis(a, t, learned, maxLevel, numNodes, arrival, s);
     return REMO_OK;
 }
 
@@ -884,7 +891,7 @@ int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, 
             if (err) *err = "hipStreamBeginCapture failed";
             return REMO_ERR_DEVICE;
         }
-        launchAnalysis(a, t, learned, maxLevel, numNodes, s);
+        launchAnalysis(a, t, learned, maxLevel, numNodes, -1.0f, s);
         launchRender(a, t, fb, pixels, s);
         if (hipStreamEndCapture(s, &g) != hipSuccess || !g) {
             if (err) *err = "hipStreamEndCapture failed";
