@@ -287,8 +287,14 @@ int remo_autotune_frame(remo_ctx* ctx, const remo_shared_uniforms* u, int mode, 
 /* The launch shape in use (ms = 0). */
 int remo_get_frame_shape(remo_ctx* ctx, remo_tune_entry* out);
 
-/* ILodPipeline::stats / diagnostics (src/pipelines/RemolodPipeline.cpp:440-470, :603-726). */
+/* ILodPipeline::stats / diagnostics (src/pipelines/RemolodPipeline.cpp:440-470, :603-726).
+ * Runs the per-node stats sweep (kernel_construct's closing sweep,
+ * remolod_octree.cu:1248-1319) and reads the counters back (synchronous). */
 int remo_get_stats(remo_ctx* ctx, remo_stats* out);
+/* The last render's DeviceDiagnostics only -- draw items, samples drawn, draw
+ * list overflow -- as RemolodPipeline::render reads them after kernel_render
+ * (src/pipelines/RemolodPipeline.cpp:592-600): one small readback, no sweep. */
+int remo_get_render_diagnostics(remo_ctx* ctx, uint32_t* draw_items, uint64_t* draw_samples, uint32_t* overflow);
 const char* remo_last_error(const remo_ctx* ctx);
 int remo_sync(remo_ctx* ctx);
 
@@ -369,9 +375,11 @@ int remo_refine(remo_ctx* ctx, float budget_ms, uint32_t max_levels, remo_refine
  * level, x, y, z), and their points are redistributed to the children exactly
  * as a split during Update does: the leaf gets a zeroed grid and voxels from its
  * own points, its chunks are recycled, the children take the points octant by
- * octant. The node pool's limit is respected and reported like Update's
- * (node_pool_overflow). Refused when the persistent store is within Update's
- * safety margin. Synchronous. */
+ * octant. The chosen leaves are a prefix of that order whose points fit both the
+ * spill list they move through (spill_capacity) and the voxel backlog
+ * (backlog_capacity); `chosen` says how many. The node pool's limit is respected
+ * and reported like Update's (node_pool_overflow). Refused when the persistent
+ * store is within Update's safety margin. Synchronous. */
 typedef struct remo_deepen_stats {
     uint32_t candidates;        /* leaves that qualified                    */
     uint32_t chosen;            /* of those, offered to the split (<= max_splits) */
@@ -413,6 +421,22 @@ int remo_dump_node_samples(remo_ctx* ctx, uint32_t node, int voxels,
 int remo_dump_grid(remo_ctx* ctx, uint32_t node, uint32_t* out_words);
 int remo_dump_accums(remo_ctx* ctx, remo_node_accum* out, uint32_t cap, uint32_t* num);
 int remo_dump_analysis(remo_ctx* ctx, remo_node_analysis* out, uint32_t cap, uint32_t* num);
+
+/* The exact sums the last Analysis rolled up for every node (no reference
+ * counterpart: the reference specifies the roll-up, README.md:80,
+ * wiki/02_KernelPipeline.md:205-224, without code). Coordinates are integers in
+ * units U = 2^-(20+D) of the root extent, D the deepest level, centred on the
+ * node's own centre; s1 = sum of q, s2 = sum of q q^T (xx xy xz yy yz zz), each a
+ * two's-complement 128-bit integer stored as [0] low and [1] high 64 bits;
+ * e = 20 + D - level (the node extent is 2^e U); closed: the closure flag. */
+typedef struct remo_node_rollup {
+    int64_t s1[3][2];
+    int64_t s2[6][2];
+    uint64_t n;
+    uint32_t e;
+    uint32_t closed;
+} remo_node_rollup;
+int remo_dump_rollup(remo_ctx* ctx, remo_node_rollup* out, uint32_t cap, uint32_t* num);
 
 /* ---- the input ring (PointSource, include/remo/PointSource.h:69-115) ---- */
 int remo_ring_create(uint32_t slot_capacity, uint32_t num_slots, remo_ring** out);

@@ -60,7 +60,7 @@ def load():
         "oracle_jacobi": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "oracle_morton": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32]),
         "oracle_refine": (C.c_uint32, [C.c_void_p, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
-        "oracle_deepen": (None, [C.c_void_p, C.c_uint32, C.c_uint32, P(C.c_uint32), P(C.c_uint32),
+        "oracle_deepen": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, P(C.c_uint32), P(C.c_uint32),
                                  P(C.c_uint64), P(C.c_uint64)]),
         "oracle_render_flat": (None, [C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
         "oracle_apply_edl": (None, [P(SharedUniforms), C.c_void_p]),
@@ -155,12 +155,14 @@ def _refine(self, min_level: int = 0):
     return n, sk.value, em.value
 
 
-def _deepen(self, max_splits: int, min_points: int = 25_000) -> dict:
-    """Refinement's local deepening (the HIP path's choice and split path)."""
+def _deepen(self, max_splits: int, min_points: int = 25_000, point_budget: int = 10_000_000) -> dict:
+    """Refinement's local deepening (the HIP path's choice and split path);
+    point_budget: the chosen leaves' points fit the spill list and voxel backlog
+    (the HIP path's default capacities are 10,000,000 each)."""
     c, sp = C.c_uint32(0), C.c_uint32(0)
     mv, vx = C.c_uint64(0), C.c_uint64(0)
-    self.lib.oracle_deepen(self.h, int(max_splits), int(min_points), C.byref(c), C.byref(sp), C.byref(mv),
-                           C.byref(vx))
+    self.lib.oracle_deepen(self.h, int(max_splits), int(min_points), int(point_budget), C.byref(c), C.byref(sp),
+                           C.byref(mv), C.byref(vx))
     return {"candidates": c.value, "splits": sp.value, "points_moved": mv.value, "new_voxels": vx.value}
 
 

@@ -52,7 +52,9 @@ constexpr float kMaxDepthGridSize = 268435456.0f;  // 2^28
 // (kernels/simlod/utils.h.cu:188-199): Chunk is 16,016 B, OccupancyGrid 262,144 B.
 constexpr uint64_t kChunkAllocBytes = 16ull * ((16016ull + 16ull) / 16ull);    // 16,032
 constexpr uint64_t kGridAllocBytes = 16ull * ((262144ull + 16ull) / 16ull);    // 262,160
-constexpr uint64_t kSafetyMargin = 200000000ull;  // remolod_octree.cu:1195
+// remolod_octree.cu:1191-1195: 200 MB, or 1 GB for the analytic-bucketing arm
+constexpr uint64_t kSafetyMargin = 200000000ull;
+constexpr uint64_t kSafetyMarginFixed = 1000000000ull;
 constexpr uint32_t kSpillDropIndex = 3000000;     // remolod_octree.cu:872
 
 // Saturating float -> u32 conversion, as the device's cvt does: NaN and
@@ -463,8 +465,10 @@ struct oracle_tree {
     // each accepted leaf's stored points go through the spill list to its
     // children (counters = the points per octant), with voxel sampling, chunk
     // acquisition and insertion as for a batch of no new points.
-    void deepen(uint32_t maxSplits, uint32_t minPoints, uint32_t* candidatesOut, uint32_t* splitsOut,
-                uint64_t* movedOut, uint64_t* voxelsOut) {
+    // The chosen prefix also stops before its points would pass pointBudget
+    // (the HIP path's spill list and voxel backlog capacities).
+    void deepen(uint32_t maxSplits, uint32_t minPoints, uint64_t pointBudget, uint32_t* candidatesOut,
+                uint32_t* splitsOut, uint64_t* movedOut, uint64_t* voxelsOut) {
         struct Cand {
             uint32_t np, level, X, Y, Z, id;
         };
@@ -481,6 +485,10 @@ struct oracle_tree {
             return Key{a.level, a.X, a.Y, a.Z} < Key{b.level, b.X, b.Y, b.Z};
         });
         if (cands.size() > maxSplits) cands.resize(maxSplits);
+        uint64_t points = 0;
+        size_t fit = 0;
+        while (fit < cands.size() && points + cands[fit].np <= pointBudget) points += cands[fit++].np;
+        cands.resize(fit);
         spilling.clear();
         for (const Cand& c : cands) spilling.push_back(c.id);
         std::sort(spilling.begin(), spilling.end(), [&](uint32_t a, uint32_t b) {
@@ -557,7 +565,8 @@ struct oracle_tree {
         for (uint32_t b = first; b < last; ++b) {
             const uint32_t slot = b % numSlots;
             const uint32_t size = batchSizes[slot];
-            const bool reached = persistentOffset + kSafetyMargin >= persistentCapacity;
+            const uint64_t margin = fixedDepth ? kSafetyMarginFixed : kSafetyMargin;
+            const bool reached = persistentOffset + margin >= persistentCapacity;
             memCapacityReached = reached;
             if (reached) break;
             addBatch(slots + uint64_t(slot) * slotCapacity, size, batchletIndex);
@@ -1317,9 +1326,9 @@ void oracle_apply_edl(const remo_shared_uniforms* u, uint64_t* fb) {
         }
 }
 
-void oracle_deepen(oracle_tree* t, uint32_t maxSplits, uint32_t minPoints, uint32_t* candidates, uint32_t* splits,
-                   uint64_t* moved, uint64_t* voxels) {
-    t->deepen(maxSplits, minPoints, candidates, splits, moved, voxels);
+void oracle_deepen(oracle_tree* t, uint32_t maxSplits, uint32_t minPoints, uint64_t pointBudget, uint32_t* candidates,
+                   uint32_t* splits, uint64_t* moved, uint64_t* voxels) {
+    t->deepen(maxSplits, minPoints, pointBudget, candidates, splits, moved, voxels);
 }
 
 uint32_t oracle_refine(oracle_tree* t, uint32_t minLevel, uint32_t* skipped, uint32_t* empty) {

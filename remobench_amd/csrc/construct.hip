@@ -40,9 +40,11 @@ __device__ __forceinline__ remo_point streamPoint(const DevTree& t, BatchRef b, 
     return i < batchSize ? b.points[i] : t.spill[i - batchSize];
 }
 
-// Spilled points actually held (the counter runs past the capacity on overflow).
+// Spilled points actually held. On overflow the counter runs past the capacity,
+// and a split whose range straddles the capacity writes nothing, so the list
+// ends at the lowest offset of a split that did not fit.
 __device__ __forceinline__ uint32_t spilledCount(const DevTree& t) {
-    return min(t.g->numSpilled, t.spillCap);
+    return min(min(t.g->numSpilled, t.spillCap), t.g->spillLimit);
 }
 
 __device__ __forceinline__ uint32_t batchSizeOf(const DevTree& t, BatchRef b) {
@@ -121,6 +123,7 @@ __global__ void kResetRoot(DevTree t) {
     }
     for (uint32_t i = 0; i < kPointDir; ++i) t.pointDir[i] = 0;
     g->numNodes = 1;
+    g->spillLimit = kNone;
     g->arenaOffset = kArenaFirstOffset;
     g->numGrids = 1;
     t.gridOffset[0] = g->arenaOffset;
@@ -150,8 +153,8 @@ __global__ void kBeginStep(DevTree t) {
 }
 
 // The batch's opening update by one thread; 0 when the step's budget stops it.
-__device__ uint32_t beginBatchThread0(Globals* g, uint64_t capacity, uint32_t checkBudget, uint64_t budgetTicks,
-                                      uint32_t beginStep) {
+__device__ uint32_t beginBatchThread0(Globals* g, uint64_t capacity, uint64_t margin, uint32_t checkBudget,
+                                      uint64_t budgetTicks, uint32_t beginStep) {
     if (beginStep) {
         // the step's first batch also opens the step (kBeginStep, one launch fewer)
         g->stepStart = wall_clock64();
@@ -167,13 +170,14 @@ __device__ uint32_t beginBatchThread0(Globals* g, uint64_t capacity, uint32_t ch
         return 0;
     }
     g->numSpilled = 0;
+    g->spillLimit = kNone;
     g->numBacklog = 0;
     g->numCandidates = 0;
     g->numAccepted = 0;
     g->numPending = 0;
     // remolod_octree.cu:1181-1210: stop before a batch once the persistent store
     // is within the safety margin of its capacity.
-    const bool reached = g->arenaOffset + kSafetyMargin >= capacity;
+    const bool reached = g->arenaOffset + margin >= capacity;
     g->memCapacityReached = reached ? 1u : 0u;
     g->stop = reached ? 1u : 0u;
     return 1;
@@ -185,7 +189,8 @@ __global__ void kBeginBatch(DevTree t, BatchRef b, uint64_t capacity, uint32_t c
                             uint64_t budgetTicks, uint32_t beginStep) {
     Globals* g = t.g;
     __shared__ uint32_t opened;
-    if (threadIdx.x == 0) opened = beginBatchThread0(g, capacity, checkBudget, budgetTicks, beginStep);
+    if (threadIdx.x == 0)
+        opened = beginBatchThread0(g, capacity, safetyMargin(t.fixedDepth), checkBudget, budgetTicks, beginStep);
     __syncthreads();
     if (opened) t.backlogFill[threadIdx.x * kShardStride] = 0;
     (void)b;
@@ -408,10 +413,14 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         // leaf's ~50 chunks here was a chain of dependent loads and stores)
         uint32_t spillOff = kNone;
         if (np > 0) {
-            if (uint64_t(spillAt) + np <= t.spillCap)
+            if (uint64_t(spillAt) + np <= t.spillCap) {
                 spillOff = spillAt;
-            else
+            } else {
+                // its points are lost (flagged); later passes stop reading the
+                // spill list where this range starts
                 g->spillOverflow = 1;
+                atomicMin(&g->spillLimit, spillAt);
+            }
         }
         t.acceptedSpill[4 * slot] = node;
         t.acceptedSpill[4 * slot + 1] = spillOff;
@@ -1075,7 +1084,7 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (closeBatch) {
             // the batch loop's closing update (remolod_octree.cu:1223-1226)
-            const uint64_t spilled = min(g->numSpilled, t.spillCap);
+            const uint64_t spilled = spilledCount(t);
             g->batchletIndex += 1;
             g->numPointsProcessed += *b.sizePtr;
             g->batches += 1;
@@ -1102,7 +1111,56 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
     }
 }
 
-// The per-node stats sweep (remolod_octree.cu:1248-1319).
+// The per-node stats sweep (remolod_octree.cu:1248-1319), in two passes: every
+// workgroup reduces its nodes (wave shuffles, then LDS) into its own partial
+// record, and one workgroup adds the records into Globals. One atomic per wave
+// on the Globals line, as this once did, serialised ~2,000 waves on one address.
+// Partial record words: 0 inner, 1 leaves, 2 nonempty, 3 chunksP, 4 chunksV,
+// 5 underflows, 6-7 points, 8-9 voxels, 10 max points, 11 max depth.
+struct StatsAcc {
+    uint32_t inner, leaves, nonempty, chunksP, chunksV, under, maxPts, maxDepth;
+    uint64_t pts, vox;
+};
+
+__device__ __forceinline__ void statsWaveReduce(StatsAcc& a) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a.inner += __shfl_down(a.inner, off);
+        a.leaves += __shfl_down(a.leaves, off);
+        a.nonempty += __shfl_down(a.nonempty, off);
+        a.chunksP += __shfl_down(a.chunksP, off);
+        a.chunksV += __shfl_down(a.chunksV, off);
+        a.under += __shfl_down(a.under, off);
+        a.pts += __shfl_down(a.pts, off);
+        a.vox += __shfl_down(a.vox, off);
+        a.maxPts = max(a.maxPts, __shfl_down(a.maxPts, off));
+        a.maxDepth = max(a.maxDepth, __shfl_down(a.maxDepth, off));
+    }
+}
+
+// Lane 0 of every wave holds its wave's totals; the block's totals go to wave 0.
+__device__ __forceinline__ void statsBlockReduce(StatsAcc& a) {
+    __shared__ StatsAcc waves[kBlock / 64];
+    statsWaveReduce(a);
+    const uint32_t w = threadIdx.x / 64;
+    if (laneId() == 0) waves[w] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 1; k < kBlock / 64; ++k) {
+            const StatsAcc& o = waves[k];
+            a.inner += o.inner;
+            a.leaves += o.leaves;
+            a.nonempty += o.nonempty;
+            a.chunksP += o.chunksP;
+            a.chunksV += o.chunksV;
+            a.under += o.under;
+            a.pts += o.pts;
+            a.vox += o.vox;
+            a.maxPts = max(a.maxPts, o.maxPts);
+            a.maxDepth = max(a.maxDepth, o.maxDepth);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void kStatsSweep(DevTree t) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
@@ -1126,38 +1184,55 @@ __global__ __launch_bounds__(kBlock) void kStatsSweep(DevTree t) {
             chunksV += (nv + kPointsPerChunk - 1) / kPointsPerChunk;
         }
     }
-    // wave reductions, then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-        inner += __shfl_down(inner, off);
-        leaves += __shfl_down(leaves, off);
-        nonempty += __shfl_down(nonempty, off);
-        chunksP += __shfl_down(chunksP, off);
-        chunksV += __shfl_down(chunksV, off);
-        under += __shfl_down(under, off);
-        pts += __shfl_down(pts, off);
-        vox += __shfl_down(vox, off);
-        maxPts = max(maxPts, __shfl_down(maxPts, off));
-        maxDepth = max(maxDepth, __shfl_down(maxDepth, off));
-    }
-    if (laneId() == 0) {
-        atomicAdd(&g->sInner, inner);
-        atomicAdd(&g->sLeaves, leaves);
-        atomicAdd(&g->sNonempty, nonempty);
-        atomicAdd(&g->sChunksPoints, chunksP);
-        atomicAdd(&g->sChunksVoxels, chunksV);
-        atomicAdd(&g->sUnderflows, under);
-        atomicAdd((unsigned long long*)&g->sPoints, (unsigned long long)pts);
-        atomicAdd((unsigned long long*)&g->sVoxels, (unsigned long long)vox);
-        atomicMax(&g->sMaxPoints, maxPts);
-        atomicMax(&g->sMaxDepth, maxDepth);
+    StatsAcc a{inner, leaves, nonempty, chunksP, chunksV, under, maxPts, maxDepth, pts, vox};
+    statsBlockReduce(a);
+    if (threadIdx.x == 0) {
+        uint32_t* p = t.statsPartials + uint64_t(blockIdx.x) * kStatsWords;
+        p[0] = a.inner;
+        p[1] = a.leaves;
+        p[2] = a.nonempty;
+        p[3] = a.chunksP;
+        p[4] = a.chunksV;
+        p[5] = a.under;
+        p[6] = uint32_t(a.pts);
+        p[7] = uint32_t(a.pts >> 32);
+        p[8] = uint32_t(a.vox);
+        p[9] = uint32_t(a.vox >> 32);
+        p[10] = a.maxPts;
+        p[11] = a.maxDepth;
     }
 }
 
-__global__ void kStatsClear(DevTree t) {
-    Globals* g = t.g;
-    g->sInner = g->sLeaves = g->sNonempty = g->sMaxPoints = 0;
-    g->sPoints = g->sVoxels = 0;
-    g->sChunksPoints = g->sChunksVoxels = g->sUnderflows = g->sMaxDepth = 0;
+// One workgroup: the partial records of `blocks` sweep workgroups into Globals.
+__global__ __launch_bounds__(kBlock) void kStatsFinish(DevTree t, uint32_t blocks) {
+    StatsAcc a{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+        const uint32_t* p = t.statsPartials + uint64_t(b) * kStatsWords;
+        a.inner += p[0];
+        a.leaves += p[1];
+        a.nonempty += p[2];
+        a.chunksP += p[3];
+        a.chunksV += p[4];
+        a.under += p[5];
+        a.pts += uint64_t(p[6]) | (uint64_t(p[7]) << 32);
+        a.vox += uint64_t(p[8]) | (uint64_t(p[9]) << 32);
+        a.maxPts = max(a.maxPts, p[10]);
+        a.maxDepth = max(a.maxDepth, p[11]);
+    }
+    statsBlockReduce(a);
+    if (threadIdx.x == 0) {
+        Globals* g = t.g;
+        g->sInner = a.inner;
+        g->sLeaves = a.leaves;
+        g->sNonempty = a.nonempty;
+        g->sChunksPoints = a.chunksP;
+        g->sChunksVoxels = a.chunksV;
+        g->sUnderflows = a.under;
+        g->sPoints = a.pts;
+        g->sVoxels = a.vox;
+        g->sMaxPoints = a.maxPts;
+        g->sMaxDepth = a.maxDepth;
+    }
 }
 
 }  // namespace
@@ -1243,14 +1318,21 @@ struct DeepenCandidate {
     uint32_t numPoints, level, x, y, z, node;
 };
 
-__global__ __launch_bounds__(kBlock) void kDeepenList(DevTree t, DeepenCandidate* out, uint32_t* count,
+// count[0]: candidates found (may pass cap: only the first cap are written);
+// count[1]: 0, the size of the empty batch the split path reads.
+__global__ void kDeepenClear(uint32_t* count) {
+    if (threadIdx.x < 2) count[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void kDeepenList(DevTree t, DeepenCandidate* out, uint32_t* count, uint32_t cap,
                                                       uint32_t minPoints) {
     const uint32_t n = t.g->numNodes;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (t.firstChild[i] >= 0 || t.level[i] >= uint32_t(kMaxDepth)) continue;
         const uint32_t np = t.numPoints[i];
         if (np < minPoints || t.counter[i] != np) continue;
-        out[atomicAdd(count, 1u)] = DeepenCandidate{np, t.level[i], t.nx[i], t.ny[i], t.nz[i], i};
+        const uint32_t at = atomicAdd(count, 1u);
+        if (at < cap) out[at] = DeepenCandidate{np, t.level[i], t.nx[i], t.ny[i], t.nz[i], i};
     }
 }
 
@@ -1263,14 +1345,16 @@ __global__ void kDeepenBegin(DevTree t, uint32_t numChosen) {
     g->numAccepted = 0;
     g->numPending = 0;
     g->numSpilled = 0;
+    g->spillLimit = kNone;
     g->numBacklog = 0;
     g->stop = 0;
 }
 
-void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t minPoints, const LaunchShape& ls,
-                      hipStream_t s) {
+void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t cap, uint32_t minPoints,
+                      const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kDeepenClear, dim3(1), dim3(64), 0, s, count);
     hipLaunchKernelGGL(kDeepenList, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, static_cast<DeepenCandidate*>(out),
-                       count, minPoints);
+                       count, cap, minPoints);
 }
 
 void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, const LaunchShape& ls, hipStream_t s) {
@@ -1284,8 +1368,9 @@ void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, con
 }
 
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(kStatsClear, dim3(1), dim3(1), 0, s, t);
-    hipLaunchKernelGGL(kStatsSweep, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
+    const uint32_t blocks = std::min(ls.nodeBlocks, kStatsBlocks);
+    hipLaunchKernelGGL(kStatsSweep, dim3(blocks), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kStatsFinish, dim3(1), dim3(kBlock), 0, s, t, blocks);
 }
 
 }  // namespace remo_hip

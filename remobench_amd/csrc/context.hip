@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -99,6 +100,11 @@ struct Scope {
     }
 };
 
+// A deepening candidate as kDeepenList writes it.
+struct DeepenHostRec {
+    uint32_t numPoints, level, x, y, z, node;
+};
+
 }  // namespace
 
 struct remo_ctx {
@@ -123,6 +129,8 @@ struct remo_ctx {
     RefineState refine;
     void* deepenRecords = nullptr;    // [maxNodes] candidate records (first remo_deepen)
     uint32_t* deepenCount = nullptr;  // device counter; deepenCount[1] stays 0: an empty batch's size
+    std::unique_ptr<DeepenHostRec[]> deepenHost;  // [maxNodes] host copy of the records (sorted there)
+    std::unique_ptr<uint32_t[]> deepenIds;        // [maxNodes] the chosen leaves
     AccumWork accWork;
     Profiler prof;
 };
@@ -301,6 +309,12 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         delete c;
         return REMO_ERR_INVALID;
     }
+    c->deepenHost.reset(new (std::nothrow) DeepenHostRec[cfg.max_nodes]);
+    c->deepenIds.reset(new (std::nothrow) uint32_t[cfg.max_nodes]);
+    if (!c->deepenHost || !c->deepenIds) {
+        delete c;
+        return REMO_ERR_OOM;
+    }
 
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
@@ -379,6 +393,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.voxelShardStart, uint64_t(kBacklogShards) * t.maxGrids, "voxel slot starts per grid and backlog segment");
     A(&t.candidates, t.candidateCap, "split candidates");
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 4, "accepted splits");
+    A(&t.statsPartials, uint64_t(kStatsBlocks) * kStatsWords, "stats sweep partials");
     A(&t.g, 1, "device counters");
     A(&t.accums, N, "the accumulator side array");
     if (t.fixedDepth) A(&t.cellCount, fixedPyramidEntries(t.fixedDepth), "the fixed-depth cell counts");
@@ -660,62 +675,77 @@ int remo_refine(remo_ctx* c, float budgetMs, uint32_t maxLevels, remo_refine_sta
     return REMO_OK;
 }
 
-int remo_deepen(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out) {
-    if (!c || !out) return REMO_ERR_INVALID;
-    if (c->dead) return REMO_ERR_DEVICE;
-    std::memset(out, 0, sizeof(*out));
+// remo_deepen's work with the candidate records in `records` (room for `cap`)
+// and the two counter words in `count`, device memory the caller provides.
+// Host memory: the context's deepenHost / deepenIds, allocated at create.
+static int deepenWith(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, void* records, uint32_t* count,
+                      uint32_t cap, remo_deepen_stats* out) {
     DevTree& t = c->t;
     hipStream_t s = c->stream;
     int rc = readGlobals(c);
     if (rc != REMO_OK) return rc;
-    if (c->hostG->memCapacityReached || c->hostG->arenaOffset + kSafetyMargin >= t.arenaCapacity) {
+    if (c->hostG->memCapacityReached || c->hostG->arenaOffset + safetyMargin(t.fixedDepth) >= t.arenaCapacity) {
         out->refused = 1;
         return REMO_OK;
     }
     if (maxSplits == 0) return REMO_OK;
-    if (!c->deepenRecords) {
-        rc = devAlloc(c, reinterpret_cast<uint8_t**>(&c->deepenRecords), uint64_t(t.maxNodes) * kDeepenRecordBytes,
-                      "deepening candidates");
-        if (rc == REMO_OK) rc = devAlloc(c, &c->deepenCount, 2, "deepening counter");
-        if (rc != REMO_OK) return rc;
-        CK(hipMemsetAsync(c->deepenCount, 0, 8, s), "deepening counter");
-    }
-    CK(hipMemsetAsync(c->deepenCount, 0, 4, s), "deepening counter");
-    launchDeepenList(t, c->deepenRecords, c->deepenCount, std::max(1u, minPoints), c->shape, s);
+    launchDeepenList(t, records, count, cap, std::max(1u, minPoints), c->shape, s);
     CK(hipGetLastError(), "deepening list kernel");
-    uint32_t n = 0;
-    CK(hipMemcpyAsync(c->hostScratch, c->deepenCount, 4, hipMemcpyDeviceToHost, s), "deepening count");
+    CK(hipMemcpyAsync(c->hostScratch, count, 4, hipMemcpyDeviceToHost, s), "deepening count");
     CK(hipStreamSynchronize(s), "deepening list");
-    n = std::min(c->hostScratch[0], t.maxNodes);
+    const uint32_t n = c->hostScratch[0];
     out->candidates = n;
+    if (n > cap) {
+        c->err = "deepening: " + std::to_string(n) + " candidate leaves exceed the " + std::to_string(cap) +
+                 " records the candidate buffer holds";
+        return REMO_ERR_CAPACITY;
+    }
     if (n == 0) return REMO_OK;
-    struct Rec {
-        uint32_t numPoints, level, x, y, z, node;
-    };
-    static_assert(sizeof(Rec) == kDeepenRecordBytes, "record layout");
-    std::vector<Rec> recs(n);
-    CK(hipMemcpy(recs.data(), c->deepenRecords, sizeof(Rec) * n, hipMemcpyDeviceToHost), "deepening records");
+    DeepenHostRec* recs = c->deepenHost.get();
+    static_assert(sizeof(DeepenHostRec) == kDeepenRecordBytes, "record layout");
+    CK(hipMemcpy(recs, records, sizeof(DeepenHostRec) * n, hipMemcpyDeviceToHost), "deepening records");
     // the densest first; ties by canonical key, so the choice is deterministic
-    std::sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) {
+    std::sort(recs, recs + n, [](const DeepenHostRec& a, const DeepenHostRec& b) {
         if (a.numPoints != b.numPoints) return a.numPoints > b.numPoints;
         if (a.level != b.level) return a.level < b.level;
         if (a.x != b.x) return a.x < b.x;
         if (a.y != b.y) return a.y < b.y;
         return a.z < b.z;
     });
-    const uint32_t chosen = std::min(std::min(n, maxSplits), t.candidateCap);
-    std::vector<uint32_t> ids(chosen);
-    for (uint32_t k = 0; k < chosen; ++k) ids[k] = recs[k].node;
-    CK(hipMemcpy(t.candidates, ids.data(), 4ull * chosen, hipMemcpyHostToDevice), "deepening candidates");
+    // the chosen leaves' points must fit the spill list they move through, and
+    // their new voxels (at most one per point) the voxel backlog
+    const uint64_t pointBudget = std::min<uint64_t>(t.spillCap, t.backlogCap);
+    uint32_t chosen = 0;
+    uint64_t points = 0;
+    const uint32_t limit = std::min(std::min(n, maxSplits), t.candidateCap);
+    while (chosen < limit && points + recs[chosen].numPoints <= pointBudget) points += recs[chosen++].numPoints;
     out->chosen = chosen;
-    launchDeepenSplit(t, chosen, BatchRef{t.spill, c->deepenCount + 1}, c->shape, s);
+    if (chosen == 0) return REMO_OK;
+    uint32_t* ids = c->deepenIds.get();
+    for (uint32_t k = 0; k < chosen; ++k) ids[k] = recs[k].node;
+    CK(hipMemcpy(t.candidates, ids, 4ull * chosen, hipMemcpyHostToDevice), "deepening candidates");
+    launchDeepenSplit(t, chosen, BatchRef{t.spill, count + 1}, c->shape, s);
     CK(hipGetLastError(), "deepening kernels");
     rc = readGlobals(c);
     if (rc != REMO_OK) return rc;
-    out->splits = c->hostG->numAccepted;
-    out->points_moved = std::min<uint64_t>(c->hostG->numSpilled, t.spillCap);
-    out->new_voxels = std::min<uint64_t>(c->hostG->numBacklog, t.backlogCap);
+    const Globals& g = *c->hostG;
+    out->splits = g.numAccepted;
+    out->points_moved = std::min<uint64_t>(std::min<uint64_t>(g.numSpilled, t.spillCap), g.spillLimit);
+    out->new_voxels = std::min<uint64_t>(g.numBacklog, t.backlogCap);
     return REMO_OK;
+}
+
+int remo_deepen(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    std::memset(out, 0, sizeof(*out));
+    if (maxSplits != 0 && !c->deepenRecords) {
+        int rc = devAlloc(c, reinterpret_cast<uint8_t**>(&c->deepenRecords),
+                          uint64_t(c->t.maxNodes) * kDeepenRecordBytes, "deepening candidates");
+        if (rc == REMO_OK) rc = devAlloc(c, &c->deepenCount, 2, "deepening counter");
+        if (rc != REMO_OK) return rc;
+    }
+    return deepenWith(c, maxSplits, minPoints, c->deepenRecords, c->deepenCount, c->t.maxNodes, out);
 }
 
 int remo_render_flat(remo_ctx* c, const remo_point* points, uint64_t n, const remo_shared_uniforms* u, uint64_t* fb) {
@@ -887,6 +917,17 @@ int remo_get_stats(remo_ctx* c, remo_stats* out) {
     return REMO_OK;
 }
 
+int remo_get_render_diagnostics(remo_ctx* c, uint32_t* items, uint64_t* samples, uint32_t* overflow) {
+    if (!c) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    analysisReadDiagnostics(c->analysis, c->stream);
+    CK(hipGetLastError(), "render diagnostics");
+    if (items) *items = c->analysis.drawItems;
+    if (samples) *samples = c->analysis.drawSamples;
+    if (overflow) *overflow = c->analysis.drawListOverflow;
+    return REMO_OK;
+}
+
 int remo_dump_nodes(remo_ctx* c, remo_node_record* out, uint32_t cap, uint32_t* num) {
     if (!c || !num) return REMO_ERR_INVALID;
     if (c->dead) return REMO_ERR_DEVICE;
@@ -1002,6 +1043,24 @@ int remo_dump_analysis(remo_ctx* c, remo_node_analysis* out, uint32_t cap, uint3
     if (!out) return REMO_OK;
     if (cap < n) return REMO_ERR_INVALID;
     CK(hipMemcpy(out, c->analysis.results, sizeof(remo_node_analysis) * n, hipMemcpyDeviceToHost), "dump");
+    return REMO_OK;
+}
+
+int remo_dump_rollup(remo_ctx* c, remo_node_rollup* out, uint32_t cap, uint32_t* num) {
+    if (!c || !num) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    const uint32_t n = c->hostG->numNodes;
+    *num = n;
+    if (!out) return REMO_OK;
+    if (cap < n) return REMO_ERR_INVALID;
+    static_assert(sizeof(RollupSums) == sizeof(remo_node_rollup), "roll-up record layout");
+    std::vector<uint8_t> closed(n);
+    CK(hipMemcpy(out, c->analysis.rollup, sizeof(RollupSums) * n, hipMemcpyDeviceToHost), "dump");
+    CK(hipMemcpy(closed.data(), c->analysis.closed, n, hipMemcpyDeviceToHost), "dump");
+    // RollupSums ends in {n, e, pad}: the pad word carries the closure flag here
+    for (uint32_t i = 0; i < n; ++i) out[i].closed = closed[i];
     return REMO_OK;
 }
 

@@ -29,7 +29,14 @@ constexpr int kMaxDepth = 20;
 constexpr uint64_t kChunkAllocBytes = 16032;   // 16 * ((16,016 + 16) / 16)
 constexpr uint64_t kGridAllocBytes = 262160;   // 16 * ((262,144 + 16) / 16)
 constexpr uint64_t kArenaFirstOffset = 16;
-constexpr uint64_t kSafetyMargin = 200000000ull;  // remolod_octree.cu:1195
+// The store margin the batch loop stops within (remolod_octree.cu:1191-1195):
+// 200 MB for the iterative expand, 1 GB for the analytic-bucketing arm, whose
+// batches split hundreds of nodes at once.
+constexpr uint64_t kSafetyMargin = 200000000ull;
+constexpr uint64_t kSafetyMarginFixed = 1000000000ull;
+__host__ __device__ inline uint64_t safetyMargin(uint32_t fixedDepth) {
+    return fixedDepth ? kSafetyMarginFixed : kSafetyMargin;
+}
 constexpr uint32_t kPointDir = 256;    // chunk slots per leaf: 256,000 points (fixed-depth leaves exceed 50,000)
 constexpr uint32_t kVoxelDir = 2112;   // >= ceil(2^21 / 1000) chunk slots per grid
 constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -68,7 +75,8 @@ struct Globals {
     uint32_t dirOverflow, candidateOverflow, budgetStop, pad1;
     uint64_t stepStart;            // wall_clock64() at the start of the step
     uint32_t maxLevel;             // deepest node level (kept by the splits)
-    uint32_t pad3;
+    uint32_t spillLimit;           // lowest spill offset of a split that did not fit (kNone: none);
+                                   // spill slots from here on were never written
     // per-step counters
     uint64_t batches, expandIters, nodesSplit, spilledPoints, newVoxels;
     // the same, cumulative since reset
@@ -130,9 +138,15 @@ struct DevTree {
     float octreeSize;
     uint32_t fixedDepth;    // 0: iterative expand; else the analytic-bucketing arm's depth
     uint32_t* cellCount;    // fixed-depth cell-count pyramid, levels 0..fixedDepth
+    uint32_t* statsPartials;  // [kStatsBlocks * kStatsWords] per-workgroup partials of the stats sweep
     Globals* g;
     remo_node_accum* accums;
 };
+
+// The stats sweep reduces per workgroup into statsPartials, then one workgroup
+// adds the partials (no same-address atomics).
+constexpr uint32_t kStatsBlocks = 512;
+constexpr uint32_t kStatsWords = 16;
 
 // Partial sums of one Accumulate tile.
 struct AccumPartial {
@@ -273,11 +287,12 @@ void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hip
 // closeBatch: also the batch loop's closing update (batch index, processed points)
 void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool closeBatch = true);
 // Refinement's local deepening: list the leaves that may split (records of 24 B:
-// numPoints, level, x, y, z, node), then split the chosen ones (already in
-// t.candidates) and redistribute their points; `empty` is a batch of size 0.
+// numPoints, level, x, y, z, node; at most cap written, count[0] = all found,
+// count[1] = 0), then split the chosen ones (already in t.candidates) and
+// redistribute their points; `empty` is a batch of size 0.
 constexpr uint32_t kDeepenRecordBytes = 24;
-void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t minPoints, const LaunchShape& ls,
-                      hipStream_t s);
+void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t cap, uint32_t minPoints,
+                      const LaunchShape& ls, hipStream_t s);
 void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, const LaunchShape& ls, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);

@@ -212,11 +212,12 @@ void RemolodHipPipeline::render(const FrameContext& frame) {
         return;
     }
     if (frame.strictTiming) remo_sync(m_ctx);
-    remo_stats s{};
-    if (remo_get_stats(m_ctx, &s) == REMO_OK) {
-        m_stats.numVisibleNodes = s.draw_items;
-        m_stats.numVisiblePoints = s.draw_samples;
-        if (s.draw_list_overflow) m_stats.allocOverflow = true;
+    uint32_t items = 0, overflow = 0;
+    uint64_t samples = 0;
+    if (remo_get_render_diagnostics(m_ctx, &items, &samples, &overflow) == REMO_OK) {
+        m_stats.numVisibleNodes = items;
+        m_stats.numVisiblePoints = samples;
+        if (overflow) m_stats.allocOverflow = true;
     }
 }
 

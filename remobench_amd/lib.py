@@ -127,6 +127,17 @@ ANALYSIS_DTYPE = np.dtype([
 ])
 
 
+ROLLUP_DTYPE = np.dtype([
+    ("s1", "<i8", (3, 2)), ("s2", "<i8", (6, 2)), ("n", "<u8"), ("e", "<u4"), ("closed", "<u4"),
+])
+
+
+def int128(pairs: np.ndarray) -> list:
+    """[..., 2] (low, high) int64 halves -> Python ints (two's complement 128-bit)."""
+    a = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    return [(int(hi) << 64) | (int(lo) & 0xFFFFFFFFFFFFFFFF) for lo, hi in a]
+
+
 SCOPES = ["remolod.construct.count", "remolod.construct.split", "remolod.construct.voxelSampling",
           "remolod.construct.allocChunks", "remolod.construct.insertPoints",
           "remolod.construct.insertVoxels", "remolod.accumulate", "remolod.analysis", "remolod.render",
@@ -193,6 +204,7 @@ def load() -> C.CDLL:
         "remo_analysis": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int]),
         "remo_render": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_void_p]),
         "remo_get_stats": (C.c_int, [C.c_void_p, P(Stats)]),
+        "remo_get_render_diagnostics": (C.c_int, [C.c_void_p, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]),
         "remo_last_error": (C.c_char_p, [C.c_void_p]),
         "remo_sync": (C.c_int, [C.c_void_p]),
         "remo_dump_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
@@ -201,6 +213,7 @@ def load() -> C.CDLL:
         "remo_dump_grid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
         "remo_dump_accums": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_dump_analysis": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
+        "remo_dump_rollup": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_ring_create": (C.c_int, [C.c_uint32, C.c_uint32, P(C.c_void_p)]),
         "remo_ring_destroy": (None, [C.c_void_p]),
         "remo_ring_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),

@@ -349,6 +349,14 @@ class Tree:
                  "analysis dump")
         return out
 
+    def rollup(self) -> np.ndarray:
+        """The exact int128 roll-up sums of every node from the last analysis()."""
+        n = C.c_uint32(0)
+        self._ck(self.lib.remo_dump_rollup(self.h, None, 0, C.byref(n)), "rollup dump")
+        out = np.zeros(n.value, dtype=L.ROLLUP_DTYPE)
+        self._ck(self.lib.remo_dump_rollup(self.h, out.ctypes.data, len(out), C.byref(n)), "rollup dump")
+        return out
+
 
 class RemolodPipeline:
     """The `remolod` plugin (src/pipelines/RemolodPipeline.{h,cpp}) on the HIP path."""

@@ -60,3 +60,67 @@ def tiny_cases():
 
 def lidar_scene(n=2_097_152, seed=1, kind="lidar"):
     return L.generate_scene(n, seed, kind)
+
+
+# ---- hand-derived cases ------------------------------------------------------
+
+def closure_case():
+    """Batches of uneven size in the unit box, one per step. Batch 0: 60,000 points
+    over every octant but 7 (the root splits in it; octant 7 stays empty); batch b
+    (1..7): 100 points in octant b-1. Accumulating with the consumed count after
+    each step, octant j (0..6) was last touched at batch j + 2 and octant 7 never;
+    after 8 batches a leaf is closed iff 8 >= last + margin, i.e. octant j closes
+    iff j <= 6 - margin, the empty octant always, and the root iff all eight are.
+    Returns (list of batches, meta)."""
+    rng = np.random.default_rng(21)
+
+    def inside(p, hi):
+        # float32 rounding can land a draw on the open end; keep it below
+        for a, ax in enumerate(("x", "y", "z")):
+            p[ax] = np.minimum(p[ax], np.nextafter(np.float32(hi[a]), np.float32(0)))
+        return p
+
+    b0 = inside(uniform_in_box(60_000, (0, 0, 0), (1, 1, 1), 22, None), (1, 1, 1))
+    top = (b0["x"] >= 0.5) & (b0["y"] >= 0.5) & (b0["z"] >= 0.5)
+    b0["x"][top] -= np.float32(0.5)  # octant 7 -> octant 3
+    batches = [b0]
+    for b in range(1, 8):
+        o = b - 1
+        lo = (0.5 * ((o >> 2) & 1), 0.5 * ((o >> 1) & 1), 0.5 * (o & 1))
+        hi = tuple(v + 0.5 for v in lo)
+        batches.append(inside(uniform_in_box(100, lo, hi, int(rng.integers(1 << 30)), None), hi))
+    pts = np.concatenate(batches)
+    return batches, make_meta(pts, [1.0, 1.0, 1.0])
+
+
+def closure_expected(margin: int) -> dict:
+    """octant -> closed, and the root's closure, for closure_case()."""
+    leaves = {j: (j <= 6 - margin) for j in range(7)}
+    leaves[7] = True
+    return {"leaves": leaves, "root": all(leaves.values())}
+
+
+def cross_case(a: int, b: int, c: int):
+    """Six points at the unit box's centre +-a, +-b, +-c units of 2^-20 along x, y, z:
+    the root stays a leaf with fixed-point sums S1 = 0 and S2 = diag(2a^2, 2b^2,
+    2c^2), so the covariance in node-extent units is diag(a^2, b^2, c^2) / (3 4^20)
+    and its eigenvalues over the trace are (a^2, b^2, c^2) / (a^2 + b^2 + c^2)."""
+    u = np.float32(2.0 ** -20)
+    h = np.float32(0.5)
+    p = np.zeros(6, dtype=L.POINT_DTYPE)
+    p["x"], p["y"], p["z"] = h, h, h
+    p["x"][0], p["x"][1] = h + np.float32(a) * u, h - np.float32(a) * u
+    p["y"][2], p["y"][3] = h + np.float32(b) * u, h - np.float32(b) * u
+    p["z"][4], p["z"][5] = h + np.float32(c) * u, h - np.float32(c) * u
+    p["color"] = 0xFF808080
+    return p, make_meta(p, [1.0, 1.0, 1.0])
+
+
+def stacked_batches(batches):
+    """(slots, sizes, slot_capacity) of a non-wrapping ring holding `batches`."""
+    cap = max(len(b) for b in batches)
+    slots = np.zeros(cap * len(batches), dtype=L.POINT_DTYPE)
+    for i, b in enumerate(batches):
+        slots[i * cap: i * cap + len(b)] = b
+    sizes = np.array([len(b) for b in batches], dtype=np.uint32)
+    return slots, sizes, cap

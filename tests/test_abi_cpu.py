@@ -38,6 +38,26 @@ def test_struct_sizes_match_the_header():
     assert L.NODE_DTYPE.itemsize == 48
     assert L.ACCUM_DTYPE.itemsize == 72 + 72 + 24 + 16 + 8
     assert L.ANALYSIS_DTYPE.itemsize == 48
+    assert L.ROLLUP_DTYPE.itemsize == 9 * 16 + 8 + 4 + 4
+
+
+def test_rollup_record_layout_matches_the_header(tmp_path):
+    fields = ["s1", "s2", "n", "e", "closed"]
+    lines = ['printf("%zu\\n", sizeof(remo_node_rollup));']
+    lines += [f'printf("%zu\\n", offsetof(remo_node_rollup, {f}));' for f in fields]
+    src = tmp_path / "rollup.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "remolod.h"\nint main(void) {\n' +
+                   "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "rollup"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == L.ROLLUP_DTYPE.itemsize
+    assert got[1:] == [L.ROLLUP_DTYPE.fields[f][1] for f in fields]
+
+
+def test_int128_halves():
+    pairs = np.array([[0, 0], [-1, -1], [5, 0], [0, 1], [-(2**63), 2**62]], dtype=np.int64)
+    assert L.int128(pairs) == [0, -1, 5, 2**64, (2**62 << 64) + 2**63]
 
 
 @pytest.mark.parametrize("cname,mirror", [("remo_refine_stats", "RefineStats"), ("remo_deepen_stats", "DeepenStats"),

@@ -58,6 +58,18 @@ def test_analysis_parity(built, pose):
 
 
 @pytest.mark.gpu
+def test_rollup_bit_exact(built):
+    """The exact roll-up itself, not only what the eigen-solve makes of it: every
+    node's int128 sums, count, exponent and closure flag equal the oracle's."""
+    from .test_gpu_pins import compare_rollup
+    points, meta, tree, ring, o = built
+    u = camera.poses(points, meta.box_size)["overview"]
+    tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
+    got = compare_rollup(tree, o, o.analyse(u))
+    assert int(got["n"].max()) == len(points)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("pose", ["overview", "street", "closeup"])
 def test_render_parity(built, pose):
     points, meta, tree, ring, o = built

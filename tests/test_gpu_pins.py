@@ -1,0 +1,194 @@
+"""GPU checks of Analysis state against hand-derived expectations, of the exact
+roll-up against the oracle bit for bit, and of the capacity rules (persistent
+store stop, spill list, deepening's point budget) against the oracle."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleTree
+from remobench_amd import camera
+from remobench_amd import lib as L
+from remobench_amd.pipeline import PointRing, Tree
+
+from .cases import closure_case, closure_expected, cross_case, lidar_scene, tiny_cases
+from .helpers import batch_arrays, compare_trees, keyed, point_rows, stats_subset
+from .test_gpu_construct import build_hip, build_oracle, compare_accums
+
+
+def ring_of(batches):
+    """A device ring holding `batches` (uneven sizes) in slots of the largest."""
+    cap = max(len(b) for b in batches)
+    ring = PointRing(cap, len(batches))
+    for i, b in enumerate(batches):
+        b = np.ascontiguousarray(b, dtype=L.POINT_DTYPE)
+        L.check(ring.lib.remo_ring_upload(ring.h, i, b.ctypes.data, len(b)), "upload")
+    ring.num_batches_total = len(batches)
+    ring.produced = len(batches)
+    return ring, cap
+
+
+def compare_rollup(tree, o, res_o):
+    """Every node's int128 sums, count, exponent and closure flag, bit for bit."""
+    got = tree.rollup()
+    kg, ko = keyed(tree.nodes()), keyed(o.nodes())
+    assert set(kg) == set(ko)
+    for k, ig in kg.items():
+        g, w = got[ig], res_o[ko[k]]
+        assert int(g["n"]) == int(w["n"]), k
+        assert int(g["e"]) == int(w["e"]), k
+        assert L.int128(g["s1"]) == [(int(h) << 64) | (int(lo) & (2**64 - 1)) for h, lo in zip(w["s1_hi"], w["s1_lo"])], k
+        assert L.int128(g["s2"]) == [(int(h) << 64) | (int(lo) & (2**64 - 1)) for h, lo in zip(w["s2_hi"], w["s2_lo"])], k
+        assert (int(g["closed"]) == 1) == (int(w["r"]["state"]) == 2), k
+    return got
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("margin", [1, 2, 6, 7])
+def test_closure_rule_on_gpu(gpu, margin):
+    batches, meta = closure_case()
+    print(f"closure case, margin {margin}", flush=True)
+    ring, cap = ring_of(batches)
+    tree = Tree(meta, slot_capacity=cap, ring_slots=len(batches), batches_per_step=1, step_budget_ms=-1.0,
+                closure_batches=margin)
+    for b in range(len(batches)):
+        rc, consumed = tree.build_step(ring.view())
+        assert consumed == b + 1
+        tree.accumulate(consumed)
+    pts = np.concatenate(batches)
+    u = camera.poses(pts, meta.box_size, 64, 64)["overview"]
+    tree.analysis(u)
+    res = tree.analysis_results()
+    roll = tree.rollup()
+    k = keyed(tree.nodes())
+    assert len(k) == 9
+    exp = closure_expected(margin)
+    for j in range(8):
+        i = k[(1, (j >> 2) & 1, (j >> 1) & 1, j & 1)]
+        assert (res[i]["state"] == 2) == exp["leaves"][j], (margin, j)
+        assert bool(roll[i]["closed"]) == exp["leaves"][j], (margin, j)
+    root = k[(0, 0, 0, 0)]
+    assert (res[root]["state"] == 2) == exp["root"]
+    tree.close()
+    ring.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c,fallback", [(4300, True), (4200, False)])
+def test_fp64_fallback_threshold_on_gpu(gpu, c, fallback):
+    a, b = 1000, 1001
+    pts, meta = cross_case(a, b, c)
+    tree, ring = build_hip(pts, meta, len(pts))
+    u = camera.poses(pts, meta.box_size, 64, 64)["overview"]
+    tree.analysis(u)
+    r = tree.analysis_results()[0]
+    assert int(r["flags"] & 1) == int(fallback)
+    T = a * a + b * b + c * c
+    lam = np.array([a * a, b * b, c * c], dtype=np.float64) / (3.0 * 4.0 ** 20)
+    np.testing.assert_allclose(r["lambda"], lam, rtol=2e-7)
+    np.testing.assert_allclose(r["surface_variation"], a * a / T, rtol=1e-6)
+    tol = 1e-12 if fallback else 2 * 2.0 ** -24 * (b * b) / (c * c)
+    np.testing.assert_allclose(r["planarity"], (b * b - a * a) / (c * c), rtol=1e-6, atol=tol)
+    roll = tree.rollup()[0]
+    # S1 = 0, S2 = diag(2a^2, 2b^2, 2c^2), n = 6, e = 20 (the root is the deepest level)
+    assert L.int128(roll["s1"]) == [0, 0, 0]
+    assert L.int128(roll["s2"]) == [2 * a * a, 0, 0, 2 * b * b, 0, 2 * c * c]
+    assert int(roll["n"]) == 6 and int(roll["e"]) == 20
+    tree.close()
+    ring.close()
+
+
+@pytest.mark.gpu
+def test_rollup_bit_exact_on_a_deep_skewed_tree(gpu):
+    """The int128 roll-up of a tree 10+ levels deep (shifts of 2^(18+D-L) across
+    many levels), every node bit for bit against the oracle."""
+    points, meta, batch = tiny_cases()["skewed_ragged"]
+    print("skewed_ragged: roll-up", flush=True)
+    tree, _ = build_hip(points, meta, batch)
+    o = build_oracle(points, meta, batch)
+    nodes = tree.nodes()
+    assert int(nodes["level"].max()) >= 10
+    u = camera.poses(points, meta.box_size, 640, 360)["overview"]
+    tree.analysis(u)
+    compare_rollup(tree, o, o.analyse(u))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fixed_depth", [0, 3])
+def test_persistent_store_stop(gpu, fixed_depth):
+    """remolod_octree.cu:1181-1210: a store that reaches its margin (200 MB, or
+    1 GB for the fixed-depth arm) after batch 6 stops before batch 7; the
+    truncated tree, batch index and allocated bytes are the oracle's."""
+    points, meta = lidar_scene(1_200_000, seed=6)
+    batch = 100_000
+    slots, sizes, nb = batch_arrays(points, batch)
+    probe = OracleTree(meta.box_size, batches_per_step=1, fixed_depth=fixed_depth)
+    offsets = [16 + 262_160]
+    for _ in range(nb):
+        probe.build_step(slots, sizes, nb, batch, nb)
+        offsets.append(probe.stats()["allocated_bytes_persistent"])
+    margin = 1_000_000_000 if fixed_depth else 200_000_000
+    cap = margin + (offsets[6] + offsets[7]) // 2
+    print(f"store stop: fixed depth {fixed_depth}, capacity {cap}", flush=True)
+    tree, _ = build_hip(points, meta, batch, persistent_bytes=cap, fixed_depth=fixed_depth)
+    o = build_oracle(points, meta, batch, persistent_bytes=cap, fixed_depth=fixed_depth)
+    s, so = tree.stats(), o.stats()
+    assert s["mem_capacity_reached"] == so["mem_capacity_reached"] == 1
+    assert s["batchlet_index"] == so["batchlet_index"] == 7
+    assert s["allocated_bytes_persistent"] == so["allocated_bytes_persistent"]
+    assert s["num_points"] == 7 * batch
+    compare_trees(tree, o, max(meta.box_size), points[: 7 * batch])
+    assert stats_subset(s) == stats_subset(so)
+    compare_accums(tree, o)
+    tree.close()
+
+
+@pytest.mark.gpu
+def test_spill_overflow_reads_no_stale_points(gpu):
+    """A spill list smaller than one split's stored points: the split that does
+    not fit loses its points (spill_overflow), and nothing after it reads the
+    unwritten part of the list, so no stored point is a stale copy: the stored
+    points are a sub-multiset of the input and every count is consistent."""
+    points, meta = lidar_scene()
+    batch = 262_144
+    print("spill overflow: 30,000-point spill list", flush=True)
+    tree, _ = build_hip(points, meta, batch, spill_capacity=30_000)
+    s = tree.stats()
+    assert s["spill_overflow"] == 1
+    nodes = tree.nodes()
+    stored = [tree.samples(i) for i in range(len(nodes)) if nodes[i]["first_child"] < 0 and nodes[i]["num_points"]]
+    got = point_rows(np.concatenate(stored))
+    assert len(got) == s["num_points"] < len(points)
+    # sub-multiset of the input: count each distinct record on both sides
+    want = point_rows(points)
+    gk, gc = np.unique(got, axis=0, return_counts=True)
+    wk, wc = np.unique(want, axis=0, return_counts=True)
+    wmap = {tuple(r): int(c) for r, c in zip(wk.tolist(), wc.tolist())}
+    for r, c in zip(gk.tolist(), gc.tolist()):
+        assert wmap.get(tuple(r), 0) >= c, f"point {r} stored {c} times, {wmap.get(tuple(r), 0)} in the input"
+    assert s["counter_underflows"] == 0
+    assert s["num_nodes"] == 1 + 8 * s["num_inner"]
+
+
+@pytest.mark.gpu
+def test_deepen_stops_at_the_spill_capacity(gpu):
+    """Deepening chooses the densest leaves only while their points fit the spill
+    list: the same choice, tree and counters as the oracle's under that budget."""
+    points, meta = lidar_scene(2_097_152, seed=11)
+    size = max(meta.box_size)
+    cap = 1_000_000
+    tree, _ = build_hip(points, meta, 262_144, spill_capacity=cap)
+    o = build_oracle(points, meta, 262_144)
+    assert tree.stats()["spill_overflow"] == 0
+    compare_trees(tree, o, size, points)
+    got = tree.deepen(64, 10_000)
+    want = o.deepen(64, 10_000, point_budget=cap)
+    print("deepen under a 1M spill list", got, want, flush=True)
+    assert got["candidates"] == want["candidates"] > got["chosen"]
+    assert got["splits"] == want["splits"] and got["points_moved"] == want["points_moved"] <= cap
+    assert got["new_voxels"] == want["new_voxels"]
+    assert tree.stats()["spill_overflow"] == 0
+    compare_trees(tree, o, size, points)
+    assert stats_subset(tree.stats()) == stats_subset(o.stats())
