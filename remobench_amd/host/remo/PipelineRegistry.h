@@ -1,11 +1,16 @@
-// Registry of pipeline factories and the switch lifecycle, with the method set of
-// the reference's remo::PipelineRegistry (include/remo/PipelineRegistry.h:13-80,
-// src/shell/PipelineRegistry.cpp): add(factory) keyed by info().id, the budget
-// checks (minBytesRequired / refusalFloor / fits / predictedCompletion /
-// unsupportedReason) and switchTo: release the old pipeline -> source.start(mode,
-// ringSlots, budget) -> factory() -> initPrograms -> allocate -> reset.
+// The set of pipelines a shell can switch between, keyed by PipelineInfo::id.
+//
+// Interface kept from the reference (include/remo/PipelineRegistry.h): add a
+// factory, look pipelines up, switch the active one. The switch follows the
+// plugin lifecycle the reference documents (wiki/01_Overview.md:135-173): the
+// old pipeline is released before anything of the new one exists, the point
+// source is configured for the new pipeline's ingest shape, then the new
+// pipeline is built, initialised, allocated and reset. Whether a pipeline fits
+// the device budget is decided by its own allocate(), which refuses with a
+// reason; the reference's separate budget estimates are not part of this path.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -16,33 +21,31 @@ namespace remo {
 
 class PipelineRegistry {
 public:
-    // Refuses (with a stderr line) a null factory, an empty id or a duplicate id.
-    void add(PipelineFactory factory);
+    // Probes the factory once for its PipelineInfo. Returns false (and says why on
+    // stderr) when the factory yields nothing, the id is empty or already taken.
+    bool add(PipelineFactory factory);
 
-    const std::vector<PipelineInfo>& list() const { return m_infos; }
-    ILodPipeline* active() const { return m_active.get(); }
-    const std::string& activeId() const { return m_activeId; }
+    // Registered pipelines, in registration order.
+    const std::vector<PipelineInfo>& list() const { return m_order; }
     const PipelineInfo* find(const std::string& id) const;
 
-    uint64_t minBytesRequired(const PipelineInfo& info, const CloudMeta& meta) const;
-    uint64_t refusalFloor(const PipelineInfo& info, const CloudMeta& meta) const;
-    bool fits(const PipelineInfo& info, const CloudMeta& meta, const DeviceBudget& budget) const;
-    double predictedCompletion(const PipelineInfo& info, const CloudMeta& meta, const DeviceBudget& budget) const;
-    std::string unsupportedReason(const PipelineInfo& info, const CloudMeta& meta,
-                                  const DeviceBudget& budget) const;
+    ILodPipeline* active() const { return m_active.get(); }
+    const std::string& activeId() const { return m_activeId; }
 
-    // On failure the registry holds no pipeline and *err says why.
+    // Makes `id` the active pipeline for `meta` within `budget`. An unknown id
+    // changes nothing; a step that fails after the old pipeline was released
+    // leaves no pipeline active. *err names the step that failed.
     bool switchTo(const std::string& id, PointSource* source, const CloudMeta& meta, const DeviceBudget& budget,
                   std::string* err);
     void releaseActive();
 
 private:
-    struct Entry {
+    struct Slot {
         PipelineInfo info;
-        PipelineFactory factory;
+        PipelineFactory make;
     };
-    std::vector<Entry> m_entries;
-    std::vector<PipelineInfo> m_infos;
+    std::map<std::string, Slot> m_slots;
+    std::vector<PipelineInfo> m_order;
     std::unique_ptr<ILodPipeline> m_active;
     std::string m_activeId;
 };

@@ -31,6 +31,37 @@ def test_host_library_and_driver_are_built():
     assert r.returncode == 0 and b"usage" in r.stderr
 
 
+def test_registry_lifecycle(tmp_path):
+    """The registry (no device): ids are unique and non-empty, a switch releases
+    the active pipeline before the source is started for the next one, then
+    builds, initialises, allocates and resets it; an unknown id changes nothing,
+    a failed step after the release leaves no pipeline active and names itself."""
+    exe = tmp_path / "registry_check"
+    root = B.ROOT
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-I", str(root / "include"), "-I", str(B.HOST),
+                    str(root / "tests" / "native" / "registry_check.cpp"), "-L", str(B.LIBDIR), "-lremolod_host",
+                    "-lremolod", f"-Wl,-rpath,{B.LIBDIR}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=60)
+    assert r.returncode == 0, r.stderr.decode()
+    out = json.loads(r.stdout.decode().strip())
+    assert out["add"] == ["added", "added", "refused", "refused", "refused", "refused", "added"]
+    assert out["list"] == ["stream", "whole", "broken"]
+    assert out["find_whole"] and not out["find_nope"]
+    assert out["switches"][0] == "ok:stream" and out["switches"][1] == "ok:whole"
+    assert out["switches"][2].startswith("error:") and "no such pipeline" in out["switches"][2]
+    assert out["switches"][2].endswith(":active")  # an unknown id leaves the active pipeline alone
+    assert "allocate" in out["switches"][3] and "budget too small" in out["switches"][3]
+    assert out["switches"][3].endswith(":none")
+    log = out["log"]
+    seg = log[log.index("-- switch whole") + 1: log.index("-- switch nope")]
+    assert seg == ["release stream", "destroy stream", "source start whole 7 5678", "source rewind", "make whole",
+                   "init whole", "allocate whole 1234 5678", "reset whole"]
+    seg = log[log.index("-- switch broken") + 1: log.index("-- release")]
+    assert seg[-2:] == ["allocate broken 1234 5678", "destroy broken"]
+    first = log[log.index("-- switch stream") + 1: log.index("-- switch whole")]
+    assert first[0] == "source start stream 7 5678"
+
+
 def test_driver_rejects_bad_arguments():
     r = _run("--analysis", "bogus")
     assert r.returncode == 2
