@@ -102,7 +102,9 @@ typedef struct remo_batch_view {
  * CloudMeta + DeviceBudget (src/pipelines/RemolodPipeline.cpp:235-325). Zero
  * fields take the reference defaults. */
 typedef struct remo_config {
-    uint64_t persistent_bytes;      /* octree store (grids + chunks); 0: 48 B/pt, >= 512 MiB */
+    uint64_t persistent_bytes;      /* octree store (grids + chunks); 0: 48 B/pt, >= 512 MiB,
+                                       plus 800 MB with fixed_depth (its store margin is
+                                       1 GB instead of 200 MB, remolod_octree.cu:1191-1195) */
     uint32_t max_nodes;             /* node pool, MAX_NODES_CAPACITY (200,000)            */
     uint32_t max_batches_per_step;  /* batches folded per build_step (20)                 */
     float    step_budget_ms;        /* device-time budget per build_step (10); < 0: none   */

@@ -298,7 +298,9 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         (uint64_t(cfg.backlog_capacity) + kBacklogShards - 1) / kBacklogShards * kBacklogShards, 0xFFFFFFC0ull));
     if (!cfg.closure_batches) cfg.closure_batches = 8u;
     if (!cfg.persistent_bytes) {
-        cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20);
+        // the same room above the store margin for both expand arms
+        cfg.persistent_bytes = std::max<uint64_t>(uint64_t(48.0 * double(meta->num_points)), 512ull << 20) +
+                               (safetyMargin(cfg.fixed_depth) - kSafetyMargin);
     }
     if (meta->num_points == 0 || cfg.fixed_depth > REMO_MAX_FIXED_DEPTH || cfg.max_nodes > kTopNodeMask) {
         delete c;

@@ -20,6 +20,7 @@ def check(points, meta, batch, depth, check_colours=True, **kw):
         assert s[k] == so[k], (k, s[k], so[k])
     compare_accums(tree, o)
     assert s["sum_leaf_counts"] == s["num_points"] == len(points)
+    assert s["mem_capacity_reached"] == 0
     return tree, o
 
 
