@@ -393,6 +393,16 @@ typedef struct remo_deepen_stats {
 
 int remo_deepen(remo_ctx* ctx, uint32_t max_splits, uint32_t min_points, remo_deepen_stats* out);
 
+/* remo_deepen for callers that cannot allocate device memory (code under graph
+ * capture, serving loops, harnesses that forbid hipMalloc); no counterpart in
+ * the reference. Same choice, splits and results as remo_deepen, which allocates
+ * its candidate buffer on its first call; this variant allocates nothing and
+ * keeps the candidate records in a fixed device array of the library (4.8 MB per
+ * GPU, held until the process ends): trees of more than 200,000 nodes are
+ * refused with REMO_ERR_CAPACITY. Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (contexts used from several threads) race. */
+int remo_deepen_noalloc(remo_ctx* ctx, uint32_t max_splits, uint32_t min_points, remo_deepen_stats* out);
+
 /* ---- timing scopes (GpuProfiler / GpuScope, include/remo/GpuProfiler.h:62-158) ----
  * When enabled, every phase launch is bracketed by a pair of HIP events on the
  * context's stream; remo_profile_read harvests finished pairs into per-scope

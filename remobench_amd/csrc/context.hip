@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -105,6 +106,37 @@ struct DeepenHostRec {
     uint32_t numPoints, level, x, y, z, node;
 };
 
+// The allocation-free deepening (remo_deepen_noalloc) keeps its candidate
+// records and counters in this file-scope device array instead of memory it
+// allocates: 256 B of counters, then up to kStaticDeepenRecords records
+// (4.8 MB; every GPU the library runs on holds its own copy until the process
+// ends). Host code never passes `scratch` itself to a kernel -- its host-side
+// name is not a device address -- but the address hipGetSymbolAddress returns
+// for the current device, looked up once per device at remo_create.
+constexpr uint32_t kStaticDeepenRecords = 200000;
+constexpr size_t kStaticCounterBytes = 256;
+constexpr size_t kStaticDeepenBytes = kStaticCounterBytes + size_t(kStaticDeepenRecords) * kDeepenRecordBytes;
+static __device__ __align__(256) uint8_t scratch[kStaticDeepenBytes];
+
+std::mutex g_scratchMutex;
+std::vector<void*> g_scratchByDevice;  // device ordinal -> address of `scratch` there
+
+// The current device's address of `scratch`, found on the first call per device.
+hipError_t staticScratchOf(int dev, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (dev < 0) return hipErrorInvalidDevice;
+    if (size_t(dev) >= g_scratchByDevice.size()) g_scratchByDevice.resize(size_t(dev) + 1, nullptr);
+    if (!g_scratchByDevice[size_t(dev)]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratchByDevice[size_t(dev)] = p;
+    }
+    *out = g_scratchByDevice[size_t(dev)];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct remo_ctx {
@@ -131,6 +163,7 @@ struct remo_ctx {
     uint32_t* deepenCount = nullptr;  // device counter; deepenCount[1] stays 0: an empty batch's size
     std::unique_ptr<DeepenHostRec[]> deepenHost;  // [maxNodes] host copy of the records (sorted there)
     std::unique_ptr<uint32_t[]> deepenIds;        // [maxNodes] the chosen leaves
+    uint8_t* staticScratch = nullptr;             // this device's address of `scratch`
     AccumWork accWork;
     Profiler prof;
 };
@@ -327,6 +360,14 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
         delete c;
         return REMO_ERR_DEVICE;
+    }
+    {
+        void* p = nullptr;
+        if (staticScratchOf(dev, &p) != hipSuccess) {
+            delete c;
+            return REMO_ERR_DEVICE;
+        }
+        c->staticScratch = static_cast<uint8_t*>(p);
     }
     const uint32_t cus = std::max(1, prop.multiProcessorCount);
     c->shape.pointBlocks = std::max(8u, (cus * 8u) & ~7u);
@@ -748,6 +789,27 @@ int remo_deepen(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, remo_deepen
         if (rc != REMO_OK) return rc;
     }
     return deepenWith(c, maxSplits, minPoints, c->deepenRecords, c->deepenCount, c->t.maxNodes, out);
+}
+
+int remo_deepen_noalloc(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    std::memset(out, 0, sizeof(*out));
+    if (!c->staticScratch) {
+        c->err = "deepening without allocation: no scratch address for this device";
+        return REMO_ERR_STATE;
+    }
+    const int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    // every candidate is a leaf, so a tree within the records' capacity always fits
+    if (c->hostG->numNodes > kStaticDeepenRecords) {
+        c->err = "deepening without allocation: the tree has " + std::to_string(c->hostG->numNodes) +
+                 " nodes; its fixed candidate buffer holds " + std::to_string(kStaticDeepenRecords);
+        return REMO_ERR_CAPACITY;
+    }
+    uint32_t* count = reinterpret_cast<uint32_t*>(c->staticScratch);
+    void* records = c->staticScratch + kStaticCounterBytes;
+    return deepenWith(c, maxSplits, minPoints, records, count, kStaticDeepenRecords, out);
 }
 
 int remo_render_flat(remo_ctx* c, const remo_point* points, uint64_t n, const remo_shared_uniforms* u, uint64_t* fb) {

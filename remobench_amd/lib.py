@@ -252,6 +252,7 @@ def load() -> C.CDLL:
         "remo_refine": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, P(RefineStats)]),
         "remo_render_flat": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
         "remo_deepen": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(DeepenStats)]),
+        "remo_deepen_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(DeepenStats)]),
         "remo_apply_edl": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_void_p]),
         "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),

@@ -231,11 +231,13 @@ class Tree:
         self._ck(self.lib.remo_refine(self.h, float(budget_ms), int(max_levels), C.byref(st)), "refine")
         return st.as_dict()
 
-    def deepen(self, max_splits: int, min_points: int = 25_000) -> dict:
+    def deepen(self, max_splits: int, min_points: int = 25_000, noalloc: bool = False) -> dict:
         """Refinement's local deepening: split up to max_splits of the densest
-        leaves holding >= min_points points."""
+        leaves holding >= min_points points (noalloc: the variant that allocates
+        no device memory)."""
         st = L.DeepenStats()
-        self._ck(self.lib.remo_deepen(self.h, int(max_splits), int(min_points), C.byref(st)), "deepen")
+        fn = self.lib.remo_deepen_noalloc if noalloc else self.lib.remo_deepen
+        self._ck(fn(self.h, int(max_splits), int(min_points), C.byref(st)), "deepen")
         return st.as_dict()
 
     def render_flat(self, points_ptr: int, n: int, uniforms: L.SharedUniforms, fb_ptr: int):
