@@ -84,9 +84,9 @@ def cpu_baseline(points, meta, batch: int, sample_batches: int) -> dict:
     dt = time.perf_counter() - t0
     o.close()
     return {"value": round(n / dt / 1e6, 3), "unit": "M pts/s", "cores": 1, "kind": "port",
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"first {nb} of the workload's batches ({n:,} points), one step per batch "
-                      f"(Update+Accumulate+Analysis), serial C++ oracle, {dt:.1f} s on "
-                      f"{platform.processor() or platform.machine() or 'host CPU'}"}
+                      f"(Update+Accumulate+Analysis), serial C++ oracle, {dt:.1f} s"}
 
 
 # Kernels behind each timing scope (a scope launch may run several kernels).
@@ -122,6 +122,60 @@ def measured_traffic(scope: str, workload: str):
     return rows[scope][0] + rows[scope][1]
 
 
+ATOMICS_FILE = ROOT / "profiles" / "atomics_lidar100M.txt"
+
+
+def measured_atomics(kernels, workload: str):
+    """Memory-side atomics per dispatch (rocprofv3 TCC_ATOMIC_sum) of the first of
+    `kernels` in the committed summary, when measured on this same workload."""
+    if not ATOMICS_FILE.exists():
+        return None
+    wl, rows = None, {}
+    for line in ATOMICS_FILE.read_text().splitlines():
+        f = line.split()
+        if len(f) >= 2 and f[0] == "workload":
+            wl = line.split(None, 1)[1].strip()
+        elif len(f) >= 3 and f[0] == "kernel":
+            rows[f[1]] = float(f[2])
+    if wl != workload:
+        return None
+    for k in kernels:
+        if k in rows:
+            return {"kernel": k, "tcc_atomic": rows[k]}
+    return None
+
+
+def atomic_rates(L, mib: int = 256, per_lane: int = 64):
+    """Device-scope atomicOr rates of the microbenchmark kernel (remo_atomic_rate)
+    on a `mib` MiB buffer: scattered returning (the peak the voxel climb's
+    atomics are held against), scattered non-returning, and one word per wave."""
+    import ctypes as C
+    lib = L.load()
+    buf = C.c_void_p()
+    if lib.remo_dev_alloc(mib << 20, C.byref(buf)) != 0:
+        return None
+    out = {"buffer_mib": mib, "atomics_per_launch": 4096 * 256 * per_lane}
+    try:
+        for mode, name in ((0, "scattered_returning"), (1, "scattered_nonreturning"), (2, "one_word_per_wave")):
+            r = C.c_double(0.0)
+            L.check(lib.remo_atomic_rate(buf, mib << 20, mode, per_lane if mode < 2 else 4, C.byref(r)),
+                    "atomic rate")
+            out[f"{name}_g_per_s"] = round(r.value / 1e9, 3)
+    finally:
+        lib.remo_dev_free(buf)
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine() or "host CPU"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +200,14 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run remo_copy_kernel over this many MiB 5 times (the known-size "
                          "reference for HBM byte counters) and exit")
+    ap.add_argument("--atomic-microbench", action="store_true",
+                    help="only run the atomicOr microbenchmark (remo_atomic_kernel) and exit")
     args = ap.parse_args()
+
+    if args.atomic_microbench:
+        from remobench_amd import lib as L
+        print(json.dumps({"atomic_microbench": atomic_rates(L)}), flush=True)
+        return 0
 
     if args.calibrate_copy_mib:
         import ctypes as C
@@ -302,8 +363,44 @@ def main() -> int:
                     "kernels": SCOPE_KERNELS.get(dominant, [])}
         if traffic:
             roofline["traffic_source"] = f"{TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc)"
-    path_bytes = sum(s["algorithmic_bytes"] for s in scopes.values())
-    path_gbs = path_bytes / dt / 1e9
+    # The path's compulsory HBM bytes per point: read the input, store it in its
+    # leaf, store the new voxels, Accumulate's re-read, zero the new grids
+    # (16 + 16 + 16 v + 16 + 262,144 s, v new voxels and s splits per point over
+    # the timed steps). The per-scope figure above also counts re-reads (count
+    # rounds, spill re-descents, the backlog) and is reported beside it.
+    v_pp = totals["voxels"] / max(1, ingested)
+    s_pp = totals["splits"] / max(1, ingested)
+    compulsory_pp = 48.0 + 16.0 * v_pp + 262144.0 * s_pp
+    path_gbs = compulsory_pp * ingested / dt / 1e9
+    scopes_bytes = sum(s["algorithmic_bytes"] for s in scopes.values())
+    scopes_gbs = scopes_bytes / dt / 1e9
+
+    # Atomic rate of the dominant kernel (memory-side atomics per dispatch from the
+    # committed rocprofv3 TCC_ATOMIC summary over its live average launch time)
+    # against the microbenchmark kernel's scattered atomicOr rate, measured here.
+    atomics = atomic_rates(L)
+    if roofline and atomics:
+        per = measured_atomics(roofline["kernels"], workload)
+        if per:
+            atomics["kernel"] = per["kernel"]
+            atomics["tcc_atomic_per_launch"] = per["tcc_atomic"]
+            atomics["achieved_g_per_s"] = round(per["tcc_atomic"] / (roofline["avg_launch_us"] * 1e-6) / 1e9, 3)
+            atomics["frac_of_scattered_returning"] = round(
+                atomics["achieved_g_per_s"] / atomics["scattered_returning_g_per_s"], 4)
+            atomics["source"] = f"{ATOMICS_FILE.relative_to(ROOT)} (rocprofv3 --pmc TCC_ATOMIC_sum)"
+    log(f"atomics: {atomics}")
+
+    # the rest of the scene, untimed: the Rasterize, image-quality, Refinement and
+    # streaming legs run on (or rebuild) the whole cloud, whatever --steps is
+    consumed = st["batchlet_index"]
+    t0 = time.perf_counter()
+    while consumed < nb:
+        _, consumed = tree.build_step(view)
+        tree.accumulate(consumed)
+    tree.sync()
+    st_full = tree.stats()
+    log(f"whole scene ingested ({nb} batches, {time.perf_counter() - t0:.2f} s): nodes {st_full['num_nodes']}, "
+        f"voxels {st_full['num_voxels']:,}")
 
     # Rasterize, and Analysis + Rasterize as one captured graph (remo_frame), three
     # poses, on the full tree
@@ -402,7 +499,7 @@ def main() -> int:
         tree.sync()
         calls = []
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(nb):
             step()
             c0 = time.perf_counter()
             rs = tree.refine(budget)
@@ -444,7 +541,7 @@ def main() -> int:
             tree.sync()
             frames_ms = []
             t0 = time.perf_counter()
-            for k in range(steps):
+            for k in range(nb):
                 f0 = time.perf_counter()
                 _, consumed = tree.build_step(view)
                 tree.accumulate(consumed)
@@ -452,7 +549,7 @@ def main() -> int:
                 tree.refine(args.refine_budget_ms)
                 tree.deepen(2, 25_000)
                 frames_ms.append((time.perf_counter() - f0) * 1e3)
-                if name == "learned" and k == steps // 2:
+                if name == "learned" and k == nb // 2:
                     tree.analysis(u_loop, L.REMO_ANALYSIS_ANALYTIC)
                     sa = tree.analysis_results()["score"].astype(np.float64)
                     tree.analysis(u_loop, L.REMO_ANALYSIS_LEARNED)
@@ -483,7 +580,7 @@ def main() -> int:
         stree.sync()
         t0 = time.perf_counter()
         consumed = 0
-        while consumed < steps:
+        while consumed < nb:
             sring.push_async(stree)
             sring.order_before(stree, consumed, 1)
             _, consumed = stree.build_step(sring.view())
@@ -495,8 +592,10 @@ def main() -> int:
         dts = time.perf_counter() - t0
         sst = stree.stats()
         streaming = {"value": round(sst["num_points_processed"] / dts / 1e6, 2), "unit": "M pts/s",
-                     "ring_slots": depth, "ms_per_step": round(dts * 1e3 / steps, 4),
-                     "same_tree": sst["num_nodes"] == st["num_nodes"] and sst["num_voxels"] == st["num_voxels"]}
+                     "ring_slots": depth, "batches": nb, "pushes": sring.produced,
+                     "ms_per_step": round(dts * 1e3 / nb, 4),
+                     "same_tree": sst["num_nodes"] == st_full["num_nodes"] and
+                     sst["num_voxels"] == st_full["num_voxels"]}
         log(f"streaming from pinned host memory: {streaming}")
         stree.close()
         sring.close()
@@ -533,14 +632,18 @@ def main() -> int:
         "streaming": streaming,
         "upload_gbs": round(16.0 * len(points) / upload_s / 1e9, 2),
         "pcie_inclusive_mpts": round(ingested / (dt + upload_s * ingested / len(points)) / 1e6, 2),
+        "path_compulsory_bytes_per_point": round(compulsory_pp, 2),
         "path_hbm_gbs": round(path_gbs, 2),
         "path_hbm_frac": round(path_gbs / HBM_PEAK_GBS, 5),
+        "scopes_hbm_gbs": round(scopes_gbs, 2),
+        "scopes_hbm_frac": round(scopes_gbs / HBM_PEAK_GBS, 5),
+        "atomics": atomics,
         "scopes": scopes,
         "rasterize": raster,
         "refinement": refinement,
         "image_quality": image_quality,
         "full_loop": full_loop,
-        "tree": {k: st[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "max_points_per_node",
+        "tree": {k: st_full[k] for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "max_points_per_node",
                                     "total_expand_iters", "total_spilled_points",
                                     "num_voxels", "max_depth", "allocated_bytes_persistent",
                                     "mem_capacity_reached", "node_pool_overflow", "spill_overflow",

@@ -486,6 +486,12 @@ int remo_memcpy_d2h(void* dst, const void* src, size_t bytes);
  * known-size reference that HBM byte counters are calibrated against. bytes must
  * be a multiple of 16; synchronous. */
 int remo_dev_copy(void* dst, const void* src, size_t bytes);
+/* Device-scope atomicOr throughput on `bytes` of device memory (a power of two,
+ * >= 4 KiB) by a plain microbenchmark kernel (remo_atomic_kernel): the peak the
+ * Update kernels' atomic rates are reported against. mode 0: scattered
+ * addresses, returning; 1: scattered, non-returning; 2: all 64 lanes of a wave
+ * on one word. 2^20 lanes issue per_lane atomics each; synchronous. */
+int remo_atomic_rate(void* words, size_t bytes, uint32_t mode, uint32_t per_lane, double* atomics_per_s);
 int remo_device_count(int* out);
 /* Free and total device memory: CudaContext::freeMemory / totalMemory, which
  * App::computeBudget turns into DeviceBudget (src/shell/App.cpp:97-114). */

@@ -1309,6 +1309,29 @@ __global__ void remo_copy_kernel(uint4* __restrict__ dst, const uint4* __restric
         dst[i] = src[i];
 }
 
+// Device-scope atomicOr throughput: every lane issues `perLane` atomics on
+// 32-bit words of a buffer of mask + 1 words. Mode 0: scattered addresses, the
+// result used (returning atomics, as voxel sampling's climb issues them); mode 1:
+// scattered, result unused (non-returning); mode 2: every lane of a wave on one
+// word (a hot address). Addresses come from a per-lane 64-bit LCG.
+__global__ void remo_atomic_kernel(uint32_t* __restrict__ words, uint64_t mask, uint32_t perLane, uint32_t mode,
+                                   uint32_t* __restrict__ sink) {
+    const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t x = tid * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    uint32_t acc = 0;
+    const uint64_t waveWord = ((tid >> 6) * 0x9E3779B97F4A7C15ull >> 20) & mask;
+    for (uint32_t k = 0; k < perLane; ++k) {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        const uint64_t idx = mode == 2 ? waveWord : ((x >> 24) & mask);
+        const uint32_t bit = 1u << ((x >> 59) & 31u);
+        if (mode == 1)
+            __hip_atomic_fetch_or(words + idx, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            acc |= atomicOr(words + idx, bit);
+    }
+    if (acc == 0xFFFFFFFFu && (tid & 0xFFFFF) == 0) sink[tid >> 20] = acc;  // keeps the results live
+}
+
 extern "C" {
 
 int remo_dev_copy(void* dst, const void* src, size_t bytes) {
@@ -1317,6 +1340,30 @@ int remo_dev_copy(void* dst, const void* src, size_t bytes) {
                        static_cast<const uint4*>(src), uint64_t(bytes / 16));
     if (hipGetLastError() != hipSuccess) return REMO_ERR_DEVICE;
     return hipDeviceSynchronize() == hipSuccess ? REMO_OK : REMO_ERR_DEVICE;
+}
+
+int remo_atomic_rate(void* words, size_t bytes, uint32_t mode, uint32_t per_lane, double* atomics_per_s) {
+    if (!words || !atomics_per_s || mode > 2 || per_lane == 0) return REMO_ERR_INVALID;
+    const uint64_t n = bytes / 4;
+    if (n < 1024 || (n & (n - 1)) != 0) return REMO_ERR_INVALID;  // a power of two of words
+    constexpr uint32_t kBlocks = 4096, kThreads = 256;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return REMO_ERR_DEVICE;
+    uint32_t* w = static_cast<uint32_t*>(words);
+    // the first launch warms up; the sink is the buffer's last word
+    hipLaunchKernelGGL(remo_atomic_kernel, dim3(kBlocks), dim3(kThreads), 0, 0, w, n - 1, 4u, mode, w + n - 1);
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(remo_atomic_kernel, dim3(kBlocks), dim3(kThreads), 0, 0, w, n - 1, per_lane, mode, w + n - 1);
+    (void)hipEventRecord(e1, 0);
+    int rc = REMO_OK;
+    float ms = 0.0f;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        rc = REMO_ERR_DEVICE;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == REMO_OK) *atomics_per_s = double(kBlocks) * kThreads * per_lane / (double(ms) * 1e-3);
+    return rc;
 }
 
 int remo_device_count(int* out) {

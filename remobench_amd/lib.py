@@ -229,6 +229,7 @@ def load() -> C.CDLL:
         "remo_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "remo_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "remo_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+        "remo_atomic_rate": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint32, C.c_uint32, P(C.c_double)]),
         "remo_device_count": (C.c_int, [P(C.c_int)]),
         "remo_device_mem_info": (C.c_int, [P(C.c_size_t), P(C.c_size_t)]),
         "remo_profile_enable": (C.c_int, [C.c_void_p, C.c_uint32]),
