@@ -636,67 +636,100 @@ __global__ __launch_bounds__(kBlock) void kFixedSeed(DevTree t) {
 // gridded node on the root->leaf path, set the point's 128^3 cell; whoever sets
 // a bit appends the cell-centre voxel, coloured by its point.
 //
-// The walk goes bottom-up. A cell set at a node implies that the point's cells
-// at every gridded ancestor are set too: every point of a region is sampled into
-// every gridded node above it, and a node only gets (or re-zeroes) a grid in the
-// batch it splits, when all its region's points pass through the spill list. So
-// a point peeks its deepest cell first and climbs only while it keeps finding
-// clear bits; the lane whose atomicOr sets a bit owns that voxel and climbs on.
-// At steady state most points stop after one peek.
+// A point's path is a chain of dependent loads (the descent) and, per gridded
+// level, an atomicOr whose return says whether this lane set the bit (and so
+// owns that voxel). The chain is kept short:
+//  * during the descent each level's cell word is peeked (a plain load, in
+//    flight beside the next level's walk load), and the path's (node, grid,
+//    peek) stay in registers at fixed level indices;
+//  * after it, the atomicOr of every level whose peek was clear is issued at
+//    once: one round trip for the whole path instead of one per level. A stale
+//    clear peek only costs an atomic; a set bit is never stale (bits are only
+//    set while this kernel runs). Every bit still gets exactly one owner, the
+//    lane whose atomic set it, so each new cell becomes exactly one voxel;
+//  * a node's coordinates at level L are the point's own 20-bit coordinate bits
+//    19..20-L (the descent's child bits), so a voxel centre needs no node load.
+// A leaf deeper than the register window climbs its levels below the window
+// one at a time (stopping at the first bit already set: its setter covers the
+// levels above), then takes the window like every other point.
 //
 // New voxels are staged per wave in LDS and reach the global backlog with one
 // atomicAdd per flush, on the counter of the block's backlog segment, and are
-// ranked with one more per (flush, grid) on that grid's counter for the segment;
-// kAllocChunks sums those counters into numVoxels. Bumping per-node counters in
-// the climb sent every wave's atomics to the few lines that hold the counters of
-// the nodes a batch touches (~33 us per 1M-point batch).
+// ranked with one more per (flush, grid) on that grid's counter for the segment
+// (those issued together); kAllocChunks sums the counters into numVoxels.
+// Bumping per-node counters in the climb sent every wave's atomics to the few
+// lines that hold the counters of the nodes a batch touches (~33 us per
+// 1M-point batch).
 constexpr uint32_t kStage = 128;  // staged voxels per wave
 
 struct VoxelStage {
     remo_point pts[kStage];
     uint32_t node[kStage];
-    uint32_t grid[kStage];  // the node's grid number
 };
 
-// Moves a wave's staged voxels to the global backlog (one atomic).
+// Lanes holding the same key form a group: its lowest lane leads, every lane
+// gets its rank within the group and the leader the group's size (ballots and
+// shuffles only).
+__device__ __forceinline__ void groupByKey(uint32_t key, bool held, uint32_t& leader, uint32_t& rank,
+                                           uint32_t& size) {
+    const uint32_t lane = laneId();
+    leader = lane;
+    rank = 0;
+    size = 0;
+    uint64_t pend = __ballot(held);
+    while (pend) {
+        const int l = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t lk = __shfl(key, l);
+        const uint64_t grp = __ballot(held && key == lk) & pend;
+        if ((grp >> lane) & 1ull) {
+            leader = uint32_t(l);
+            rank = uint32_t(__popcll(grp & lanesBelow()));
+        }
+        if (lane == uint32_t(l)) size = uint32_t(__popcll(grp));
+        pend &= ~grp;
+    }
+}
+
+// Moves a wave's staged voxels to the global backlog: the grids of the staged
+// voxels (loads in flight together), the segment reservation (one atomic), then
+// the per-(grid, segment) ranks of the held entries, every grid's atomic at once.
 __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uint32_t& count) {
     if (count == 0) return;
     // the wave's own LDS writes must land before other lanes read them
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    static_assert(kStage == 128, "two 64-entry halves");
     const uint32_t shard = blockIdx.x & (kBacklogShards - 1u);
     const uint32_t lane = laneId();
+    const bool in0 = lane < count, in1 = lane + 64 < count;
+    const uint32_t node0 = in0 ? st.node[lane] : 0u, node1 = in1 ? st.node[lane + 64] : 0u;
+    const uint32_t grid0 = in0 ? t.gridNo[node0] : kNone, grid1 = in1 ? t.gridNo[node1] : kNone;
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&t.backlogFill[shard * kShardStride], count);
     base = __shfl(base, 0);
-    for (uint32_t k0 = 0; k0 < count; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        const uint32_t at = base + k;
-        const bool held = k < count && at < t.backlogSegCap;
-        const uint64_t slot = uint64_t(shard) * t.backlogSegCap + at;
-        // rank among this batch's entries of the grid in this segment: one
-        // returning atomic per (flush, grid) on the (grid, segment) counter
-        const uint32_t key = held ? st.grid[k] : kNone;
-        uint32_t rank = 0;
-        uint64_t pend = __ballot(held);
-        while (pend) {
-            const int leader = __ffsll((unsigned long long)pend) - 1;
-            const uint32_t lk = __shfl(key, leader);
-            const uint64_t grp = __ballot(held && key == lk) & pend;
-            uint32_t gbase = 0;
-            if (lane == uint32_t(leader))
-                gbase = atomicAdd(&t.voxelShardCount[uint64_t(lk) * kBacklogShards + shard], uint32_t(__popcll(grp)));
-            gbase = __shfl(gbase, leader);
-            if ((grp >> lane) & 1ull) rank = gbase + uint32_t(__popcll(grp & lanesBelow()));
-            pend &= ~grp;
-        }
+    const bool held0 = in0 && base + lane < t.backlogSegCap;
+    const bool held1 = in1 && base + lane + 64 < t.backlogSegCap;
+    uint32_t lead0, rank0, size0, lead1, rank1, size1;
+    groupByKey(grid0, held0, lead0, rank0, size0);
+    groupByKey(grid1, held1, lead1, rank1, size1);
+    uint32_t gb0 = 0, gb1 = 0;
+    if (held0 && lead0 == lane)
+        gb0 = atomicAdd(&t.voxelShardCount[uint64_t(grid0) * kBacklogShards + shard], size0);
+    if (held1 && lead1 == lane)
+        gb1 = atomicAdd(&t.voxelShardCount[uint64_t(grid1) * kBacklogShards + shard], size1);
+    gb0 = __shfl(gb0, lead0);
+    gb1 = __shfl(gb1, lead1);
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint32_t k = lane + 64 * h;
+        const bool in = h ? in1 : in0, held = h ? held1 : held0;
+        const uint64_t slot = uint64_t(shard) * t.backlogSegCap + base + k;
         if (held) {
             t.backlogPts[slot] = st.pts[k];
-            t.backlogNode[slot] = st.node[k];
-            t.backlogRank[slot] = rank;
-        } else if (k < count) {
+            t.backlogNode[slot] = h ? node1 : node0;
+            t.backlogRank[slot] = h ? gb1 + rank1 : gb0 + rank0;
+        } else if (in) {
             // a dropped voxel still counts (no backlog entry will)
             t.g->backlogOverflow = 1;
-            atomicAdd(&t.numVoxels[st.node[k]], 1u);
+            atomicAdd(&t.numVoxels[h ? node1 : node0], 1u);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -714,6 +747,47 @@ constexpr uint32_t kVoxelTile = kBlock;
 // there, the climb lost the ancestors above it and cost 8 us more per batch.)
 // Three or six ancestors time the same as four (172.9 and 171.1 against 172.7 us).
 constexpr uint32_t kPathRegs = 4;
+
+// Cell of a 28-bit point coordinate in the 128^3 grid of a node at `level`.
+__device__ __forceinline__ uint32_t cellIndex(uint32_t pX, uint32_t pY, uint32_t pZ, uint32_t level) {
+    const uint32_t sh = (kMaxDepth + 1) - level;
+    return ((pX >> sh) & (kGridSize - 1)) + ((pY >> sh) & (kGridSize - 1)) * kGridSize +
+           ((pZ >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
+}
+
+__device__ __forceinline__ uint32_t* cellWord(const DevTree& t, uint32_t gref, uint32_t vi) {
+    return reinterpret_cast<uint32_t*>(t.arena + (uint64_t(gref) << 4)) + (vi >> 5);
+}
+
+// Stages the wave's new voxels of one step (isNew per lane) into LDS. Every lane
+// of the wave calls it together, so the staged count stays wave-uniform.
+__device__ __forceinline__ void stageLevel(const DevTree& t, VoxelStage& st, uint32_t& staged, bool isNew,
+                                           uint32_t level, uint32_t nd, uint32_t vi, uint32_t X, uint32_t Y,
+                                           uint32_t Z, uint32_t col, float size) {
+    const uint64_t nm = __ballot(isNew);
+    const uint32_t cnt = uint32_t(__popcll(nm));
+    if (cnt == 0) return;
+    if (staged + cnt > kStage) flushStage(t, st, staged);
+    if (isNew) {
+        // the node's coordinates at `level`: bits 19..20-level of the point's
+        // 20-bit coordinate (a point on boxMax wraps to 0, as the descent does)
+        const uint32_t sh = kMaxDepth - level, m = (1u << level) - 1u;
+        const float nodeSize = size / float(1u << level);
+        const float minx = float((X >> sh) & m) * nodeSize;
+        const float miny = float((Y >> sh) & m) * nodeSize;
+        const float minz = float((Z >> sh) & m) * nodeSize;
+        const uint32_t cx = vi & (kGridSize - 1), cy = (vi >> 7) & (kGridSize - 1), cz = vi >> 14;
+        remo_point v;
+        v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
+        v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
+        v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
+        v.color = col;
+        const uint32_t slot = staged + uint32_t(__popcll(nm & lanesBelow()));
+        st.pts[slot] = v;
+        st.node[slot] = nd;
+    }
+    staged += cnt;
+}
 
 __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
     __shared__ VoxelStage stage[kBlock / 64];
@@ -736,6 +810,7 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
             pZ = quant28(p.z, size);
             col = p.color;
         }
+        const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
         // descend to the leaf; pathNode[0] / pathGrid[0] is the leaf's parent
         uint32_t pathNode[kPathRegs], pathGrid[kPathRegs];
         for (uint32_t k = 0; k < kPathRegs; ++k) {
@@ -744,7 +819,6 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
         }
         uint32_t node = 0, lvl = 0, gref = 0;
         if (active) {
-            const uint32_t X = pX >> 8, Y = pY >> 8, Z = pZ >> 8;
             for (;;) {
                 const uint2 rec = t.walk[node];
                 if (rec.x == kNone || lvl == uint32_t(kMaxDepth)) {
@@ -762,71 +836,40 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                 lvl += 1;
             }
         }
-        // the climb starts at the deepest node with a grid: the root while it is
-        // still a leaf, else the leaf's parent
-        auto up = [&]() {
-            lvl -= 1;
-            if (pathNode[0] != kNone) {
-                node = pathNode[0];
-                gref = pathGrid[0];
-                for (uint32_t k = 0; k + 1 < kPathRegs; ++k) {
-                    pathNode[k] = pathNode[k + 1];
-                    pathGrid[k] = pathGrid[k + 1];
-                }
-                pathNode[kPathRegs - 1] = kNone;
-            } else {
-                node = t.parent[node];
-                gref = t.walk[node].y;
-            }
-        };
-        if (active && gref == 0) {
-            if (lvl == 0)
-                active = false;  // a root without a grid: nothing to sample
-            else
-                up();
-        }
-        // climb while new cells are set
+        // The climb, from the leaf up: one loop with a single entry and a
+        // uniform trip count, so every lane runs each step's staging together. A
+        // node without a grid (the leaf, unless it is the root) is passed over; at
+        // one with a grid the lane peeks its cell (a stale peek only costs an
+        // atomic), sets it, and climbs on while the bit was new: a set bit at
+        // level L implies the point's cells at every gridded ancestor are set (or
+        // are being set by the lane that set it).
         for (uint32_t step = 0; step <= uint32_t(kMaxDepth); ++step) {
             if (__ballot(active) == 0) break;
-            if (gref == 0) active = false;  // no grid here (allocation overflow)
-            const uint32_t sh = (kMaxDepth + 1) - lvl;
-            const uint32_t vi = ((pX >> sh) & (kGridSize - 1)) + ((pY >> sh) & (kGridSize - 1)) * kGridSize +
-                                ((pZ >> sh) & (kGridSize - 1)) * kGridSize * kGridSize;
+            const uint32_t vi = cellIndex(pX, pY, pZ, lvl);
             const uint32_t mask = 1u << (vi & 31u);
-            uint32_t* word = reinterpret_cast<uint32_t*>(t.arena + (uint64_t(gref) << 4)) + (vi >> 5);
-            // a stale peek only costs an atomicOr
-            const uint32_t peek = active ? *word : 0xFFFFFFFFu;
-            const bool isNew = (peek & mask) == 0 && (atomicOr(word, mask) & mask) == 0;
-            const uint32_t nd = node;
-            const uint64_t nm = __ballot(isNew);
-            const uint32_t cnt = uint32_t(__popcll(nm));
-            if (cnt) {
-                if (staged + cnt > kStage) flushStage(t, st, staged);
-                if (isNew) {
-                    const uint32_t cx = vi & (kGridSize - 1);
-                    const uint32_t cy = (vi >> 7) & (kGridSize - 1);
-                    const uint32_t cz = vi >> 14;
-                    const float nodeSize = size / float(1u << lvl);
-                    const float minx = float(t.nx[nd]) * nodeSize;
-                    const float miny = float(t.ny[nd]) * nodeSize;
-                    const float minz = float(t.nz[nd]) * nodeSize;
-                    remo_point v;
-                    v.x = minx + (nodeSize * (float(cx) + 0.5f)) / 128.0f;
-                    v.y = miny + (nodeSize * (float(cy) + 0.5f)) / 128.0f;
-                    v.z = minz + (nodeSize * (float(cz) + 0.5f)) / 128.0f;
-                    v.color = col;
-                    const uint32_t slot = staged + uint32_t(__popcll(nm & lanesBelow()));
-                    st.pts[slot] = v;
-                    st.node[slot] = nd;
-                    st.grid[slot] = t.gridNo[nd];
-                }
-                staged += cnt;
+            bool isNew = false, set = false;
+            if (active && gref != 0) {
+                uint32_t* word = cellWord(t, gref, vi);
+                isNew = (*word & mask) == 0 && (atomicOr(word, mask) & mask) == 0;
+                set = !isNew;
             }
-            // a set bit means every ancestor cell is set: stop; a new one climbs
-            if (!isNew || lvl == 0)
-                active = false;
-            else
-                up();
+            stageLevel(t, st, staged, isNew, lvl, node, vi, X, Y, Z, col, size);
+            if (set || lvl == 0) active = false;
+            if (active) {
+                lvl -= 1;
+                if (pathNode[0] != kNone) {
+                    node = pathNode[0];
+                    gref = pathGrid[0];
+                    for (uint32_t k = 0; k + 1 < kPathRegs; ++k) {
+                        pathNode[k] = pathNode[k + 1];
+                        pathGrid[k] = pathGrid[k + 1];
+                    }
+                    pathNode[kPathRegs - 1] = kNone;
+                } else {
+                    node = t.parent[node];
+                    gref = t.walk[node].y;
+                }
+            }
         }
     }
     flushStage(t, st, staged);
