@@ -192,7 +192,7 @@ typedef struct remo_node_analysis {
     float score;
     uint64_t n;                     /* points under the node (rolled up) */
     uint32_t state;                 /* 0 open, 1 closed, 2 finalized     */
-    uint32_t flags;                 /* bit0: FP64 fallback used; bit1: learned score used */
+    uint32_t flags;                 /* bit1: learned score used (bit0 unused, 0)           */
 } remo_node_analysis;
 
 #define REMO_ANALYSIS_ANALYTIC 0
@@ -247,6 +247,13 @@ int remo_accum_fp32_atomics(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* n
  * roll-up of the exact sums, 3x3 eigen-solve, score = geometric x coverage,
  * work queue. mode: REMO_ANALYSIS_ANALYTIC or REMO_ANALYSIS_LEARNED. */
 int remo_analysis(remo_ctx* ctx, const remo_shared_uniforms* u, int mode);
+
+/* Limits of the incremental roll-up (no reference counterpart; a tuning and test
+ * knob): at most lds_slots marked nodes keep their sums in LDS (more: device
+ * memory), and levels of at most wave_slots marked nodes run in one wave without
+ * workgroup barriers. 0 restores a default (384, 64); larger values are clamped
+ * to those. The results do not depend on the limits. */
+int remo_set_rollup_limits(remo_ctx* ctx, uint32_t lds_slots, uint32_t wave_slots);
 
 /* ILodPipeline::render -> kernel_render selection + shared rasteriser
  * (kernels/remolod/remolod_render.cu:72-181, kernels/shared/remo_draw.cuh:118-167,

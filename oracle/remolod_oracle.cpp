@@ -824,8 +824,10 @@ struct oracle_tree {
 //    +-2^(18+D-L) per axis; everything in __int128;
 //  * covariance in node-extent units from num = n*S2 - S1*S1^T (exact int128 when
 //    it fits, FP64 otherwise), cov = num / n^2 / 4^(20+D-L);
-//  * eigenvalues: cyclic Jacobi on cov / trace, in FP32, redone in FP64 when the
-//    smallest gap is below 1e-4 of the trace; FP64 Jacobi is the reference;
+//  * eigenvalues: cyclic Jacobi on cov / trace in FP32 (the eigenvalues of a
+//    symmetric matrix are perturbed by at most the norm of the perturbation, so
+//    FP32 holds them to ~1e-7 of the trace at any eigen-gap); FP64 Jacobi is
+//    computed beside it as the accuracy reference;
 //  * surface variation l0/(l0+l1+l2), planarity (l1-l0)/l2, linearity (l2-l1)/l2;
 //  * coverage: projected 8-corner screen extent dx*dy, clamped to the screen;
 //    score = 3 * surface variation * coverage;
@@ -1051,14 +1053,8 @@ void oracle_tree::analyse(const remo_shared_uniforms& u, uint32_t closureMargin,
         jacobi3<double>(a64, ev64);
         float ev32[3];
         jacobi3<float>(a32, ev32);
-        const float gap = std::fmin(ev32[1] - ev32[0], ev32[2] - ev32[1]);
         double ev[3];
-        if (gap < 1e-4f) {
-            r.flags |= 1u;
-            for (int i = 0; i < 3; ++i) ev[i] = ev64[i];
-        } else {
-            for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
-        }
+        for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
         for (int i = 0; i < 3; ++i) {
             o.lambda64[i] = ev64[i] * trace;
             r.lambda[i] = float(ev[i] * trace);

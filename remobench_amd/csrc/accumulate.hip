@@ -244,14 +244,16 @@ __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, u
     if (blockIdx.x == 0 && threadIdx.x == 0) *w.numTiles = 0;
     unsigned long long sumCounts = 0, folded = 0, wm = 0;
     uint32_t inner = 0, leaves = 0;
-    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
-        remo_node_accum* a = t.accums + node;
-        if (t.firstChild[node] >= 0) {
-            if (a->count != 0) inner++;
-            continue;
-        }
-        const uint32_t nt = w.nodeTiles[node];
+    // wave-uniform steps (the dirty-list append below is one atomic per wave)
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t node = base + threadIdx.x;
+        bool changed = false;
+        remo_node_accum* a = t.accums + (node < n ? node : 0u);
+        const bool isInner = node < n && t.firstChild[node] >= 0;
+        if (isInner && a->count != 0) inner++;
+        const uint32_t nt = (node < n && !isInner) ? w.nodeTiles[node] : 0u;
         if (nt) {
+            changed = true;
             const uint32_t base = w.nodeBase[node];
             long long q[9];
             double gs[9];
@@ -284,7 +286,18 @@ __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, u
             folded += to - from;
             leaves++;
         }
-        sumCounts += a->count;
+        if (node < n && !isInner) sumCounts += a->count;
+        // the folded leaves go on Analysis' dirty list
+        const bool add = changed && t.dirtyListed[node] == 0;
+        if (add) t.dirtyListed[node] = 1;
+        const uint64_t m = __ballot(add);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t at = 0;
+            if (laneId() == uint32_t(leader)) at = atomicAdd(t.dirtyCount, uint32_t(__popcll(m)));
+            at = __shfl(at, leader);
+            if (add) t.dirtyList[at + uint32_t(__popcll(m & lanesBelow()))] = node;
+        }
     }
     sumCounts = waveSum(sumCounts);
     folded = waveSum(folded);

@@ -5,16 +5,19 @@
 // test closure, roll leaf accumulators up the tree and finalise, score =
 // geometric score x screen coverage, rebuild the work queue each frame.
 //
-//  * Roll-up is level-synchronous, bottom-up: one launch writes every leaf's
-//    sums, then one launch per level adds each inner node's eight children in
-//    child order (a launch per level beats a one-workgroup walk with barriers
-//    and an arrival-counter chain on these tree sizes). The sums are __int128 in
-//    units U = 2^-(20+D) of the root extent (D the deepest level), centred on
-//    each node: a child's sums move to its parent's centre with the exact offset
-//    +-2^(18+D-L) per axis, so the roll-up is exact and order-independent.
-//  * Finalise: covariance in node-extent units from num = n*S2 - S1*S1^T (exact
-//    int128 while it fits), cyclic Jacobi on cov/trace in FP32, redone in FP64
-//    when the smallest eigen-gap is below 1e-4 of the trace.
+//  * Roll-up: the sums are __int128 integers in units U = 2^-(20+D) of the root
+//    extent (D the deepest level) in one frame for the whole tree, so a parent's
+//    sums are the plain sum of its children's; a node's own (centred) frame is
+//    taken, exactly, only where its shape or features are read. The first
+//    Analysis after a reset or a deepening of the tree rolls up every node (one
+//    launch for the leaves, one per two levels); later ones roll up only what
+//    changed since (the leaves Accumulate folded, the nodes that split, their
+//    ancestors) in one workgroup.
+//  * Shape: covariance in node-extent units from num = n*S2 - S1*S1^T (exact
+//    int128 while it fits), cyclic Jacobi on cov/trace in FP32 (eigenvalues of a
+//    symmetric matrix move by at most the norm of a perturbation, so FP32 holds
+//    them to ~1e-7 of the trace whatever the eigen-gap), solved once per change
+//    of a node's sums and cached; the finalize reads the cache.
 //
 // Rasterize follows kernel_render's selection and the shared rasteriser
 // (kernels/remolod/remolod_render.cu:30-181, kernels/shared/remo_draw.cuh:118-167,
@@ -46,87 +49,129 @@ __device__ __forceinline__ int bitsI128(i128 x) {
     return bitsU128(x < 0 ? (unsigned __int128)(-x) : (unsigned __int128)x);
 }
 
-// Cyclic Jacobi on a symmetric 3x3 (a00 a01 a02 a11 a12 a22); eigenvalues ascending.
-template <typename T>
-__device__ __forceinline__ void jacobi3(const T a[6], T ev[3]) {
-    T m[3][3] = {{a[0], a[1], a[2]}, {a[1], a[3], a[4]}, {a[2], a[4], a[5]}};
-    for (int sweep = 0; sweep < 10; ++sweep) {
-        const T off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
-        if (!(off > T(0))) break;
-        for (int pq = 0; pq < 3; ++pq) {
-            const int p = pq == 2 ? 1 : 0;
-            const int q = pq == 0 ? 1 : 2;
-            const T apq = m[p][q];
-            if (apq == T(0)) continue;
-            const T theta = (m[q][q] - m[p][p]) / (T(2) * apq);
-            const T at = theta < T(0) ? -theta : theta;
-            T t;
-            if (at > T(1e15))
-                t = T(0.5) / theta;
-            else
-                t = (theta < T(0) ? T(-1) : T(1)) / (at + sqrt(theta * theta + T(1)));
-            const T c = T(1) / sqrt(t * t + T(1));
-            const T sn = t * c;
-            const int r = 3 - p - q;
-            const T arp = m[r][p], arq = m[r][q];
-            m[p][p] = m[p][p] - t * apq;
-            m[q][q] = m[q][q] + t * apq;
-            m[p][q] = m[q][p] = T(0);
-            const T nrp = c * arp - sn * arq;
-            const T nrq = sn * arp + c * arq;
-            m[r][p] = m[p][r] = nrp;
-            m[r][q] = m[q][r] = nrq;
-        }
-    }
-    ev[0] = m[0][0];
-    ev[1] = m[1][1];
-    ev[2] = m[2][2];
-    T tmp;
-    if (ev[0] > ev[1]) { tmp = ev[0]; ev[0] = ev[1]; ev[1] = tmp; }
-    if (ev[1] > ev[2]) { tmp = ev[1]; ev[1] = ev[2]; ev[2] = tmp; }
-    if (ev[0] > ev[1]) { tmp = ev[0]; ev[0] = ev[1]; ev[1] = tmp; }
-}
-
 // ---- roll-up -------------------------------------------------------------------
 
 // Left shift of a possibly negative __int128 (two's complement, no UB).
 __device__ __forceinline__ i128 shl(i128 x, int k) { return i128((unsigned __int128)x << k); }
 
-__device__ __forceinline__ void leafSums(const DevTree& t, uint32_t node, uint32_t D, RollupSums& S) {
-    const remo_node_accum& a = t.accums[node];
-    const int sh = int(D - t.level[node]);
-    for (int i = 0; i < 3; ++i) S.s1[i] = shl(i128(a.q1[i]), sh);
-    for (int i = 0; i < 6; ++i) S.s2[i] = shl(i128(a.q2[i]), 2 * sh);
-    S.n = a.count;
+// a * b modulo 2^128 for a small unsigned factor (a 64x32 and a 32x32 product of
+// limbs instead of the general 128x128 expansion).
+__device__ __forceinline__ i128 mulU32(i128 a, uint32_t b) {
+    const unsigned __int128 u = (unsigned __int128)a;
+    const uint64_t lo = uint64_t(u), hi = uint64_t(u >> 64);
+    const unsigned __int128 plo = (unsigned __int128)lo * b;
+    return i128(plo + ((unsigned __int128)(hi * b) << 64));
 }
 
-// The child offsets are +-h with h = 2^k, so every product is a shift: child c
-// contributes S1 += C1 + n o, S2 += C2 + o_a C1_b + C1_a o_b + n o_a o_b to its
-// parent, o the offset of c's centre from the parent's.
-__device__ __forceinline__ void childContribution(const RollupSums& C, uint32_t ci, int k, RollupSums& S) {
+// a * b modulo 2^128 for an unsigned 64-bit factor.
+__device__ __forceinline__ i128 mulU64(i128 a, uint64_t b) {
+    const unsigned __int128 u = (unsigned __int128)a;
+    const uint64_t lo = uint64_t(u), hi = uint64_t(u >> 64);
+    const unsigned __int128 plo = (unsigned __int128)lo * b;
+    return i128(plo + ((unsigned __int128)(hi * b) << 64));
+}
+
+// The sums are kept in one frame for the whole tree: integers in units
+// U = 2^-(20+D) of the root extent (D the deepest level), origin at the root's
+// minimum corner, so a parent's sums are the plain sum of its children's. Every
+// operation is on __int128 modulo 2^128: a record's node-centred form (below)
+// fits in 128 bits, so it comes out exact even where a root-frame sum wrapped.
+// A node's centre in these units is (2 i + 1) 2^(e-1) per axis, i its integer
+// coordinate at its level and e = 20 + D - level (its extent is 2^e units).
+
+// The accumulator fields and node coordinates a leaf's record is made from.
+struct LeafIn {
+    int64_t q1[3], q2[6];
+    uint32_t count, last, level, x, y, z;
+};
+
+__device__ __forceinline__ LeafIn loadLeaf(const DevTree& t, uint32_t node) {
+    const remo_node_accum& a = t.accums[node];
+    LeafIn l;
+    for (int i = 0; i < 3; ++i) l.q1[i] = a.q1[i];
+    for (int i = 0; i < 6; ++i) l.q2[i] = a.q2[i];
+    l.count = a.count;
+    l.last = a.last_touched_batch;
+    l.level = t.level[node];
+    l.x = t.nx[node];
+    l.y = t.ny[node];
+    l.z = t.nz[node];
+    return l;
+}
+
+// A leaf's sums from its accumulator (units 2^-(20+L), centred on the leaf):
+// x = c + 2^(D-L) u for each point, so S1 = 2^(D-L) Q1 + n c and
+// S2_ab = 4^(D-L) Q2_ab + c_a 2^(D-L) Q1_b + 2^(D-L) Q1_a c_b + n c_a c_b.
+__device__ __forceinline__ void leafSumsOf(const LeafIn& a, uint32_t D, RollupSums& S) {
+    const int sh = int(D - a.level), s = int(19 + D - a.level);  // c = odd * 2^s
+    const uint32_t odd[3] = {2u * a.x + 1u, 2u * a.y + 1u, 2u * a.z + 1u};
     const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-    const bool pos[3] = {(ci & 4) != 0, (ci & 2) != 0, (ci & 1) != 0};
-    const i128 n = i128(C.n);
-    const i128 nh = shl(n, k);
-    for (int i = 0; i < 3; ++i) S.s1[i] = C.s1[i] + (pos[i] ? nh : -nh);
-    const i128 nn = shl(n, 2 * k);
+    const i128 n = i128(a.count);
+    i128 q1[3];
+    for (int i = 0; i < 3; ++i) {
+        q1[i] = shl(i128(a.q1[i]), sh);
+        S.s1[i] = q1[i] + shl(mulU32(n, odd[i]), s);
+    }
     for (int q = 0; q < 6; ++q) {
         const int a_ = pi[q], b_ = pj[q];
-        const i128 t1 = shl(C.s1[b_], k), t2 = shl(C.s1[a_], k);
-        const i128 cross = (pos[a_] ? t1 : -t1) + (pos[b_] ? t2 : -t2);
-        S.s2[q] = C.s2[q] + cross + (pos[a_] == pos[b_] ? nn : -nn);
+        const i128 cross = mulU32(q1[b_], odd[a_]) + mulU32(q1[a_], odd[b_]);
+        S.s2[q] = shl(i128(a.q2[q]), 2 * sh) + shl(cross, s) + shl(mulU32(mulU32(n, odd[a_]), odd[b_]), 2 * s);
     }
-    S.n = C.n;
+    S.n = a.count;
+    S.e = 20 + D - a.level;
+    S.pad = 0;
 }
 
-// Sum over the 8 lanes of a group (xor butterfly within width 8).
-__device__ __forceinline__ i128 groupSum8(i128 v) {
-    for (int m = 1; m < 8; m <<= 1) {
-        const unsigned __int128 u = (unsigned __int128)v;
-        const unsigned long long lo = __shfl_xor((unsigned long long)uint64_t(u), m, 8);
-        const unsigned long long hi = __shfl_xor((unsigned long long)uint64_t(u >> 64), m, 8);
-        v += i128(((unsigned __int128)hi << 64) | lo);
+__device__ __forceinline__ void leafSums(const DevTree& t, uint32_t node, uint32_t D, RollupSums& S) {
+    leafSumsOf(loadLeaf(t, node), D, S);
+}
+
+// A record in the node's own frame (centred on the node, same units): what the
+// shape, the learned-score inputs and the dump read. S1 - n c and
+// S2_ab - c_a S1_b - S1_a c_b + n c_a c_b, c = odd * 2^(e-1).
+__device__ __forceinline__ RollupSums centred(const DevTree& t, uint32_t node, const RollupSums& R) {
+    const int s = int(R.e) - 1;
+    const uint32_t odd[3] = {2u * t.nx[node] + 1u, 2u * t.ny[node] + 1u, 2u * t.nz[node] + 1u};
+    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+    const unsigned __int128 n = R.n;
+    RollupSums S;
+    for (int i = 0; i < 3; ++i) S.s1[i] = R.s1[i] - shl(mulU32(i128(n), odd[i]), s);
+    for (int q = 0; q < 6; ++q) {
+        const int a_ = pi[q], b_ = pj[q];
+        const i128 cross = mulU32(R.s1[b_], odd[a_]) + mulU32(R.s1[a_], odd[b_]);
+        S.s2[q] = R.s2[q] - shl(cross, s) + shl(mulU32(mulU32(i128(n), odd[a_]), odd[b_]), 2 * s);
     }
+    S.n = R.n;
+    S.e = R.e;
+    S.pad = 0;
+    return S;
+}
+
+// Lane exchange within rows of 16 lanes by DPP (a VALU operation, no LDS
+// round trip): quad_perm [1,0,3,2] pairs lanes i and i^1, [2,3,0,1] lanes i and
+// i^2, row_half_mirror lanes i and 7-i of each group of 8.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141;
+
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), kCtrl, 0xF, 0xF, false));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ i128 dpp128(i128 v) {
+    const unsigned __int128 u = (unsigned __int128)v;
+    const uint32_t w0 = dpp32<kCtrl>(uint32_t(u)), w1 = dpp32<kCtrl>(uint32_t(u >> 32));
+    const uint32_t w2 = dpp32<kCtrl>(uint32_t(u >> 64)), w3 = dpp32<kCtrl>(uint32_t(u >> 96));
+    return i128((unsigned __int128)w0 | ((unsigned __int128)w1 << 32) | ((unsigned __int128)w2 << 64) |
+                ((unsigned __int128)w3 << 96));
+}
+
+// Sum over the 8 lanes of a group (every lane of the group active), in every
+// lane: pairs, then quads, then the two quads of the group.
+__device__ __forceinline__ i128 groupSum8(i128 v) {
+    v += dpp128<kDppXor1>(v);
+    v += dpp128<kDppXor2>(v);
+    v += dpp128<kDppHalfMirror>(v);
     return v;
 }
 
@@ -183,25 +228,139 @@ struct AnalysisArgs {
     uint32_t* queueCount;
     const FrameParams* frame;    // uniforms and arrival of this frame (device)
     uint32_t margin;
+    uint32_t* closeAt;
+    uint32_t* rollStamp;
+    uint32_t* shapeStamp;
+    ShapeCache* shapes;
+    uint32_t* markList;
+    uint32_t* stamp;
+    // device slots of the incremental roll-up when its marked nodes exceed LDS
+    RollupSums* slotSums;
+    uint32_t* slotNode;
+    uint32_t* slotFc;
+    uint32_t* slotClose;
+    uint32_t* slotChild;
+    uint32_t* slotNk;
+    uint32_t* slotOf;
+    uint32_t ldsSlots;   // the roll-up's limits (at most kLdsSlots and 64)
+    uint32_t waveSlots;
 };
 
-// Closure, covariance, eigen-solve and score of one node from its rolled-up sums.
-// With `pred` (the learned score's surface variation and planarity for this
-// node), an open node takes the prediction.
+// Cyclic Jacobi on a symmetric 3x3 (a00 a01 a02 a11 a12 a22) in FP32 with the
+// hardware reciprocal, square root and reciprocal square root (about 1 ulp each;
+// a rotation is then orthogonal to within a few ulps, so the eigenvalues stay
+// within ~1e-7 of the trace): a rotation costs a few VALU operations instead of
+// three correctly rounded divisions and two square roots. Sweeps stop when the
+// off-diagonal mass is below 2^-48 of the diagonal's (the eigenvalues then sit
+// within an FP32 rounding of their limit); eigenvalues ascending.
+__device__ __forceinline__ void jacobi3Fast(const float a[6], float ev[3]) {
+    float m[3][3] = {{a[0], a[1], a[2]}, {a[1], a[3], a[4]}, {a[2], a[4], a[5]}};
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        const float off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+        const float dg = m[0][0] * m[0][0] + m[1][1] * m[1][1] + m[2][2] * m[2][2];
+        if (!(off > 3.5527137e-15f * dg)) break;
+        for (int pq = 0; pq < 3; ++pq) {
+            const int p = pq == 2 ? 1 : 0;
+            const int q = pq == 0 ? 1 : 2;
+            const float apq = m[p][q];
+            if (apq == 0.0f) continue;
+            const float theta = (m[q][q] - m[p][p]) * 0.5f * __builtin_amdgcn_rcpf(apq);
+            const float at = fabsf(theta);
+            const float t = at > 1e15f ? 0.5f * __builtin_amdgcn_rcpf(theta)
+                                       : copysignf(__builtin_amdgcn_rcpf(at + __builtin_amdgcn_sqrtf(at * at + 1.0f)),
+                                                   theta);
+            const float c = __builtin_amdgcn_rsqf(t * t + 1.0f);
+            const float sn = t * c;
+            const int r = 3 - p - q;
+            const float arp = m[r][p], arq = m[r][q];
+            m[p][p] = m[p][p] - t * apq;
+            m[q][q] = m[q][q] + t * apq;
+            m[p][q] = m[q][p] = 0.0f;
+            const float nrp = c * arp - sn * arq;
+            const float nrq = sn * arp + c * arq;
+            m[r][p] = m[p][r] = nrp;
+            m[r][q] = m[q][r] = nrq;
+        }
+    }
+    ev[0] = m[0][0];
+    ev[1] = m[1][1];
+    ev[2] = m[2][2];
+    float tmp;
+    if (ev[0] > ev[1]) { tmp = ev[0]; ev[0] = ev[1]; ev[1] = tmp; }
+    if (ev[1] > ev[2]) { tmp = ev[1]; ev[1] = ev[2]; ev[2] = tmp; }
+    if (ev[0] > ev[1]) { tmp = ev[0]; ev[0] = ev[1]; ev[1] = tmp; }
+}
+
+// The analytic shape of a node from its centred sums: covariance numerators
+// num = n*S2 - S1*S1^T (exact int128 while they fit, FP64 otherwise), the
+// matrix num / trace(num) in FP32, Jacobi as above; eigenvalues ascending, in
+// node-extent units (times trace(num) / n^2 / 4^e), surface variation
+// l0/(l0+l1+l2), planarity (l1-l0)/l2, linearity (l2-l1)/l2.
+__device__ void shapeOf(const RollupSums& S, ShapeCache& sh) {
+    memset(&sh, 0, sizeof(sh));
+    sh.n = S.n;
+    if (S.n < 3) return;
+    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+    int maxS2 = 0, maxS1 = 0;
+    for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
+    for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
+    // exact: n * S2 and S1 * S1 fit (then |S1| < 2^63, a signed 64-bit value)
+    const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
+    const double dn = double(S.n);
+    double num[6];
+    for (int k = 0; k < 6; ++k) {
+        if (exact)
+            num[k] = i128ToDouble(mulU64(S.s2[k], S.n) -
+                                  i128(int64_t(S.s1[pi[k]])) * i128(int64_t(S.s1[pj[k]])));
+        else
+            num[k] = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
+    }
+    const double tn = num[0] + num[3] + num[5];
+    if (!(tn > 0.0)) return;
+    const double inv = 1.0 / tn;
+    float a32[6];
+    for (int k = 0; k < 6; ++k) a32[k] = float(num[k] * inv);
+    float ev[3];
+    jacobi3Fast(a32, ev);
+    const double scale = ldexp(1.0, -int(S.e));
+    const double trace = tn / (dn * dn) * scale * scale;
+    for (int i = 0; i < 3; ++i) sh.lambda[i] = float(double(ev[i]) * trace);
+    const float sum = ev[0] + ev[1] + ev[2];
+    sh.sv = sum > 0.0f ? ev[0] * __builtin_amdgcn_rcpf(sum) : 0.0f;
+    const float r2 = ev[2] > 0.0f ? __builtin_amdgcn_rcpf(ev[2]) : 0.0f;
+    sh.planarity = (ev[1] - ev[0]) * r2;
+    sh.linearity = (ev[2] - ev[1]) * r2;
+}
+
+// Recomputes the cached shape of every node whose sums changed since it was
+// solved (after the full roll-up; the incremental one solves its own nodes).
+__global__ __launch_bounds__(kBlock) void kShapes(DevTree t, AnalysisArgs aa) {
+    const uint32_t n = t.g->numNodes;
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        const uint32_t rs = aa.rollStamp[node];
+        if (aa.shapeStamp[node] == rs) continue;
+        ShapeCache sh;
+        shapeOf(centred(t, node, aa.rollup[node]), sh);
+        aa.shapes[node] = sh;
+        aa.shapeStamp[node] = rs;
+    }
+}
+
+// Closure, shape, coverage and score of one node. With `pred` (the learned
+// score's surface variation and planarity for this node), an open node takes the
+// prediction unless it says the node is not planar (surface variation > 0.05).
 __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
-                             uint32_t node, const RollupSums& S, bool closed, uint32_t D,
-                             const float* pred = nullptr) {
+                             uint32_t node, bool closed, const float* pred = nullptr) {
+    const ShapeCache sh = aa.shapes[node];  // solved by the roll-up that last changed its sums
     remo_node_analysis r;
     memset(&r, 0, sizeof(r));
-    r.n = S.n;
+    r.n = sh.n;
     float dx, dy;
     projectedExtent(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, dx, dy);
     r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
     r.state = closed ? 2u : 0u;
     bool scored = false;
-    if (pred && !closed && S.n >= 3) {
-        // learned score for open nodes; the analytic path is the fallback when the
-        // prediction says the node is not planar (surface variation > 0.05)
+    if (pred && !closed && sh.n >= 3) {
         const float sv = pred[0], pl = pred[1];
         if (sv <= kLearnedFallbackSv) {
             r.surface_variation = fmaxf(sv, 0.0f);
@@ -211,87 +370,64 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
             scored = true;
         }
     }
-    if (!scored && S.n >= 3) {
-        const uint32_t e = 20 + D - t.level[node];
-        const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-        int maxS2 = 0, maxS1 = 0;
-        for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
-        for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
-        const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
-        const double scale = ldexp(1.0, -int(e));
-        const double dn = double(S.n);
-        double cov[6];
-        for (int k = 0; k < 6; ++k) {
-            double num;
-            if (exact)
-                num = i128ToDouble(i128(S.n) * S.s2[k] - S.s1[pi[k]] * S.s1[pj[k]]);
-            else
-                num = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
-            cov[k] = num / (dn * dn) * scale * scale;
-        }
-        const double trace = cov[0] + cov[3] + cov[5];
-        if (trace > 0.0) {
-            double a64[6];
-            float a32[6];
-            for (int k = 0; k < 6; ++k) {
-                a64[k] = cov[k] / trace;
-                a32[k] = float(a64[k]);
-            }
-            float ev32[3];
-            jacobi3<float>(a32, ev32);
-            const float gap = fminf(ev32[1] - ev32[0], ev32[2] - ev32[1]);
-            double ev[3];
-            if (gap < 1e-4f) {
-                double ev64[3];
-                jacobi3<double>(a64, ev64);
-                for (int i = 0; i < 3; ++i) ev[i] = ev64[i];
-                r.flags |= 1u;
-            } else {
-                for (int i = 0; i < 3; ++i) ev[i] = double(ev32[i]);
-            }
-            for (int i = 0; i < 3; ++i) r.lambda[i] = float(ev[i] * trace);
-            const double sum = ev[0] + ev[1] + ev[2];
-            r.surface_variation = sum > 0.0 ? float(ev[0] / sum) : 0.0f;
-            r.planarity = ev[2] > 0.0 ? float((ev[1] - ev[0]) / ev[2]) : 0.0f;
-            r.linearity = ev[2] > 0.0 ? float((ev[2] - ev[1]) / ev[2]) : 0.0f;
-        }
+    if (!scored) {
+        for (int i = 0; i < 3; ++i) r.lambda[i] = sh.lambda[i];
+        r.surface_variation = sh.sv;
+        r.planarity = sh.planarity;
+        r.linearity = sh.linearity;
     }
     r.score = 3.0f * r.surface_variation * r.coverage;
     aa.results[node] = r;
+    aa.closed[node] = closed ? 1 : 0;
     remo_node_accum* a = t.accums + node;
     a->state = r.state;
     a->score = r.score;
 }
 
-// Leaves: sums from the fixed-point accumulators, closure from the batch age,
-// then their analysis.
-// A leaf's rolled-up sums and closure (written for kFinalize, returned for the
-// parent's roll-up).
-__device__ __forceinline__ uint8_t leafRecord(const DevTree& t, const AnalysisArgs& aa, uint32_t node, uint32_t D,
-                                              uint32_t current, RollupSums& S) {
-    leafSums(t, node, D, S);
-    S.e = 20 + D - t.level[node];
-    const remo_node_accum& a = t.accums[node];
-    const uint8_t cl = (a.count == 0 || current >= a.last_touched_batch + aa.margin) ? 1 : 0;
-    aa.rollup[node] = S;
-    aa.closed[node] = cl;
-    return cl;
+// The batch index from which a leaf counts as closed: untouched for `margin`
+// batches, or empty (closed from the start).
+__device__ __forceinline__ uint32_t leafCloseAt(const remo_node_accum& a, uint32_t margin) {
+    return a.count == 0 ? 0u : a.last_touched_batch + margin;
 }
 
-// Leaves: sums from the fixed-point accumulators, closure from the batch age.
-// (Forming leaf children's sums inside the level launches instead was slower:
-// 61 against 53 us per Analysis on the 100M-point tree.)
+// A leaf's rolled-up sums, closing batch and stamp (written for kFinalize,
+// returned for the parent's roll-up).
+__device__ __forceinline__ uint32_t leafRecord(const DevTree& t, const AnalysisArgs& aa, uint32_t node, uint32_t D,
+                                               uint32_t stamp, RollupSums& S) {
+    leafSums(t, node, D, S);
+    const uint32_t ca = leafCloseAt(t.accums[node], aa.margin);
+    aa.rollup[node] = S;
+    aa.closeAt[node] = ca;
+    aa.rollStamp[node] = stamp;
+    return ca;
+}
+
+// The full roll-up (the first Analysis, or one after the tree got deeper, which
+// rescales every sum): every leaf, then the inner nodes level by level. Its
+// records carry the stamp the incremental kernel launched right after it takes
+// (the device counter + 1); it empties the dirty list.
+// Leaves: sums from the fixed-point accumulators, closing batch from the batch
+// age. (Forming leaf children's sums inside the level launches instead was
+// slower: 61 against 53 us per Analysis on the 100M-point tree.)
 __global__ __launch_bounds__(kBlock) void kAnalysisLeaves(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
-    const uint32_t current = g->batchletIndex;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *aa.queueCount = 0;  // kFinalize fills the queue afterwards
+    const uint32_t stamp = *aa.stamp + 1u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *t.dirtyCount = 0;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        t.dirtyListed[node] = 0;
         if (t.firstChild[node] >= 0) continue;
         RollupSums S;
-        leafRecord(t, aa, node, D, current, S);
+        leafRecord(t, aa, node, D, stamp, S);
     }
+}
+
+// Max over the 8 lanes of a group (every lane of the group active).
+__device__ __forceinline__ uint32_t groupMax8(uint32_t v) {
+    v = max(v, dpp32<kDppXor1>(v));
+    v = max(v, dpp32<kDppXor2>(v));
+    return max(v, dpp32<kDppHalfMirror>(v));
 }
 
 // Inner nodes of one level, bottom-up, one launch per level: the eight children
@@ -303,6 +439,7 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
+    const uint32_t stamp = *aa.stamp + 1u;
     const uint32_t sub = threadIdx.x & 7u;
     const uint32_t groups = (gridDim.x * blockDim.x) >> 3;
     // every lane of a group sees the same node, so the group's lanes stay together
@@ -311,19 +448,18 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs
         const int32_t fc = t.firstChild[node];
         if (fc < 0) continue;
         const uint32_t child = uint32_t(fc) + sub;
-        RollupSums C;
-        childContribution(aa.rollup[child], sub, int(18 + D - level), C);
+        const RollupSums C = aa.rollup[child];
         RollupSums P;
         for (int i = 0; i < 3; ++i) P.s1[i] = groupSum8(C.s1[i]);
         for (int q = 0; q < 6; ++q) P.s2[q] = groupSum8(C.s2[q]);
         P.n = uint64_t(groupSum8(i128(C.n)));
         P.e = 20 + D - level;
         P.pad = 0;
-        uint32_t cl = aa.closed[child];
-        for (int m = 1; m < 8; m <<= 1) cl &= __shfl_xor(cl, m, 8);
+        const uint32_t ca = groupMax8(aa.closeAt[child]);
         if (sub == 0) {
             aa.rollup[node] = P;
-            aa.closed[node] = uint8_t(cl);
+            aa.closeAt[node] = ca;
+            aa.rollStamp[node] = stamp;
         }
     }
 }
@@ -348,6 +484,7 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
+    const uint32_t stamp = *aa.stamp + 1u;
     const uint32_t lane = laneId(), c = lane >> 3, gi = lane & 7u;
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t node = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; node < n; node += waves) {
@@ -357,40 +494,275 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
         const uint32_t child = uint32_t(fc) + c;
         const int32_t gfc = t.firstChild[child];  // the same for the 8 lanes of a group
         RollupSums C;
-        uint32_t cl;
+        uint32_t ca;
         if (gfc >= 0) {
             const uint32_t gc = uint32_t(gfc) + gi;
-            RollupSums G;
-            childContribution(aa.rollup[gc], gi, int(18 + D - (level + 1)), G);
+            const RollupSums G = aa.rollup[gc];
             for (int i = 0; i < 3; ++i) C.s1[i] = groupSum8(G.s1[i]);
             for (int q = 0; q < 6; ++q) C.s2[q] = groupSum8(G.s2[q]);
             C.n = uint64_t(groupSum8(i128(G.n)));
             C.e = 20 + D - (level + 1);
             C.pad = 0;
-            cl = aa.closed[gc];
-            for (int m = 1; m < 8; m <<= 1) cl &= __shfl_xor(cl, m, 8);
+            ca = groupMax8(aa.closeAt[gc]);
             if (gi == 0) {
                 aa.rollup[child] = C;
-                aa.closed[child] = uint8_t(cl);
+                aa.closeAt[child] = ca;
+                aa.rollStamp[child] = stamp;
             }
         } else {
             C = aa.rollup[child];
-            cl = aa.closed[child];
+            ca = aa.closeAt[child];
         }
-        RollupSums T;
-        childContribution(C, c, int(18 + D - level), T);
         RollupSums P;
-        for (int i = 0; i < 3; ++i) P.s1[i] = strideSum8(T.s1[i]);
-        for (int q = 0; q < 6; ++q) P.s2[q] = strideSum8(T.s2[q]);
-        P.n = uint64_t(strideSum8(i128(T.n)));
+        for (int i = 0; i < 3; ++i) P.s1[i] = strideSum8(C.s1[i]);
+        for (int q = 0; q < 6; ++q) P.s2[q] = strideSum8(C.s2[q]);
+        P.n = uint64_t(strideSum8(i128(C.n)));
         P.e = 20 + D - level;
         P.pad = 0;
-        for (int m = 8; m < 64; m <<= 1) cl &= __shfl_xor(cl, m);
+        for (int m = 8; m < 64; m <<= 1) ca = max(ca, uint32_t(__shfl_xor(ca, m)));
         if (lane == 0) {
             aa.rollup[node] = P;
-            aa.closed[node] = uint8_t(cl);
+            aa.closeAt[node] = ca;
+            aa.rollStamp[node] = stamp;
         }
     }
+}
+
+// The incremental roll-up, one workgroup. The nodes on the dirty list and every
+// ancestor of them are marked (an LDS bitmap, so each is taken once) and given a
+// slot, by level. Then, for every slot at once, the eight children: a child that
+// is not marked adds its contribution to the slot's partial sums (a leaf whose
+// sums changed, folded by Accumulate or new from a split, from its accumulator,
+// with a new stamp; any other from its stored record), and a marked child leaves
+// its slot number. Then the slots level by level from the deepest: the partial
+// plus the marked children's slots. With at most kLdsSlots marked nodes the slots
+// live in LDS, so every load happens in the partial pass and the level passes
+// touch LDS only; when every level's slots also fit one wave, that wave runs the
+// levels alone (no workgroup barrier per level). Integer sums: the result equals
+// a full roll-up's. Last the slots' records are written and the shapes of every
+// node whose sums changed are solved, in one pass: a few hundred per step, which
+// in the node-parallel finalize would put an eigen-solve into nearly every wave
+// (a wave running the levels beside waves that solve shapes shares their SIMD
+// and took longer than both run one after the other). It also bumps the
+// Analysis stamp (the full roll-up, when it ran first, already wrote this stamp)
+// and clears the work queue for the finalize.
+constexpr uint32_t kRollThreads = 512;
+constexpr uint32_t kLevels = kMaxDepth + 1;
+constexpr uint32_t kLdsSlots = 384;
+static_assert(AnalysisState{}.rollLdsSlots == kLdsSlots, "default LDS slot limit");
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void addSums(RollupSums& P, const RollupSums& T) {
+    for (int i = 0; i < 3; ++i) P.s1[i] += T.s1[i];
+    for (int q = 0; q < 6; ++q) P.s2[q] += T.s2[q];
+    P.n += T.n;
+}
+
+__device__ __forceinline__ void solveShape(const DevTree& t, const AnalysisArgs& aa, uint32_t node,
+                                           const RollupSums& S, uint32_t stamp) {
+    ShapeCache sh;
+    shapeOf(centred(t, node, S), sh);
+    aa.shapes[node] = sh;
+    aa.shapeStamp[node] = stamp;
+}
+
+// One slot: its partial plus its marked children's final sums (their slots
+// listed first in the slot's eight child links, nk of them).
+__device__ __forceinline__ void rollSlot(uint32_t sl, RollupSums* sSums, uint32_t* sClose, const uint32_t* sChild,
+                                         const uint32_t* sNk) {
+    RollupSums P = sSums[sl];
+    uint32_t ca = sClose[sl];
+    const uint32_t nk = sNk[sl];
+    for (uint32_t j = 0; j < nk; ++j) {
+        const uint32_t cs = sChild[sl * 8 + j];
+        addSums(P, sSums[cs]);
+        ca = max(ca, sClose[cs]);
+    }
+    sSums[sl] = P;
+    sClose[sl] = ca;
+}
+
+// The slot phases (placement, partials, levels, records and shapes) over slot
+// arrays in LDS or in device memory (the caller picks; inlined for each).
+__device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t stamp,
+                                          uint32_t M, bool oneWave, const uint32_t* markBits,
+                                          const uint32_t* changedBits, const uint32_t* count, const uint32_t* start,
+                                          uint32_t* cursor, uint32_t* leaves, RollupSums* sSums, uint32_t* sNode,
+                                          uint32_t* sFc, uint32_t* sClose, uint32_t* sChild, uint32_t* sNk) {
+    const uint32_t tid = threadIdx.x;
+    // slots by level; a slot's first child carries the level in its top byte
+    for (uint32_t k = tid; k < M; k += kRollThreads) {
+        const uint32_t v = aa.markList[k];
+        const uint32_t p = v & 0xFFFFFFu, L = v >> 24;
+        const uint32_t pos = atomicAdd(&cursor[L], 1u);
+        aa.slotOf[p] = pos;
+        sNode[pos] = p;
+        sFc[pos] = uint32_t(t.firstChild[p]) | (L << 24);
+    }
+    __syncthreads();
+    // partials: 8 lanes per slot, one child each. Every lane loads what any of
+    // the three cases needs and picks after, so the group makes one memory round
+    // trip instead of one per case.
+    const uint32_t sub = tid & 7u;
+    for (uint32_t grp = tid >> 3; grp < M; grp += kRollThreads / 8) {
+        const uint32_t fcw = sFc[grp];
+        const uint32_t L = fcw >> 24, child = (fcw & 0xFFFFFFu) + sub;
+        const bool mk = (markBits[child >> 5] >> (child & 31u)) & 1u;
+        const bool ch = !mk && ((changedBits[child >> 5] >> (child & 31u)) & 1u);
+        const uint32_t slotv = aa.slotOf[child];
+        const RollupSums R = aa.rollup[child];
+        const uint32_t caR = aa.closeAt[child];
+        const LeafIn lf = loadLeaf(t, child);
+        RollupSums F;
+        leafSumsOf(lf, D, F);
+        const uint32_t caF = lf.count == 0 ? 0u : lf.last + aa.margin;
+        if (ch) {  // a leaf whose sums changed: its record, and its shape below
+            aa.rollup[child] = F;
+            aa.closeAt[child] = caF;
+            aa.rollStamp[child] = stamp;
+            aa.markList[atomicAdd(leaves, 1u)] = child;
+        }
+        RollupSums T;
+        for (int i = 0; i < 3; ++i) T.s1[i] = mk ? i128(0) : (ch ? F.s1[i] : R.s1[i]);
+        for (int q = 0; q < 6; ++q) T.s2[q] = mk ? i128(0) : (ch ? F.s2[q] : R.s2[q]);
+        T.n = mk ? 0ull : (ch ? F.n : R.n);
+        uint32_t ca = mk ? 0u : (ch ? caF : caR);
+        const uint32_t cs = mk ? slotv : kNoSlot;
+        // the marked children's slots, packed to the front of the slot's links
+        const uint32_t gbits = uint32_t(__ballot(mk) >> (laneId() & ~7u)) & 0xFFu;
+        if (mk) sChild[grp * 8 + __popc(gbits & ((1u << sub) - 1u))] = cs;
+        RollupSums P;
+        for (int i = 0; i < 3; ++i) P.s1[i] = groupSum8(T.s1[i]);
+        for (int q = 0; q < 6; ++q) P.s2[q] = groupSum8(T.s2[q]);
+        P.n = uint64_t(groupSum8(i128(T.n)));
+        P.e = 20 + D - L;
+        P.pad = 0;
+        ca = groupMax8(ca);
+        if (sub == 0) {
+            sSums[grp] = P;
+            sClose[grp] = ca;
+            sNk[grp] = __popc(gbits);
+        }
+    }
+    __syncthreads();
+    if (oneWave) {
+        // levels, deepest first, by wave 0 alone: its LDS operations complete in
+        // order, so a fence between levels is enough
+        if (tid < 64) {
+            for (int L = int(D) - 1; L >= 0; --L) {
+                if (tid < count[L]) rollSlot(start[L] + tid, sSums, sClose, sChild, sNk);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        }
+    } else {
+        for (int L = int(D) - 1; L >= 0; --L) {
+            const uint32_t cnt = count[L];
+            if (cnt == 0) continue;  // uniform: the barrier below is skipped by every thread
+            for (uint32_t sl = start[L] + tid; sl < start[L] + cnt; sl += kRollThreads)
+                rollSlot(sl, sSums, sClose, sChild, sNk);
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    // the slots' records, and the shapes of the slots and of the changed leaves
+    const uint32_t total = M + *leaves;
+    for (uint32_t k = tid; k < total; k += kRollThreads) {
+        uint32_t node;
+        RollupSums S;
+        if (k < M) {
+            node = sNode[k];
+            S = sSums[k];
+            aa.rollup[node] = S;
+            aa.closeAt[node] = sClose[k];
+            aa.rollStamp[node] = stamp;
+        } else {
+            node = aa.markList[k - M];
+            S = aa.rollup[node];
+        }
+        solveShape(t, aa, node, S, stamp);
+    }
+}
+
+__global__ __launch_bounds__(kRollThreads) void kRollupDirty(DevTree t, AnalysisArgs aa) {
+    __shared__ uint32_t markBits[kMarkBits / 32];     // marked (rolled-up) inner nodes
+    __shared__ uint32_t changedBits[kMarkBits / 32];  // leaves whose sums changed
+    __shared__ RollupSums lSums[kLdsSlots];
+    __shared__ uint32_t lNode[kLdsSlots], lFc[kLdsSlots], lClose[kLdsSlots], lChild[kLdsSlots * 8], lNk[kLdsSlots];
+    __shared__ uint32_t count[kLevels], start[kLevels], cursor[kLevels];
+    __shared__ uint32_t marked, stampS, leaves, widest;
+    const uint32_t tid = threadIdx.x;
+    Globals* g = t.g;
+    const uint32_t D = g->maxLevel;
+    for (uint32_t k = tid; k < kMarkBits / 128; k += kRollThreads) {
+        reinterpret_cast<uint4*>(markBits)[k] = make_uint4(0, 0, 0, 0);
+        reinterpret_cast<uint4*>(changedBits)[k] = make_uint4(0, 0, 0, 0);
+    }
+    if (tid < kLevels) count[tid] = 0;
+    if (tid == 0) {
+        marked = 0;
+        leaves = 0;
+        stampS = *aa.stamp + 1u;
+        *aa.stamp = stampS;
+        *aa.queueCount = 0;
+    }
+    __syncthreads();
+    const uint32_t stamp = stampS;
+    // (pools beyond the bitmap always take the full roll-up, which empties the list)
+    const uint32_t nd = t.maxNodes <= kMarkBits ? min(*t.dirtyCount, t.maxNodes) : 0u;
+    // mark: an inner node on the list (one that split) is rolled up itself and its
+    // children count as changed; a leaf is changed and rolled up from its parent.
+    // A mark carries the node's level in its top byte.
+    for (uint32_t e = tid; e < nd; e += kRollThreads) {
+        const uint32_t node = t.dirtyList[e];
+        t.dirtyListed[node] = 0;
+        const int32_t fc = t.firstChild[node];
+        const uint32_t lvl = t.level[node];
+        uint32_t p, lp;
+        if (fc >= 0) {
+            for (uint32_t c = uint32_t(fc); c < uint32_t(fc) + 8u; ++c) atomicOr(&changedBits[c >> 5], 1u << (c & 31u));
+            p = node;
+            lp = lvl;
+        } else {
+            atomicOr(&changedBits[node >> 5], 1u << (node & 31u));
+            p = t.parent[node];
+            lp = lvl - 1u;
+        }
+        if (p == kNone) {  // the root while it is a leaf
+            RollupSums S;
+            leafRecord(t, aa, node, D, stamp, S);
+            solveShape(t, aa, node, S, stamp);
+            continue;
+        }
+        while (p != kNone) {
+            const uint32_t next = t.parent[p];
+            const uint32_t bit = 1u << (p & 31u);
+            if (atomicOr(&markBits[p >> 5], bit) & bit) break;  // marked, with its ancestors
+            aa.markList[atomicAdd(&marked, 1u)] = p | (lp << 24);
+            atomicAdd(&count[lp], 1u);
+            p = next;
+            --lp;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        *t.dirtyCount = 0;
+        uint32_t acc = 0, w = 0;
+        for (uint32_t L = 0; L < kLevels; ++L) {
+            start[L] = acc;
+            cursor[L] = acc;
+            acc += count[L];
+            w = max(w, count[L]);
+        }
+        widest = w;
+    }
+    __syncthreads();
+    const uint32_t M = marked;
+    if (M <= min(aa.ldsSlots, kLdsSlots))  // uniform
+        rollSlots(t, aa, D, stamp, M, widest <= min(aa.waveSlots, 64u), markBits, changedBits, count, start, cursor, &leaves, lSums,
+                  lNode, lFc, lClose, lChild, lNk);
+    else
+        rollSlots(t, aa, D, stamp, M, false, markBits, changedBits, count, start, cursor, &leaves, aa.slotSums,
+                  aa.slotNode, aa.slotFc, aa.slotClose, aa.slotChild, aa.slotNk);
 }
 
 // Appends the wave's scored nodes to the work queue (one atomic per wave).
@@ -404,17 +776,18 @@ __device__ inline void queueScored(const AnalysisArgs& aa, uint32_t node, bool q
     if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
 }
 
-// Every node's eigen-solve and score, and the work queue of scored nodes.
+// Every node's closure, shape (the cached one: the roll-up that changed a node's
+// sums solved it), coverage and score, and the work queue of scored nodes.
 __global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
-    const uint32_t D = g->maxLevel;
+    const uint32_t current = g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t node = base + threadIdx.x;
         bool q = false;
         if (node < n) {
-            finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D);
+            finalizeNode(t, aa, u, node, current >= aa.closeAt[node]);
             q = aa.results[node].score > 0.0f;
         }
         queueScored(aa, node, q);
@@ -438,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     if (blockIdx.x * kPerBlock >= n) return;  // the whole block: before the barrier
-    const uint32_t D = g->maxLevel;
+    const uint32_t current = g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
     if (arrival < 0.0f) arrival = aa.frame->arrival;
     const uint32_t wave = threadIdx.x / 64, lane = laneId();
@@ -452,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
         if (w0 < n) {
             if (lane < 16) {
                 float f[kMlpIn];
-                if (node < n) nodeFeatures(t, node, aa.rollup[node], arrival, f);
+                if (node < n) nodeFeatures(t, node, centred(t, node, aa.rollup[node]), arrival, f);
                 for (uint32_t k = 0; k < 16; ++k) act[lane][k] = (node < n && k < kMlpIn) ? f[k] : 0.0f;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -461,10 +834,20 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
         }
         bool q = false;
         if (mine) {
-            finalizeNode(t, aa, u, node, aa.rollup[node], aa.closed[node] != 0, D, &pred[wave][lane * kMlpOut]);
+            finalizeNode(t, aa, u, node, current >= aa.closeAt[node], &pred[wave][lane * kMlpOut]);
             q = aa.results[node].score > 0.0f;
         }
         queueScored(aa, node, q);
+    }
+}
+
+// Every node's record in its own frame (the roll-up dump). A record never
+// written by a roll-up (e = 0) is copied as it is.
+__global__ __launch_bounds__(kBlock) void kCentredRollups(DevTree t, const RollupSums* in, RollupSums* out,
+                                                         uint32_t n) {
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
+        const RollupSums R = in[node];
+        out[node] = R.e ? centred(t, node, R) : R;
     }
 }
 
@@ -473,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void kFeatures(DevTree t, AnalysisArgs aa, 
     const uint32_t n = min(t.g->numNodes, cap);
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
         float f[kMlpIn];
-        nodeFeatures(t, node, aa.rollup[node], aa.frame->arrival, f);
+        nodeFeatures(t, node, centred(t, node, aa.rollup[node]), aa.frame->arrival, f);
         for (uint32_t k = 0; k < kMlpIn; ++k) out[uint64_t(node) * kMlpIn + k] = f[k];
     }
 }
@@ -781,6 +1164,37 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.units = static_cast<RasterUnit*>(p);
     A(&p, sizeof(FrameParams), "frame parameters");
     a.frame = static_cast<FrameParams*>(p);
+    A(&p, 4 * maxNodes, "closing batches");
+    a.closeAt = static_cast<uint32_t*>(p);
+    A(&p, 4 * maxNodes, "roll-up stamps");
+    a.rollStamp = static_cast<uint32_t*>(p);
+    A(&p, 4 * maxNodes, "shape stamps");
+    a.shapeStamp = static_cast<uint32_t*>(p);
+    A(&p, sizeof(ShapeCache) * maxNodes, "shape cache");
+    a.shapes = static_cast<ShapeCache*>(p);
+    A(&p, 4 * maxNodes, "marked nodes");
+    a.markList = static_cast<uint32_t*>(p);
+    const uint64_t slots = maxNodes < kMarkBits ? maxNodes : kMarkBits;
+    A(&p, sizeof(RollupSums) * slots, "roll-up slot sums");
+    a.slotSums = static_cast<RollupSums*>(p);
+    A(&p, 4 * slots, "roll-up slot nodes");
+    a.slotNode = static_cast<uint32_t*>(p);
+    A(&p, 4 * slots, "roll-up slot children");
+    a.slotFc = static_cast<uint32_t*>(p);
+    A(&p, 4 * slots, "roll-up slot closing batches");
+    a.slotClose = static_cast<uint32_t*>(p);
+    A(&p, 32 * slots, "roll-up slot child links");
+    a.slotChild = static_cast<uint32_t*>(p);
+    A(&p, 4 * slots, "roll-up slot marked children");
+    a.slotNk = static_cast<uint32_t*>(p);
+    A(&p, 4 * maxNodes, "roll-up slot of each node");
+    a.slotOf = static_cast<uint32_t*>(p);
+    A(&p, 16, "analysis stamp");
+    a.stamp = static_cast<uint32_t*>(p);
+    // stamps start at 0 and only grow, so no cached shape can match a later roll-up
+    if (rc == REMO_OK && (hipMemset(a.stamp, 0, 16) != hipSuccess ||
+                          hipMemset(a.shapeStamp, 0, 4 * maxNodes) != hipSuccess))
+        rc = REMO_ERR_DEVICE;
     if (rc == REMO_OK) rc = mlpAllocate(a, alloc);
     return rc;
 }
@@ -789,6 +1203,7 @@ void analysisReset(AnalysisState& a) {
     a.drawItems = 0;
     a.drawListOverflow = 0;
     a.drawSamples = 0;
+    a.needFull = true;
 }
 
 namespace {
@@ -807,20 +1222,45 @@ void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, bo
     a.frameSet = true;
 }
 
-// The Analysis kernels alone (no host work, so they can be captured).
-// The learned score's arrival fraction goes in as an argument when arrival >= 0,
-// else it is read from the frame parameters.
-void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, uint32_t maxLevel, uint32_t numNodes,
-                    float arrival, hipStream_t s) {
+AnalysisArgs argsOf(const AnalysisState& a) {
+    return AnalysisArgs{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin,
+                        a.closeAt, a.rollStamp, a.shapeStamp, a.shapes, a.markList, a.stamp,
+                        a.slotSums, a.slotNode, a.slotFc, a.slotClose, a.slotChild, a.slotNk, a.slotOf,
+                        a.rollLdsSlots, a.rollWaveSlots};
+}
+
+// Whether this Analysis must roll up every node: the first one after a reset,
+// one after the tree got deeper (the sums' units are 2^-(20 + depth) of the root
+// extent, so every sum rescales), or a node pool beyond the marking bitmap.
+bool takeFull(AnalysisState& a, const DevTree& t, uint32_t maxLevel) {
+    const bool full = a.needFull || maxLevel != a.lastD || t.maxNodes > kMarkBits;
+    a.needFull = false;
+    a.lastD = maxLevel;
+    return full;
+}
+
+// The full roll-up: every leaf, then the inner nodes one launch per two levels
+// (faster than one workgroup walking all levels with barriers: 157 against 101 us
+// per Analysis on a 10,073-node tree, one launch per level).
+void launchFullRollup(AnalysisState& a, const DevTree& t, uint32_t maxLevel, hipStream_t s) {
     const uint32_t nb = a.shape.nodeBlocks;
-    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin};
-    // one launch per level: measured faster than one workgroup walking all levels
-    // with barriers (157 us against 101 us per Analysis on a 10,073-node tree)
+    const AnalysisArgs aa = argsOf(a);
     hipLaunchKernelGGL(kAnalysisLeaves, dim3(nb), dim3(kBlock), 0, s, t, aa);
     int level = int(maxLevel) - 1;  // the deepest level with inner nodes
     for (; level >= 1; level -= 2)
         hipLaunchKernelGGL(kAnalysisLevel2, dim3(nb), dim3(kBlock), 0, s, t, aa, uint32_t(level - 1));
     if (level == 0) hipLaunchKernelGGL(kAnalysisLevel, dim3(nb), dim3(kBlock), 0, s, t, aa, 0u);
+    hipLaunchKernelGGL(kShapes, dim3(nb), dim3(kBlock), 0, s, t, aa);
+}
+
+// The per-frame Analysis kernels (no host work, so they can be captured): the
+// incremental roll-up of what changed since the last Analysis, then the finalize
+// pass. The learned score's arrival fraction goes in as an argument when
+// arrival >= 0, else it is read from the frame parameters.
+void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, float arrival, hipStream_t s) {
+    const uint32_t nb = a.shape.nodeBlocks;
+    const AnalysisArgs aa = argsOf(a);
+    hipLaunchKernelGGL(kRollupDirty, dim3(1), dim3(kRollThreads), 0, s, t, aa);
     if (learned)
         hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa, a.mlpWeights, arrival);
     else
@@ -839,6 +1279,10 @@ void launchRender(AnalysisState& a, const DevTree& t, uint64_t* fb, uint64_t pix
 
 }  // namespace
 
+void launchCentredRollups(const AnalysisState& a, const DevTree& t, RollupSums* out, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(kCentredRollups, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, t, a.rollup, out, n);
+}
+
 int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
                 uint32_t numNodes, float arrival, hipStream_t s, std::string* err) {
     const bool learned = mode == REMO_ANALYSIS_LEARNED;
@@ -850,14 +1294,16 @@ int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& 
     // the arrival fraction, which changes every step, goes to the kernel as an
     // argument: the frame parameters are written only when the camera changes
     setFrame(a, u, arrival, false, s);
-    launchAnalysis(a, t, learned, maxLevel, numNodes, arrival, s);
+    (void)numNodes;
+    if (takeFull(a, t, maxLevel)) launchFullRollup(a, t, maxLevel, s);
+    launchAnalysis(a, t, learned, arrival, s);
     return REMO_OK;
 }
 
 int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s) {
     // the inputs of the last Analysis: its arrival fraction on the device
     if (a.frameSet) setFrame(a, a.lastFrame.u, a.arrival, true, s);
-    AnalysisArgs aa{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin};
+    const AnalysisArgs aa = argsOf(a);
     hipLaunchKernelGGL(kFeatures, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, aa, out, cap);
     return REMO_OK;
 }
@@ -878,7 +1324,11 @@ int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, 
         return REMO_ERR_STATE;
     }
     const uint64_t pixels = uint64_t(u.width) * uint64_t(u.height);
-    const uint64_t key[6] = {maxLevel, uint64_t(mode),
+    (void)numNodes;
+    // the graph holds the incremental roll-up and the finalize pass; a full
+    // roll-up, when one is due, runs eagerly before it
+    if (takeFull(a, t, maxLevel)) launchFullRollup(a, t, maxLevel, s);
+    const uint64_t key[6] = {(uint64_t(a.rollLdsSlots) << 32) | a.rollWaveSlots, uint64_t(mode),
                              reinterpret_cast<uint64_t>(fb), pixels,
                              (uint64_t(a.shape.nodeBlocks) << 32) | a.shape.rasterBlocks, a.shape.rasterTile};
     a.arrival = arrival;
@@ -891,7 +1341,7 @@ int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, 
             if (err) *err = "hipStreamBeginCapture failed";
             return REMO_ERR_DEVICE;
         }
-        launchAnalysis(a, t, learned, maxLevel, numNodes, -1.0f, s);
+        launchAnalysis(a, t, learned, -1.0f, s);
         launchRender(a, t, fb, pixels, s);
         if (hipStreamEndCapture(s, &g) != hipSuccess || !g) {
             if (err) *err = "hipStreamEndCapture failed";

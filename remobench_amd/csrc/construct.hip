@@ -446,6 +446,8 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         // publish the children last: later kernels descend through them
         t.firstChild[node] = int32_t(c0);
         t.walk[node] = make_uint2(c0, gno == kNone ? 0u : uint32_t(gridOff >> 4));
+        // its sums become its (new) children's: Analysis rolls it up again
+        listDirty(t, node);
         if (lvl <= kTopLevel) topSplits[atomicAdd(&numTopSplits, 1u)] = make_uint4(X, Y, Z, c0 | (lvl << kTopLevelShift));
     }
     // the split nodes' top-index cells now start at the children holding them

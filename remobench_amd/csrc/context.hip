@@ -437,6 +437,9 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.candidates, t.candidateCap, "split candidates");
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 4, "accepted splits");
     A(&t.statsPartials, uint64_t(kStatsBlocks) * kStatsWords, "stats sweep partials");
+    A(&t.dirtyList, N, "the analysis dirty list");
+    A(&t.dirtyCount, 1, "the analysis dirty count");
+    A(&t.dirtyListed, N, "the analysis dirty flags");
     A(&t.g, 1, "device counters");
     A(&t.accums, N, "the accumulator side array");
     if (t.fixedDepth) A(&t.cellCount, fixedPyramidEntries(t.fixedDepth), "the fixed-depth cell counts");
@@ -500,6 +503,8 @@ int remo_reset(remo_ctx* c) {
     CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
     CK(hipMemsetAsync(t.backlogFill, 0, 4ull * kBacklogShards * kShardStride, c->stream), "reset");
     CK(hipMemsetAsync(t.voxelShardCount, 0, 4ull * kBacklogShards * t.maxGrids, c->stream), "reset");
+    CK(hipMemsetAsync(t.dirtyListed, 0, 4ull * t.maxNodes, c->stream), "reset");
+    CK(hipMemsetAsync(t.dirtyCount, 0, 4, c->stream), "reset");
     launchResetTree(t, c->stream);
     refineReset(c->refine, t, c->stream);
     CK(hipGetLastError(), "reset kernel");
@@ -846,6 +851,15 @@ int remo_accum_fp32_atomics(remo_ctx* c, float* out, uint32_t cap, uint32_t* num
     return REMO_OK;
 }
 
+int remo_set_rollup_limits(remo_ctx* c, uint32_t ldsSlots, uint32_t waveSlots) {
+    if (!c) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    const AnalysisState defaults{};
+    c->analysis.rollLdsSlots = ldsSlots == 0 ? defaults.rollLdsSlots : std::min(ldsSlots, defaults.rollLdsSlots);
+    c->analysis.rollWaveSlots = waveSlots == 0 ? defaults.rollWaveSlots : std::min(waveSlots, defaults.rollWaveSlots);
+    return REMO_OK;
+}
+
 int remo_set_score_model(remo_ctx* c, const remo_mlp* mlp) {
     if (!c || !mlp) return REMO_ERR_INVALID;
     if (c->dead) return REMO_ERR_DEVICE;
@@ -1121,7 +1135,16 @@ int remo_dump_rollup(remo_ctx* c, remo_node_rollup* out, uint32_t cap, uint32_t*
     if (cap < n) return REMO_ERR_INVALID;
     static_assert(sizeof(RollupSums) == sizeof(remo_node_rollup), "roll-up record layout");
     std::vector<uint8_t> closed(n);
-    CK(hipMemcpy(out, c->analysis.rollup, sizeof(RollupSums) * n, hipMemcpyDeviceToHost), "dump");
+    if (n) {
+        // the records are kept in the root frame; the dump gives each in its node's frame
+        void* tmp = nullptr;
+        CK(hipMalloc(&tmp, sizeof(RollupSums) * n), "dump");
+        launchCentredRollups(c->analysis, c->t, static_cast<RollupSums*>(tmp), n, c->stream);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, tmp, sizeof(RollupSums) * n, hipMemcpyDeviceToHost);
+        hipFree(tmp);
+        CK(e, "dump");
+    }
     CK(hipMemcpy(closed.data(), c->analysis.closed, n, hipMemcpyDeviceToHost), "dump");
     // RollupSums ends in {n, e, pad}: the pad word carries the closure flag here
     for (uint32_t i = 0; i < n; ++i) out[i].closed = closed[i];

@@ -9,10 +9,13 @@
 
 namespace remo_hip {
 
-// Exact roll-up record of one node, in the node's own frame: coordinates are
-// integers in units of 2^-(20 + E) of the node extent, centred on the node
-// centre, where E = deepest level in the tree - node level. __int128 keeps the
-// sums exact (see analysis.hip for the bound).
+// Exact roll-up record of one node: sums of point coordinates (integers in
+// units 2^-(20 + D) of the root extent, D the deepest level) and of their
+// products, over the points under the node; e = 20 + D - level, the node extent
+// being 2^e units. Stored in the root frame (origin at the root's minimum
+// corner, sums modulo 2^128), so a parent's record is the sum of its children's;
+// read in the node's own frame (centred on the node), where the sums are exact
+// __int128 values.
 struct RollupSums {
     __int128 s1[3];
     __int128 s2[6];
@@ -60,10 +63,41 @@ struct DrawItem {
     uint32_t level;
 };
 
+// A node's analytic shape, kept until its roll-up sums change.
+struct ShapeCache {
+    float lambda[3];
+    float sv, planarity, linearity;
+    uint32_t pad[2];
+    unsigned long long n;
+};
+
+// The incremental roll-up marks a dirty node's ancestors in an LDS bitmap of
+// this many node indices; larger node pools always take the full roll-up.
+constexpr uint32_t kMarkBits = 262144;
+
 struct AnalysisState {
     remo_node_analysis* results = nullptr;
     RollupSums* rollup = nullptr;
-    uint8_t* closed = nullptr;        // closure flag per node
+    uint8_t* closed = nullptr;        // closure flag per node (last Analysis)
+    uint32_t* closeAt = nullptr;      // batch index from which the node is closed (max over its leaves)
+    uint32_t* rollStamp = nullptr;    // Analysis stamp of the node's last roll-up write
+    uint32_t* shapeStamp = nullptr;   // stamp of the sums its cached shape was computed from
+    ShapeCache* shapes = nullptr;     // [maxNodes] cached analytic shape per node
+    uint32_t* markList = nullptr;     // [maxNodes] the incremental roll-up's marked nodes, then changed leaves
+    // the incremental roll-up's slots in device memory, for calls that mark more
+    // nodes than its LDS holds ([min(maxNodes, kMarkBits)], child links x8)
+    RollupSums* slotSums = nullptr;
+    uint32_t* slotNode = nullptr;
+    uint32_t* slotFc = nullptr;
+    uint32_t* slotClose = nullptr;
+    uint32_t* slotChild = nullptr;
+    uint32_t* slotNk = nullptr;
+    uint32_t* slotOf = nullptr;       // [maxNodes] slot of each marked node
+    uint32_t* stamp = nullptr;        // [1] Analysis counter (device; never reset)
+    bool needFull = true;             // the next Analysis rolls up every node
+    uint32_t rollLdsSlots = 384;      // incremental roll-up: marked nodes kept in LDS at most
+    uint32_t rollWaveSlots = 64;      // incremental roll-up: widest level run by one wave
+    uint32_t lastD = 0;               // tree depth of the last roll-up
     uint32_t closureMargin = 8;       // batches untouched before a leaf is closed
     float arrival = 0.0f;             // fraction of the cloud's batches ingested (last Analysis)
     uint32_t* queue = nullptr;        // work queue of nodes (score > 0)
@@ -104,11 +138,15 @@ int runAnalysis(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& 
 int runFeatures(AnalysisState& a, const DevTree& t, float* out, uint32_t cap, hipStream_t s);
 int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, uint64_t* fb, hipStream_t s,
               std::string* err);
-// Analysis + Rasterize as one captured graph, re-captured when the tree depth,
-// the mode, the framebuffer, its size or the launch shape change.
+// Analysis + Rasterize as one captured graph (the incremental roll-up, the
+// finalize and the Rasterize kernels; a full roll-up, when one is due, runs
+// eagerly before it), re-captured when the mode, the framebuffer, its size, the
+// launch shape or the roll-up limits change.
 int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
              uint32_t numNodes, float arrival, uint64_t* fb, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);
+// The first n roll-up records, each in its own node's frame, into `out` (device).
+void launchCentredRollups(const AnalysisState& a, const DevTree& t, RollupSums* out, uint32_t n, hipStream_t s);
 // The flat (LOD-free) Rasterize of n resident points, and eye-dome lighting.
 void launchRenderFlat(const remo_point* pts, uint64_t n, const remo_shared_uniforms& u, uint64_t* fb, uint32_t blocks,
                       hipStream_t s);

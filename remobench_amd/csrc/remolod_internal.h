@@ -139,6 +139,11 @@ struct DevTree {
     uint32_t fixedDepth;    // 0: iterative expand; else the analytic-bucketing arm's depth
     uint32_t* cellCount;    // fixed-depth cell-count pyramid, levels 0..fixedDepth
     uint32_t* statsPartials;  // [kStatsBlocks * kStatsWords] per-workgroup partials of the stats sweep
+    // Analysis' dirty list: every node whose roll-up sums changed since the last
+    // Analysis -- leaves Accumulate folded, nodes that split -- listed once.
+    uint32_t* dirtyList;    // [maxNodes]
+    uint32_t* dirtyCount;   // [1]
+    uint32_t* dirtyListed;  // [maxNodes] 1 while the node is in the list
     Globals* g;
     remo_node_accum* accums;
 };
@@ -259,6 +264,15 @@ __device__ __forceinline__ uint64_t lanesBelow() {
 
 __device__ __forceinline__ remo_point* chunkPtr(uint8_t* arena, uint32_t ref) {
     return reinterpret_cast<remo_point*>(arena + (uint64_t(ref) << 4));
+}
+
+// Puts `node` on Analysis' dirty list unless it is already there. Producers
+// (Accumulate's finish, the split kernel) hand each node to one thread, so the
+// flag needs no atomic; the list slot is one atomic per call.
+__device__ __forceinline__ void listDirty(const DevTree& t, uint32_t node) {
+    if (t.dirtyListed[node] != 0) return;
+    t.dirtyListed[node] = 1;
+    t.dirtyList[atomicAdd(t.dirtyCount, 1u)] = node;
 }
 
 // ---- host launchers --------------------------------------------------------

@@ -339,6 +339,11 @@ class Tree:
         finally:
             buf.free()
 
+    def set_rollup_limits(self, lds_slots: int = 0, wave_slots: int = 0) -> None:
+        """Limits of the incremental roll-up (0 = default); the results do not
+        depend on them, smaller ones take its device-memory and barrier paths."""
+        self._ck(self.lib.remo_set_rollup_limits(self.h, int(lds_slots), int(wave_slots)), "set_rollup_limits")
+
     def set_score_model(self, mlp) -> None:
         """Loads a trained score model (remobench_amd.score_model.ScoreMLP)."""
         self._ck(self.lib.remo_set_score_model(self.h, mlp.h), "set_score_model")

@@ -11,9 +11,8 @@ from the leaf's stored points (numpy, float64, node-local coordinates):
           (remo_accum_fp32_atomics).
 
 Errors are max |cov - cov_ref| over the six entries, relative to the trace of
-cov_ref. Also reports the Analysis eigenvalues (FP32 Jacobi, FP64 fallback when
-the eigen-gap is below 1e-4 of the trace) against numpy.linalg.eigvalsh of the
-FP64 reference covariance, for the same leaves.
+cov_ref. Also reports the Analysis eigenvalues (FP32 Jacobi) against
+numpy.linalg.eigvalsh of the FP64 reference covariance, for the same leaves.
 
     python -m remobench_amd.precision --scene skewed --points 300000000
 """
@@ -70,7 +69,6 @@ def run(scene: str, num_points: int, batch: int, seed: int, sample: int, log=pri
     pick = rng.choice(leaves, min(sample, len(leaves)), replace=False)
     err = {"int64": [], "fp64": [], "fp32": [], "eig": []}
     counts = []
-    fallback = 0
     for i in pick:
         rec = nodes[i]
         pts = tree.samples(int(i))
@@ -91,7 +89,6 @@ def run(scene: str, num_points: int, batch: int, seed: int, sample: int, log=pri
         s = f32[i]
         err["fp32"].append(np.abs(cov_from_sums(float(s[0]), s[1:4], s[4:10]) - ref).max() / tr)
         err["eig"].append(np.abs(res[i]["lambda"].astype(np.float64) - np.linalg.eigvalsh(c)).max() / tr)
-        fallback += int(res[i]["flags"] & 1)
     tree.close()
     ring.close()
 
@@ -104,7 +101,6 @@ def run(scene: str, num_points: int, batch: int, seed: int, sample: int, log=pri
         "leaf_points": {"min": int(min(counts)), "median": int(np.median(counts)), "max": int(max(counts))},
         "cov_error_rel_trace": {k: summary(err[k]) for k in ("int64", "fp64", "fp32")},
         "eigen_error_rel_trace": summary(err["eig"]),
-        "fp64_fallback_fraction": fallback / max(1, len(counts)),
     }
 
 

@@ -24,7 +24,7 @@ def main():
         lam = np.sort(10.0 ** rng.uniform(-6, 0, size=3))
         q = rotation(rng)
         mats.append(q @ np.diag(lam) @ q.T)
-    # near-degenerate pairs (the FP64 fallback region) and exact repeats
+    # near-degenerate pairs (small eigen-gaps) and exact repeats
     for gap in (1e-3, 1e-5, 1e-7, 0.0):
         for _ in range(25):
             a = rng.uniform(0.1, 1.0)

@@ -9,6 +9,7 @@ import pytest
 from oracle.oracle import OracleTree
 from remobench_amd import camera
 from remobench_amd import lib as L
+from remobench_amd.pipeline import PointRing, Tree
 
 from .cases import lidar_scene
 from .helpers import keyed
@@ -49,12 +50,15 @@ def test_analysis_parity(built, pose):
         if w["n"] < 3:
             continue
         trace = float(w["lambda64"].sum())
-        # FP32 Jacobi on the GPU vs the oracle's identical FP32 path
+        # the GPU's FP32 Jacobi (hardware reciprocal / square root) vs the oracle's
+        # correctly rounded FP32 Jacobi: both within a few 1e-7 of the trace
         np.testing.assert_allclose(g["lambda"], r["lambda"], rtol=0, atol=1e-6 * trace + 1e-30)
         # the stated accuracy: eigenvalues within 1e-5 of the trace of FP64
         np.testing.assert_allclose(g["lambda"], w["lambda64"], rtol=0, atol=1e-5 * trace + 1e-30)
         np.testing.assert_allclose(g["surface_variation"], r["surface_variation"], rtol=0, atol=1e-5)
-        np.testing.assert_allclose(g["score"], r["score"], rtol=1e-4, atol=1e-3)
+        np.testing.assert_allclose(g["planarity"], r["planarity"], rtol=0, atol=1e-4)
+        # score = 3 * surface variation * coverage, the coverage identical
+        np.testing.assert_allclose(g["score"], r["score"], rtol=1e-5, atol=3e-5 * float(g["coverage"]) + 1e-6)
 
 
 @pytest.mark.gpu
@@ -67,6 +71,62 @@ def test_rollup_bit_exact(built):
     tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
     got = compare_rollup(tree, o, o.analyse(u))
     assert int(got["n"].max()) == len(points)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("limits", [(0, 0), (384, 1), (8, 1)], ids=["lds-one-wave", "lds-barriers", "device-slots"])
+def test_incremental_analysis_every_step(gpu, limits):
+    """Analysis after every 1-batch step rolls up only what changed (the leaves
+    Accumulate folded, the nodes that split, their ancestors) and fully when the
+    tree got deeper; a deepening midway and steps through the captured frame graph
+    included. At every step every node's int128 sums, count, closure and results
+    equal the oracle's full analysis of the same tree. The limits take the roll-up
+    through its three paths: slots in LDS with the levels in one wave, in LDS with
+    a barrier per level, and in device memory."""
+    from .helpers import batch_arrays
+    from .test_gpu_pins import compare_rollup
+    points, meta = lidar_scene(2_097_152, seed=3)
+    batch = 131_072
+    slots, sizes, nb = batch_arrays(points, batch)
+    ring = PointRing(batch, nb)
+    ring.set_cloud(points, batch)
+    ring.pump()
+    tree = Tree(meta, slot_capacity=batch, ring_slots=nb, batches_per_step=1, step_budget_ms=-1.0)
+    tree.set_rollup_limits(*limits)
+    o = OracleTree(meta.box_size, batches_per_step=1)
+    u = camera.poses(points, meta.box_size, 320, 180)["overview"]
+    fb = _device_fb(320, 180)
+    try:
+        for step in range(nb):
+            _, consumed = tree.build_step(ring.view())
+            tree.accumulate(consumed)
+            o.accumulate(o.build_step(slots, sizes, nb, batch, nb))
+            if step == nb // 2:
+                got, want = tree.deepen(4, 10_000), o.deepen(4, 10_000)
+                assert got["splits"] == want["splits"] > 0
+                tree.accumulate(consumed)
+                o.accumulate(consumed)
+            if step % 3 == 2:
+                tree.frame(u, fb.value)
+            else:
+                tree.analysis(u)
+            want = o.analyse(u)
+            print(f"step {step}: {len(want)} nodes", flush=True)
+            compare_rollup(tree, o, want)
+            got = tree.analysis_results()
+            kg, ko = keyed(tree.nodes()), keyed(o.nodes())
+            for k, ig in kg.items():
+                g, r = got[ig], want[ko[k]]["r"]
+                assert g["n"] == r["n"] and g["state"] == r["state"], (step, k)
+                assert (g["flags"] & 1) == (r["flags"] & 1), (step, k)
+                assert g["coverage"] == r["coverage"], (step, k)
+                np.testing.assert_allclose(g["lambda"], r["lambda"], rtol=0,
+                                           atol=1e-6 * float(want[ko[k]]["lambda64"].sum()) + 1e-30)
+    finally:
+        L.load().remo_dev_free(fb)
+        tree.close()
+        ring.close()
+        o.close()
 
 
 @pytest.mark.gpu

@@ -7,9 +7,10 @@ restating the oracle's formula:
     do): a two-level tree whose octants were last touched at known batches,
     at margins 0, 1, 2, 6, 7 and 8 -- an off-by-one in the comparison, a
     missing empty-leaf rule or an OR instead of an AND fails one of them;
-  * the FP64 fallback (taken when the smallest gap of the eigenvalues over the
-    trace is below 1e-4): a diagonal covariance with a gap just below and just
-    above 1e-4 of the trace, eigenvalues known in closed form;
+  * the FP32 eigen-solve at a near-degenerate pair: a diagonal covariance
+    whose two smaller eigenvalues differ by just under and just over 1e-4 of
+    the trace, eigenvalues known in closed form, planarity (a difference of
+    the pair) within two float ulps of the larger;
   * the store margin of the fixed-depth arm (1 GB instead of 200 MB,
     remolod_octree.cu:1191-1195): a capacity the iterative arm's margin would
     let run that the fixed arm's stops;
@@ -69,8 +70,8 @@ def test_closure_rule_by_hand(margin):
     assert (res[root]["r"]["state"] == 2) == exp["root"], margin
 
 
-@pytest.mark.parametrize("c,fallback", [(4300, True), (4200, False)])
-def test_fp64_fallback_threshold_by_hand(c, fallback):
+@pytest.mark.parametrize("c", [4300, 4200])
+def test_fp32_solve_near_degenerate_by_hand(c):
     a, b = 1000, 1001
     pts, meta = cross_case(a, b, c)
     slots, sizes, nb = batch_arrays(pts, len(pts))
@@ -83,16 +84,16 @@ def test_fp64_fallback_threshold_by_hand(c, fallback):
     r = res[0]
     T = a * a + b * b + c * c
     gap = (b * b - a * a) / T
-    assert (gap < 1e-4) == fallback  # the case straddles the threshold as intended
-    assert int(r["r"]["flags"] & 1) == int(fallback)
+    assert (gap < 1e-4) == (c == 4300)  # the pair straddles 1e-4 of the trace
+    assert int(r["r"]["flags"]) == 0
     # eigenvalues in node-extent units: (a^2, b^2, c^2) / (3 * 4^20)
     lam = np.array([a * a, b * b, c * c], dtype=np.float64) / (3.0 * 4.0 ** 20)
     np.testing.assert_allclose(r["lambda64"], lam, rtol=1e-12)
     np.testing.assert_allclose(r["r"]["lambda"], lam, rtol=2e-7)
     np.testing.assert_allclose(r["r"]["surface_variation"], a * a / T, rtol=1e-6)
-    # without the fallback the gap is a difference of FP32 eigenvalues (of the
-    # trace-normalised matrix): one float ulp of b^2/T each, over c^2/T
-    tol = 1e-12 if fallback else 2 * 2.0 ** -24 * (b * b) / (c * c)
+    # the pair's difference from FP32 eigenvalues of the trace-normalised matrix:
+    # one float ulp of b^2/T each, over c^2/T
+    tol = 2 * 2.0 ** -24 * (b * b) / (c * c)
     np.testing.assert_allclose(r["r"]["planarity"], (b * b - a * a) / (c * c), rtol=1e-6, atol=tol)
 
 
