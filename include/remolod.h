@@ -248,12 +248,12 @@ int remo_accum_fp32_atomics(remo_ctx* ctx, float* out, uint32_t cap, uint32_t* n
  * work queue. mode: REMO_ANALYSIS_ANALYTIC or REMO_ANALYSIS_LEARNED. */
 int remo_analysis(remo_ctx* ctx, const remo_shared_uniforms* u, int mode);
 
-/* Limits of the incremental roll-up (no reference counterpart; a tuning and test
- * knob): at most lds_slots marked nodes keep their sums in LDS (more: device
- * memory), and levels of at most wave_slots marked nodes run in one wave without
- * workgroup barriers. 0 restores a default (384, 64); larger values are clamped
- * to those. The results do not depend on the limits. */
-int remo_set_rollup_limits(remo_ctx* ctx, uint32_t lds_slots, uint32_t wave_slots);
+/* Limit of the incremental roll-up (no reference counterpart; a tuning and test
+ * knob): at most lds_slots marked nodes keep their sums in LDS, where each adds
+ * its partial sums to its marked ancestors at once; more take device memory and
+ * a pass per level. 0 restores the default (384); larger values are clamped to
+ * it. The results do not depend on the limit. */
+int remo_set_rollup_limits(remo_ctx* ctx, uint32_t lds_slots);
 
 /* ILodPipeline::render -> kernel_render selection + shared rasteriser
  * (kernels/remolod/remolod_render.cu:72-181, kernels/shared/remo_draw.cuh:118-167,

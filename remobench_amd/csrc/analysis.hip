@@ -242,8 +242,7 @@ struct AnalysisArgs {
     uint32_t* slotChild;
     uint32_t* slotNk;
     uint32_t* slotOf;
-    uint32_t ldsSlots;   // the roll-up's limits (at most kLdsSlots and 64)
-    uint32_t waveSlots;
+    uint32_t ldsSlots;   // marked nodes the roll-up keeps in LDS at most (<= kLdsSlots)
 };
 
 // Cyclic Jacobi on a symmetric 3x3 (a00 a01 a02 a11 a12 a22) in FP32 with the
@@ -531,25 +530,23 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
 // The incremental roll-up, one workgroup. The nodes on the dirty list and every
 // ancestor of them are marked (an LDS bitmap, so each is taken once) and given a
 // slot, by level. Then, for every slot at once, the eight children: a child that
-// is not marked adds its contribution to the slot's partial sums (a leaf whose
-// sums changed, folded by Accumulate or new from a split, from its accumulator,
-// with a new stamp; any other from its stored record), and a marked child leaves
-// its slot number. Then the slots level by level from the deepest: the partial
-// plus the marked children's slots. With at most kLdsSlots marked nodes the slots
-// live in LDS, so every load happens in the partial pass and the level passes
-// touch LDS only; when every level's slots also fit one wave, that wave runs the
-// levels alone (no workgroup barrier per level). Integer sums: the result equals
-// a full roll-up's. Last the slots' records are written and the shapes of every
-// node whose sums changed are solved, in one pass: a few hundred per step, which
-// in the node-parallel finalize would put an eigen-solve into nearly every wave
-// (a wave running the levels beside waves that solve shapes shares their SIMD
-// and took longer than both run one after the other). It also bumps the
-// Analysis stamp (the full roll-up, when it ran first, already wrote this stamp)
-// and clears the work queue for the finalize.
+// is not marked adds its sums to the slot's partial sums (a leaf whose sums
+// changed, folded by Accumulate or new from a split, from its accumulator, with
+// a new stamp; any other from its stored record). A slot's final sums are its
+// partial plus the partials of the marked nodes below it (the sums are plain
+// integer sums in the root frame); with at most kLdsSlots marked nodes the slots
+// live in LDS and each marked node adds its partial to its marked ancestors with
+// exact LDS atomics, all at once. Integer sums: the result equals a full
+// roll-up's. Last the slots' records are written and the shapes of every node
+// whose sums changed are solved, in one pass: a few hundred per step, which in
+// the node-parallel finalize would put an eigen-solve into nearly every wave. It
+// also bumps the Analysis stamp (the full roll-up, when it ran first, already
+// wrote this stamp) and clears the work queue for the finalize.
 constexpr uint32_t kRollThreads = 512;
 constexpr uint32_t kLevels = kMaxDepth + 1;
 constexpr uint32_t kLdsSlots = 384;
 static_assert(AnalysisState{}.rollLdsSlots == kLdsSlots, "default LDS slot limit");
+static_assert(kLdsSlots <= kRollThreads, "one LDS slot per thread when partials are added up");
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 __device__ __forceinline__ void addSums(RollupSums& P, const RollupSums& T) {
@@ -582,13 +579,41 @@ __device__ __forceinline__ void rollSlot(uint32_t sl, RollupSums* sSums, uint32_
     sClose[sl] = ca;
 }
 
-// The slot phases (placement, partials, levels, records and shapes) over slot
-// arrays in LDS or in device memory (the caller picks; inlined for each).
+// dst += v in LDS, exactly (modulo 2^128 per value) and safe against other lanes
+// adding to the same record: each value as 32-bit limbs, lowest first, a limb's
+// carry out taken from the value its atomic add found. The ten values (S1, S2, n)
+// go limb by limb together, so a record costs four rounds of LDS atomics.
+__device__ __forceinline__ void ldsAddSums(RollupSums* dst, const RollupSums& v) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(dst);
+    uint32_t x[38];
+    memcpy(x, &v, sizeof(x));  // s1: words 0..11, s2: 12..35, n: 36..37
+    uint32_t carry[10];
+    for (int j = 0; j < 10; ++j) carry[j] = 0;
+    for (int k = 0; k < 4; ++k) {
+        for (int j = 0; j < 10; ++j) {
+            if (j == 9 && k >= 2) continue;  // n: two limbs
+            const int word = j < 9 ? 4 * j + k : 36 + k;
+            const uint64_t add = uint64_t(x[word]) + carry[j];
+            const uint32_t lo = uint32_t(add);
+            const uint32_t old = atomicAdd(&w[word], lo);
+            carry[j] = uint32_t(add >> 32) + uint32_t((uint64_t(old) + lo) >> 32);
+        }
+    }
+}
+
+// The slot phases over slot arrays in LDS (kLds: up to kLdsSlots marked nodes)
+// or in device memory: placement by level, partials, the sums of marked
+// descendants, then the slots' records and the shapes. In LDS a slot's final sum
+// is its partial plus the partials of every marked node below it, so each marked
+// node adds its partial to each marked ancestor (exact LDS atomics, all nodes at
+// once: no pass per level); in device memory the levels go deepest first, a slot
+// adding its marked children's final sums, with a barrier per level.
+template <bool kLds>
 __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t stamp,
-                                          uint32_t M, bool oneWave, const uint32_t* markBits,
-                                          const uint32_t* changedBits, const uint32_t* count, const uint32_t* start,
-                                          uint32_t* cursor, uint32_t* leaves, RollupSums* sSums, uint32_t* sNode,
-                                          uint32_t* sFc, uint32_t* sClose, uint32_t* sChild, uint32_t* sNk) {
+                                          uint32_t M, const uint32_t* markBits, const uint32_t* changedBits,
+                                          const uint32_t* count, const uint32_t* start, uint32_t* cursor,
+                                          uint32_t* leaves, RollupSums* sSums, uint32_t* sNode, uint32_t* sFc,
+                                          uint32_t* sClose, uint32_t* sChild, uint32_t* sNk, uint32_t* sUp) {
     const uint32_t tid = threadIdx.x;
     // slots by level; a slot's first child carries the level in its top byte
     for (uint32_t k = tid; k < M; k += kRollThreads) {
@@ -598,6 +623,7 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         aa.slotOf[p] = pos;
         sNode[pos] = p;
         sFc[pos] = uint32_t(t.firstChild[p]) | (L << 24);
+        if constexpr (kLds) sUp[pos] = t.parent[p];  // made the parent's slot below
     }
     __syncthreads();
     // partials: 8 lanes per slot, one child each. Every lane loads what any of
@@ -613,6 +639,13 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         const RollupSums R = aa.rollup[child];
         const uint32_t caR = aa.closeAt[child];
         const LeafIn lf = loadLeaf(t, child);
+        uint32_t up = kNoSlot;
+        if constexpr (kLds) {
+            if (sub == 0) {
+                const uint32_t par = sUp[grp];
+                if (par != kNone) up = aa.slotOf[par];  // every ancestor of a marked node is marked
+            }
+        }
         RollupSums F;
         leafSumsOf(lf, D, F);
         const uint32_t caF = lf.count == 0 ? 0u : lf.last + aa.margin;
@@ -627,10 +660,12 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         for (int q = 0; q < 6; ++q) T.s2[q] = mk ? i128(0) : (ch ? F.s2[q] : R.s2[q]);
         T.n = mk ? 0ull : (ch ? F.n : R.n);
         uint32_t ca = mk ? 0u : (ch ? caF : caR);
-        const uint32_t cs = mk ? slotv : kNoSlot;
-        // the marked children's slots, packed to the front of the slot's links
-        const uint32_t gbits = uint32_t(__ballot(mk) >> (laneId() & ~7u)) & 0xFFu;
-        if (mk) sChild[grp * 8 + __popc(gbits & ((1u << sub) - 1u))] = cs;
+        uint32_t gbits = 0;
+        if constexpr (!kLds) {
+            // the marked children's slots, packed to the front of the slot's links
+            gbits = uint32_t(__ballot(mk) >> (laneId() & ~7u)) & 0xFFu;
+            if (mk) sChild[grp * 8 + __popc(gbits & ((1u << sub) - 1u))] = slotv;
+        }
         RollupSums P;
         for (int i = 0; i < 3; ++i) P.s1[i] = groupSum8(T.s1[i]);
         for (int q = 0; q < 6; ++q) P.s2[q] = groupSum8(T.s2[q]);
@@ -641,18 +676,28 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         if (sub == 0) {
             sSums[grp] = P;
             sClose[grp] = ca;
-            sNk[grp] = __popc(gbits);
+            if constexpr (kLds)
+                sUp[grp] = up;
+            else
+                sNk[grp] = __popc(gbits);
         }
     }
     __syncthreads();
-    if (oneWave) {
-        // levels, deepest first, by wave 0 alone: its LDS operations complete in
-        // order, so a fence between levels is enough
-        if (tid < 64) {
-            for (int L = int(D) - 1; L >= 0; --L) {
-                if (tid < count[L]) rollSlot(start[L] + tid, sSums, sClose, sChild, sNk);
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            }
+    if constexpr (kLds) {
+        // each marked node's partial into every marked ancestor (M <= kRollThreads:
+        // one slot per thread; the partials are read before any is added to)
+        const bool mine = tid < M;
+        RollupSums own;
+        uint32_t caOwn = 0, up = kNoSlot;
+        if (mine) {
+            own = sSums[tid];
+            caOwn = sClose[tid];
+            up = sUp[tid];
+        }
+        __syncthreads();
+        for (uint32_t a = up; a != kNoSlot; a = sUp[a]) {
+            ldsAddSums(sSums + a, own);
+            atomicMax(&sClose[a], caOwn);
         }
     } else {
         for (int L = int(D) - 1; L >= 0; --L) {
@@ -687,9 +732,9 @@ __global__ __launch_bounds__(kRollThreads) void kRollupDirty(DevTree t, Analysis
     __shared__ uint32_t markBits[kMarkBits / 32];     // marked (rolled-up) inner nodes
     __shared__ uint32_t changedBits[kMarkBits / 32];  // leaves whose sums changed
     __shared__ RollupSums lSums[kLdsSlots];
-    __shared__ uint32_t lNode[kLdsSlots], lFc[kLdsSlots], lClose[kLdsSlots], lChild[kLdsSlots * 8], lNk[kLdsSlots];
+    __shared__ uint32_t lNode[kLdsSlots], lFc[kLdsSlots], lClose[kLdsSlots], lUp[kLdsSlots];
     __shared__ uint32_t count[kLevels], start[kLevels], cursor[kLevels];
-    __shared__ uint32_t marked, stampS, leaves, widest;
+    __shared__ uint32_t marked, stampS, leaves;
     const uint32_t tid = threadIdx.x;
     Globals* g = t.g;
     const uint32_t D = g->maxLevel;
@@ -746,23 +791,21 @@ __global__ __launch_bounds__(kRollThreads) void kRollupDirty(DevTree t, Analysis
     __syncthreads();
     if (tid == 0) {
         *t.dirtyCount = 0;
-        uint32_t acc = 0, w = 0;
+        uint32_t acc = 0;
         for (uint32_t L = 0; L < kLevels; ++L) {
             start[L] = acc;
             cursor[L] = acc;
             acc += count[L];
-            w = max(w, count[L]);
         }
-        widest = w;
     }
     __syncthreads();
     const uint32_t M = marked;
     if (M <= min(aa.ldsSlots, kLdsSlots))  // uniform
-        rollSlots(t, aa, D, stamp, M, widest <= min(aa.waveSlots, 64u), markBits, changedBits, count, start, cursor, &leaves, lSums,
-                  lNode, lFc, lClose, lChild, lNk);
+        rollSlots<true>(t, aa, D, stamp, M, markBits, changedBits, count, start, cursor, &leaves, lSums, lNode, lFc,
+                        lClose, nullptr, nullptr, lUp);
     else
-        rollSlots(t, aa, D, stamp, M, false, markBits, changedBits, count, start, cursor, &leaves, aa.slotSums,
-                  aa.slotNode, aa.slotFc, aa.slotClose, aa.slotChild, aa.slotNk);
+        rollSlots<false>(t, aa, D, stamp, M, markBits, changedBits, count, start, cursor, &leaves, aa.slotSums,
+                         aa.slotNode, aa.slotFc, aa.slotClose, aa.slotChild, aa.slotNk, nullptr);
 }
 
 // Appends the wave's scored nodes to the work queue (one atomic per wave).
@@ -1226,7 +1269,7 @@ AnalysisArgs argsOf(const AnalysisState& a) {
     return AnalysisArgs{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin,
                         a.closeAt, a.rollStamp, a.shapeStamp, a.shapes, a.markList, a.stamp,
                         a.slotSums, a.slotNode, a.slotFc, a.slotClose, a.slotChild, a.slotNk, a.slotOf,
-                        a.rollLdsSlots, a.rollWaveSlots};
+                        a.rollLdsSlots};
 }
 
 // Whether this Analysis must roll up every node: the first one after a reset,
@@ -1328,7 +1371,7 @@ int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, 
     // the graph holds the incremental roll-up and the finalize pass; a full
     // roll-up, when one is due, runs eagerly before it
     if (takeFull(a, t, maxLevel)) launchFullRollup(a, t, maxLevel, s);
-    const uint64_t key[6] = {(uint64_t(a.rollLdsSlots) << 32) | a.rollWaveSlots, uint64_t(mode),
+    const uint64_t key[6] = {a.rollLdsSlots, uint64_t(mode),
                              reinterpret_cast<uint64_t>(fb), pixels,
                              (uint64_t(a.shape.nodeBlocks) << 32) | a.shape.rasterBlocks, a.shape.rasterTile};
     a.arrival = arrival;

@@ -851,12 +851,11 @@ int remo_accum_fp32_atomics(remo_ctx* c, float* out, uint32_t cap, uint32_t* num
     return REMO_OK;
 }
 
-int remo_set_rollup_limits(remo_ctx* c, uint32_t ldsSlots, uint32_t waveSlots) {
+int remo_set_rollup_limits(remo_ctx* c, uint32_t ldsSlots) {
     if (!c) return REMO_ERR_INVALID;
     if (c->dead) return REMO_ERR_DEVICE;
     const AnalysisState defaults{};
     c->analysis.rollLdsSlots = ldsSlots == 0 ? defaults.rollLdsSlots : std::min(ldsSlots, defaults.rollLdsSlots);
-    c->analysis.rollWaveSlots = waveSlots == 0 ? defaults.rollWaveSlots : std::min(waveSlots, defaults.rollWaveSlots);
     return REMO_OK;
 }
 

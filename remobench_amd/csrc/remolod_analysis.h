@@ -96,7 +96,6 @@ struct AnalysisState {
     uint32_t* stamp = nullptr;        // [1] Analysis counter (device; never reset)
     bool needFull = true;             // the next Analysis rolls up every node
     uint32_t rollLdsSlots = 384;      // incremental roll-up: marked nodes kept in LDS at most
-    uint32_t rollWaveSlots = 64;      // incremental roll-up: widest level run by one wave
     uint32_t lastD = 0;               // tree depth of the last roll-up
     uint32_t closureMargin = 8;       // batches untouched before a leaf is closed
     float arrival = 0.0f;             // fraction of the cloud's batches ingested (last Analysis)
@@ -141,7 +140,7 @@ int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u,
 // Analysis + Rasterize as one captured graph (the incremental roll-up, the
 // finalize and the Rasterize kernels; a full roll-up, when one is due, runs
 // eagerly before it), re-captured when the mode, the framebuffer, its size, the
-// launch shape or the roll-up limits change.
+// launch shape or the roll-up's LDS limit change.
 int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
              uint32_t numNodes, float arrival, uint64_t* fb, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);

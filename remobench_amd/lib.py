@@ -245,7 +245,7 @@ def load() -> C.CDLL:
         "remo_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                           P(C.c_float)]),
         "remo_set_score_model": (C.c_int, [C.c_void_p, C.c_void_p]),
-        "remo_set_rollup_limits": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+        "remo_set_rollup_limits": (C.c_int, [C.c_void_p, C.c_uint32]),
         "remo_accum_fp32_atomics": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_frame": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int, C.c_void_p]),
         "remo_autotune_frame": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int, C.c_void_p, C.c_void_p,

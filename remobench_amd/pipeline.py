@@ -339,10 +339,10 @@ class Tree:
         finally:
             buf.free()
 
-    def set_rollup_limits(self, lds_slots: int = 0, wave_slots: int = 0) -> None:
-        """Limits of the incremental roll-up (0 = default); the results do not
-        depend on them, smaller ones take its device-memory and barrier paths."""
-        self._ck(self.lib.remo_set_rollup_limits(self.h, int(lds_slots), int(wave_slots)), "set_rollup_limits")
+    def set_rollup_limits(self, lds_slots: int = 0) -> None:
+        """The incremental roll-up's LDS limit (0 = default); the results do not
+        depend on it, a smaller one takes its device-memory path more often."""
+        self._ck(self.lib.remo_set_rollup_limits(self.h, int(lds_slots)), "set_rollup_limits")
 
     def set_score_model(self, mlp) -> None:
         """Loads a trained score model (remobench_amd.score_model.ScoreMLP)."""

@@ -74,15 +74,15 @@ def test_rollup_bit_exact(built):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("limits", [(0, 0), (384, 1), (8, 1)], ids=["lds-one-wave", "lds-barriers", "device-slots"])
-def test_incremental_analysis_every_step(gpu, limits):
+@pytest.mark.parametrize("lds_slots", [0, 8], ids=["lds-slots", "device-slots"])
+def test_incremental_analysis_every_step(gpu, lds_slots):
     """Analysis after every 1-batch step rolls up only what changed (the leaves
     Accumulate folded, the nodes that split, their ancestors) and fully when the
     tree got deeper; a deepening midway and steps through the captured frame graph
     included. At every step every node's int128 sums, count, closure and results
-    equal the oracle's full analysis of the same tree. The limits take the roll-up
-    through its three paths: slots in LDS with the levels in one wave, in LDS with
-    a barrier per level, and in device memory."""
+    equal the oracle's full analysis of the same tree. The LDS limit takes the
+    roll-up through both paths: slots in LDS (marked nodes add their partial sums
+    to their ancestors with LDS atomics) and in device memory (a pass per level)."""
     from .helpers import batch_arrays
     from .test_gpu_pins import compare_rollup
     points, meta = lidar_scene(2_097_152, seed=3)
@@ -92,7 +92,7 @@ def test_incremental_analysis_every_step(gpu, limits):
     ring.set_cloud(points, batch)
     ring.pump()
     tree = Tree(meta, slot_capacity=batch, ring_slots=nb, batches_per_step=1, step_budget_ms=-1.0)
-    tree.set_rollup_limits(*limits)
+    tree.set_rollup_limits(lds_slots)
     o = OracleTree(meta.box_size, batches_per_step=1)
     u = camera.poses(points, meta.box_size, 320, 180)["overview"]
     fb = _device_fb(320, 180)
