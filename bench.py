@@ -56,7 +56,10 @@ def algorithmic_bytes(scope: str, t: dict) -> float:
     if scope == "remolod.accumulate":
         return 16.0 * t["folded"] + 192.0 * t["nodes"] * t["steps"]
     if scope == "remolod.analysis":
-        return 400.0 * t["nodes"] * t["steps"]
+        # the finalize of every node: its cached shape, closing batch and coordinates
+        # in (56 B), its result, closure flag and accumulator state out (57 B); the
+        # incremental roll-up adds some 0.1 MB a step (a few hundred changed records)
+        return 113.0 * t["nodes"] * t["steps"]
     return 0.0
 
 
@@ -98,7 +101,8 @@ SCOPE_KERNELS = {
     "remolod.construct.insertPoints": ["kInsertPoints"],
     "remolod.construct.insertVoxels": ["kInsertVoxels"],
     "remolod.accumulate": ["kAccumPlan", "kAccumTiles", "kAccumFinish"],
-    "remolod.analysis": ["kAnalysisLeaves", "kAnalysisLevel", "kFinalize", "kSetFrame"],
+    "remolod.analysis": ["kRollupDirty", "kFinalize", "kAnalysisLeaves", "kAnalysisLevel", "kAnalysisLevel2",
+                         "kShapes", "kSetFrame"],
 }
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_lidar100M.txt"
 
