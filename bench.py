@@ -331,6 +331,31 @@ def main() -> int:
                    "analysis_speedup": round((ms_a / max(1, n_a)) / max(1e-9, ms_l / max(1, n_l)), 3),
                    "model": str(DEFAULT_MODEL.relative_to(ROOT))}
         log(f"learned score: {learned}")
+    # The same steps in the sequence the C++ plugin's build runs per frame at one
+    # batch per step (RemolodHipPipeline::build, host/RemolodHipPipeline.cpp:110-170:
+    # build step, Accumulate, Analysis, then readStats -> remo_get_stats, whose
+    # per-node stats sweep the C-ABI step leaves out)
+    plugin_path = None
+    if analysis_ok and analysis_mode == L.REMO_ANALYSIS_ANALYTIC:
+        import ctypes as C
+        raw = L.Stats()
+
+        def plugin_step():
+            step()
+            tree._ck(tree.lib.remo_get_stats(tree.h, C.byref(raw)), "stats")
+
+        tree.reset()
+        tree.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plugin_step()
+        tree.sync()
+        dt_plugin = time.perf_counter() - t0
+        plugin_path = {"ms_per_step": round(dt_plugin * 1e3 / steps, 4),
+                       "vs_c_abi_step": round(dt_plugin / dt, 4),
+                       "sequence": "remo_build_step, remo_accumulate, remo_analysis, remo_get_stats "
+                                   "(RemolodHipPipeline::build with readStats)"}
+        log(f"plugin build path: {plugin_path}")
     ingested = st["num_points_processed"]
     value = ingested / dt / 1e6
     log(f"timed {steps} steps: {dt * 1e3:.1f} ms, {value:.1f} M pts/s ({ingested / dt_all / 1e6:.1f} with "
@@ -642,6 +667,7 @@ def main() -> int:
         "scopes_hbm_gbs": round(scopes_gbs, 2),
         "scopes_hbm_frac": round(scopes_gbs / HBM_PEAK_GBS, 5),
         "atomics": atomics,
+        "plugin_build_path": plugin_path,
         "scopes": scopes,
         "rasterize": raster,
         "refinement": refinement,

@@ -1411,15 +1411,23 @@ void analysisRelease(AnalysisState& a) {
     a.frameExec = nullptr;
 }
 
+void analysisQueueDiagnostics(const AnalysisState& a, uint32_t* host, hipStream_t s) {
+    (void)hipMemcpyAsync(host, a.drawCounters, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+}
+
+void analysisTakeDiagnostics(AnalysisState& a, const uint32_t* host) {
+    a.drawItems = host[0];
+    a.drawListOverflow = host[1] | host[5];
+    uint64_t samples;
+    std::memcpy(&samples, host + 2, 8);
+    a.drawSamples = samples;
+}
+
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s) {
     uint32_t c[6] = {0, 0, 0, 0, 0, 0};
     if (hipMemcpyAsync(c, a.drawCounters, sizeof(c), hipMemcpyDeviceToHost, s) != hipSuccess) return;
     if (hipStreamSynchronize(s) != hipSuccess) return;
-    a.drawItems = c[0];
-    a.drawListOverflow = c[1] | c[5];
-    uint64_t samples;
-    std::memcpy(&samples, c + 2, 8);
-    a.drawSamples = samples;
+    analysisTakeDiagnostics(a, c);
 }
 
 }  // namespace remo_hip

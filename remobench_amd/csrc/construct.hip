@@ -1206,7 +1206,12 @@ __device__ __forceinline__ void statsBlockReduce(StatsAcc& a) {
     }
 }
 
+// The per-node stats sweep in one launch: every workgroup reduces its nodes in
+// LDS and writes one partial record; the workgroup that finishes last (a count
+// on Globals, the partials released before it and acquired after) adds the
+// records up into Globals and clears the count.
 __global__ __launch_bounds__(kBlock) void kStatsSweep(DevTree t) {
+    __shared__ bool last;
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     uint32_t inner = 0, leaves = 0, nonempty = 0, maxPts = 0, chunksP = 0, chunksV = 0, under = 0,
@@ -1245,38 +1250,39 @@ __global__ __launch_bounds__(kBlock) void kStatsSweep(DevTree t) {
         p[9] = uint32_t(a.vox >> 32);
         p[10] = a.maxPts;
         p[11] = a.maxDepth;
+        __threadfence();  // the partial before the count
+        last = atomicAdd(&g->statsDone, 1u) == gridDim.x - 1;
     }
-}
-
-// One workgroup: the partial records of `blocks` sweep workgroups into Globals.
-__global__ __launch_bounds__(kBlock) void kStatsFinish(DevTree t, uint32_t blocks) {
-    StatsAcc a{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+    __syncthreads();
+    if (!last) return;  // uniform over the workgroup
+    __threadfence();    // every other workgroup's partial after the count
+    StatsAcc r{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
         const uint32_t* p = t.statsPartials + uint64_t(b) * kStatsWords;
-        a.inner += p[0];
-        a.leaves += p[1];
-        a.nonempty += p[2];
-        a.chunksP += p[3];
-        a.chunksV += p[4];
-        a.under += p[5];
-        a.pts += uint64_t(p[6]) | (uint64_t(p[7]) << 32);
-        a.vox += uint64_t(p[8]) | (uint64_t(p[9]) << 32);
-        a.maxPts = max(a.maxPts, p[10]);
-        a.maxDepth = max(a.maxDepth, p[11]);
+        r.inner += p[0];
+        r.leaves += p[1];
+        r.nonempty += p[2];
+        r.chunksP += p[3];
+        r.chunksV += p[4];
+        r.under += p[5];
+        r.pts += uint64_t(p[6]) | (uint64_t(p[7]) << 32);
+        r.vox += uint64_t(p[8]) | (uint64_t(p[9]) << 32);
+        r.maxPts = max(r.maxPts, p[10]);
+        r.maxDepth = max(r.maxDepth, p[11]);
     }
-    statsBlockReduce(a);
+    statsBlockReduce(r);
     if (threadIdx.x == 0) {
-        Globals* g = t.g;
-        g->sInner = a.inner;
-        g->sLeaves = a.leaves;
-        g->sNonempty = a.nonempty;
-        g->sChunksPoints = a.chunksP;
-        g->sChunksVoxels = a.chunksV;
-        g->sUnderflows = a.under;
-        g->sPoints = a.pts;
-        g->sVoxels = a.vox;
-        g->sMaxPoints = a.maxPts;
-        g->sMaxDepth = a.maxDepth;
+        g->sInner = r.inner;
+        g->sLeaves = r.leaves;
+        g->sNonempty = r.nonempty;
+        g->sChunksPoints = r.chunksP;
+        g->sChunksVoxels = r.chunksV;
+        g->sUnderflows = r.under;
+        g->sPoints = r.pts;
+        g->sVoxels = r.vox;
+        g->sMaxPoints = r.maxPts;
+        g->sMaxDepth = r.maxDepth;
+        g->statsDone = 0;
     }
 }
 
@@ -1415,7 +1421,6 @@ void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, con
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
     const uint32_t blocks = std::min(ls.nodeBlocks, kStatsBlocks);
     hipLaunchKernelGGL(kStatsSweep, dim3(blocks), dim3(kBlock), 0, s, t);
-    hipLaunchKernelGGL(kStatsFinish, dim3(1), dim3(kBlock), 0, s, t, blocks);
 }
 
 }  // namespace remo_hip

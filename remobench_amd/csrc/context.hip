@@ -987,9 +987,12 @@ int remo_get_stats(remo_ctx* c, remo_stats* out) {
     if (c->dead) return REMO_ERR_DEVICE;
     launchStatsSweep(c->t, c->shape, c->stream);
     CK(hipGetLastError(), "stats sweep");
-    const int rc = readGlobals(c);
-    if (rc != REMO_OK) return rc;
-    analysisReadDiagnostics(c->analysis, c->stream);
+    // the counters and the draw diagnostics in one synchronisation
+    CK(hipMemcpyAsync(c->hostG, c->t.g, sizeof(Globals), hipMemcpyDeviceToHost, c->stream),
+       "reading device counters");
+    analysisQueueDiagnostics(c->analysis, c->hostScratch + 8, c->stream);
+    CK(hipStreamSynchronize(c->stream), "device work");
+    analysisTakeDiagnostics(c->analysis, c->hostScratch + 8);
     fillStats(c, out);
     return REMO_OK;
 }

@@ -28,6 +28,7 @@
 // the cell centre kVoxelSample computes, with colour (sum + n/2) / n per channel.
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "remolod_refine.h"
@@ -673,6 +674,7 @@ int refineGrow(RefineState& r, const DevTree& t, uint64_t entries, hipStream_t s
 int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetMs, uint32_t maxLevels,
               const LaunchShape& ls, hipStream_t s, RefineStatsOut* out, std::string* err) {
     std::memset(out, 0, sizeof(*out));
+    const auto wall0 = std::chrono::steady_clock::now();
     const RefineArgs ra = argsOf(r);
     const uint32_t nb = ls.nodeBlocks;
     hipLaunchKernelGGL(kRefineClear, dim3(nb), dim3(kBlock), 0, s, ra, numNodes);
@@ -707,7 +709,12 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
     }
     // Levels from the deepest while the estimate fits. The last level taken may be
     // cut to the nodes that fit at its average node cost (nodes of one level do
-    // not depend on each other; the rest stay dirty); at least one node runs.
+    // not depend on each other; the rest stay dirty); at least one node runs. The
+    // kernels get the budget less the call's own cost (the marking kernels, two
+    // readbacks, the launches: a decaying peak of what recent calls measured),
+    // with 15 % kept back for the model's error, so the call's wall time holds it.
+    const double kernelBudget =
+        budgetMs > 0.0f ? std::max(0.0, (double(budgetMs) - r.overheadMs) * 0.85) : double(budgetMs);
     double est = 0.0;
     int lo = deepest;
     uint32_t taken = 0, lastCount = 0;
@@ -717,8 +724,8 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
         // the fitted model, corrected by what recent calls measured against it
         const double fixed = kLevelUs * 1e-3 * r.modelScale, nodes = nodesCostUs(c, uint32_t(L)) * 1e-3 * r.modelScale;
         uint32_t k = c.levelCount[L];
-        if (budgetMs > 0.0f && est + fixed + nodes > double(budgetMs)) {
-            const double room = double(budgetMs) - est - fixed;
+        if (budgetMs > 0.0f && est + fixed + nodes > kernelBudget) {
+            const double room = kernelBudget - est - fixed;
             const double per = nodes / double(c.levelCount[L]);
             k = room > 0.0 ? uint32_t(std::min<double>(double(c.levelCount[L]), room / per)) : 0u;
             if (k == 0 && taken > 0) break;
@@ -788,6 +795,9 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
         r.modelScale = std::min(8.0, std::max(0.25, 0.7 * r.modelScale + 0.3 * ratio));
     }
     out->deviceMs = ms;
+    const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    const double over = std::max(0.0, wall - double(ms));
+    r.overheadMs = std::max(over, 0.97 * r.overheadMs + 0.03 * over);
     return 0;
 }
 

@@ -144,6 +144,10 @@ int runRender(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u,
 int runFrame(AnalysisState& a, const DevTree& t, const remo_shared_uniforms& u, int mode, uint32_t maxLevel,
              uint32_t numNodes, float arrival, uint64_t* fb, hipStream_t s, std::string* err);
 void analysisReadDiagnostics(AnalysisState& a, hipStream_t s);
+// The same in two halves: queue the copy of the draw counters into pinned host
+// memory (6 words), then, after the caller's synchronisation, take them.
+void analysisQueueDiagnostics(const AnalysisState& a, uint32_t* host, hipStream_t s);
+void analysisTakeDiagnostics(AnalysisState& a, const uint32_t* host);
 // The first n roll-up records, each in its own node's frame, into `out` (device).
 void launchCentredRollups(const AnalysisState& a, const DevTree& t, RollupSums* out, uint32_t n, hipStream_t s);
 // The flat (LOD-free) Rasterize of n resident points, and eye-dome lighting.

@@ -83,7 +83,8 @@ struct Globals {
     uint64_t totBatches, totExpandIters, totNodesSplit, totSpilledPoints, totNewVoxels;
     // AccumGlobals (include/remo/RemoAccum.h:33-40)
     uint64_t mortonWatermark, numAccumulated, sumLeafCounts;
-    uint32_t pointsFolded, leavesFolded, innerWithSums, pad2;
+    uint32_t pointsFolded, leavesFolded, innerWithSums;
+    uint32_t statsDone;            // stats-sweep workgroups finished (the last reduces, then clears it)
     // stats sweep
     uint32_t sInner, sLeaves, sNonempty, sMaxPoints;
     uint64_t sPoints, sVoxels;

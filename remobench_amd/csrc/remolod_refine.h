@@ -53,6 +53,7 @@ struct RefineState {
     RefineCounters* host = nullptr;  // pinned mirror
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double modelScale = 1.0;         // measured / estimated device time of recent calls (running)
+    double overheadMs = 0.3;         // a call's wall time beyond its timed kernels (a decaying peak)
 };
 
 using RefineAllocFn = std::function<int(void**, uint64_t, const char*)>;
