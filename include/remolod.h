@@ -132,7 +132,9 @@ typedef struct remo_config {
  * (include/remo/HostDeviceCommon.h:102-166) + AccumGlobals
  * (include/remo/RemoAccum.h:33-40) + DeviceDiagnostics. */
 typedef struct remo_stats {
-    uint32_t num_nodes, num_inner, num_leaves, num_nonempty_leaves;
+    uint32_t num_nodes;             /* nodes in the tree: the pool's high-water mark less the
+                                       slots of blocks pooled by remo_collapse            */
+    uint32_t num_inner, num_leaves, num_nonempty_leaves;
     uint64_t num_points;            /* sum of leaf numPoints      */
     uint64_t num_voxels;            /* sum of inner-node numVoxels */
     uint64_t num_points_processed;
@@ -144,14 +146,16 @@ typedef struct remo_stats {
     uint64_t num_allocated_chunks, chunk_pool_size;
     uint64_t allocated_bytes_persistent, persistent_capacity;
     uint32_t mem_capacity_reached, node_capacity_reached, node_pool_overflow, alloc_overflow;
-    uint32_t spill_overflow, backlog_overflow, dir_overflow, pad0;
+    uint32_t spill_overflow, backlog_overflow, dir_overflow;
+    uint32_t free_node_blocks;      /* 8-node blocks freed by remo_collapse, taken first by splits */
     /* per-step counters (DeviceTimeline counter block semantics, summed over launches) */
     uint64_t batches, expand_iters, nodes_split, spilled_points, new_voxels;
     /* the same five, cumulative since the last reset */
     uint64_t total_batches, total_expand_iters, total_nodes_split, total_spilled_points, total_new_voxels;
     /* AccumGlobals */
     uint64_t morton_watermark, num_accumulated, sum_leaf_counts;
-    uint32_t points_folded, leaves_folded, inner_with_sums, pad1;
+    uint32_t points_folded, leaves_folded, inner_with_sums;
+    uint32_t pooled_grids;          /* grids freed by remo_collapse, taken first by splits */
     /* render diagnostics (DeviceDiagnostics) */
     uint32_t draw_items, draw_list_overflow;
     uint64_t draw_samples;
@@ -165,7 +169,9 @@ typedef struct remo_node_record {
                                        child i = (cx<<2)|(cy<<1)|cz */
     uint32_t counter, num_points, num_voxels, num_voxels_stored;
     int32_t grid;                   /* occupancy-grid number, -1: none */
-    uint32_t pad0, pad1;
+    uint32_t free_slot;             /* 1: a slot of a block remo_collapse pooled: not in the
+                                       tree (its other fields are zero or stale)        */
+    uint32_t pad1;
 } remo_node_record;
 
 /* Per-node Accumulate record. g/c/count/last_touched_batch/state/score have the
@@ -399,6 +405,31 @@ typedef struct remo_deepen_stats {
 } remo_deepen_stats;
 
 int remo_deepen(remo_ctx* ctx, uint32_t max_splits, uint32_t min_points, remo_deepen_stats* out);
+
+/* Refinement's collapsing (README.md:81 and wiki/02_KernelPipeline.md:238: "merge
+ * a node back up", which "needs a grid pool first"; no reference code). Up to
+ * max_collapses inner nodes below the root whose eight children are leaves with
+ * every counted point stored, holding at most min(max_points, 50,000) points
+ * together, are merged into one leaf each, the sparsest first (ties by level,
+ * x, y, z); the chosen are a prefix of that order whose points fit the spill list
+ * they move through (spill_capacity). A merged node takes its children's points
+ * (its chunks from the chunk queue, as in Update's insertion), returns its
+ * children's point chunks to that queue, its occupancy grid to the grid pool and
+ * the grid's voxel chunks to the chunk queue, and drops its voxels (a leaf draws
+ * its points); its children's 8-node block goes to the node-block pool. Splits
+ * (Update, the fixed-depth arm, remo_deepen) take pooled blocks and grids before
+ * they grow the node pool or the persistent store, so a collapsed region grows
+ * back without new memory. Accumulate folds a merged leaf from scratch; the next
+ * Analysis rolls up every node. Synchronous. */
+typedef struct remo_collapse_stats {
+    uint32_t candidates;        /* inner nodes that qualified                   */
+    uint32_t collapsed;         /* merged (a prefix of the sparsest, <= max_collapses) */
+    uint64_t points_moved;      /* stored points moved into the merged leaves   */
+    uint32_t grids_pooled;      /* grids returned to the pool                   */
+    uint32_t voxels_dropped;    /* voxels the merged nodes held                 */
+} remo_collapse_stats;
+
+int remo_collapse(remo_ctx* ctx, uint32_t max_collapses, uint32_t max_points, remo_collapse_stats* out);
 
 /* remo_deepen for callers that cannot allocate device memory (code under graph
  * capture, serving loops, harnesses that forbid hipMalloc); no counterpart in

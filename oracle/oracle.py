@@ -62,6 +62,8 @@ def load():
         "oracle_refine": (C.c_uint32, [C.c_void_p, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
         "oracle_deepen": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, P(C.c_uint32), P(C.c_uint32),
                                  P(C.c_uint64), P(C.c_uint64)]),
+        "oracle_collapse": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64, P(C.c_uint32), P(C.c_uint32),
+                                   P(C.c_uint64)]),
         "oracle_render_flat": (None, [C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
         "oracle_apply_edl": (None, [P(SharedUniforms), C.c_void_p]),
     }
@@ -166,7 +168,19 @@ def _deepen(self, max_splits: int, min_points: int = 25_000, point_budget: int =
     return {"candidates": c.value, "splits": sp.value, "points_moved": mv.value, "new_voxels": vx.value}
 
 
+def _collapse(self, max_collapses: int, max_points: int = 50_000, point_budget: int = 10_000_000) -> dict:
+    """Refinement's collapsing (the HIP path's choice and merge); point_budget:
+    the merged nodes' points fit the spill list (the HIP path's default
+    capacity is 10,000,000)."""
+    c, n = C.c_uint32(0), C.c_uint32(0)
+    mv = C.c_uint64(0)
+    self.lib.oracle_collapse(self.h, int(max_collapses), int(max_points), int(point_budget), C.byref(c),
+                             C.byref(n), C.byref(mv))
+    return {"candidates": c.value, "collapsed": n.value, "points_moved": mv.value}
+
+
 OracleTree.analyse = _analyse
+OracleTree.collapse = _collapse
 OracleTree.deepen = _deepen
 OracleTree.refine = _refine
 OracleTree.render = _render

@@ -84,7 +84,7 @@ struct ONode {
     uint32_t countIteration = 0;
     int32_t grid = -1;
     uint32_t numVoxels = 0, numVoxelsStored = 0;
-    uint32_t voxelChunks = 0;  // length of the voxel chunk list
+    bool dead = false;         // a pooled slot of a block freed by a collapse: not in the tree
     std::vector<remo_point> points;
     std::vector<remo_point> voxels;
     // Accumulate side array (include/remo/RemoAccum.h:18-30) plus the fixed point.
@@ -122,6 +122,13 @@ struct oracle_tree {
 
     std::vector<ONode> nodes;
     std::vector<std::vector<uint32_t>> grids;
+    std::vector<uint32_t> gridChunks;  // voxel chunks held by each grid (the chunk list stays with the grid)
+    // Refinement's collapse returns what it frees to pools the splits draw from
+    // first: 8-node child blocks (by first index) and grids.
+    std::vector<uint32_t> freeBlocks;
+    std::vector<int32_t> freeGrids;
+    uint32_t maxLevel = 0;             // deepest level a split has created since reset
+    uint64_t totCollapsed = 0;
 
     // Stats (kernels/simlod/HostDeviceInterface.h:51-75)
     uint32_t batchletIndex = 0;
@@ -150,6 +157,11 @@ struct oracle_tree {
     void reset() {
         nodes.clear();
         grids.clear();
+        gridChunks.clear();
+        freeBlocks.clear();
+        freeGrids.clear();
+        maxLevel = 0;
+        totCollapsed = 0;
         nodes.reserve(std::min<uint32_t>(maxNodes, 1u << 20));
         persistentOffset = 16;
         batchletIndex = 0;
@@ -168,10 +180,42 @@ struct oracle_tree {
         std::memset(&lastStats, 0, sizeof(lastStats));
     }
 
+    // A pooled grid first (no store bytes: its memory is already counted), then
+    // a new one from the store (remolod_octree.cu:365-370).
     int32_t allocGrid() {
+        if (!freeGrids.empty()) {
+            const int32_t g = freeGrids.back();
+            freeGrids.pop_back();
+            return g;
+        }
         persistentOffset += kGridAllocBytes;
         grids.emplace_back(kGridWords, 0u);
+        gridChunks.push_back(0);
         return int32_t(grids.size() - 1);
+    }
+
+    // The node pool's acceptance of split candidates, given in canonical key
+    // order (remolod_octree.cu:309-321, where the pool accepts a prefix of the
+    // atomic order): pooled 8-node blocks first, then the unused pool while a
+    // block of 8 fits below maxNodes; the rest are refused and flagged.
+    void acceptByNodePool(std::vector<uint32_t>& cands) {
+        std::vector<uint32_t> accepted;
+        size_t pooled = freeBlocks.size();
+        uint64_t room = nodes.size();
+        for (uint32_t id : cands) {
+            if (pooled > 0) {
+                --pooled;
+                accepted.push_back(id);
+                continue;
+            }
+            if (room + 8 > maxNodes) {
+                nodePoolOverflow = true;
+                continue;
+            }
+            room += 8;
+            accepted.push_back(id);
+        }
+        cands.swap(accepted);
     }
 
     // Root -> leaf descent (remolod_octree.cu:182-199); stops at a node without
@@ -215,18 +259,10 @@ struct oracle_tree {
             const ONode &na = nodes[a], &nb = nodes[b];
             return Key{na.level, na.X, na.Y, na.Z} < Key{nb.level, nb.X, nb.Y, nb.Z};
         });
-        std::vector<uint32_t> accepted;
-        uint64_t room = nodes.size();
-        for (uint32_t id : spilling) {
-            if (nodes[id].level >= uint32_t(kMaxDepth)) continue;
-            if (room + 8 > maxNodes) {
-                nodePoolOverflow = true;
-                continue;
-            }
-            room += 8;
-            accepted.push_back(id);
-        }
-        spilling.swap(accepted);
+        spilling.erase(std::remove_if(spilling.begin(), spilling.end(),
+                                      [&](uint32_t id) { return nodes[id].level >= uint32_t(kMaxDepth); }),
+                       spilling.end());
+        acceptByNodePool(spilling);
 
         // spill loop (:250-285): copy each spilling leaf's stored points
         for (uint32_t id : spilling) {
@@ -241,7 +277,13 @@ struct oracle_tree {
     // doSplitting (remolod_octree.cu:303-400)
     void doSplitting() {
         for (uint32_t id : spilling) {
-            const uint32_t childOffset = uint32_t(nodes.size());
+            // a pooled block first (the last one pooled), else the end of the pool
+            uint32_t childOffset = uint32_t(nodes.size());
+            const bool pooled = !freeBlocks.empty();
+            if (pooled) {
+                childOffset = freeBlocks.back();
+                freeBlocks.pop_back();
+            }
             for (int i = 0; i < 8; ++i) {
                 ONode child;
                 const ONode& parent = nodes[id];
@@ -249,8 +291,12 @@ struct oracle_tree {
                 child.X = 2 * parent.X + ((i >> 2) & 1);
                 child.Y = 2 * parent.Y + ((i >> 1) & 1);
                 child.Z = 2 * parent.Z + ((i >> 0) & 1);
-                nodes.push_back(child);
+                if (pooled)
+                    nodes[childOffset + i] = child;
+                else
+                    nodes.push_back(child);
             }
+            maxLevel = std::max(maxLevel, nodes[id].level + 1);
             ONode& sn = nodes[id];
             for (int i = 0; i < 8; ++i) sn.children[i] = int32_t(childOffset + i);
             // chunks back to the queue (:349-360)
@@ -336,7 +382,7 @@ struct oracle_tree {
             spilling.clear();
             for (uint32_t id = 0; id < nodes.size(); ++id) {
                 const ONode& nd = nodes[id];
-                if (!nd.isLeaf() || nd.level != L || nd.numPoints != 0) continue;
+                if (nd.dead || !nd.isLeaf() || nd.level != L || nd.numPoints != 0) continue;
                 if (cells[L][fixedMorton(nd.X, nd.Y, nd.Z, L)] == 0) continue;
                 spilling.push_back(id);
             }
@@ -344,22 +390,12 @@ struct oracle_tree {
                 const ONode &na = nodes[a], &nb = nodes[b];
                 return Key{na.level, na.X, na.Y, na.Z} < Key{nb.level, nb.X, nb.Y, nb.Z};
             });
-            std::vector<uint32_t> accepted;
-            uint64_t room = nodes.size();
-            for (uint32_t id : spilling) {
-                if (room + 8 > maxNodes) {
-                    nodePoolOverflow = true;
-                    continue;
-                }
-                room += 8;
-                accepted.push_back(id);
-            }
-            spilling.swap(accepted);
+            acceptByNodePool(spilling);
             if (!spilling.empty()) doSplitting();
         }
         spilling.clear();
         for (ONode& nd : nodes) {
-            if (!nd.isLeaf() || nd.level > D) continue;
+            if (nd.dead || !nd.isLeaf() || nd.level > D) continue;
             nd.counter += cells[nd.level][fixedMorton(nd.X, nd.Y, nd.Z, nd.level)];
         }
         ++expandIters;
@@ -426,10 +462,11 @@ struct oracle_tree {
     void allocateVoxelChunks() {
         for (ONode& nd : nodes) {
             const uint32_t req = (nd.numVoxels + kPointsPerChunk - 1) / kPointsPerChunk;
-            if (req == 0) continue;
-            while (nd.voxelChunks < req) {
+            if (req == 0 || nd.grid < 0) continue;
+            uint32_t& have = gridChunks[size_t(nd.grid)];
+            while (have < req) {
                 persistentOffset += kChunkAllocBytes;
-                ++nd.voxelChunks;
+                ++have;
             }
         }
     }
@@ -495,17 +532,7 @@ struct oracle_tree {
             const ONode &na = nodes[a], &nb = nodes[b];
             return Key{na.level, na.X, na.Y, na.Z} < Key{nb.level, nb.X, nb.Y, nb.Z};
         });
-        std::vector<uint32_t> accepted;
-        uint64_t room = nodes.size();
-        for (uint32_t id : spilling) {
-            if (room + 8 > maxNodes) {
-                nodePoolOverflow = true;
-                continue;
-            }
-            room += 8;
-            accepted.push_back(id);
-        }
-        spilling.swap(accepted);
+        acceptByNodePool(spilling);
         spilled.clear();
         backlogVoxels.clear();
         backlogTargets.clear();
@@ -534,6 +561,98 @@ struct oracle_tree {
         *splitsOut = uint32_t(spilling.size());
         *movedOut = spilled.size();
         spilling.clear();
+        spilled.clear();
+        sweepStats();
+    }
+
+    // Refinement's collapsing (README.md:81, wiki/02_KernelPipeline.md:238: "merge
+    // a node back up", which "needs a grid pool first"; no reference code), as the
+    // HIP path does it. Candidates: inner nodes below the root whose 8 children
+    // are all settled leaves (counter == numPoints) holding at most
+    // min(maxPoints, 50,000) points together, so the merged leaf is one Update
+    // would leave unsplit. The sparsest first, ties by (level, X, Y, Z), up to
+    // maxCollapses, and a prefix whose points fit pointBudget (the HIP path's
+    // spill list). Each chosen node:
+    //   * takes its children's stored points back (they pass through the spill
+    //     list and are inserted as for a batch of no new points: the node's
+    //     counter becomes their count, its chunks come from the chunk queue);
+    //   * returns its children's point chunks to the chunk queue, its grid to
+    //     the grid pool and the grid's voxel chunks to the chunk queue (above its
+    //     high-water mark, so later point chunks are taken from them first), and
+    //     drops its voxels: as a leaf it draws its points;
+    //   * frees its children's 8-node block to the node-block pool; the slots
+    //     are dead (in no pass) until a split takes the block.
+    // Accumulate then folds the merged leaf from scratch (its sums are cleared
+    // as an inner node's are), so its fixed point is relative to its own extent.
+    void collapse(uint32_t maxCollapses, uint32_t maxPoints, uint64_t pointBudget, uint32_t* candidatesOut,
+                  uint32_t* collapsedOut, uint64_t* movedOut) {
+        struct Cand {
+            uint32_t np, level, X, Y, Z, id;
+        };
+        const uint32_t limit = std::min(maxPoints, kMaxPointsPerNode);
+        std::vector<Cand> cands;
+        for (uint32_t id = 0; id < nodes.size(); ++id) {
+            const ONode& nd = nodes[id];
+            if (nd.dead || nd.isLeaf() || nd.level == 0) continue;
+            uint64_t total = 0;
+            bool ok = true;
+            for (int c : nd.children) {
+                const ONode& ch = nodes[size_t(c)];
+                if (!ch.isLeaf() || ch.counter != ch.numPoints || ch.grid >= 0) ok = false;
+                total += ch.numPoints;
+            }
+            if (!ok || total > limit) continue;
+            cands.push_back({uint32_t(total), nd.level, nd.X, nd.Y, nd.Z, id});
+        }
+        *candidatesOut = uint32_t(cands.size());
+        std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) {
+            if (a.np != b.np) return a.np < b.np;
+            return Key{a.level, a.X, a.Y, a.Z} < Key{b.level, b.X, b.Y, b.Z};
+        });
+        if (cands.size() > maxCollapses) cands.resize(maxCollapses);
+        uint64_t points = 0;
+        size_t fit = 0;
+        while (fit < cands.size() && points + cands[fit].np <= pointBudget) points += cands[fit++].np;
+        cands.resize(fit);
+        spilled.clear();
+        backlogVoxels.clear();
+        backlogTargets.clear();
+        for (const Cand& cd : cands) {
+            ONode& nd = nodes[cd.id];
+            const uint32_t first = uint32_t(nd.children[0]);
+            for (int i = 0; i < 8; ++i) {
+                ONode& ch = nodes[first + uint32_t(i)];
+                spilled.insert(spilled.end(), ch.points.begin(), ch.points.end());
+                numAllocatedChunks -= (ch.numPoints + kPointsPerChunk - 1) / kPointsPerChunk;
+                const uint32_t lvl = ch.level, X = ch.X, Y = ch.Y, Z = ch.Z;
+                ch = ONode();
+                ch.level = lvl;
+                ch.X = X;
+                ch.Y = Y;
+                ch.Z = Z;
+                ch.dead = true;
+            }
+            freeBlocks.push_back(first);
+            for (int& c : nd.children) c = -1;
+            if (nd.grid >= 0) {
+                chunkPoolSize += gridChunks[size_t(nd.grid)];
+                gridChunks[size_t(nd.grid)] = 0;
+                freeGrids.push_back(nd.grid);
+                nd.grid = -1;
+            }
+            nd.voxels.clear();
+            nd.voxels.shrink_to_fit();
+            nd.numVoxels = nd.numVoxelsStored = 0;
+            nd.points.clear();
+            nd.numPoints = 0;
+            nd.counter = cd.np;
+            std::memset(&nd.acc, 0, sizeof(nd.acc));
+            ++totCollapsed;
+        }
+        allocatePointChunks();
+        insertPoints(nullptr, 0);
+        *collapsedOut = uint32_t(cands.size());
+        *movedOut = spilled.size();
         spilled.clear();
         sweepStats();
     }
@@ -632,11 +751,11 @@ struct oracle_tree {
                 }
             }
         }
-        uint32_t maxLevel = 0;
-        for (const ONode& nd : nodes) maxLevel = std::max(maxLevel, nd.level);
+        uint32_t deepest = 0;
+        for (const ONode& nd : nodes) deepest = std::max(deepest, nd.level);
         std::vector<char> done(N, 0), skipped(N, 0);
         uint32_t refined = 0, nSkipped = 0, empty = 0;
-        for (int L = int(maxLevel); L >= int(minLevel); --L) {
+        for (int L = int(deepest); L >= int(minLevel); --L) {
             for (size_t i = 0; i < N; ++i) {
                 ONode& nd = nodes[i];
                 if (nd.grid < 0 || int(nd.level) != L) continue;
@@ -689,8 +808,9 @@ struct oracle_tree {
     void sweepStats() {
         remo_stats& s = lastStats;
         std::memset(&s, 0, sizeof(s));
-        s.num_nodes = uint32_t(nodes.size());
+        s.num_nodes = uint32_t(nodes.size() - 8 * freeBlocks.size());
         for (const ONode& nd : nodes) {
+            if (nd.dead) continue;
             s.max_depth = std::max(s.max_depth, nd.level);
             if (nd.isLeaf()) {
                 s.num_leaves++;
@@ -712,7 +832,9 @@ struct oracle_tree {
         s.allocated_bytes_persistent = persistentOffset;
         s.persistent_capacity = persistentCapacity;
         s.mem_capacity_reached = memCapacityReached;
-        s.node_capacity_reached = s.num_nodes >= maxNodes;
+        s.node_capacity_reached = nodes.size() >= maxNodes && freeBlocks.empty();
+        s.free_node_blocks = uint32_t(freeBlocks.size());
+        s.pooled_grids = uint32_t(freeGrids.size());
         s.node_pool_overflow = nodePoolOverflow;
         s.spill_overflow = spillOverflow;
         s.batches = stepBatches;
@@ -955,8 +1077,9 @@ void oracle_tree::analyse(const remo_shared_uniforms& u, uint32_t closureMargin,
     using namespace anal;
     const size_t N = nodes.size();
     out.assign(N, oracle_analysis_out{});
-    uint32_t D = 0;
-    for (const ONode& nd : nodes) D = std::max(D, nd.level);
+    // units 2^-(20+D) with D the deepest level any split created (a collapse does
+    // not lower it), as the HIP path keeps it
+    const uint32_t D = maxLevel;
     std::vector<Sums> sums(N);
     std::vector<int> closed(N, 0);
     // bottom-up: children always have larger indices than their parent here
@@ -1241,6 +1364,7 @@ void oracle_dump_nodes(const oracle_tree* t, remo_node_record* out) {
         r.num_voxels = nd.numVoxels;
         r.num_voxels_stored = nd.numVoxelsStored;
         r.grid = nd.grid;
+        r.free_slot = nd.dead ? 1u : 0u;
     }
 }
 
@@ -1325,6 +1449,11 @@ void oracle_apply_edl(const remo_shared_uniforms* u, uint64_t* fb) {
 void oracle_deepen(oracle_tree* t, uint32_t maxSplits, uint32_t minPoints, uint64_t pointBudget, uint32_t* candidates,
                    uint32_t* splits, uint64_t* moved, uint64_t* voxels) {
     t->deepen(maxSplits, minPoints, pointBudget, candidates, splits, moved, voxels);
+}
+
+void oracle_collapse(oracle_tree* t, uint32_t maxCollapses, uint32_t maxPoints, uint64_t pointBudget,
+                     uint32_t* candidates, uint32_t* collapsed, uint64_t* moved) {
+    t->collapse(maxCollapses, maxPoints, pointBudget, candidates, collapsed, moved);
 }
 
 uint32_t oracle_refine(oracle_tree* t, uint32_t minLevel, uint32_t* skipped, uint32_t* empty) {

@@ -323,6 +323,9 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
     if (threadIdx.x == 0) numTopSplits = 0;
     const uint32_t nc = min(g->numCandidates, t.candidateCap);
     const uint32_t base = g->numNodes;
+    // blocks and grids a collapse pooled go first (kSpillAndZero settles the counts)
+    const uint32_t pooledBlocks = g->numFreeBlocks;
+    const uint32_t pooledGrids = g->numFreeGrids;
     const bool staged = nc <= kRankSlots;
     if (staged) {
         for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
@@ -340,8 +343,12 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         } else {
             for (uint32_t k = 0; k < nc; ++k) rank += keyLess(t, t.candidates[k], node) ? 1u : 0u;
         }
-        const uint64_t childOffset = uint64_t(base) + 8ull * rank;
-        if (childOffset + 8 > t.maxNodes) {
+        // the lowest ranks take the pooled blocks, the rest extend the pool's
+        // used range: the node pool accepts a prefix of the key order
+        const bool fromPool = rank < pooledBlocks;
+        const uint64_t childOffset = fromPool ? uint64_t(t.freeBlocks[pooledBlocks - 1 - rank])
+                                              : uint64_t(base) + 8ull * (rank - pooledBlocks);
+        if (!fromPool && childOffset + 8 > t.maxNodes) {
             g->nodePoolOverflow = 1;  // stays a leaf; its points stay where they are
             uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
             bo[0] = make_uint4(0, 0, 0, 0);
@@ -374,9 +381,21 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
                                                        (unsigned long long)(-int64_t(nChunks))))
                                    : 0;
         const uint32_t slot = atomicAdd(&g->numAccepted, 1u);
-        const uint32_t gi = needGrid ? atomicAdd(&g->numGrids, 1u) : 0u;
-        const unsigned long long gridAt =
-            needGrid ? atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes) : 0ull;
+        // a pooled grid when there is one (pooled grids are alike: no voxel
+        // chunks, already counted in the store), else a new one from the store
+        uint32_t gi = 0, poolTake = kNone;
+        unsigned long long gridAt = 0;
+        if (needGrid && pooledGrids != 0) {
+            poolTake = atomicAdd(&g->gridsTaken, 1u);
+            if (poolTake >= pooledGrids) poolTake = kNone;
+        }
+        if (needGrid && poolTake != kNone) {
+            gi = t.freeGrids[pooledGrids - 1 - poolTake];
+            gridAt = t.gridOffset[gi];
+        } else if (needGrid) {
+            gi = atomicAdd(&g->numGrids, 1u);
+            gridAt = atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes);
+        }
         atomicMax(&g->maxLevel, lvl);
         for (uint32_t i = 0; i < 8; ++i) {
             const uint32_t c = c0 + i;
@@ -430,7 +449,12 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
 
         uint32_t gno = gno0;
         uint64_t gridOff = gridOff0;
-        if (needGrid) {
+        if (needGrid && poolTake != kNone) {
+            t.voxelChunks[gi] = 0;
+            t.gridNo[node] = gi;
+            gno = gi;
+            gridOff = gridAt;
+        } else if (needGrid) {
             if (gi < t.maxGrids && gridAt + kGridAllocBytes <= t.arenaCapacity) {
                 t.gridOffset[gi] = gridAt;
                 t.voxelChunks[gi] = 0;
@@ -479,7 +503,15 @@ __global__ __launch_bounds__(kBlock) void kSpillAndZero(DevTree t, uint32_t coun
     if (g->stop) return;
     const uint32_t na = g->numAccepted;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        g->numNodes += 8u * na;
+        // the accepted splits took the pooled blocks first, then new ones
+        const uint32_t pooled = g->numFreeBlocks, fromPool = min(na, pooled);
+        g->numFreeBlocks = pooled - fromPool;
+        g->numNodes += 8u * (na - fromPool);
+        const uint32_t taken = g->gridsTaken;
+        if (taken) {
+            g->numFreeGrids -= min(taken, g->numFreeGrids);
+            g->gridsTaken = 0;
+        }
         if (countRound) {
             g->expandIters += 1;
             g->totExpandIters += 1;
@@ -605,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void kFixedDecide(DevTree t, uint32_t level
     const uint32_t n = g->numNodes;
     const uint32_t base = fixedLevelOffset(level);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (t.firstChild[i] >= 0 || t.level[i] != level || t.numPoints[i] != 0) continue;
+        if (t.firstChild[i] >= 0 || t.level[i] != level || t.numPoints[i] != 0 || isFreeSlot(t, i)) continue;
         if (t.cellCount[base + fixedMorton(t.nx[i], t.ny[i], t.nz[i], level)] == 0) continue;
         const uint32_t slot = atomicAdd(&g->numCandidates, 1u);
         if (slot < t.candidateCap)
@@ -627,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void kFixedSeed(DevTree t) {
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t L = t.level[i];
-        if (t.firstChild[i] >= 0 || L > D) continue;
+        if (t.firstChild[i] >= 0 || L > D || isFreeSlot(t, i)) continue;
         const uint32_t cnt = t.cellCount[fixedLevelOffset(L) + fixedMorton(t.nx[i], t.ny[i], t.nz[i], L)];
         if (cnt) t.counter[i] += cnt;
     }
@@ -1218,6 +1250,7 @@ __global__ __launch_bounds__(kBlock) void kStatsSweep(DevTree t) {
              maxDepth = 0;
     uint64_t pts = 0, vox = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (isFreeSlot(t, i)) continue;
         maxDepth = max(maxDepth, t.level[i]);
         if (t.firstChild[i] < 0) {
             const uint32_t np = t.numPoints[i];
@@ -1398,7 +1431,223 @@ __global__ void kDeepenBegin(DevTree t, uint32_t numChosen) {
     g->numSpilled = 0;
     g->spillLimit = kNone;
     g->numBacklog = 0;
+    g->droppedVoxels = 0;
     g->stop = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Refinement's collapse (no reference code; README.md:81,
+// wiki/02_KernelPipeline.md:238): an inner node whose 8 children are leaves is
+// merged into one leaf. Its children's points pass through the spill list and
+// are inserted as a batch of no new points would be (its counter is their count,
+// kAllocChunks gives it chunks, kInsertPoints stores them); the children's point
+// chunks go back to the chunk queue, the node's grid to the grid pool and the
+// grid's voxel chunks to the chunk queue above its high-water mark (so the pool
+// of recyclable chunks grows by them), and the children's 8-node block to the
+// block pool. kSplitApply takes pooled blocks and grids before new ones.
+
+// Inner nodes below the root whose children are all leaves with every counted
+// point stored and no grid, holding at most maxPoints points together.
+__global__ __launch_bounds__(kBlock) void kCollapseList(DevTree t, DeepenCandidate* out, uint32_t* count,
+                                                        uint32_t cap, uint32_t maxPoints) {
+    const uint32_t n = t.g->numNodes;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t fc = t.firstChild[i];
+        if (fc < 0 || i == 0) continue;  // free slots are leaves
+        uint32_t total = 0;
+        bool ok = true;
+        for (uint32_t c = 0; c < 8; ++c) {
+            const uint32_t ch = uint32_t(fc) + c;
+            const uint32_t np = t.numPoints[ch];
+            if (t.firstChild[ch] >= 0 || t.counter[ch] != np || t.gridNo[ch] != kNone) ok = false;
+            total += np;
+        }
+        if (!ok || total > maxPoints) continue;
+        const uint32_t at = atomicAdd(count, 1u);
+        if (at < cap) out[at] = DeepenCandidate{total, t.level[i], t.nx[i], t.ny[i], t.nz[i], i};
+    }
+}
+
+// Per chosen node: its ranges in the spill list (its children's points), in the
+// chunk queue (their chunks, below the in-use count it lowers) and above the
+// queue's high-water mark (its grid's voxel chunks, which raise it), recorded
+// as (node, spill offset, queue top, pool position). Nothing is moved yet.
+__global__ __launch_bounds__(kBlock) void kCollapsePlan(DevTree t, uint32_t numChosen) {
+    Globals* g = t.g;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < numChosen; k += gridDim.x * blockDim.x) {
+        const uint32_t node = t.candidates[k];
+        const uint32_t fc = uint32_t(t.firstChild[node]);
+        const uint32_t gi = t.gridNo[node];
+        uint32_t np[8];
+        for (uint32_t c = 0; c < 8; ++c) np[c] = t.numPoints[fc + c];
+        // (a grid's directory holds every chunk its 128^3 cells can need)
+        const uint32_t vChunks = gi != kNone ? min(t.voxelChunks[gi], kVoxelDir) : 0u;
+        const uint32_t nv = t.numVoxels[node];
+        if (nv) atomicAdd(&g->droppedVoxels, nv);
+        uint32_t total = 0, chunks = 0;
+        for (uint32_t c = 0; c < 8; ++c) {
+            total += np[c];
+            chunks += (np[c] + kPointsPerChunk - 1) / kPointsPerChunk;
+        }
+        const uint32_t spillAt = total ? atomicAdd(&g->numSpilled, total) : 0u;
+        const int64_t top = chunks ? int64_t(atomicAdd((unsigned long long*)&g->numAllocatedChunks,
+                                                       (unsigned long long)(-int64_t(chunks))))
+                                   : 0;
+        const int64_t poolAt =
+            vChunks ? int64_t(atomicAdd((unsigned long long*)&g->chunkPoolSize, (unsigned long long)vChunks)) : 0;
+        const bool fits = uint64_t(spillAt) + total <= t.spillCap;  // the caller sizes the choice to fit
+        if (!fits) {
+            g->spillOverflow = 1;
+            atomicMin(&g->spillLimit, spillAt);
+        }
+        t.acceptedSpill[4 * k] = node;
+        t.acceptedSpill[4 * k + 1] = fits ? spillAt : kNone;
+        t.acceptedSpill[4 * k + 2] = uint32_t(top);
+        t.acceptedSpill[4 * k + 3] = uint32_t(poolAt);
+    }
+}
+
+// The children's points into the spill list and their chunks into the queue, one
+// (chosen node, child, chunk) per block step; then the grids' voxel chunks.
+constexpr uint32_t kCollapseChunkSlots = 64;  // >= 50 chunks of a child (at most 50,000 points)
+
+__global__ __launch_bounds__(kBlock) void kCollapseMove(DevTree t, uint32_t numChosen) {
+    const uint32_t pairs = numChosen * 8 * kCollapseChunkSlots;
+    for (uint32_t pair = blockIdx.x; pair < pairs; pair += gridDim.x) {
+        const uint32_t k = pair / (8 * kCollapseChunkSlots);
+        const uint32_t c = (pair / kCollapseChunkSlots) % 8, j = pair % kCollapseChunkSlots;
+        const uint32_t node = t.acceptedSpill[4 * k];
+        const uint32_t off = t.acceptedSpill[4 * k + 1];
+        const uint32_t fc = uint32_t(t.firstChild[node]);
+        // this child's place among its siblings' points and chunks
+        uint32_t pointsBefore = 0, chunksBefore = 0, np = 0;
+        for (uint32_t q = 0; q < 8; ++q) {
+            const uint32_t m = t.numPoints[fc + q];
+            if (q < c) {
+                pointsBefore += m;
+                chunksBefore += (m + kPointsPerChunk - 1) / kPointsPerChunk;
+            } else if (q == c) {
+                np = m;
+            }
+        }
+        if (j * kPointsPerChunk >= np) continue;
+        const uint32_t ref = t.pointDir[uint64_t(fc + c) * kPointDir + j];
+        if (threadIdx.x == 0) {
+            const int64_t qi = int64_t(t.acceptedSpill[4 * k + 2]) - 1 - int64_t(chunksBefore + j);
+            if (qi >= 0 && uint64_t(qi) < t.chunkQueueCap) t.chunkQueue[qi] = ref;
+        }
+        if (off == kNone) continue;
+        const uint32_t m = min(kPointsPerChunk, np - j * kPointsPerChunk);
+        const remo_point* src = chunkPtr(t.arena, ref);
+        remo_point* dst = t.spill + off + pointsBefore + j * kPointsPerChunk;
+        for (uint32_t q = threadIdx.x; q < m; q += blockDim.x) dst[q] = src[q];
+    }
+    const uint64_t slots = uint64_t(numChosen) * kVoxelDir;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < slots;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t k = uint32_t(i / kVoxelDir), v = uint32_t(i % kVoxelDir);
+        const uint32_t gi = t.gridNo[t.acceptedSpill[4 * k]];
+        if (gi == kNone || v >= t.voxelChunks[gi]) continue;
+        const uint64_t qi = uint64_t(t.acceptedSpill[4 * k + 3]) + v;
+        if (qi < t.chunkQueueCap) t.chunkQueue[qi] = t.voxelDir[uint64_t(gi) * kVoxelDir + v];
+    }
+}
+
+// One workgroup per chosen node: the children become free slots, the node a
+// leaf waiting for its points (counter = their count), its grid and block go to
+// the pools, and the top-level descent index enters at the node again.
+__global__ __launch_bounds__(kBlock) void kCollapseFinish(DevTree t, uint32_t numChosen, uint32_t* refPoints,
+                                                          uint32_t* refVoxels) {
+    __shared__ uint32_t npS[8];
+    Globals* g = t.g;
+    for (uint32_t k = blockIdx.x; k < numChosen; k += gridDim.x) {
+        const uint32_t node = t.acceptedSpill[4 * k];
+        const uint32_t fc = uint32_t(t.firstChild[node]);
+        if (threadIdx.x < 8) npS[threadIdx.x] = t.numPoints[fc + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const uint32_t c = fc + threadIdx.x;
+            const uint32_t used = (npS[threadIdx.x] + kPointsPerChunk - 1) / kPointsPerChunk;
+            for (uint32_t j = 0; j < used && j < kPointDir; ++j) t.pointDir[uint64_t(c) * kPointDir + j] = 0;
+            t.firstChild[c] = -1;
+            t.walk[c] = make_uint2(kNone, 0u);
+            t.parent[c] = kNone;  // a free slot from here on
+            t.counter[c] = 0;
+            t.numPoints[c] = 0;
+            t.numVoxels[c] = 0;
+            t.numVoxelsStored[c] = 0;
+            t.gridNo[c] = kNone;
+            t.pending[c] = 0;
+            uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * c);
+            so[0] = so[1] = make_uint4(0, 0, 0, 0);
+            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * c);
+            bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
+            t.accums[c] = remo_node_accum{};
+            t.dirtyListed[c] = 0;
+            if (refPoints) refPoints[c] = 0;
+            if (refVoxels) refVoxels[c] = kNone;
+        }
+        const uint32_t lvl = t.level[node];
+        if (threadIdx.x == 8) {
+            uint32_t total = 0;
+            for (uint32_t c = 0; c < 8; ++c) total += npS[c];
+            const uint32_t gi = t.gridNo[node];
+            t.firstChild[node] = -1;
+            t.walk[node] = make_uint2(kNone, 0u);
+            t.gridNo[node] = kNone;
+            t.numVoxels[node] = 0;
+            t.numVoxelsStored[node] = 0;
+            t.counter[node] = total;
+            t.numPoints[node] = 0;
+            t.pending[node] = 0;
+            uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * node);
+            so[0] = so[1] = make_uint4(0, 0, 0, 0);
+            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
+            bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
+            t.accums[node] = remo_node_accum{};
+            if (refPoints) refPoints[node] = kNone;  // its parent folds it again
+            if (refVoxels) refVoxels[node] = kNone;
+            t.freeBlocks[atomicAdd(&g->numFreeBlocks, 1u)] = fc;
+            if (gi != kNone) {
+                t.voxelChunks[gi] = 0;
+                t.freeGrids[atomicAdd(&g->numFreeGrids, 1u)] = gi;
+            }
+            atomicAdd((unsigned long long*)&g->totCollapsed, 1ull);
+        }
+        // descents into the node's region enter at the node again
+        if (lvl < kTopLevel) {
+            const uint32_t sb = kTopLevel - lvl;  // the node spans 2^sb top cells per axis
+            const uint32_t X = t.nx[node], Y = t.ny[node], Z = t.nz[node];
+            const uint32_t m = (1u << sb) - 1u;
+            for (uint32_t q = threadIdx.x; q < (1u << (3 * sb)); q += blockDim.x) {
+                const uint32_t cx = (X << sb) | ((q >> (2 * sb)) & m);
+                const uint32_t cy = (Y << sb) | ((q >> sb) & m);
+                const uint32_t cz = (Z << sb) | (q & m);
+                t.topCell[(cx << (2 * kTopLevel)) | (cy << kTopLevel) | cz] = node | (lvl << kTopLevelShift);
+            }
+        }
+        __syncthreads();  // npS is rewritten for the next node
+    }
+}
+
+void launchCollapseList(const DevTree& t, void* out, uint32_t* count, uint32_t cap, uint32_t maxPoints,
+                        const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kDeepenClear, dim3(1), dim3(64), 0, s, count);
+    hipLaunchKernelGGL(kCollapseList, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, static_cast<DeepenCandidate*>(out),
+                       count, cap, maxPoints);
+}
+
+void launchCollapse(const DevTree& t, uint32_t numChosen, BatchRef empty, uint32_t* refPoints, uint32_t* refVoxels,
+                    const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kDeepenBegin, dim3(1), dim3(kBacklogShards), 0, s, t, numChosen);
+    const uint32_t planBlocks = std::max(1u, std::min(256u, (numChosen + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(kCollapsePlan, dim3(planBlocks), dim3(kBlock), 0, s, t, numChosen);
+    hipLaunchKernelGGL(kCollapseMove, dim3(1024), dim3(kBlock), 0, s, t, numChosen);
+    hipLaunchKernelGGL(kCollapseFinish, dim3(std::max(1u, std::min(numChosen, 1024u))), dim3(kBlock), 0, s, t,
+                       numChosen, refPoints, refVoxels);
+    hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
+    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
+    hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);
 }
 
 void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t cap, uint32_t minPoints,

@@ -260,7 +260,7 @@ float arrivalOf(const remo_ctx* c) {
 void fillStats(const remo_ctx* c, remo_stats* s) {
     const Globals& g = *c->hostG;
     std::memset(s, 0, sizeof(*s));
-    s->num_nodes = g.numNodes;
+    s->num_nodes = g.numNodes - 8u * g.numFreeBlocks;
     s->num_inner = g.sInner;
     s->num_leaves = g.sLeaves;
     s->num_nonempty_leaves = g.sNonempty;
@@ -278,7 +278,9 @@ void fillStats(const remo_ctx* c, remo_stats* s) {
     s->allocated_bytes_persistent = g.arenaOffset;
     s->persistent_capacity = c->t.arenaCapacity;
     s->mem_capacity_reached = g.memCapacityReached;
-    s->node_capacity_reached = g.numNodes >= c->t.maxNodes ? 1u : 0u;
+    s->node_capacity_reached = g.numNodes >= c->t.maxNodes && g.numFreeBlocks == 0 ? 1u : 0u;
+    s->free_node_blocks = g.numFreeBlocks;
+    s->pooled_grids = g.numFreeGrids;
     s->node_pool_overflow = g.nodePoolOverflow;
     s->alloc_overflow = g.allocOverflow;
     s->spill_overflow = g.spillOverflow;
@@ -437,6 +439,8 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.candidates, t.candidateCap, "split candidates");
     A(&t.acceptedSpill, uint64_t(t.candidateCap) * 4, "accepted splits");
     A(&t.statsPartials, uint64_t(kStatsBlocks) * kStatsWords, "stats sweep partials");
+    A(&t.freeBlocks, N / 8 + 1, "the node-block pool");
+    A(&t.freeGrids, t.maxGrids, "the grid pool");
     A(&t.dirtyList, N, "the analysis dirty list");
     A(&t.dirtyCount, 1, "the analysis dirty count");
     A(&t.dirtyListed, N, "the analysis dirty flags");
@@ -817,6 +821,66 @@ int remo_deepen_noalloc(remo_ctx* c, uint32_t maxSplits, uint32_t minPoints, rem
     return deepenWith(c, maxSplits, minPoints, records, count, kStaticDeepenRecords, out);
 }
 
+int remo_collapse(remo_ctx* c, uint32_t maxCollapses, uint32_t maxPoints, remo_collapse_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    std::memset(out, 0, sizeof(*out));
+    if (maxCollapses == 0) return REMO_OK;
+    DevTree& t = c->t;
+    hipStream_t s = c->stream;
+    // the candidate records share remo_deepen's buffer (an inner node's record
+    // has the leaf record's layout)
+    if (!c->deepenRecords) {
+        int rc = devAlloc(c, reinterpret_cast<uint8_t**>(&c->deepenRecords), uint64_t(t.maxNodes) * kDeepenRecordBytes,
+                          "collapse candidates");
+        if (rc == REMO_OK) rc = devAlloc(c, &c->deepenCount, 2, "collapse counter");
+        if (rc != REMO_OK) return rc;
+    }
+    const uint32_t limit = std::min(maxPoints, kMaxPointsPerNode);
+    launchCollapseList(t, c->deepenRecords, c->deepenCount, t.maxNodes, limit, c->shape, s);
+    CK(hipGetLastError(), "collapse list kernel");
+    CK(hipMemcpyAsync(c->hostScratch, c->deepenCount, 4, hipMemcpyDeviceToHost, s), "collapse count");
+    CK(hipStreamSynchronize(s), "collapse list");
+    const uint32_t n = std::min(c->hostScratch[0], t.maxNodes);
+    out->candidates = n;
+    if (n == 0) return REMO_OK;
+    DeepenHostRec* recs = c->deepenHost.get();
+    CK(hipMemcpy(recs, c->deepenRecords, sizeof(DeepenHostRec) * n, hipMemcpyDeviceToHost), "collapse records");
+    // the sparsest first; ties by canonical key, so the choice is deterministic
+    std::sort(recs, recs + n, [](const DeepenHostRec& a, const DeepenHostRec& b) {
+        if (a.numPoints != b.numPoints) return a.numPoints < b.numPoints;
+        if (a.level != b.level) return a.level < b.level;
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        return a.z < b.z;
+    });
+    // the merged nodes' points move through the spill list
+    uint32_t chosen = 0;
+    uint64_t points = 0;
+    const uint32_t most = std::min(std::min(n, maxCollapses), t.candidateCap);
+    while (chosen < most && points + recs[chosen].numPoints <= t.spillCap) points += recs[chosen++].numPoints;
+    if (chosen == 0) return REMO_OK;
+    uint32_t* ids = c->deepenIds.get();
+    for (uint32_t k = 0; k < chosen; ++k) ids[k] = recs[k].node;
+    int rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    CK(hipMemcpy(t.candidates, ids, 4ull * chosen, hipMemcpyHostToDevice), "collapse candidates");
+    const uint32_t gridsBefore = c->hostG->numFreeGrids;
+    launchCollapse(t, chosen, BatchRef{t.spill, c->deepenCount + 1}, c->refine.refPoints, c->refine.refVoxels,
+                   c->shape, s);
+    CK(hipGetLastError(), "collapse kernels");
+    // every node's roll-up record is rebuilt: the merged leaves' sums are folded anew
+    c->analysis.needFull = true;
+    rc = readGlobals(c);
+    if (rc != REMO_OK) return rc;
+    const Globals& g = *c->hostG;
+    out->collapsed = chosen;
+    out->points_moved = std::min<uint64_t>(std::min<uint64_t>(g.numSpilled, t.spillCap), g.spillLimit);
+    out->grids_pooled = g.numFreeGrids - gridsBefore;
+    out->voxels_dropped = g.droppedVoxels;
+    return REMO_OK;
+}
+
 int remo_render_flat(remo_ctx* c, const remo_point* points, uint64_t n, const remo_shared_uniforms* u, uint64_t* fb) {
     if (!c || !u || !fb || (n && !points)) return REMO_ERR_INVALID;
     if (c->dead) return REMO_ERR_DEVICE;
@@ -1017,7 +1081,7 @@ int remo_dump_nodes(remo_ctx* c, remo_node_record* out, uint32_t cap, uint32_t* 
     *num = n;
     if (!out || cap < n) return out ? REMO_ERR_INVALID : REMO_OK;
     std::vector<int32_t> fc(n);
-    std::vector<uint32_t> lv(n), X(n), Y(n), Z(n), cnt(n), np(n), nv(n), nvs(n), gr(n);
+    std::vector<uint32_t> lv(n), X(n), Y(n), Z(n), cnt(n), np(n), nv(n), nvs(n), gr(n), par(n);
     auto D = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
     };
@@ -1032,6 +1096,7 @@ int remo_dump_nodes(remo_ctx* c, remo_node_record* out, uint32_t cap, uint32_t* 
     CK(D(nv.data(), t.numVoxels, 4ull * n), "dump");
     CK(D(nvs.data(), t.numVoxelsStored, 4ull * n), "dump");
     CK(D(gr.data(), t.gridNo, 4ull * n), "dump");
+    CK(D(par.data(), t.parent, 4ull * n), "dump");
     CK(hipStreamSynchronize(c->stream), "dump");
     for (uint32_t i = 0; i < n; ++i) {
         remo_node_record& r = out[i];
@@ -1046,6 +1111,7 @@ int remo_dump_nodes(remo_ctx* c, remo_node_record* out, uint32_t cap, uint32_t* 
         r.num_voxels = nv[i];
         r.num_voxels_stored = nvs[i];
         r.grid = gr[i] == kNone ? -1 : int32_t(gr[i]);
+        r.free_slot = (i != 0 && par[i] == kNone) ? 1u : 0u;
     }
     return REMO_OK;
 }

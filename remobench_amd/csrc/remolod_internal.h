@@ -89,6 +89,12 @@ struct Globals {
     uint32_t sInner, sLeaves, sNonempty, sMaxPoints;
     uint64_t sPoints, sVoxels;
     uint32_t sChunksPoints, sChunksVoxels, sUnderflows, sMaxDepth;
+    // pools fed by Refinement's collapse, drawn from first by splits
+    uint32_t numFreeBlocks;        // 8-node child blocks on DevTree::freeBlocks
+    uint32_t numFreeGrids;         // grids on DevTree::freeGrids
+    uint32_t gridsTaken;           // pooled grids taken by the current split round
+    uint32_t droppedVoxels;        // voxels of the nodes the last collapse merged
+    uint64_t totCollapsed;
 };
 
 // Everything a kernel needs, passed by value (one kernel argument).
@@ -145,6 +151,10 @@ struct DevTree {
     uint32_t* dirtyList;    // [maxNodes]
     uint32_t* dirtyCount;   // [1]
     uint32_t* dirtyListed;  // [maxNodes] 1 while the node is in the list
+    // Collapse's pools: first node index of each freed 8-node block, number of
+    // each freed grid (stacks; splits take from the top)
+    uint32_t* freeBlocks;   // [maxNodes / 8 + 1]
+    uint32_t* freeGrids;    // [maxGrids]
     Globals* g;
     remo_node_accum* accums;
 };
@@ -242,6 +252,13 @@ __device__ __forceinline__ void descendN(const int32_t* __restrict__ firstChild,
     }
 }
 
+// A slot of an 8-node block that a collapse returned to the pool: no parent and
+// not the root. It is in no pass until a split takes the block (every field a
+// split does not rewrite is zero: no points, voxels, grid or sums).
+__device__ __forceinline__ bool isFreeSlot(const DevTree& t, uint32_t i) {
+    return i != 0 && t.parent[i] == kNone;
+}
+
 __device__ __forceinline__ uint32_t octantAt(uint32_t X, uint32_t Y, uint32_t Z, uint32_t level) {
     const uint32_t sh = kMaxDepth - level - 1;
     return (((X >> sh) & 1u) << 2) | (((Y >> sh) & 1u) << 1) | ((Z >> sh) & 1u);
@@ -309,6 +326,17 @@ constexpr uint32_t kDeepenRecordBytes = 24;
 void launchDeepenList(const DevTree& t, void* out, uint32_t* count, uint32_t cap, uint32_t minPoints,
                       const LaunchShape& ls, hipStream_t s);
 void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, const LaunchShape& ls, hipStream_t s);
+// Refinement's collapse: list the inner nodes whose 8 children are settled
+// leaves holding at most maxPoints together (records of 24 B: points, level, x,
+// y, z, node; count[0] = all found, count[1] = 0), then merge the chosen ones
+// (already in t.candidates): points through the spill list into the merged
+// leaf, chunks and grids to their pools. refPoints / refVoxels: Refinement's
+// per-node marks (null when it never ran), reset for the merged nodes and the
+// freed slots.
+void launchCollapseList(const DevTree& t, void* out, uint32_t* count, uint32_t cap, uint32_t maxPoints,
+                        const LaunchShape& ls, hipStream_t s);
+void launchCollapse(const DevTree& t, uint32_t numChosen, BatchRef empty, uint32_t* refPoints, uint32_t* refVoxels,
+                    const LaunchShape& ls, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);
 void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,

@@ -203,6 +203,19 @@ bool RemolodHipPipeline::deepen(uint32_t maxSplits, uint32_t minPoints, remo_dee
     return true;
 }
 
+bool RemolodHipPipeline::collapse(uint32_t maxCollapses, uint32_t maxPoints, remo_collapse_stats* out,
+                                  std::string* err) {
+    if (!m_ctx || m_needsReset || !out) return false;
+    const int rc = remo_collapse(m_ctx, maxCollapses, maxPoints, out);
+    if (rc != REMO_OK) {
+        fail("collapse", rc);
+        if (err) *err = m_lastError;
+        return false;
+    }
+    readStats();
+    return true;
+}
+
 void RemolodHipPipeline::render(const FrameContext& frame) {
     // src/pipelines/RemolodPipeline.cpp:559-601.
     if (!m_ctx || m_needsReset || !frame.targets.framebuffer) return;

@@ -52,6 +52,9 @@ public:
     bool refine(float budgetMs, remo_refine_stats* out, std::string* err);
     // Refinement's local deepening: split up to maxSplits of the densest leaves.
     bool deepen(uint32_t maxSplits, uint32_t minPoints, remo_deepen_stats* out, std::string* err);
+    // Refinement's collapsing: merge up to maxCollapses of the sparsest inner nodes
+    // whose children are all leaves (at most maxPoints points together).
+    bool collapse(uint32_t maxCollapses, uint32_t maxPoints, remo_collapse_stats* out, std::string* err);
 
     remo_ctx* context() const { return m_ctx; }
     const remo_stats& rawStats() const { return m_raw; }

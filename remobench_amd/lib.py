@@ -91,7 +91,7 @@ class Stats(C.Structure):
         ("mem_capacity_reached", C.c_uint32), ("node_capacity_reached", C.c_uint32),
         ("node_pool_overflow", C.c_uint32), ("alloc_overflow", C.c_uint32),
         ("spill_overflow", C.c_uint32), ("backlog_overflow", C.c_uint32),
-        ("dir_overflow", C.c_uint32), ("pad0", C.c_uint32),
+        ("dir_overflow", C.c_uint32), ("free_node_blocks", C.c_uint32),
         ("batches", C.c_uint64), ("expand_iters", C.c_uint64), ("nodes_split", C.c_uint64),
         ("spilled_points", C.c_uint64), ("new_voxels", C.c_uint64),
         ("total_batches", C.c_uint64), ("total_expand_iters", C.c_uint64),
@@ -100,7 +100,7 @@ class Stats(C.Structure):
         ("morton_watermark", C.c_uint64), ("num_accumulated", C.c_uint64),
         ("sum_leaf_counts", C.c_uint64),
         ("points_folded", C.c_uint32), ("leaves_folded", C.c_uint32),
-        ("inner_with_sums", C.c_uint32), ("pad1", C.c_uint32),
+        ("inner_with_sums", C.c_uint32), ("pooled_grids", C.c_uint32),
         ("draw_items", C.c_uint32), ("draw_list_overflow", C.c_uint32), ("draw_samples", C.c_uint64),
     ]
 
@@ -111,7 +111,7 @@ class Stats(C.Structure):
 NODE_DTYPE = np.dtype([
     ("level", "<u4"), ("x", "<u4"), ("y", "<u4"), ("z", "<u4"), ("first_child", "<i4"),
     ("counter", "<u4"), ("num_points", "<u4"), ("num_voxels", "<u4"),
-    ("num_voxels_stored", "<u4"), ("grid", "<i4"), ("pad0", "<u4"), ("pad1", "<u4"),
+    ("num_voxels_stored", "<u4"), ("grid", "<i4"), ("free_slot", "<u4"), ("pad1", "<u4"),
 ])
 
 ACCUM_DTYPE = np.dtype([
@@ -171,6 +171,15 @@ class DeepenStats(C.Structure):
     """remo_deepen_stats (Refinement, local deepening)."""
     _fields_ = [("candidates", C.c_uint32), ("chosen", C.c_uint32), ("splits", C.c_uint32),
                 ("refused", C.c_uint32), ("points_moved", C.c_uint64), ("new_voxels", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class CollapseStats(C.Structure):
+    """remo_collapse_stats (Refinement, collapsing)."""
+    _fields_ = [("candidates", C.c_uint32), ("collapsed", C.c_uint32), ("points_moved", C.c_uint64),
+                ("grids_pooled", C.c_uint32), ("voxels_dropped", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -255,6 +264,7 @@ def load() -> C.CDLL:
         "remo_render_flat": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, P(SharedUniforms), C.c_void_p]),
         "remo_deepen": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(DeepenStats)]),
         "remo_deepen_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(DeepenStats)]),
+        "remo_collapse": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, P(CollapseStats)]),
         "remo_apply_edl": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_void_p]),
         "remo_mlp_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, P(C.c_uint32)]),
         "remo_scene_default": (None, [P(SceneSpec)]),

@@ -240,6 +240,15 @@ class Tree:
         self._ck(fn(self.h, int(max_splits), int(min_points), C.byref(st)), "deepen")
         return st.as_dict()
 
+    def collapse(self, max_collapses: int, max_points: int = 50_000) -> dict:
+        """Refinement's collapsing: merge up to max_collapses of the sparsest
+        inner nodes whose 8 children are leaves holding <= max_points points
+        together into one leaf each; their blocks and grids go to the pools that
+        later splits draw from."""
+        st = L.CollapseStats()
+        self._ck(self.lib.remo_collapse(self.h, int(max_collapses), int(max_points), C.byref(st)), "collapse")
+        return st.as_dict()
+
     def render_flat(self, points_ptr: int, n: int, uniforms: L.SharedUniforms, fb_ptr: int):
         """Every one of n device points splatted into fb (no LOD): the image ground truth."""
         self._ck(self.lib.remo_render_flat(self.h, C.c_void_p(points_ptr), n, C.byref(uniforms),

@@ -16,11 +16,13 @@ STRUCT_KEYS = ["num_nodes", "num_inner", "num_leaves", "num_nonempty_leaves", "n
                "num_voxels", "num_points_processed", "batchlet_index", "num_chunks_points",
                "num_chunks_voxels", "max_points_per_node", "counter_underflows", "max_depth",
                "num_allocated_chunks", "chunk_pool_size", "allocated_bytes_persistent",
-               "mem_capacity_reached", "node_pool_overflow"]
+               "mem_capacity_reached", "node_pool_overflow", "free_node_blocks", "pooled_grids"]
 
 
 def keyed(nodes: np.ndarray) -> dict:
-    return {(int(r["level"]), int(r["x"]), int(r["y"]), int(r["z"])): i for i, r in enumerate(nodes)}
+    """(level, x, y, z) -> pool index of every node in the tree (pooled slots left out)."""
+    return {(int(r["level"]), int(r["x"]), int(r["y"]), int(r["z"])): i for i, r in enumerate(nodes)
+            if not r["free_slot"]}
 
 
 def point_rows(p: np.ndarray) -> np.ndarray:
