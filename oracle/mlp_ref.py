@@ -3,9 +3,10 @@
 TEST INFRASTRUCTURE ONLY: used by tests/, never by the product path.
 
 13 -> 64 -> 64 -> 2 with exact GELU, parameters in the C ABI's order
-(W1[64][13], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]); the forward and
-backward passes run on BF16-rounded parameters (round to nearest even) with
-float32 (or float64, for finite-difference checks) activations; the loss is the
+(W1[64][13], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]); the training forward
+and backward passes run on BF16-rounded parameters (round to nearest even) with
+float32 (or float64, for finite-difference checks) activations, the inference
+forward rounds each layer's inputs to BF16 as well; the loss is the
 mean squared error over n x 2 outputs; Adam with bias correction. There is no
 reference implementation to follow (the reference's plans/04_NeuralPass.md is
 empty); the gradient is pinned by finite differences in tests/test_oracle_mlp.py.
@@ -61,6 +62,17 @@ def forward(theta, x, dtype=np.float32, cache=False):
     if cache:
         return y, (x, z1, h1, z2, h2)
     return y
+
+
+def forward_infer(theta, x):
+    """The inference kernels' forward: BF16 parameters (theta as stored), every
+    layer's inputs rounded to BF16 for the matrix cores, float32 accumulation and
+    bias, exact GELU."""
+    p = {k: v.astype(np.float32) for k, v in unpack(bf16(theta)).items()}
+    h0 = bf16(np.asarray(x, dtype=np.float32))
+    h1 = bf16(gelu(h0 @ p["W1"].T + p["b1"]))
+    h2 = bf16(gelu(h1 @ p["W2"].T + p["b2"]))
+    return h2 @ p["W3"].T + p["b3"]
 
 
 def loss_and_grad(theta, x, t, dtype=np.float32):

@@ -346,8 +346,10 @@ __global__ __launch_bounds__(kBlock) void kShapes(DevTree t, AnalysisArgs aa) {
 }
 
 // Closure, shape, coverage and score of one node. With `pred` (the learned
-// score's surface variation and planarity for this node), an open node takes the
-// prediction unless it says the node is not planar (surface variation > 0.05).
+// score's predicted change of this node's surface variation and planarity), an
+// open node takes its analytic values plus that change where the change is large
+// (above 0.05 in surface variation or 0.1 in planarity); otherwise, and for
+// closed nodes, the analytic values of its current sums stand.
 __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
                              uint32_t node, bool closed, const float* pred = nullptr) {
     const ShapeCache sh = aa.shapes[node];  // solved by the roll-up that last changed its sums
@@ -358,22 +360,18 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
     projectedExtent(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, dx, dy);
     r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
     r.state = closed ? 2u : 0u;
-    bool scored = false;
+    for (int i = 0; i < 3; ++i) r.lambda[i] = sh.lambda[i];
+    r.surface_variation = sh.sv;
+    r.planarity = sh.planarity;
+    r.linearity = sh.linearity;
     if (pred && !closed && sh.n >= 3) {
-        const float sv = pred[0], pl = pred[1];
-        if (sv <= kLearnedFallbackSv) {
-            r.surface_variation = fmaxf(sv, 0.0f);
-            r.planarity = fminf(fmaxf(pl, 0.0f), 1.0f);
-            r.linearity = 0.0f;
+        // the predicted change to the analytic values of the current sums
+        const float dsv = pred[0], dpl = pred[1];
+        if (fabsf(dsv) > kLearnedChangeSv || fabsf(dpl) > kLearnedChangePl) {
+            r.surface_variation = fminf(fmaxf(sh.sv + dsv, 0.0f), 1.0f / 3.0f);
+            r.planarity = fminf(fmaxf(sh.planarity + dpl, 0.0f), 1.0f);
             r.flags |= 2u;
-            scored = true;
         }
-    }
-    if (!scored) {
-        for (int i = 0; i < 3; ++i) r.lambda[i] = sh.lambda[i];
-        r.surface_variation = sh.sv;
-        r.planarity = sh.planarity;
-        r.linearity = sh.linearity;
     }
     r.score = 3.0f * r.surface_variation * r.coverage;
     aa.results[node] = r;
@@ -849,18 +847,17 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
                                                            float arrival) {
     static_assert(kBlock == kMlpInferThreads, "one MLP wave per finalize wave");
     constexpr uint32_t kPerBlock = 16 * kMlpWavesPerBlock;
-    __shared__ MlpInferShared sh;
+    __shared__ MlpWaveShared sh[kMlpWavesPerBlock];
     __shared__ float pred[kMlpWavesPerBlock][16 * kMlpOut];
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
-    if (blockIdx.x * kPerBlock >= n) return;  // the whole block: before the barrier
     const uint32_t current = g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
     if (arrival < 0.0f) arrival = aa.frame->arrival;
     const uint32_t wave = threadIdx.x / 64, lane = laneId();
-    float(*act)[kMlpActRow] = sh.act[wave];
-    mlpStageWeights(wb, sh.w);
-    __syncthreads();
+    // every lane's weight fragments, in registers for the whole kernel (no barrier)
+    MlpFrag frag;
+    if (blockIdx.x * kPerBlock + wave * 16 < n) mlpLoadFrag(wb, frag);
     for (uint32_t base = blockIdx.x * kPerBlock; base < n; base += gridDim.x * kPerBlock) {
         const uint32_t w0 = base + wave * 16;  // this wave's nodes
         const uint32_t node = w0 + lane;
@@ -869,10 +866,10 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
             if (lane < 16) {
                 float f[kMlpIn];
                 if (node < n) nodeFeatures(t, node, centred(t, node, aa.rollup[node]), arrival, f);
-                for (uint32_t k = 0; k < 16; ++k) act[lane][k] = (node < n && k < kMlpIn) ? f[k] : 0.0f;
+                for (uint32_t k = 0; k < 16; ++k) sh[wave].in[lane][k] = (node < n && k < kMlpIn) ? f[k] : 0.0f;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            mlpInferTile(sh.w, act, n, w0, pred[wave]);
+            mlpInferTile(frag, sh[wave].in, sh[wave].act, n, w0, pred[wave]);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the predictions are in LDS
         }
         bool q = false;

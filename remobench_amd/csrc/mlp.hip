@@ -12,7 +12,7 @@
 //              partial gradient (5,186 values) in sample order;
 //   kMlpAdam   one thread per parameter: sums the groups' partials in group order
 //              (deterministic), Adam update of the master, BF16 copy;
-//   kMlpInfer  inference on the matrix cores (FP32 MFMA), 16 samples per wave.
+//   kMlpInfer  inference on the matrix cores (BF16 MFMA), 16 samples per wave.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,15 +54,16 @@ __global__ void kMlpReset(const float* master, uint16_t* bf16, float* m, float* 
 // row count comes from nPtr (device, e.g. the node count) when given, else n.
 __global__ __launch_bounds__(kMlpInferThreads) void kMlpInfer(const uint16_t* wb, const float* x, const uint32_t* nPtr,
                                                               uint32_t n, float* y) {
-    __shared__ MlpInferShared sh;
-    mlpStageWeights(wb, sh.w);
-    __syncthreads();
+    __shared__ MlpWaveShared sh[kMlpWavesPerBlock];
     const uint32_t rows = nPtr ? *nPtr : n;
     const uint32_t wave = threadIdx.x / 64;
+    if ((blockIdx.x * kMlpWavesPerBlock + wave) * 16 >= rows) return;  // the wave has no tile (no barrier below)
+    MlpFrag f;
+    mlpLoadFrag(wb, f);
     for (uint32_t tile = blockIdx.x * kMlpWavesPerBlock + wave; tile * 16 < rows;
          tile += gridDim.x * kMlpWavesPerBlock) {
-        mlpLoadRows(x, rows, tile * 16, sh.act[wave]);
-        mlpInferTile(sh.w, sh.act[wave], rows, tile * 16, y + uint64_t(tile) * 16 * kMlpOut);
+        mlpLoadRows(x, rows, tile * 16, sh[wave].in);
+        mlpInferTile(f, sh[wave].in, sh[wave].act, rows, tile * 16, y + uint64_t(tile) * 16 * kMlpOut);
     }
 }
 

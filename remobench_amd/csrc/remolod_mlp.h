@@ -1,8 +1,10 @@
-// Learned score: a 13 -> 64 -> 64 -> 2 MLP with exact (erf) GELU that predicts a
-// node's final surface variation and planarity from its partial accumulators.
-// BF16 weights, FP32 activations and accumulation; trained with hand-written
-// forward / backward / Adam kernels (mlp.hip) on (partial, final) pairs logged
-// from the analytic pipeline.
+// Learned score: a 13 -> 64 -> 64 -> 2 MLP with exact (erf) GELU that predicts
+// how much a node's surface variation and planarity will still change before the
+// node is complete, from its partial accumulators: the prediction is a correction
+// to the analytic values of the partial sums. BF16 weights, FP32 accumulation
+// (inference rounds each layer's inputs to BF16 for the matrix cores; training
+// keeps FP32 activations); trained with hand-written forward / backward / Adam
+// kernels (mlp.hip) on (partial, final) pairs logged from the analytic pipeline.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,9 +26,13 @@ constexpr uint32_t kB3 = kW3 + kMlpOut * kMlpHid;
 constexpr uint32_t kMlpParams = kB3 + kMlpOut;
 static_assert(kMlpParams == REMO_MLP_PARAMS, "parameter count");
 
-// The fallback threshold: a predicted surface variation above this sends the
-// node to the analytic eigen-solve.
-constexpr float kLearnedFallbackSv = 0.05f;
+// An open node takes the corrected values only where the model predicts a change
+// of its surface variation above kLearnedChangeSv or of its planarity above
+// kLearnedChangePl; otherwise (and for every closed node) the analytic values of
+// its current sums stand, which are exact for a node whose points have all
+// arrived.
+constexpr float kLearnedChangeSv = 0.05f;
+constexpr float kLearnedChangePl = 0.10f;
 
 // ---- BF16 (round to nearest even) ------------------------------------------------
 
@@ -92,158 +98,158 @@ __device__ __forceinline__ void mlpForward(const float* w, const float x[kMlpIn]
 }
 static_assert(kMlpOut == 2, "mlpForward writes two outputs");
 
-// ---- batched inference on the matrix cores ------------------------------------------
+// ---- batched inference on the matrix cores (BF16) ----------------------------------
 //
-// One wave computes 16 rows with v_mfma_f32_16x16x4_f32 (FP32 operands: the BF16
-// weights expanded, FP32 activations and accumulation). Operand layout of that
-// instruction for lane l: A[row l % 16][k l / 16], B[k l / 16][col l % 16], and
-// the 16x16 result D[row 4 (l / 16) + r][col l % 16] in element r. Layer outputs
-// go through LDS to become the next layer's A operand.
+// One wave computes 16 rows with v_mfma_f32_16x16x32_bf16: BF16 operands (the
+// weights as stored; each layer's inputs rounded to BF16), FP32 accumulation.
+// Operand layout for lane l: A[row l % 16][k 8 (l / 16) .. 8 (l / 16) + 7],
+// B[k 8 (l / 16) .. + 7][col l % 16], and the 16x16 result D[row 4 (l / 16) + r]
+// [col l % 16] in element r. The layers are 13 (padded to 32) -> 64 -> 64 -> 2
+// (padded to 16 columns): 4 + 8 + 2 MFMAs per 16 rows.
+//
+// Every lane keeps its own weight fragments in registers for the whole kernel
+// (14 x 8 BF16 and 9 biases, loaded once from global memory), so no block stages
+// the parameters; a layer's outputs go through a per-wave LDS tile (BF16 rows) to
+// become the next layer's A operand.
 typedef float mlpF4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlpBf8 __attribute__((ext_vector_type(8)));
 constexpr uint32_t kMlpWavesPerBlock = 4;
 constexpr uint32_t kMlpInferThreads = 64 * kMlpWavesPerBlock;
-// LDS rows padded to 68 floats: lane (r16, k4) of an operand read touches bank
-// 4 r16 + k4, and a result write (row 4 k4 + r, column r16) bank 16 k4 + r16, so
-// neither conflicts.
-constexpr uint32_t kMlpActRow = kMlpHid + 4;
-constexpr uint32_t kMlpW2Row = kMlpHid + 4;
-// the parameters in LDS: the global layout with W2's rows padded to kMlpW2Row
-constexpr uint32_t kMlpShW2 = kW2;
-constexpr uint32_t kMlpShB2 = kMlpShW2 + kMlpHid * kMlpW2Row;
-constexpr uint32_t kMlpShW3 = kMlpShB2 + kMlpHid;
-constexpr uint32_t kMlpShB3 = kMlpShW3 + kMlpOut * kMlpHid;
-constexpr uint32_t kMlpShParams = kMlpShB3 + kMlpOut;
+constexpr uint32_t kMlpK1 = 32;                 // layer 1's K: 13 inputs, zero-padded
+// per-wave activation tile: 16 rows of 64 BF16, padded to 72 (144 B: rows stay
+// 16-byte aligned for the operand loads, and consecutive rows start 4 banks apart)
+constexpr uint32_t kMlpActRow = kMlpHid + 8;
 
-struct MlpInferShared {
-    float w[kMlpShParams];
-    float act[kMlpWavesPerBlock][16][kMlpActRow];
+struct MlpFrag {
+    mlpBf8 w1[4];      // layer 1: column tile nt, k = 8 (l / 16) + 0..7 (k >= 13: zero)
+    mlpBf8 w2[4][2];   // layer 2: column tile nt, k-step ks: k = 32 ks + 8 (l / 16) + 0..7
+    mlpBf8 w3[2];      // layer 3: k-step ks, column l % 16 (columns >= 2: zero)
+    float b1[4], b2[4], b3;
 };
 
-__device__ __forceinline__ uint32_t mlpShIndex(uint32_t p) {
-    if (p < kW2) return p;
-    if (p < kB2) return kMlpShW2 + (p - kW2) / kMlpHid * kMlpW2Row + (p - kW2) % kMlpHid;
-    return p + kMlpHid * (kMlpW2Row - kMlpHid);
+__device__ __forceinline__ __bf16 mlpBf(uint16_t h) { return __builtin_bit_cast(__bf16, h); }
+__device__ __forceinline__ __bf16 mlpToBf(float f) { return __builtin_bit_cast(__bf16, f32ToBf16(f)); }
+
+// Eight consecutive BF16 parameters starting at p (16-byte aligned) as one operand.
+__device__ __forceinline__ mlpBf8 mlpLoad8(const uint16_t* wb, uint32_t p) {
+    const uint4 v = *reinterpret_cast<const uint4*>(wb + p);
+    const uint32_t h[4] = {v.x, v.y, v.z, v.w};
+    mlpBf8 r;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        r[2 * q] = mlpBf(uint16_t(h[q] & 0xFFFFu));
+        r[2 * q + 1] = mlpBf(uint16_t(h[q] >> 16));
+    }
+    return r;
 }
 
-// The BF16 parameters (16-byte aligned, as every device allocation is) expanded
-// into the LDS layout by the whole block of kMlpInferThreads: 16-byte loads, all
-// issued before the first store.
-__device__ inline void mlpStageWeights(const uint16_t* wb, float* w) {
-    constexpr uint32_t kGroups = kMlpParams / 8;
-    constexpr uint32_t kPer = (kGroups + kMlpInferThreads - 1) / kMlpInferThreads;
-    const uint4* v = reinterpret_cast<const uint4*>(wb);
-    uint4 r[kPer];
+// This lane's fragments of the BF16 parameters (the global layout of the C ABI:
+// W[out][in] row-major, the buffer 16-byte aligned; W2 and W3 start at multiples
+// of 8 parameters, W1's 13-wide rows are read element by element).
+__device__ inline void mlpLoadFrag(const uint16_t* wb, MlpFrag& f) {
+    const uint32_t lane = __lane_id(), c16 = lane % 16, g = lane / 16;
+    static_assert(kW2 % 8 == 0 && kW3 % 8 == 0, "aligned weight blocks");
 #pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) {
-        const uint32_t g = threadIdx.x + i * kMlpInferThreads;
-        if (g < kGroups) r[i] = v[g];
+    for (uint32_t nt = 0; nt < 4; ++nt) {
+        const uint32_t col = nt * 16 + c16;
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) {
+            const uint32_t k = 8 * g + i;
+            f.w1[nt][i] = k < kMlpIn ? mlpBf(wb[kW1 + col * kMlpIn + k]) : mlpBf(0);
+        }
+#pragma unroll
+        for (uint32_t ks = 0; ks < 2; ++ks) f.w2[nt][ks] = mlpLoad8(wb, kW2 + col * kMlpHid + 32 * ks + 8 * g);
+        f.b1[nt] = bf16ToF32(wb[kB1 + col]);
+        f.b2[nt] = bf16ToF32(wb[kB2 + col]);
     }
 #pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) {
-        const uint32_t g = threadIdx.x + i * kMlpInferThreads;
-        if (g >= kGroups) continue;
-        const uint32_t h[4] = {r[i].x, r[i].y, r[i].z, r[i].w};
+    for (uint32_t ks = 0; ks < 2; ++ks) {
+        if (c16 < kMlpOut) {
+            f.w3[ks] = mlpLoad8(wb, kW3 + c16 * kMlpHid + 32 * ks + 8 * g);
+        } else {
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t p = g * 8 + 2 * q;
-            w[mlpShIndex(p)] = bf16ToF32(uint16_t(h[q] & 0xFFFFu));
-            w[mlpShIndex(p + 1)] = bf16ToF32(uint16_t(h[q] >> 16));
+            for (uint32_t i = 0; i < 8; ++i) f.w3[ks][i] = mlpBf(0);
         }
     }
-    for (uint32_t p = kGroups * 8 + threadIdx.x; p < kMlpParams; p += kMlpInferThreads) w[mlpShIndex(p)] = bf16ToF32(wb[p]);
+    f.b3 = c16 < kMlpOut ? bf16ToF32(wb[kB3 + c16]) : 0.0f;
 }
 
-// Rows [row0, row0 + 16) of x (n rows of kMlpIn) into act[0..16)[0..16), zero
+// One wave: 16 input rows (FP32, kMlpIn used of 16 per row, in `in`) -> the
+// outputs of rows row0 .. row0 + 15 that are below n, at y[(row - row0) kMlpOut + o].
+// `act` is the wave's BF16 tile. All of a layer's operands are read before its
+// MFMAs issue; every column tile sums its k-steps in order.
+__device__ inline void mlpInferTile(const MlpFrag& f, const float (*in)[16], __bf16 (*act)[kMlpActRow], uint32_t n,
+                                    uint32_t row0, float* y) {
+    const uint32_t lane = __lane_id();
+    const uint32_t r16 = lane % 16, g = lane / 16;
+    mlpF4 acc[4];
+    // layer 1: [16 x 32 (13 used)] x [32 x 64]
+    {
+        mlpBf8 a;
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) {
+            const uint32_t k = 8 * g + i;
+            a[i] = mlpToBf(k < 16 ? in[r16][k] : 0.0f);
+        }
+#pragma unroll
+        for (uint32_t nt = 0; nt < 4; ++nt)
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, f.w1[nt], mlpF4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    }
+#pragma unroll
+    for (uint32_t nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) act[4 * g + r][nt * 16 + r16] = mlpToBf(gelu(acc[nt][r] + f.b1[nt]));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 2: [16 x 64] x [64 x 64]
+    {
+        const mlpBf8 a0 = *reinterpret_cast<const mlpBf8*>(&act[r16][8 * g]);
+        const mlpBf8 a1 = *reinterpret_cast<const mlpBf8*>(&act[r16][32 + 8 * g]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read layer 1
+#pragma unroll
+        for (uint32_t nt = 0; nt < 4; ++nt) {
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, f.w2[nt][0], mlpF4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, f.w2[nt][1], acc[nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (uint32_t nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) act[4 * g + r][nt * 16 + r16] = mlpToBf(gelu(acc[nt][r] + f.b2[nt]));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 3: [16 x 64] x [64 x 16 (2 used)]
+    mlpF4 out;
+    {
+        const mlpBf8 a0 = *reinterpret_cast<const mlpBf8*>(&act[r16][8 * g]);
+        const mlpBf8 a1 = *reinterpret_cast<const mlpBf8*>(&act[r16][32 + 8 * g]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // act is reused by the next tile
+        out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, f.w3[0], mlpF4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+        out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, f.w3[1], out, 0, 0, 0);
+    }
+    if (r16 < kMlpOut)
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint32_t orow = row0 + 4 * g + r;
+            if (orow < n) y[(4 * g + r) * kMlpOut + r16] = out[r] + f.b3;
+        }
+}
+
+// Rows [row0, row0 + 16) of x (n rows of kMlpIn) into in[0..16)[0..16), zero
 // past the inputs and past n.
-__device__ inline void mlpLoadRows(const float* x, uint32_t n, uint32_t row0, float (*act)[kMlpActRow]) {
+__device__ inline void mlpLoadRows(const float* x, uint32_t n, uint32_t row0, float (*in)[16]) {
     const uint32_t lane = __lane_id();
     const uint32_t r16 = lane % 16, k4 = lane / 16;
     const uint32_t row = row0 + r16;
     for (uint32_t ks = 0; ks < 4; ++ks) {
         const uint32_t k = ks * 4 + k4;
-        act[r16][k] = (row < n && k < kMlpIn) ? x[uint64_t(row) * kMlpIn + k] : 0.0f;
+        in[r16][k] = (row < n && k < kMlpIn) ? x[uint64_t(row) * kMlpIn + k] : 0.0f;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// One wave: the 16 input rows in act[0..16)[0..16) (w in the LDS layout) -> the
-// outputs of rows row0 .. row0 + 15 that are below n, at y[(row - row0) kMlpOut + o]. Each layer reads all its operands
-// first and then issues its MFMAs k-step by k-step across the four independent
-// column tiles, so neither LDS latency nor the MFMA dependency is exposed per
-// instruction; every column tile still sums its k-steps in order.
-__device__ inline void mlpInferTile(const float* w, float (*act)[kMlpActRow], uint32_t n, uint32_t row0, float* y) {
-    const uint32_t lane = __lane_id();
-    const uint32_t r16 = lane % 16, k4 = lane / 16;
-    mlpF4 acc[4];
-    // layer 1: [16 x 16 (13 used)] x [16 x 64]
-    {
-        float a[4], b[4][4];
-#pragma unroll
-        for (uint32_t ks = 0; ks < 4; ++ks) {
-            const uint32_t k = ks * 4 + k4;
-            a[ks] = act[r16][k];
-#pragma unroll
-            for (uint32_t nt = 0; nt < 4; ++nt) b[nt][ks] = k < kMlpIn ? w[kW1 + (nt * 16 + r16) * kMlpIn + k] : 0.0f;
-        }
-#pragma unroll
-        for (uint32_t nt = 0; nt < 4; ++nt) acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (uint32_t ks = 0; ks < 4; ++ks)
-#pragma unroll
-            for (uint32_t nt = 0; nt < 4; ++nt)
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[nt][ks], acc[nt], 0, 0, 0);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read the inputs
-#pragma unroll
-    for (uint32_t nt = 0; nt < 4; ++nt)
-#pragma unroll
-        for (uint32_t r = 0; r < 4; ++r) act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kB1 + nt * 16 + r16]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // layer 2: [16 x 64] x [64 x 64]
-    {
-        float a[16], b[4][16];
-#pragma unroll
-        for (uint32_t ks = 0; ks < 16; ++ks) {
-            const uint32_t k = ks * 4 + k4;
-            a[ks] = act[r16][k];
-#pragma unroll
-            for (uint32_t nt = 0; nt < 4; ++nt) b[nt][ks] = w[kMlpShW2 + (nt * 16 + r16) * kMlpW2Row + k];
-        }
-#pragma unroll
-        for (uint32_t nt = 0; nt < 4; ++nt) acc[nt] = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (uint32_t ks = 0; ks < 16; ++ks)
-#pragma unroll
-            for (uint32_t nt = 0; nt < 4; ++nt)
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[nt][ks], acc[nt], 0, 0, 0);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read layer 1
-#pragma unroll
-    for (uint32_t nt = 0; nt < 4; ++nt)
-#pragma unroll
-        for (uint32_t r = 0; r < 4; ++r)
-            act[4 * k4 + r][nt * 16 + r16] = gelu(acc[nt][r] + w[kMlpShB2 + nt * 16 + r16]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // layer 3: [16 x 64] x [64 x 16 (2 used)]
-    mlpF4 out = mlpF4{0.0f, 0.0f, 0.0f, 0.0f};
-    {
-        float a[16], b[16];
-#pragma unroll
-        for (uint32_t ks = 0; ks < 16; ++ks) {
-            const uint32_t k = ks * 4 + k4;
-            a[ks] = act[r16][k];
-            b[ks] = r16 < kMlpOut ? w[kMlpShW3 + r16 * kMlpHid + k] : 0.0f;
-        }
-#pragma unroll
-        for (uint32_t ks = 0; ks < 16; ++ks) out = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ks], b[ks], out, 0, 0, 0);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // act is reused by the next tile
-    if (r16 < kMlpOut)
-        for (uint32_t r = 0; r < 4; ++r) {
-            const uint32_t orow = row0 + 4 * k4 + r;
-            if (orow < n) y[(4 * k4 + r) * kMlpOut + r16] = out[r] + w[kMlpShB3 + r16];
-        }
-}
+// The per-wave LDS of the inference kernels.
+struct alignas(16) MlpWaveShared {
+    float in[16][16];
+    __bf16 act[16][kMlpActRow];
+};
 
 // ---- features --------------------------------------------------------------------
 

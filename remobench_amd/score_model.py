@@ -1,18 +1,27 @@
 """The learned score: training-pair logging, training and evaluation on the GPU.
 
 A 13 -> 64 -> 64 -> 2 MLP (BF16 weights, FP32 accumulation, exact GELU) predicts
-a node's final (surface variation, planarity) from its partial accumulators, so
-Analysis can score open nodes before they close; the analytic eigen-solve stays
-the fallback (predicted surface variation above 0.05, and every closed node) and
-the judge of its error. All arithmetic runs in the HIP kernels of
+how a node's (surface variation, planarity) will still change before all its
+points have arrived, from its partial accumulators, so Analysis can score open
+nodes before they close; the analytic eigen-solve stays the base, the fallback
+(every closed node, and every predicted change below the thresholds) and the
+judge of its error. All arithmetic runs in the HIP kernels of
 `csrc/mlp.hip` through the C ABI; this module only moves arrays.
 
-Pairs are logged from plain-pipeline runs of the 2,097,152-point scene (8 batches
-of 262,144): after 1, 2, 4 and 6 batches (12.5 / 25 / 50 / 75 % arrival) every
-node's 13 inputs are recorded, and paired with the same node's surface variation
-and planarity once the whole cloud has arrived.
+Pairs are logged from plain-pipeline runs of the 2,097,152-point scene streamed
+in 20 batches: after 2, 5, 10 and 15 batches (10 / 25 / 50 / 75 % arrival) every
+node's 13 inputs and analytic (surface variation, planarity) are recorded, and
+paired with the same node's values once the whole cloud has arrived. Each
+training scene is streamed in several orders (its scan-line sweep started at
+other points and run backwards), which multiplies the partial states of one
+generated scene; held-out scenes are streamed in their own order.
 
-    python -m remobench_amd.score_model --seeds 100 --out remobench_amd/models/score_mlp.npy
+The model learns the change still to come (final minus the analytic values of
+the partial sums), and Analysis applies it only where it is large (see
+CHANGE_SV / CHANGE_PL): a node whose points have all arrived keeps its exact
+analytic values. The committed model was made with
+
+    python -m remobench_amd.score_model --seeds 2200 --variants 16 --eval-seeds 50 --parallel 8 --epochs 30
 """
 from __future__ import annotations
 
@@ -29,8 +38,10 @@ from . import lib as L
 from .pipeline import PointRing, Tree
 
 DEFAULT_MODEL = Path(__file__).resolve().parent / "models" / "score_mlp.npy"
-SNAPSHOT_BATCHES = (1, 2, 4, 6)
-FALLBACK_SV = 0.05
+SNAPSHOT_FRACTIONS = (0.10, 0.25, 0.50, 0.75)
+LOG_BATCHES = 20
+CHANGE_SV = 0.05   # the learned path's thresholds (kLearnedChangeSv / kLearnedChangePl)
+CHANGE_PL = 0.10
 
 
 class ScoreMLP:
@@ -106,42 +117,108 @@ class ScoreMLP:
         return m
 
 
-def log_pairs(seed: int, num_points: int = 2_097_152, batch: int = 262_144, snapshots=SNAPSHOT_BATCHES,
-              kind: str = "lidar"):
-    """Runs the plain pipeline on one seed and returns (X, Y, partial) where X are
-    node inputs at the snapshots, Y the same nodes' final (surface variation,
-    planarity), partial their analytic values at the snapshot."""
-    points, meta = L.generate_scene(num_points, seed, kind)
-    nb = (len(points) + batch - 1) // batch
-    ring = PointRing(batch, nb)
-    ring.set_cloud(points, batch)
-    ring.pump()
-    tree = Tree(meta, slot_capacity=batch, ring_slots=nb, batches_per_step=1, step_budget_ms=-1.0)
-    u = L.SharedUniforms()
-    snaps = []
-    try:
+def sweep_variant(points: np.ndarray, v: int, variants: int) -> np.ndarray:
+    """Stream order v of a scan-line-ordered scene: the sweep starts (v // 2) /
+    (variants // 2) of the way through the cloud and wraps (a sensor that starts
+    its sweep elsewhere), run backwards for odd v. Variant 0 is the scene's own
+    order."""
+    n = len(points)
+    half = max(1, variants // 2)
+    out = np.roll(points, -((n * (v // 2)) // half)) if v // 2 else points
+    if v % 2:
+        out = out[::-1]
+    return np.ascontiguousarray(out)
+
+
+class PairLogger:
+    """The plain pipeline over one scene, streamed in LOG_BATCHES batches: at each
+    snapshot (a fraction of the batches ingested) every node's 13 inputs and its
+    analytic (surface variation, planarity) are recorded, and paired with the same
+    node's values once the whole cloud has arrived. One context serves every
+    stream order of the scene (reset between them)."""
+
+    def __init__(self, meta: L.CloudMeta, num_points: int, batches: int = LOG_BATCHES):
+        self.batches = batches
+        self.batch = (num_points + batches - 1) // batches
+        self.ring = PointRing(self.batch, batches)
+        # sized for a 2M-point scene: a few hundred nodes, the whole cloud spillable
+        self.tree = Tree(meta, slot_capacity=self.batch, ring_slots=batches, batches_per_step=1, step_budget_ms=-1.0,
+                         max_nodes=16_384, spill_capacity=max(num_points, 1), backlog_capacity=num_points + 64)
+        self.u = L.SharedUniforms()
+
+    def close(self):
+        self.tree.close()
+        self.ring.close()
+
+    def run(self, points: np.ndarray, fractions=SNAPSHOT_FRACTIONS) -> dict:
+        snapshots = {max(1, int(round(f * self.batches))): f for f in fractions}
+        self.tree.reset()
+        self.ring.set_cloud(points, self.batch)
+        self.ring.pump()
+        snaps = []
         consumed = 0
-        while consumed < nb:
-            _, consumed = tree.build_step(ring.view())
-            tree.accumulate(consumed)
+        while consumed < self.batches:
+            _, consumed = self.tree.build_step(self.ring.view())
+            self.tree.accumulate(consumed)
             if consumed in snapshots:
-                tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
-                r = tree.analysis_results()
-                snaps.append((tree.features(), r["surface_variation"].copy(), r["planarity"].copy(), r["n"].copy()))
-        tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
-        final = tree.analysis_results()
+                self.tree.analysis(self.u, L.REMO_ANALYSIS_ANALYTIC)
+                r = self.tree.analysis_results()
+                nodes = self.tree.nodes()
+                snaps.append((snapshots[consumed], self.tree.features(), r["surface_variation"].copy(),
+                              r["planarity"].copy(), r["n"].copy(), nodes["level"].copy(),
+                              (nodes["first_child"] < 0).copy(), r["state"] == 2))
+        self.tree.analysis(self.u, L.REMO_ANALYSIS_ANALYTIC)
+        final = self.tree.analysis_results()
+        out = {k: [] for k in ("X", "Y", "P", "arrival", "level", "leaf", "closed", "n", "n_final")}
+        for frac, f, sv, pl, n, lvl, leaf, closed in snaps:
+            k = len(f)  # node indices are stable: nodes present at the snapshot keep theirs
+            ok = (n >= 3) & (final["n"][:k] >= 3)
+            out["X"].append(f[ok])
+            out["Y"].append(np.stack([final["surface_variation"][:k][ok], final["planarity"][:k][ok]], axis=1))
+            out["P"].append(np.stack([sv[ok], pl[ok]], axis=1))
+            out["arrival"].append(np.full(int(ok.sum()), frac, dtype=np.float32))
+            out["level"].append(lvl[ok])
+            out["leaf"].append(leaf[ok])
+            out["closed"].append(closed[ok])
+            out["n"].append(n[ok])
+            out["n_final"].append(final["n"][:k][ok])
+        return {k: np.concatenate(v) for k, v in out.items()}
+
+
+def log_pairs(seed: int, num_points: int = 2_097_152, variants: int = 1, kind: str = "lidar") -> dict:
+    """(partial, final) pairs of one seed's scene in `variants` stream orders."""
+    points, meta = L.generate_scene(num_points, seed, kind)
+    lg = PairLogger(meta, num_points)
+    try:
+        parts = [lg.run(sweep_variant(points, v, variants)) for v in range(variants)]
     finally:
-        tree.close()
-        ring.close()
-    X, Y, P = [], [], []
-    for f, sv, pl, n in snaps:
-        k = len(f)  # node indices are stable: nodes present at the snapshot keep theirs
-        ok = (n >= 3) & (final["n"][:k] >= 3)
-        X.append(f[ok])
-        Y.append(np.stack([final["surface_variation"][:k][ok], final["planarity"][:k][ok]], axis=1))
-        P.append(np.stack([sv[ok], pl[ok]], axis=1))
-    return (np.concatenate(X).astype(np.float32), np.concatenate(Y).astype(np.float32),
-            np.concatenate(P).astype(np.float32))
+        lg.close()
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+
+
+def log_many(seeds, num_points: int, variants: int, log=None) -> dict:
+    """Pairs of many seeds; the next scene is generated on a worker thread while
+    the GPU runs the current one (the generator releases the interpreter)."""
+    from concurrent.futures import ThreadPoolExecutor
+    seeds = list(seeds)
+    parts = []
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=1) as ex:
+        nxt = ex.submit(L.generate_scene, num_points, seeds[0], "lidar") if seeds else None
+        for i, seed in enumerate(seeds):
+            points, meta = nxt.result()
+            if i + 1 < len(seeds):
+                nxt = ex.submit(L.generate_scene, num_points, seeds[i + 1], "lidar")
+            lg = PairLogger(meta, num_points)
+            try:
+                for v in range(variants):
+                    parts.append(lg.run(sweep_variant(points, v, variants)))
+            finally:
+                lg.close()
+            if log and (i % 100 == 99 or i + 1 == len(seeds)):
+                n = sum(len(p["X"]) for p in parts)
+                log(f"{i + 1} scenes, {n:,} pairs, {time.perf_counter() - t0:.0f} s")
+    return {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
 
 
 def train(X: np.ndarray, Y: np.ndarray, epochs: int = 60, batch: int = 8192, lr: float = 1e-3, seed: int = 1,
@@ -167,53 +244,109 @@ def train(X: np.ndarray, Y: np.ndarray, epochs: int = 60, batch: int = 8192, lr:
     return m
 
 
-def evaluate(m: ScoreMLP, X: np.ndarray, Y: np.ndarray, P: np.ndarray) -> dict:
-    """Errors of the learned prediction and of the analytic value on partial data,
-    both against the final analytic (surface variation, planarity). "score error"
-    is the mean absolute surface-variation error over the mean final surface
-    variation (the score is 3 * surface variation * coverage, coverage shared).
-    The learned path is what Analysis uses: the prediction where it is <= 0.05,
-    the analytic value otherwise."""
-    pred = m.forward(X)
-    used = np.where((pred[:, 0] <= FALLBACK_SV)[:, None], pred, P)
-    msv = float(np.mean(np.abs(Y[:, 0])))
+def learned_path(delta: np.ndarray, partial: np.ndarray, closed: np.ndarray) -> np.ndarray:
+    """What learned-mode Analysis uses per node: the analytic values of the
+    partial sums plus the predicted change where the change is large (above
+    CHANGE_SV in surface variation or CHANGE_PL in planarity) and the node is
+    open; the analytic values otherwise (clamped as the kernel clamps)."""
+    take = ~closed & ((np.abs(delta[:, 0]) > CHANGE_SV) | (np.abs(delta[:, 1]) > CHANGE_PL))
+    out = partial.copy()
+    out[take, 0] = np.clip(partial[take, 0] + delta[take, 0], 0.0, np.float32(1.0 / 3.0))
+    out[take, 1] = np.clip(partial[take, 1] + delta[take, 1], 0.0, 1.0)
+    return out, take
+
+
+def evaluate(m: ScoreMLP, D: dict) -> dict:
+    """Errors against the final analytic (surface variation, planarity) of the
+    learned path, of the corrected value everywhere and of the analytic value on
+    the partial data. "score error" is the mean absolute surface-variation error
+    over the mean final surface variation (the score is 3 * surface variation *
+    coverage, coverage shared)."""
+    X, Y, P, closed = D["X"], D["Y"], D["P"], D["closed"].astype(bool)
+    delta = m.forward(X)
+    used, take = learned_path(delta, P, closed)
+    corrected = np.stack([np.clip(P[:, 0] + delta[:, 0], 0, 1 / 3), np.clip(P[:, 1] + delta[:, 1], 0, 1)], axis=1)
+    msv = max(float(np.mean(np.abs(Y[:, 0]))), 1e-12)
+
+    def errs(v):
+        return float(np.mean(np.abs(v[:, 0] - Y[:, 0])) / msv), float(np.mean(np.abs(v[:, 1] - Y[:, 1])))
+
+    (e_l, p_l), (e_c, p_c), (e_a, p_a) = errs(used), errs(corrected), errs(P)
     return {
         "pairs": int(len(X)),
-        "learned_fraction": float(np.mean(pred[:, 0] <= FALLBACK_SV)),
-        "sv_mae_learned_path": float(np.mean(np.abs(used[:, 0] - Y[:, 0]))),
-        "sv_mae_prediction": float(np.mean(np.abs(pred[:, 0] - Y[:, 0]))),
-        "sv_mae_partial_analytic": float(np.mean(np.abs(P[:, 0] - Y[:, 0]))),
-        "planarity_mae_prediction": float(np.mean(np.abs(pred[:, 1] - Y[:, 1]))),
-        "planarity_mae_partial_analytic": float(np.mean(np.abs(P[:, 1] - Y[:, 1]))),
-        "score_error_learned_path": float(np.mean(np.abs(used[:, 0] - Y[:, 0])) / max(msv, 1e-12)),
-        "score_error_partial_analytic": float(np.mean(np.abs(P[:, 0] - Y[:, 0])) / max(msv, 1e-12)),
+        "learned_fraction": float(np.mean(take)),
+        "score_error_learned_path": e_l, "planarity_mae_learned_path": p_l,
+        "score_error_corrected_everywhere": e_c, "planarity_mae_corrected_everywhere": p_c,
+        "score_error_partial_analytic": e_a, "planarity_mae_partial_analytic": p_a,
+        "mean_final_sv": msv,
     }
+
+
+def _load_pairs(paths) -> tuple[dict, dict]:
+    parts = [np.load(p, allow_pickle=False) for p in paths]
+    keys = [k[6:] for k in parts[0].files if k.startswith("train_")]
+    tr = {k: np.concatenate([p[f"train_{k}"] for p in parts]) for k in keys}
+    ev = {k: np.concatenate([p[f"eval_{k}"] for p in parts if f"eval_{k}" in p.files]) for k in keys}
+    return tr, ev
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--seeds", type=int, default=100, help="training scenes (seeds 1000..)")
-    ap.add_argument("--eval-seeds", type=int, default=10, help="held-out scenes (seeds 5000..)")
+    ap.add_argument("--seeds", type=int, default=2200, help="training scenes (seeds 1000..)")
+    ap.add_argument("--variants", type=int, default=16, help="stream orders per training scene")
+    ap.add_argument("--eval-seeds", type=int, default=50, help="held-out scenes (seeds 5000.., their own order)")
     ap.add_argument("--points", type=int, default=2_097_152)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--out", type=Path, default=DEFAULT_MODEL)
+    ap.add_argument("--shard", default="0/1", help="log only shard k of K of the seeds (k/K), for parallel loggers")
+    ap.add_argument("--save-pairs", type=Path, default=None, help="write the logged pairs (npz) and stop")
+    ap.add_argument("--pairs", type=Path, nargs="*", default=None, help="train on pairs saved by --save-pairs")
+    ap.add_argument("--parallel", type=int, default=1,
+                    help="log with this many processes (each a --shard), then train here on what they saved")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if args.parallel > 1 and not args.pairs and not args.save_pairs:
+        # the loggers start before this process touches the GPU
+        import subprocess
+        import tempfile
+        tmp = Path(tempfile.mkdtemp(prefix="score_pairs_"))
+        K = args.parallel
+        procs = [subprocess.Popen([sys.executable, "-m", "remobench_amd.score_model", "--seeds", str(args.seeds),
+                                   "--variants", str(args.variants), "--eval-seeds", str(args.eval_seeds),
+                                   "--points", str(args.points), "--shard", f"{k}/{K}",
+                                   "--save-pairs", str(tmp / f"pairs_{k}.npz")]) for k in range(K)]
+        codes = [p.wait() for p in procs]
+        if any(codes):
+            log(f"a logger failed: exit codes {codes}")
+            return 1
+        args.pairs = [tmp / f"pairs_{k}.npz" for k in range(K)]
     if L.device_count() < 1:
         log("no GPU visible")
         return 1
-    t0 = time.perf_counter()
-    parts = [log_pairs(1000 + s, args.points) for s in range(args.seeds)]
-    X = np.concatenate([p[0] for p in parts])
-    Y = np.concatenate([p[1] for p in parts])
-    log(f"logged {len(X):,} pairs from {args.seeds} scenes in {time.perf_counter() - t0:.1f} s")
+    if args.pairs:
+        tr, ev = _load_pairs(args.pairs)
+    else:
+        k, K = (int(v) for v in args.shard.split("/"))
+        t0 = time.perf_counter()
+        tr = log_many(range(1000 + k, 1000 + args.seeds, K), args.points, args.variants, log)
+        log(f"logged {len(tr['X']):,} pairs from {len(range(k, args.seeds, K))} scenes x {args.variants} orders in "
+            f"{time.perf_counter() - t0:.1f} s")
+        ev = log_many(range(5000 + k, 5000 + args.eval_seeds, K), args.points, 1, log)
+        if args.save_pairs:
+            args.save_pairs.parent.mkdir(parents=True, exist_ok=True)
+            np.savez(args.save_pairs, **{f"train_{k}": v for k, v in tr.items()},
+                     **{f"eval_{k}": v for k, v in ev.items()})
+            return 0
+    log(f"training on {len(tr['X']):,} pairs; {len(ev['X']):,} held-out pairs")
     t1 = time.perf_counter()
-    m = train(X, Y, epochs=args.epochs, log=log)
+    # the model learns the change still to come: final minus the analytic values of the partial sums
+    m = train(tr["X"], tr["Y"] - tr["P"], epochs=args.epochs, log=log)
     log(f"trained in {time.perf_counter() - t1:.1f} s")
-    ev = [log_pairs(5000 + s, args.points) for s in range(args.eval_seeds)]
-    res = evaluate(m, np.concatenate([e[0] for e in ev]), np.concatenate([e[1] for e in ev]),
-                   np.concatenate([e[2] for e in ev]))
-    res["train"] = evaluate(m, X, Y, np.concatenate([p[2] for p in parts]))
+    res = evaluate(m, ev)
+    res["by_arrival"] = {f"{a:.2f}": evaluate(m, {k: v[ev["arrival"] == a] for k, v in ev.items()})
+                         for a in SNAPSHOT_FRACTIONS}
+    res["train"] = evaluate(m, tr)
+    res["train_pairs"] = int(len(tr["X"]))
     m.save(args.out)
     res["model"] = str(args.out)
     print(json.dumps(res), flush=True)

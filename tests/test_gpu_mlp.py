@@ -26,8 +26,15 @@ def test_forward_matches_numpy(gpu):
     m = ScoreMLP(seed=3, max_batch=1024)
     m.set_params(theta)
     got = m.forward(x)
-    want = M.forward(M.bf16(theta), x)  # BF16 weights, FP32 activations
-    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-5)
+    # BF16 weights and BF16 layer inputs on the matrix cores, FP32 accumulation
+    want = M.forward_infer(theta, x)
+    err = np.abs(got - want)
+    # summation order and erf differ in the last bits; where such a difference
+    # flips an activation's BF16 rounding the output moves by ~1e-4
+    assert np.mean(err <= 1e-5 + 1e-5 * np.abs(want)) > 0.9, np.quantile(err, [0.5, 0.9, 0.99])
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-3)
+    # and within BF16's precision of the FP32-activation network the training kernels run
+    np.testing.assert_allclose(got, M.forward(M.bf16(theta), x), rtol=0, atol=3e-2 * np.abs(want).max())
     np.testing.assert_array_equal(m.params(), theta)  # the FP32 master is kept as given
 
 
@@ -118,14 +125,17 @@ def test_features_match_the_oracle_roll_up(built):
 
 @pytest.mark.gpu
 def test_learned_analysis_uses_the_model_for_open_nodes_only(built):
+    """The model predicts a change to the analytic values: an open node takes it
+    where it exceeds 0.05 in surface variation or 0.1 in planarity, clamped to the
+    quantities' ranges; closed nodes and smaller changes keep the analytic values."""
     points, meta, tree, ring, o = built
     u = L.SharedUniforms()
     tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
     analytic = tree.analysis_results()
     m = ScoreMLP(seed=1, max_batch=64)
     theta = np.zeros(M.PARAMS, dtype=np.float32)
-    # a constant prediction: surface variation 0.01 (<= 0.05: used), planarity 0.7
-    theta[-2:] = [0.01, 0.7]
+    # a constant prediction: surface variation +0.1 (> 0.05: taken), planarity -0.05
+    theta[-2:] = [0.1, -0.05]
     m.set_params(theta)
     tree.set_score_model(m)
     tree.analysis(u, L.REMO_ANALYSIS_LEARNED)
@@ -134,47 +144,65 @@ def test_learned_analysis_uses_the_model_for_open_nodes_only(built):
     closed = r["state"] == 2
     assert open_.sum() > 0 and closed.sum() > 0
     assert np.all(r["flags"][open_] & 2)
-    np.testing.assert_allclose(r["surface_variation"][open_], np.float32(M.bf16(np.float32(0.01))))
-    np.testing.assert_allclose(r["planarity"][open_], np.float32(M.bf16(np.float32(0.7))))
-    for k in ("surface_variation", "planarity", "score", "lambda"):
+    d = M.bf16(np.float32([0.1, -0.05]))
+    np.testing.assert_array_equal(r["surface_variation"][open_],
+                                  np.clip(analytic["surface_variation"][open_] + d[0], 0, np.float32(1 / 3)))
+    np.testing.assert_array_equal(r["planarity"][open_], np.clip(analytic["planarity"][open_] + d[1], 0, 1))
+    for k in ("linearity", "lambda"):
+        np.testing.assert_array_equal(r[k], analytic[k])
+    for k in ("surface_variation", "planarity", "score"):
         np.testing.assert_array_equal(r[k][closed], analytic[k][closed])
     assert not np.any(r["flags"][closed] & 2)
-    # a prediction above 0.05 falls back to the analytic eigen-solve everywhere
-    theta[-2:] = [0.2, 0.7]
+    # changes under both thresholds leave every node analytic
+    theta[-2:] = [0.04, -0.09]
     m.set_params(theta)
     tree.set_score_model(m)
     tree.analysis(u, L.REMO_ANALYSIS_LEARNED)
     r2 = tree.analysis_results()
     for k in ("surface_variation", "planarity", "linearity", "score", "lambda", "flags"):
         np.testing.assert_array_equal(r2[k], analytic[k])
+    # a planarity change alone is enough
+    theta[-2:] = [0.0, 0.2]
+    m.set_params(theta)
+    tree.set_score_model(m)
+    tree.analysis(u, L.REMO_ANALYSIS_LEARNED)
+    r3 = tree.analysis_results()
+    assert np.all(r3["flags"][open_] & 2)
+    np.testing.assert_array_equal(r3["planarity"][open_],
+                                  np.clip(analytic["planarity"][open_] + M.bf16(np.float32(0.2)), 0, 1))
 
 
 @pytest.mark.gpu
 def test_learned_analysis_predictions_equal_inference_on_the_features(built):
     """Analysis computes each node's inputs and prediction in one kernel: the
-    predictions it applies are those of batched inference on the features
-    entry point's rows, and exactly the open nodes predicted <= 0.05 take them."""
+    changes it applies are those of batched inference on the features entry
+    point's rows, and exactly the open nodes with a large predicted change take
+    them."""
     points, meta, tree, ring, o = built
     u = L.SharedUniforms()
     tree.analysis(u, L.REMO_ANALYSIS_ANALYTIC)
+    analytic = tree.analysis_results()
     f = tree.features()
     m = ScoreMLP(seed=1, max_batch=len(f))
     theta = _theta(7, 0.1)
     theta[-2:] = 0.0
     m.set_params(theta)
     p0 = m.forward(f)
-    theta[-2:] = [0.05 - np.median(p0[:, 0]), 0.5]  # about half the nodes under the threshold
+    # surface variation centred on its threshold (about half the nodes over it), planarity on 0
+    theta[-2:] = [0.05 - np.median(p0[:, 0]), -np.median(p0[:, 1])]
     m.set_params(theta)
     pred = m.forward(f)
     tree.set_score_model(m)
     tree.analysis(u, L.REMO_ANALYSIS_LEARNED)
     r = tree.analysis_results()
     open_ = (r["state"] == 0) & (r["n"] >= 3)
-    take = open_ & (pred[:, 0] <= np.float32(0.05))
+    take = open_ & ((np.abs(pred[:, 0]) > np.float32(0.05)) | (np.abs(pred[:, 1]) > np.float32(0.1)))
     assert take.sum() > 10 and (open_ & ~take).sum() > 10
     np.testing.assert_array_equal((r["flags"] & 2) != 0, take)
-    np.testing.assert_allclose(r["surface_variation"][take], np.maximum(pred[take, 0], 0), rtol=1e-6, atol=1e-7)
-    np.testing.assert_allclose(r["planarity"][take], np.clip(pred[take, 1], 0, 1), rtol=1e-6, atol=1e-7)
+    np.testing.assert_array_equal(r["surface_variation"][take],
+                                  np.clip(analytic["surface_variation"][take] + pred[take, 0], 0, np.float32(1 / 3)))
+    np.testing.assert_array_equal(r["planarity"][take], np.clip(analytic["planarity"][take] + pred[take, 1], 0, 1))
+    np.testing.assert_array_equal(r["surface_variation"][~take], analytic["surface_variation"][~take])
 
 
 @pytest.mark.gpu

@@ -59,3 +59,37 @@ def test_adam_first_step_is_lr_times_sign():
     g = np.array([0.3, -2.0, 1e-3])
     theta, m, v = M.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, 1e-3)
     np.testing.assert_allclose(theta, -1e-3 * np.sign(g), rtol=1e-4)
+
+
+def test_bf16_rounding_and_inference_forward_against_torch():
+    """The reference's BF16 rounding (nearest even) and its inference forward,
+    against torch's bfloat16 casts with float64 products: an independent
+    implementation of the same rounding and the same network."""
+    import torch
+    rng = np.random.default_rng(31)
+    edge = np.array([1.0, 1.00390625, 1.005859375, 1.001953125, 1.0009765625, -3.14159, 6e-39, 1e38, 0.0, -0.0],
+                    dtype=np.float32)
+    vals = np.concatenate([edge, rng.normal(0, 3, 10000).astype(np.float32)])
+    want = torch.from_numpy(vals).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(M.bf16(vals), want)
+
+    theta = rng.normal(0, 0.2, M.PARAMS).astype(np.float32)
+    x = rng.normal(0, 1, (256, M.IN)).astype(np.float32)
+    p = {k: torch.from_numpy(np.ascontiguousarray(v)).to(torch.bfloat16).to(torch.float64)
+         for k, v in M.unpack(theta).items()}
+
+    def rb(t):
+        return t.to(torch.float32).to(torch.bfloat16).to(torch.float64)
+
+    def gelu(z):
+        return 0.5 * z * (1.0 + torch.erf(z / np.sqrt(2.0)))
+
+    h0 = rb(torch.from_numpy(x).to(torch.float64))
+    h1 = rb(gelu(h0 @ p["W1"].T + p["b1"]))
+    h2 = rb(gelu(h1 @ p["W2"].T + p["b2"]))
+    y = (h2 @ p["W3"].T + p["b3"]).numpy()
+    got = M.forward_infer(theta, x)
+    # float32 against float64 accumulation; a rounding flip of one activation moves
+    # an output by ~1e-4 at these weights
+    err = np.abs(got - y)
+    assert np.mean(err < 1e-5) > 0.9 and err.max() < 2e-3
