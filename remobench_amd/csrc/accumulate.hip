@@ -8,10 +8,14 @@
 //
 // Dirty leaves carry 1k-50k new points each, so one workgroup per leaf (the
 // reference's shape) leaves most of the GPU idle behind the largest leaves.
-// Here a plan pass cuts every dirty range into tiles of 2,048 points, one
-// persistent launch folds the tiles into partial sums, and a finish pass adds
-// each leaf's partials in tile order, which keeps the FP64 and FP32 sums
-// deterministic for a given point order.
+// Here a plan pass cuts every dirty range into tiles of 2,048 points and lists
+// the leaves it planned, one persistent launch folds the tiles into partial
+// sums, and a finish pass adds each listed leaf's partials in tile order (a wave
+// per leaf), which keeps the FP64 and FP32 sums deterministic for a given point
+// order. (Finishing a leaf in the workgroup that folds its last tile needs a
+// device-scope release and acquire per tile, which write back and invalidate the
+// per-XCD L2s here: 61 against 40 us per step.) The plan gathers the verify sums
+// (sum of leaf counts, inner nodes holding sums) as it visits every node.
 //
 // Besides the reference's FP64 geometry sums g[9] and FP32 colour sums c[6], every
 // point adds its node-centred fixed-point coordinate q = floor(l * 2^20) - 2^19
@@ -77,38 +81,56 @@ __device__ __forceinline__ uint64_t waveMax(uint64_t v) {
 // Plan: per node, clear an inner node's sums; cut a dirty leaf's new range into
 // tiles. A leaf whose tiles do not fit waits for the next launch (the watermark
 // makes the pass incremental, plans/03_AccumulatorHook.md:562-573).
-__global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w) {
+__global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w, uint32_t parity) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         g->pointsFolded = 0;
         g->leavesFolded = 0;
-        g->sumLeafCounts = 0;
-        g->innerWithSums = 0;
     }
-    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
-        remo_node_accum* a = t.accums + node;
-        w.nodeTiles[node] = 0;
-        if (t.firstChild[node] >= 0) {
-            if (a->count != 0) *a = remo_node_accum{};
-            continue;
+    // the verify sums (kernel_accum_verify, remolod_accum.cu:214-233) over the
+    // state this launch leaves: a leaf's count after its fold, inner nodes' sums
+    // after they are cleared
+    unsigned long long sumCounts = 0;
+    uint32_t innerSums = 0;
+    for (uint32_t base0 = blockIdx.x * blockDim.x; base0 < n; base0 += gridDim.x * blockDim.x) {
+        const uint32_t node = base0 + threadIdx.x;
+        if (node < n) {
+            remo_node_accum* a = t.accums + node;
+            w.nodeTiles[node] = 0;
+            if (t.firstChild[node] >= 0) {
+                if (a->count != 0) *a = remo_node_accum{};
+                innerSums += a->count != 0 ? 1u : 0u;
+            } else {
+                const uint32_t from = a->count, to = t.numPoints[node];
+                uint32_t after = from;
+                if (to > from) {
+                    const uint32_t nt = (to - from + AccumWork::kTile - 1) / AccumWork::kTile;
+                    const uint32_t base = atomicAdd(w.numTiles + parity, nt);
+                    if (uint64_t(base) + nt > w.tileCap) {
+                        // no room: mark the reserved slots below the capacity empty
+                        for (uint32_t k = 0; base + k < w.tileCap && k < nt; ++k)
+                            w.tiles[base + k] = make_uint4(kNone, 0, 0, 0);
+                    } else {
+                        for (uint32_t k = 0; k < nt; ++k) {
+                            const uint32_t b = from + k * AccumWork::kTile;
+                            w.tiles[base + k] = make_uint4(node, b, min(to, b + AccumWork::kTile), k);
+                        }
+                        w.nodeTiles[node] = nt;
+                        w.nodeBase[node] = base;
+                        w.planned[atomicAdd(w.numPlanned + parity, 1u)] = node;
+                        after = to;
+                    }
+                }
+                sumCounts += after;
+            }
         }
-        const uint32_t from = a->count, to = t.numPoints[node];
-        if (to <= from) continue;
-        const uint32_t nt = (to - from + AccumWork::kTile - 1) / AccumWork::kTile;
-        const uint32_t base = atomicAdd(w.numTiles, nt);
-        if (uint64_t(base) + nt > w.tileCap) {
-            // no room: mark the reserved slots below the capacity empty
-            for (uint32_t k = 0; base + k < w.tileCap && k < nt; ++k)
-                w.tiles[base + k] = make_uint4(kNone, 0, 0, 0);
-            continue;
-        }
-        for (uint32_t k = 0; k < nt; ++k) {
-            const uint32_t b = from + k * AccumWork::kTile;
-            w.tiles[base + k] = make_uint4(node, b, min(to, b + AccumWork::kTile), k);
-        }
-        w.nodeTiles[node] = nt;
-        w.nodeBase[node] = base;
+    }
+    sumCounts = waveSum(sumCounts);
+    innerSums = waveSum(innerSums);
+    if (laneId() == 0) {
+        if (sumCounts) atomicAdd((unsigned long long*)&g->sumLeafCountsAcc, sumCounts);
+        if (innerSums) atomicAdd(&g->innerWithSumsAcc, innerSums);
     }
 }
 
@@ -116,7 +138,65 @@ __global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w) {
 // i: a launch holds fewer tiles (~760 on the 100M-point scene) than blocks, and
 // hardware blocks go round-robin to the 8 XCDs, so every XCD gets an equal share
 // (an XCD-contiguous renumbering would hand all of them to the first three).
-__global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
+// The leaf's partials [base, base + nt) added in tile order into its sums (one
+// field per thread, from LDS), its watermark count advanced, the folded counts
+// into the globals, the leaf onto Analysis' dirty list: kernel_accumulate's
+// per-leaf tail (remolod_accum.cu:190-212). The partials are loaded in one round
+// by the whole workgroup (a field's loads one after another, tile by tile, were
+// a chain of dependent memory round trips).
+constexpr uint32_t kMaxLeafTiles = (kMaxPointsPerNode + AccumWork::kTile - 1) / AccumWork::kTile + 1;
+static_assert(sizeof(AccumPartial) % 8 == 0, "partials load as 8-byte words");
+
+__device__ void finishLeaf(const DevTree& t, const AccumWork& w, uint32_t node, uint32_t batchIndex,
+                           AccumPartial* sp) {
+    const uint32_t nt = w.nodeTiles[node], base = w.nodeBase[node];
+    const uint32_t nLoad = min(nt, kMaxLeafTiles);
+    constexpr uint32_t kVecs = sizeof(AccumPartial) / 8;
+    const uint2* src = reinterpret_cast<const uint2*>(w.partials + base);
+    uint2* dst = reinterpret_cast<uint2*>(sp);
+    for (uint32_t v = threadIdx.x; v < nLoad * kVecs; v += blockDim.x) dst[v] = src[v];
+    __syncthreads();
+    remo_node_accum* a = t.accums + node;
+    const uint32_t f = threadIdx.x;
+    auto P = [&](uint32_t k) -> const AccumPartial& { return k < nLoad ? sp[k] : w.partials[base + k]; };
+    if (f < 9) {
+        long long v = 0;
+        double d = 0.0;
+        for (uint32_t k = 0; k < nt; ++k) {
+            v += P(k).q[f];
+            d += P(k).g[f];
+        }
+        if (f < 3)
+            a->q1[f] += v;
+        else
+            a->q2[f - 3] += v;
+        a->g[f] += d;
+    } else if (f < 15) {
+        float v = 0.0f;
+        for (uint32_t k = 0; k < nt; ++k) v += P(k).c[f - 9];
+        a->c[f - 9] += v;
+    } else if (f == 15) {
+        unsigned long long mm = 0;
+        for (uint32_t k = 0; k < nt; ++k) {
+            const unsigned long long m = P(k).morton;
+            mm = m > mm ? m : mm;
+        }
+        if (mm > a->morton_max) a->morton_max = mm;
+        if (mm) atomicMax((unsigned long long*)&t.g->mortonWatermark, mm);
+    } else if (f == 16) {
+        const uint32_t from = a->count, to = w.tiles[base + nt - 1].z;
+        a->count = to;
+        a->last_touched_batch = batchIndex;
+        Globals* g = t.g;
+        atomicAdd((unsigned long long*)&g->numAccumulated, (unsigned long long)(to - from));
+        atomicAdd(&g->pointsFolded, to - from);
+        atomicAdd(&g->leavesFolded, 1u);
+        listDirty(t, node);  // its sums changed: Analysis rolls it up again
+    }
+    __syncthreads();  // sp is reused for the next leaf
+}
+
+__global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w, uint32_t parity) {
     __shared__ long long sq[9][kWaves];
     __shared__ double sg[9][kWaves];
     __shared__ float sc[6][kWaves];
@@ -124,7 +204,9 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
     const uint32_t lane = laneId(), wave = threadIdx.x >> 6;
     const double invSize = 1.0 / double(t.octreeSize);
     const double fGrid = double(1u << kMaxDepth);
-    const uint32_t numTiles = min(*w.numTiles, w.tileCap);
+    const uint32_t numTiles = min(w.numTiles[parity], uint32_t(w.tileCap));
+    // the next launch plans into the other counter
+    if (blockIdx.x == 0 && threadIdx.x == 0) w.numTiles[parity ^ 1u] = 0;
 
     for (uint32_t ti = blockIdx.x; ti < numTiles; ti += gridDim.x) {
         const uint4 tile = w.tiles[ti];
@@ -235,85 +317,21 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w) {
     }
 }
 
-// Finish: add each leaf's partials in tile order, advance its watermark count,
-// and verify (kernel_accum_verify, remolod_accum.cu:214-233).
-__global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, uint32_t batchIndex) {
-    Globals* g = t.g;
-    const uint32_t n = g->numNodes;
-    // the tile counter starts at zero for the next launch (kAccumTiles is done with it)
-    if (blockIdx.x == 0 && threadIdx.x == 0) *w.numTiles = 0;
-    unsigned long long sumCounts = 0, folded = 0, wm = 0;
-    uint32_t inner = 0, leaves = 0;
-    // wave-uniform steps (the dirty-list append below is one atomic per wave)
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t node = base + threadIdx.x;
-        bool changed = false;
-        remo_node_accum* a = t.accums + (node < n ? node : 0u);
-        const bool isInner = node < n && t.firstChild[node] >= 0;
-        if (isInner && a->count != 0) inner++;
-        const uint32_t nt = (node < n && !isInner) ? w.nodeTiles[node] : 0u;
-        if (nt) {
-            changed = true;
-            const uint32_t base = w.nodeBase[node];
-            long long q[9];
-            double gs[9];
-            float cs[6];
-            for (int i = 0; i < 9; ++i) {
-                q[i] = 0;
-                gs[i] = 0.0;
-            }
-            for (int i = 0; i < 6; ++i) cs[i] = 0.0f;
-            unsigned long long mm = 0;
-            for (uint32_t k = 0; k < nt; ++k) {
-                const AccumPartial& pp = w.partials[base + k];
-                for (int i = 0; i < 9; ++i) {
-                    q[i] += pp.q[i];
-                    gs[i] += pp.g[i];
-                }
-                for (int i = 0; i < 6; ++i) cs[i] += pp.c[i];
-                mm = pp.morton > mm ? pp.morton : mm;
-            }
-            for (int i = 0; i < 3; ++i) a->q1[i] += q[i];
-            for (int i = 0; i < 6; ++i) a->q2[i] += q[3 + i];
-            for (int i = 0; i < 9; ++i) a->g[i] += gs[i];
-            for (int i = 0; i < 6; ++i) a->c[i] += cs[i];
-            const uint4 last = w.tiles[base + nt - 1];
-            const uint32_t from = a->count, to = last.z;
-            a->count = to;
-            a->last_touched_batch = batchIndex;
-            if (mm > a->morton_max) a->morton_max = mm;
-            wm = mm > wm ? mm : wm;
-            folded += to - from;
-            leaves++;
-        }
-        if (node < n && !isInner) sumCounts += a->count;
-        // the folded leaves go on Analysis' dirty list
-        const bool add = changed && t.dirtyListed[node] == 0;
-        if (add) t.dirtyListed[node] = 1;
-        const uint64_t m = __ballot(add);
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t at = 0;
-            if (laneId() == uint32_t(leader)) at = atomicAdd(t.dirtyCount, uint32_t(__popcll(m)));
-            at = __shfl(at, leader);
-            if (add) t.dirtyList[at + uint32_t(__popcll(m & lanesBelow()))] = node;
-        }
+// The folded leaves (the plan listed them), a workgroup each: finishLeaf. Block 0
+// publishes the plan's verify sums.
+__global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, uint32_t batchIndex,
+                                                       uint32_t parity) {
+    __shared__ AccumPartial sp[kMaxLeafTiles];
+    const uint32_t n = w.numPlanned[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.numPlanned[parity ^ 1u] = 0;  // the next launch lists into the other counter
+        Globals* g = t.g;
+        g->sumLeafCounts = g->sumLeafCountsAcc;
+        g->innerWithSums = g->innerWithSumsAcc;
+        g->sumLeafCountsAcc = 0;
+        g->innerWithSumsAcc = 0;
     }
-    sumCounts = waveSum(sumCounts);
-    folded = waveSum(folded);
-    inner = waveSum(inner);
-    leaves = waveSum(leaves);
-    wm = waveMax(wm);
-    if (laneId() == 0) {
-        if (sumCounts) atomicAdd((unsigned long long*)&g->sumLeafCounts, sumCounts);
-        if (inner) atomicAdd(&g->innerWithSums, inner);
-        if (folded) {
-            atomicAdd((unsigned long long*)&g->numAccumulated, folded);
-            atomicAdd(&g->pointsFolded, uint32_t(folded));
-        }
-        if (leaves) atomicAdd(&g->leavesFolded, leaves);
-        if (wm) atomicMax((unsigned long long*)&g->mortonWatermark, wm);
-    }
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) finishLeaf(t, w, w.planned[k], batchIndex, sp);
 }
 
 // The FP32-atomics comparator: every stored point of every leaf adds its
@@ -363,11 +381,11 @@ void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const La
     hipLaunchKernelGGL(kAccumFp32Atomics, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, out, cap);
 }
 
-void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(kAccumPlan, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w);
-    hipLaunchKernelGGL(kAccumTiles, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, w);
-    hipLaunchKernelGGL(kAccumFinish, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w, batchIndex);
+void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, uint32_t parity,
+                      const LaunchShape& ls, hipStream_t s) {
+    hipLaunchKernelGGL(kAccumPlan, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w, parity);
+    hipLaunchKernelGGL(kAccumTiles, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, w, parity);
+    hipLaunchKernelGGL(kAccumFinish, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t, w, batchIndex, parity);
 }
 
 }  // namespace remo_hip

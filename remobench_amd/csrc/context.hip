@@ -165,6 +165,7 @@ struct remo_ctx {
     std::unique_ptr<uint32_t[]> deepenIds;        // [maxNodes] the chosen leaves
     uint8_t* staticScratch = nullptr;             // this device's address of `scratch`
     AccumWork accWork;
+    uint32_t accumParity = 0;  // which of the two tile counters the next Accumulate plans into
     Profiler prof;
 };
 
@@ -456,7 +457,9 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         A(&w.partials, w.tileCap, "accumulate partial sums");
         A(&w.nodeTiles, N, "accumulate tiles per node");
         A(&w.nodeBase, N, "accumulate first tile per node");
-        A(&w.numTiles, 1, "accumulate tile count");
+        A(&w.planned, N, "accumulate planned leaves");
+        A(&w.numPlanned, 2, "accumulate planned leaf counts");
+        A(&w.numTiles, 2, "accumulate tile counts");
     }
     c->analysis.closureMargin = cfg.closure_batches;
     c->analysis.shape = FrameShape{c->shape.nodeBlocks, c->shape.pointBlocks, 1000u};
@@ -504,7 +507,9 @@ int remo_reset(remo_ctx* c) {
     CK(hipMemsetAsync(t.accums, 0, sizeof(remo_node_accum) * uint64_t(t.maxNodes), c->stream), "reset");
     // node indices restart at 0: every chunk directory starts empty (splits rely on it)
     CK(hipMemsetAsync(t.pointDir, 0, 4ull * kPointDir * uint64_t(t.maxNodes), c->stream), "reset");
-    CK(hipMemsetAsync(c->accWork.numTiles, 0, sizeof(uint32_t), c->stream), "reset");
+    CK(hipMemsetAsync(c->accWork.numTiles, 0, 2 * sizeof(uint32_t), c->stream), "reset");
+    CK(hipMemsetAsync(c->accWork.numPlanned, 0, 2 * sizeof(uint32_t), c->stream), "reset");
+    c->accumParity = 0;
     CK(hipMemsetAsync(t.backlogFill, 0, 4ull * kBacklogShards * kShardStride, c->stream), "reset");
     CK(hipMemsetAsync(t.voxelShardCount, 0, 4ull * kBacklogShards * t.maxGrids, c->stream), "reset");
     CK(hipMemsetAsync(t.dirtyListed, 0, 4ull * t.maxNodes, c->stream), "reset");
@@ -624,7 +629,8 @@ int remo_accumulate(remo_ctx* c, uint32_t batchIndex) {
     if (c->cfg.flags & REMO_FLAG_NO_ACCUM) return REMO_OK;
     {
         Scope sc(c->prof, kScopeAccumulate, c->stream);
-        launchAccumulate(c->t, c->accWork, batchIndex, c->shape, c->stream);
+        launchAccumulate(c->t, c->accWork, batchIndex, c->accumParity, c->shape, c->stream);
+        c->accumParity ^= 1u;
     }
     CK(hipGetLastError(), "accumulate kernels");
     return REMO_OK;

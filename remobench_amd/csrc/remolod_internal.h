@@ -95,6 +95,10 @@ struct Globals {
     uint32_t gridsTaken;           // pooled grids taken by the current split round
     uint32_t droppedVoxels;        // voxels of the nodes the last collapse merged
     uint64_t totCollapsed;
+    // kernel_accum_verify's sums as Accumulate's plan gathers them (its finish
+    // publishes them into sumLeafCounts / innerWithSums and clears these)
+    uint64_t sumLeafCountsAcc;
+    uint32_t innerWithSumsAcc, pad3;
 };
 
 // Everything a kernel needs, passed by value (one kernel argument).
@@ -180,7 +184,9 @@ struct AccumWork {
     AccumPartial* partials;  // one per tile
     uint32_t* nodeTiles;     // [maxNodes] tiles of the node this launch
     uint32_t* nodeBase;      // [maxNodes] first tile of the node
-    uint32_t* numTiles;
+    uint32_t* planned;       // [maxNodes] the leaves planned this launch
+    uint32_t* numPlanned;    // [2] leaves planned, by launch parity (a launch clears the other's)
+    uint32_t* numTiles;      // [2] tiles planned, by launch parity
     uint64_t tileCap;
 };
 
@@ -339,7 +345,8 @@ void launchCollapse(const DevTree& t, uint32_t numChosen, BatchRef empty, uint32
                     const LaunchShape& ls, hipStream_t s);
 void launchStatsSweep(const DevTree& t, const LaunchShape& ls, hipStream_t s);
 void launchAccumFp32Atomics(const DevTree& t, float* out, uint32_t cap, const LaunchShape& ls, hipStream_t s);
-void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, const LaunchShape& ls,
-                      hipStream_t s);
+// parity: this launch's tile counter (launches alternate, see AccumWork::numTiles)
+void launchAccumulate(const DevTree& t, const AccumWork& w, uint32_t batchIndex, uint32_t parity,
+                      const LaunchShape& ls, hipStream_t s);
 
 }  // namespace remo_hip
