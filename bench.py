@@ -101,7 +101,7 @@ SCOPE_KERNELS = {
     "remolod.construct.insertPoints": ["kInsertPoints"],
     "remolod.construct.insertVoxels": ["kInsertVoxels"],
     "remolod.accumulate": ["kAccumPlan", "kAccumTiles", "kAccumFinish"],
-    "remolod.analysis": ["kRollupDirty", "kFinalize", "kAnalysisLeaves", "kAnalysisLevel", "kAnalysisLevel2",
+    "remolod.analysis": ["kAnalysisStep", "kFinalizeLearned", "kAnalysisLeaves", "kAnalysisLevel", "kAnalysisLevel2",
                          "kShapes", "kSetFrame"],
 }
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_lidar100M.txt"

@@ -10,9 +10,11 @@
 // reference's shape) leaves most of the GPU idle behind the largest leaves.
 // Here a plan pass cuts every dirty range into tiles of 2,048 points and lists
 // the leaves it planned, one persistent launch folds the tiles into partial
-// sums, and a finish pass adds each listed leaf's partials in tile order (a wave
-// per leaf), which keeps the FP64 and FP32 sums deterministic for a given point
-// order. (Finishing a leaf in the workgroup that folds its last tile needs a
+// sums, and a finish pass adds each listed leaf's partials in tile order (a
+// workgroup per leaf), which keeps the FP64 and FP32 sums deterministic for a
+// given point order. The finish also writes the leaf's roll-up record for
+// Analysis (its sums in the tree's root frame), so that Analysis rolls up only
+// inner nodes. (Finishing a leaf in the workgroup that folds its last tile needs a
 // device-scope release and acquire per tile, which write back and invalidate the
 // per-XCD L2s here: 61 against 40 us per step.) The plan gathers the verify sums
 // (sum of leaf counts, inner nodes holding sums) as it visits every node.
@@ -23,7 +25,7 @@
 // into int64 sums q1[3], q2[6]. Integer addition is associative, so those sums
 // are bit-exact and independent of thread order; they are what Analysis rolls up.
 
-#include "remolod_internal.h"
+#include "remolod_analysis.h"
 
 namespace remo_hip {
 
@@ -141,59 +143,117 @@ __global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w, uin
 // The leaf's partials [base, base + nt) added in tile order into its sums (one
 // field per thread, from LDS), its watermark count advanced, the folded counts
 // into the globals, the leaf onto Analysis' dirty list: kernel_accumulate's
-// per-leaf tail (remolod_accum.cu:190-212). The partials are loaded in one round
-// by the whole workgroup (a field's loads one after another, tile by tile, were
-// a chain of dependent memory round trips).
+// per-leaf tail (remolod_accum.cu:190-212); then its roll-up record. The
+// partials are loaded in one round by the workgroup (a field's loads one after
+// another, tile by tile, were a chain of dependent memory round trips).
 constexpr uint32_t kMaxLeafTiles = (kMaxPointsPerNode + AccumWork::kTile - 1) / AccumWork::kTile + 1;
 static_assert(sizeof(AccumPartial) % 8 == 0, "partials load as 8-byte words");
 
-__device__ void finishLeaf(const DevTree& t, const AccumWork& w, uint32_t node, uint32_t batchIndex,
-                           AccumPartial* sp) {
+// The roles of the workgroup's waves: wave 0 the geometry sums (threads 0..8),
+// wave 1 the colour sums (threads 64..69) and the Morton maximum (thread 70),
+// wave 2 the leaf's roll-up record (threads 128..137), wave 3 the counters and the
+// dirty-list entry (thread 192). Each wave then waits only on its own memory
+// operations: the record's stores do not wait behind the sums' stores or the
+// watermark's atomic, and the dirty list's chain of atomics runs from the start.
+// Waves 0..2 stage the partials.
+constexpr uint32_t kColourThread = 64;
+constexpr uint32_t kMortonThread = 70;
+constexpr uint32_t kRecordThread = 128;
+constexpr uint32_t kCountThread = 192;
+constexpr uint32_t kStageThreads = 192;
+static_assert(kCountThread >= kStageThreads && kCountThread < kBlock, "the count thread stages nothing");
+
+// The folded totals a leaf's record is made from (LDS, per workgroup).
+struct LeafTotals {
+    int64_t q[9];
+    uint32_t count;
+};
+
+__device__ __forceinline__ void finishLeaf(const DevTree& t, const AccumWork& w, uint32_t node, uint32_t batchIndex,
+                           AccumPartial* sp, LeafTotals& tot) {
     const uint32_t nt = w.nodeTiles[node], base = w.nodeBase[node];
     const uint32_t nLoad = min(nt, kMaxLeafTiles);
     constexpr uint32_t kVecs = sizeof(AccumPartial) / 8;
-    const uint2* src = reinterpret_cast<const uint2*>(w.partials + base);
-    uint2* dst = reinterpret_cast<uint2*>(sp);
-    for (uint32_t v = threadIdx.x; v < nLoad * kVecs; v += blockDim.x) dst[v] = src[v];
-    __syncthreads();
-    remo_node_accum* a = t.accums + node;
     const uint32_t f = threadIdx.x;
-    auto P = [&](uint32_t k) -> const AccumPartial& { return k < nLoad ? sp[k] : w.partials[base + k]; };
-    if (f < 9) {
-        long long v = 0;
-        double d = 0.0;
-        for (uint32_t k = 0; k < nt; ++k) {
-            v += P(k).q[f];
-            d += P(k).g[f];
-        }
-        if (f < 3)
-            a->q1[f] += v;
-        else
-            a->q2[f - 3] += v;
-        a->g[f] += d;
-    } else if (f < 15) {
-        float v = 0.0f;
-        for (uint32_t k = 0; k < nt; ++k) v += P(k).c[f - 9];
-        a->c[f - 9] += v;
-    } else if (f == 15) {
-        unsigned long long mm = 0;
-        for (uint32_t k = 0; k < nt; ++k) {
-            const unsigned long long m = P(k).morton;
-            mm = m > mm ? m : mm;
-        }
-        if (mm > a->morton_max) a->morton_max = mm;
-        if (mm) atomicMax((unsigned long long*)&t.g->mortonWatermark, mm);
-    } else if (f == 16) {
+    remo_node_accum* a = t.accums + node;
+    const AccumPartial* gp = w.partials + base;  // tiles past the LDS copy (leaves over 50,000 points)
+    // what the record needs besides the totals, in flight beside the partials
+    const uint32_t rf = f - kRecordThread;
+    uint32_t lvl = 0, X = 0, Y = 0, Z = 0, D = 0;
+    if (rf < 10) {
+        lvl = t.level[node];
+        X = t.nx[node];
+        Y = t.ny[node];
+        Z = t.nz[node];
+        D = t.g->maxLevel;
+    }
+    if (f == kCountThread) {
         const uint32_t from = a->count, to = w.tiles[base + nt - 1].z;
         a->count = to;
+        tot.count = to;
         a->last_touched_batch = batchIndex;
         Globals* g = t.g;
         atomicAdd((unsigned long long*)&g->numAccumulated, (unsigned long long)(to - from));
         atomicAdd(&g->pointsFolded, to - from);
         atomicAdd(&g->leavesFolded, 1u);
-        listDirty(t, node);  // its sums changed: Analysis rolls it up again
+        listDirty(t, node);  // its sums changed: Analysis rolls its ancestors up again
     }
-    __syncthreads();  // sp is reused for the next leaf
+    const uint2* src = reinterpret_cast<const uint2*>(gp);
+    uint2* dst = reinterpret_cast<uint2*>(sp);
+    if (f < kStageThreads)
+        for (uint32_t v = f; v < nLoad * kVecs; v += kStageThreads) dst[v] = src[v];
+    __syncthreads();
+    // sums in tile order: the LDS copy, then any tiles past it
+    if (f < 9) {
+        long long v = 0;
+        double d = 0.0;
+        for (uint32_t k = 0; k < nLoad; ++k) {
+            v += sp[k].q[f];
+            d += sp[k].g[f];
+        }
+        for (uint32_t k = nLoad; k < nt; ++k) {
+            v += gp[k].q[f];
+            d += gp[k].g[f];
+        }
+        int64_t total;
+        if (f < 3)
+            a->q1[f] = total = a->q1[f] + v;
+        else
+            a->q2[f - 3] = total = a->q2[f - 3] + v;
+        tot.q[f] = total;
+        a->g[f] += d;
+    } else if (f - kColourThread < 6) {
+        const uint32_t c = f - kColourThread;
+        float v = 0.0f;
+        for (uint32_t k = 0; k < nLoad; ++k) v += sp[k].c[c];
+        for (uint32_t k = nLoad; k < nt; ++k) v += gp[k].c[c];
+        a->c[c] += v;
+    } else if (f == kMortonThread) {
+        unsigned long long mm = 0;
+        for (uint32_t k = 0; k < nLoad; ++k) mm = sp[k].morton > mm ? sp[k].morton : mm;
+        for (uint32_t k = nLoad; k < nt; ++k) mm = gp[k].morton > mm ? gp[k].morton : mm;
+        if (mm > a->morton_max) a->morton_max = mm;
+        if (mm) atomicMax((unsigned long long*)&t.g->mortonWatermark, mm);
+    }
+    __syncthreads();  // sp is reused for the next leaf; the totals are in LDS
+    if (rf < 10) {
+        // the leaf's roll-up record from its new totals, a value per thread
+        RollupSums* r = t.rollup + node;
+        const LeafFrame F = leafFrame(D, lvl, X, Y, Z);
+        const int64_t q1[3] = {tot.q[0], tot.q[1], tot.q[2]};
+        if (rf < 3) {
+            r->s1[rf] = leafRecordValue(F, q1, 0, tot.count, int(rf));
+        } else if (rf < 9) {
+            r->s2[rf - 3] = leafRecordValue(F, q1, tot.q[rf], tot.count, int(rf));
+        } else {
+            r->n = tot.count;
+            r->e = 20 + D - lvl;
+            r->pad = 0;
+            t.closeAt[node] = batchIndex + t.closureMargin;  // count > 0: last touched now
+            markRecordChanged(t, node);
+        }
+    }
+    __syncthreads();  // tot is reused for the next leaf
 }
 
 __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w, uint32_t parity) {
@@ -322,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w, ui
 __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, uint32_t batchIndex,
                                                        uint32_t parity) {
     __shared__ AccumPartial sp[kMaxLeafTiles];
+    __shared__ LeafTotals tot;
     const uint32_t n = w.numPlanned[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         w.numPlanned[parity ^ 1u] = 0;  // the next launch lists into the other counter
@@ -331,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, u
         g->sumLeafCountsAcc = 0;
         g->innerWithSumsAcc = 0;
     }
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) finishLeaf(t, w, w.planned[k], batchIndex, sp);
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) finishLeaf(t, w, w.planned[k], batchIndex, sp, tot);
 }
 
 // The FP32-atomics comparator: every stored point of every leaf adds its

@@ -51,18 +51,6 @@ __device__ __forceinline__ int bitsI128(i128 x) {
 
 // ---- roll-up -------------------------------------------------------------------
 
-// Left shift of a possibly negative __int128 (two's complement, no UB).
-__device__ __forceinline__ i128 shl(i128 x, int k) { return i128((unsigned __int128)x << k); }
-
-// a * b modulo 2^128 for a small unsigned factor (a 64x32 and a 32x32 product of
-// limbs instead of the general 128x128 expansion).
-__device__ __forceinline__ i128 mulU32(i128 a, uint32_t b) {
-    const unsigned __int128 u = (unsigned __int128)a;
-    const uint64_t lo = uint64_t(u), hi = uint64_t(u >> 64);
-    const unsigned __int128 plo = (unsigned __int128)lo * b;
-    return i128(plo + ((unsigned __int128)(hi * b) << 64));
-}
-
 // a * b modulo 2^128 for an unsigned 64-bit factor.
 __device__ __forceinline__ i128 mulU64(i128 a, uint64_t b) {
     const unsigned __int128 u = (unsigned __int128)a;
@@ -99,24 +87,11 @@ __device__ __forceinline__ LeafIn loadLeaf(const DevTree& t, uint32_t node) {
     return l;
 }
 
-// A leaf's sums from its accumulator (units 2^-(20+L), centred on the leaf):
-// x = c + 2^(D-L) u for each point, so S1 = 2^(D-L) Q1 + n c and
-// S2_ab = 4^(D-L) Q2_ab + c_a 2^(D-L) Q1_b + 2^(D-L) Q1_a c_b + n c_a c_b.
+// A leaf's record from its accumulator (leafRecordValue).
 __device__ __forceinline__ void leafSumsOf(const LeafIn& a, uint32_t D, RollupSums& S) {
-    const int sh = int(D - a.level), s = int(19 + D - a.level);  // c = odd * 2^s
-    const uint32_t odd[3] = {2u * a.x + 1u, 2u * a.y + 1u, 2u * a.z + 1u};
-    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-    const i128 n = i128(a.count);
-    i128 q1[3];
-    for (int i = 0; i < 3; ++i) {
-        q1[i] = shl(i128(a.q1[i]), sh);
-        S.s1[i] = q1[i] + shl(mulU32(n, odd[i]), s);
-    }
-    for (int q = 0; q < 6; ++q) {
-        const int a_ = pi[q], b_ = pj[q];
-        const i128 cross = mulU32(q1[b_], odd[a_]) + mulU32(q1[a_], odd[b_]);
-        S.s2[q] = shl(i128(a.q2[q]), 2 * sh) + shl(cross, s) + shl(mulU32(mulU32(n, odd[a_]), odd[b_]), 2 * s);
-    }
+    const LeafFrame F = leafFrame(D, a.level, a.x, a.y, a.z);
+    for (int f = 0; f < 3; ++f) S.s1[f] = leafRecordValue(F, a.q1, 0, a.count, f);
+    for (int q = 0; q < 6; ++q) S.s2[q] = leafRecordValue(F, a.q1, a.q2[q], a.count, 3 + q);
     S.n = a.count;
     S.e = 20 + D - a.level;
     S.pad = 0;
@@ -129,18 +104,37 @@ __device__ __forceinline__ void leafSums(const DevTree& t, uint32_t node, uint32
 // A record in the node's own frame (centred on the node, same units): what the
 // shape, the learned-score inputs and the dump read. S1 - n c and
 // S2_ab - c_a S1_b - S1_a c_b + n c_a c_b, c = odd * 2^(e-1).
+__device__ __forceinline__ i128 centredS1(const RollupSums& R, const uint32_t odd[3], int i) {
+    return R.s1[i] - shl(mulU32(i128(R.n), odd[i]), int(R.e) - 1);
+}
+// (S2_ab of the record, S1_a, S1_b, n, the odd factors of axes a and b, e - 1)
+__device__ __forceinline__ i128 centredS2Of(i128 s2q, i128 s1a, i128 s1b, unsigned long long n, uint32_t oa,
+                                            uint32_t ob, int s) {
+    const i128 cross = mulU32(s1b, oa) + mulU32(s1a, ob);
+    return s2q - shl(cross, s) + shl(mulU32(mulU32(i128(n), oa), ob), 2 * s);
+}
+__device__ __forceinline__ i128 centredS2(const RollupSums& R, const uint32_t odd[3], int q) {
+    const int a = q < 3 ? 0 : (q < 5 ? 1 : 2);
+    const int b = q < 3 ? q : (q < 5 ? q - 2 : 2);
+    return centredS2Of(R.s2[q], R.s1[a], R.s1[b], R.n, odd[a], odd[b], int(R.e) - 1);
+}
+
+// Value i of three or six, for an index known only at run time, as selects
+// (an array indexed at run time would go to scratch memory).
+template <typename T>
+__device__ __forceinline__ T pick3(T v0, T v1, T v2, int i) {
+    return i == 0 ? v0 : (i == 1 ? v1 : v2);
+}
+template <typename T>
+__device__ __forceinline__ T pick6(T v0, T v1, T v2, T v3, T v4, T v5, int i) {
+    return i < 3 ? pick3(v0, v1, v2, i) : pick3(v3, v4, v5, i - 3);
+}
+
 __device__ __forceinline__ RollupSums centred(const DevTree& t, uint32_t node, const RollupSums& R) {
-    const int s = int(R.e) - 1;
     const uint32_t odd[3] = {2u * t.nx[node] + 1u, 2u * t.ny[node] + 1u, 2u * t.nz[node] + 1u};
-    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-    const unsigned __int128 n = R.n;
     RollupSums S;
-    for (int i = 0; i < 3; ++i) S.s1[i] = R.s1[i] - shl(mulU32(i128(n), odd[i]), s);
-    for (int q = 0; q < 6; ++q) {
-        const int a_ = pi[q], b_ = pj[q];
-        const i128 cross = mulU32(R.s1[b_], odd[a_]) + mulU32(R.s1[a_], odd[b_]);
-        S.s2[q] = R.s2[q] - shl(cross, s) + shl(mulU32(mulU32(i128(n), odd[a_]), odd[b_]), 2 * s);
-    }
+    for (int i = 0; i < 3; ++i) S.s1[i] = centredS1(R, odd, i);
+    for (int q = 0; q < 6; ++q) S.s2[q] = centredS2(R, odd, q);
     S.n = R.n;
     S.e = R.e;
     S.pad = 0;
@@ -193,22 +187,30 @@ __device__ __forceinline__ Bounds nodeBounds(const DevTree& t, uint32_t node, fl
 }
 
 // projectedExtent (remolod_render.cu:40-70).
+// One corner's screen position; false when it is behind the eye (w <= 0).
+__device__ __forceinline__ bool cornerScreen(const Bounds& b, const remo_mat4& M, float width, float height,
+                                             int corner, float& sx, float& sy) {
+    const float x = (corner & 1) ? b.mx[0] : b.mn[0];
+    const float y = (corner & 2) ? b.mx[1] : b.mn[1];
+    const float z = (corner & 4) ? b.mx[2] : b.mn[2];
+    const float cx = M.rows[0][0] * x + M.rows[0][1] * y + M.rows[0][2] * z + M.rows[0][3] * 1.0f;
+    const float cy = M.rows[1][0] * x + M.rows[1][1] * y + M.rows[1][2] * z + M.rows[1][3] * 1.0f;
+    const float cw = M.rows[3][0] * x + M.rows[3][1] * y + M.rows[3][2] * z + M.rows[3][3] * 1.0f;
+    if (cw <= 0.0f) return false;
+    const float invW = 1.0f / cw;
+    sx = (cx * invW * 0.5f + 0.5f) * width;
+    sy = (cy * invW * 0.5f + 0.5f) * height;
+    return true;
+}
+
 __device__ __forceinline__ void projectedExtent(const Bounds& b, const remo_mat4& M, float width, float height,
                                                 float& dx, float& dy) {
     dx = dy = 0.0f;
     float minX = 1e30f, minY = 1e30f, maxX = -1e30f, maxY = -1e30f;
     bool any = false;
     for (int corner = 0; corner < 8; ++corner) {
-        const float x = (corner & 1) ? b.mx[0] : b.mn[0];
-        const float y = (corner & 2) ? b.mx[1] : b.mn[1];
-        const float z = (corner & 4) ? b.mx[2] : b.mn[2];
-        const float cx = M.rows[0][0] * x + M.rows[0][1] * y + M.rows[0][2] * z + M.rows[0][3] * 1.0f;
-        const float cy = M.rows[1][0] * x + M.rows[1][1] * y + M.rows[1][2] * z + M.rows[1][3] * 1.0f;
-        const float cw = M.rows[3][0] * x + M.rows[3][1] * y + M.rows[3][2] * z + M.rows[3][3] * 1.0f;
-        if (cw <= 0.0f) continue;
-        const float invW = 1.0f / cw;
-        const float sx = (cx * invW * 0.5f + 0.5f) * width;
-        const float sy = (cy * invW * 0.5f + 0.5f) * height;
+        float sx, sy;
+        if (!cornerScreen(b, M, width, height, corner, sx, sy)) continue;
         minX = fminf(minX, sx);
         maxX = fmaxf(maxX, sx);
         minY = fminf(minY, sy);
@@ -225,15 +227,12 @@ struct AnalysisArgs {
     uint8_t* closed;
     remo_node_analysis* results;
     uint32_t* queue;
-    uint32_t* queueCount;
+    uint32_t* queueCtl;          // the queue's two counters, its parity, the readers counter
     const FrameParams* frame;    // uniforms and arrival of this frame (device)
     uint32_t margin;
     uint32_t* closeAt;
-    uint32_t* rollStamp;
-    uint32_t* shapeStamp;
     ShapeCache* shapes;
-    uint32_t* markList;
-    uint32_t* stamp;
+    uint4* markList;             // marked nodes (node | level << 24, parent, first child)
     // device slots of the incremental roll-up when its marked nodes exceed LDS
     RollupSums* slotSums;
     uint32_t* slotNode;
@@ -295,25 +294,27 @@ __device__ __forceinline__ void jacobi3Fast(const float a[6], float ev[3]) {
 // matrix num / trace(num) in FP32, Jacobi as above; eigenvalues ascending, in
 // node-extent units (times trace(num) / n^2 / 4^e), surface variation
 // l0/(l0+l1+l2), planarity (l1-l0)/l2, linearity (l2-l1)/l2.
-__device__ void shapeOf(const RollupSums& S, ShapeCache& sh) {
-    memset(&sh, 0, sizeof(sh));
-    sh.n = S.n;
-    if (S.n < 3) return;
-    const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-    int maxS2 = 0, maxS1 = 0;
-    for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
-    for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
-    // exact: n * S2 and S1 * S1 fit (then |S1| < 2^63, a signed 64-bit value)
-    const bool exact = bitsU128((unsigned __int128)S.n) + maxS2 <= 125 && 2 * maxS1 <= 125;
-    const double dn = double(S.n);
-    double num[6];
-    for (int k = 0; k < 6; ++k) {
-        if (exact)
-            num[k] = i128ToDouble(mulU64(S.s2[k], S.n) -
-                                  i128(int64_t(S.s1[pi[k]])) * i128(int64_t(S.s1[pj[k]])));
-        else
-            num[k] = i128ToDouble(S.s2[k]) * dn - i128ToDouble(S.s1[pi[k]]) * i128ToDouble(S.s1[pj[k]]);
-    }
+// Whether the numerators are exact in int128: n * S2 and S1 * S1 fit (then
+// |S1| < 2^63, a signed 64-bit value).
+__device__ __forceinline__ bool shapeExact(unsigned long long n, int maxS2, int maxS1) {
+    return bitsU128((unsigned __int128)n) + maxS2 <= 125 && 2 * maxS1 <= 125;
+}
+
+// Covariance numerator S2_ab n - S1_a S1_b of centred sums.
+__device__ __forceinline__ double shapeNumOf(i128 s1a, i128 s1b, i128 s2k, unsigned long long n, bool exact) {
+    if (exact) return i128ToDouble(mulU64(s2k, n) - i128(int64_t(s1a)) * i128(int64_t(s1b)));
+    return i128ToDouble(s2k) * double(n) - i128ToDouble(s1a) * i128ToDouble(s1b);
+}
+// Numerator k (xx xy xz yy yz zz).
+__device__ __forceinline__ double shapeNum(const i128 s1[3], i128 s2k, unsigned long long n, int k, bool exact) {
+    const int a = k < 3 ? 0 : (k < 5 ? 1 : 2);
+    const int b = k < 3 ? k : (k < 5 ? k - 2 : 2);
+    return shapeNumOf(s1[a], s1[b], s2k, n, exact);
+}
+
+// The shape from the six numerators (sh zeroed, sh.n set, n >= 3).
+__device__ __forceinline__ void shapeFinish(const double num[6], unsigned long long n, uint32_t e, ShapeCache& sh) {
+    const double dn = double(n);
     const double tn = num[0] + num[3] + num[5];
     if (!(tn > 0.0)) return;
     const double inv = 1.0 / tn;
@@ -321,7 +322,7 @@ __device__ void shapeOf(const RollupSums& S, ShapeCache& sh) {
     for (int k = 0; k < 6; ++k) a32[k] = float(num[k] * inv);
     float ev[3];
     jacobi3Fast(a32, ev);
-    const double scale = ldexp(1.0, -int(S.e));
+    const double scale = ldexp(1.0, -int(e));
     const double trace = tn / (dn * dn) * scale * scale;
     for (int i = 0; i < 3; ++i) sh.lambda[i] = float(double(ev[i]) * trace);
     const float sum = ev[0] + ev[1] + ev[2];
@@ -331,17 +332,29 @@ __device__ void shapeOf(const RollupSums& S, ShapeCache& sh) {
     sh.linearity = (ev[2] - ev[1]) * r2;
 }
 
-// Recomputes the cached shape of every node whose sums changed since it was
-// solved (after the full roll-up; the incremental one solves its own nodes).
+__device__ void shapeOf(const RollupSums& S, ShapeCache& sh) {
+    memset(&sh, 0, sizeof(sh));
+    sh.n = S.n;
+    if (S.n < 3) return;
+    int maxS2 = 0, maxS1 = 0;
+    for (int k = 0; k < 6; ++k) maxS2 = max(maxS2, bitsI128(S.s2[k]));
+    for (int k = 0; k < 3; ++k) maxS1 = max(maxS1, bitsI128(S.s1[k]));
+    const bool exact = shapeExact(S.n, maxS2, maxS1);
+    double num[6];
+    for (int k = 0; k < 6; ++k) num[k] = shapeNum(S.s1, S.s2[k], S.n, k, exact);
+    shapeFinish(num, S.n, S.e, sh);
+}
+
+// Recomputes the cached shape of every node whose record changed since its
+// shape was solved (after the full roll-up, which rewrites every record).
 __global__ __launch_bounds__(kBlock) void kShapes(DevTree t, AnalysisArgs aa) {
     const uint32_t n = t.g->numNodes;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
-        const uint32_t rs = aa.rollStamp[node];
-        if (aa.shapeStamp[node] == rs) continue;
+        if (!t.shapeStale[node]) continue;
         ShapeCache sh;
         shapeOf(centred(t, node, aa.rollup[node]), sh);
         aa.shapes[node] = sh;
-        aa.shapeStamp[node] = rs;
+        t.shapeStale[node] = 0;
     }
 }
 
@@ -350,14 +363,13 @@ __global__ __launch_bounds__(kBlock) void kShapes(DevTree t, AnalysisArgs aa) {
 // open node takes its analytic values plus that change where the change is large
 // (above 0.05 in surface variation or 0.1 in planarity); otherwise, and for
 // closed nodes, the analytic values of its current sums stand.
-__device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
-                             uint32_t node, bool closed, const float* pred = nullptr) {
-    const ShapeCache sh = aa.shapes[node];  // solved by the roll-up that last changed its sums
+// (dx, dy: the node's projected extent.)
+__device__ void finalizeExtent(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
+                               uint32_t node, const ShapeCache& sh, bool closed, float dx, float dy,
+                               const float* pred = nullptr) {
     remo_node_analysis r;
     memset(&r, 0, sizeof(r));
     r.n = sh.n;
-    float dx, dy;
-    projectedExtent(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, dx, dy);
     r.coverage = fminf(dx, u.width) * fminf(dy, u.height);
     r.state = closed ? 2u : 0u;
     for (int i = 0; i < 3; ++i) r.lambda[i] = sh.lambda[i];
@@ -381,28 +393,32 @@ __device__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const rem
     a->score = r.score;
 }
 
+__device__ __forceinline__ void finalizeNode(const DevTree& t, const AnalysisArgs& aa, const remo_shared_uniforms& u,
+                                             uint32_t node, const ShapeCache& sh, bool closed,
+                                             const float* pred = nullptr) {
+    float dx, dy;
+    projectedExtent(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, dx, dy);
+    finalizeExtent(t, aa, u, node, sh, closed, dx, dy, pred);
+}
+
 // The batch index from which a leaf counts as closed: untouched for `margin`
 // batches, or empty (closed from the start).
 __device__ __forceinline__ uint32_t leafCloseAt(const remo_node_accum& a, uint32_t margin) {
     return a.count == 0 ? 0u : a.last_touched_batch + margin;
 }
 
-// A leaf's rolled-up sums, closing batch and stamp (written for kFinalize,
-// returned for the parent's roll-up).
-__device__ __forceinline__ uint32_t leafRecord(const DevTree& t, const AnalysisArgs& aa, uint32_t node, uint32_t D,
-                                               uint32_t stamp, RollupSums& S) {
+// A leaf's record and closing batch from its accumulator.
+__device__ __forceinline__ void leafRecord(const DevTree& t, const AnalysisArgs& aa, uint32_t node, uint32_t D) {
+    RollupSums S;
     leafSums(t, node, D, S);
-    const uint32_t ca = leafCloseAt(t.accums[node], aa.margin);
     aa.rollup[node] = S;
-    aa.closeAt[node] = ca;
-    aa.rollStamp[node] = stamp;
-    return ca;
+    aa.closeAt[node] = leafCloseAt(t.accums[node], aa.margin);
+    markRecordChanged(t, node);
 }
 
 // The full roll-up (the first Analysis, or one after the tree got deeper, which
-// rescales every sum): every leaf, then the inner nodes level by level. Its
-// records carry the stamp the incremental kernel launched right after it takes
-// (the device counter + 1); it empties the dirty list.
+// rescales every sum): every leaf, then the inner nodes level by level, then
+// every node's shape (kShapes); it empties the dirty list.
 // Leaves: sums from the fixed-point accumulators, closing batch from the batch
 // age. (Forming leaf children's sums inside the level launches instead was
 // slower: 61 against 53 us per Analysis on the 100M-point tree.)
@@ -410,13 +426,11 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLeaves(DevTree t, AnalysisArg
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
-    const uint32_t stamp = *aa.stamp + 1u;
     if (blockIdx.x == 0 && threadIdx.x == 0) *t.dirtyCount = 0;
     for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < n; node += gridDim.x * blockDim.x) {
         t.dirtyListed[node] = 0;
         if (t.firstChild[node] >= 0) continue;
-        RollupSums S;
-        leafRecord(t, aa, node, D, stamp, S);
+        leafRecord(t, aa, node, D);
     }
 }
 
@@ -431,12 +445,11 @@ __device__ __forceinline__ uint32_t groupMax8(uint32_t v) {
 // (one level deeper, finished by the previous launch) are added by 8 lanes, one
 // child each, and summed across the group (exact int128 addition, so the order
 // does not matter). Only the exact sums travel up the levels; the eigen-solve and
-// score of every node run afterwards in one node-parallel pass (kFinalize).
+// score of every node run afterwards (kShapes, then the Analysis launch).
 __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs aa, uint32_t level) {
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
-    const uint32_t stamp = *aa.stamp + 1u;
     const uint32_t sub = threadIdx.x & 7u;
     const uint32_t groups = (gridDim.x * blockDim.x) >> 3;
     // every lane of a group sees the same node, so the group's lanes stay together
@@ -456,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel(DevTree t, AnalysisArgs
         if (sub == 0) {
             aa.rollup[node] = P;
             aa.closeAt[node] = ca;
-            aa.rollStamp[node] = stamp;
+            markRecordChanged(t, node);
         }
     }
 }
@@ -481,7 +494,6 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
     Globals* g = t.g;
     const uint32_t n = g->numNodes;
     const uint32_t D = g->maxLevel;
-    const uint32_t stamp = *aa.stamp + 1u;
     const uint32_t lane = laneId(), c = lane >> 3, gi = lane & 7u;
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t node = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; node < n; node += waves) {
@@ -504,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
             if (gi == 0) {
                 aa.rollup[child] = C;
                 aa.closeAt[child] = ca;
-                aa.rollStamp[child] = stamp;
+                markRecordChanged(t, child);
             }
         } else {
             C = aa.rollup[child];
@@ -520,32 +532,49 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
         if (lane == 0) {
             aa.rollup[node] = P;
             aa.closeAt[node] = ca;
-            aa.rollStamp[node] = stamp;
+            markRecordChanged(t, node);
         }
     }
 }
 
-// The incremental roll-up, one workgroup. The nodes on the dirty list and every
-// ancestor of them are marked (an LDS bitmap, so each is taken once) and given a
-// slot, by level. Then, for every slot at once, the eight children: a child that
-// is not marked adds its sums to the slot's partial sums (a leaf whose sums
-// changed, folded by Accumulate or new from a split, from its accumulator, with
-// a new stamp; any other from its stored record). A slot's final sums are its
-// partial plus the partials of the marked nodes below it (the sums are plain
-// integer sums in the root frame); with at most kLdsSlots marked nodes the slots
-// live in LDS and each marked node adds its partial to its marked ancestors with
-// exact LDS atomics, all at once. Integer sums: the result equals a full
-// roll-up's. Last the slots' records are written and the shapes of every node
-// whose sums changed are solved, in one pass: a few hundred per step, which in
-// the node-parallel finalize would put an eigen-solve into nearly every wave. It
-// also bumps the Analysis stamp (the full roll-up, when it ran first, already
-// wrote this stamp) and clears the work queue for the finalize.
+// Analysis in one launch. Workgroup 0 rolls up what changed since the last
+// Analysis; the other workgroups (the sweep) finalise every other node at the
+// same time.
+//
+// The records of leaves are written where their sums change: Accumulate's finish
+// writes a folded leaf's record, a split writes its new children's (empty) ones.
+// So only inner nodes are rolled up here: the nodes on the dirty list that are
+// inner (they split) and every ancestor of a node on the list are marked (an LDS
+// bitmap, so each is taken once; ~53 per 1M-point step on the 10k-node tree) and
+// given a slot by level. For every slot at once, eight lanes take its children:
+// a child that is not marked adds its record to the slot's partial sum (summed
+// over the eight lanes by DPP), a marked one is linked to the slot. Then the
+// levels, deepest first, add each slot's marked children's final sums to its
+// partial (exact int128 sums: the result equals a full roll-up's). Last each slot's
+// record is written, its shape solved and the node finalised (with the learned
+// score the finalize is a separate launch).
+//
+// A sweep workgroup takes ranges of 512 nodes. It marks the inner nodes of its
+// ranges the same way (every node on the dirty list climbs to the root), and
+// finalises every node of its ranges that is not marked, first solving the shape
+// of a leaf whose record changed since its shape was solved.
+//
+// The dirty list is read by every workgroup, so the last to have read it (a
+// counter) empties it. The work queue of scored nodes alternates between two
+// counters: a launch appends to the one the parity word names, and the last
+// reader clears the other and flips the parity for the next launch.
 constexpr uint32_t kRollThreads = 512;
 constexpr uint32_t kLevels = kMaxDepth + 1;
 constexpr uint32_t kLdsSlots = 384;
 static_assert(AnalysisState{}.rollLdsSlots == kLdsSlots, "default LDS slot limit");
-static_assert(kLdsSlots <= kRollThreads, "one LDS slot per thread when partials are added up");
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+static_assert(kLdsSlots <= kRollThreads, "a slot per thread when the LDS slots push to their parents");
+constexpr uint32_t kSweepBlocks = 128;
+// ranges a sweep workgroup marks in LDS (pools up to kMarkBits nodes; larger
+// ones take the full roll-up every time, so nothing on them is marked)
+constexpr uint32_t kSweepRanges = (kMarkBits + kRollThreads * kSweepBlocks - 1) / (kRollThreads * kSweepBlocks);
+// the work-queue control words: cursors 0 and 1, the parity, the readers counter
+constexpr uint32_t kQueueParity = 2, kQueueReaders = 3;
 
 __device__ __forceinline__ void addSums(RollupSums& P, const RollupSums& T) {
     for (int i = 0; i < 3; ++i) P.s1[i] += T.s1[i];
@@ -553,16 +582,51 @@ __device__ __forceinline__ void addSums(RollupSums& P, const RollupSums& T) {
     P.n += T.n;
 }
 
-__device__ __forceinline__ void solveShape(const DevTree& t, const AnalysisArgs& aa, uint32_t node,
-                                           const RollupSums& S, uint32_t stamp) {
+__device__ __forceinline__ ShapeCache solveShape(const DevTree& t, const AnalysisArgs& aa, uint32_t node,
+                                                 const RollupSums& R) {
     ShapeCache sh;
-    shapeOf(centred(t, node, S), sh);
+    shapeOf(centred(t, node, R), sh);
     aa.shapes[node] = sh;
-    aa.shapeStamp[node] = stamp;
+    return sh;
 }
 
-// One slot: its partial plus its marked children's final sums (their slots
-// listed first in the slot's eight child links, nk of them).
+// The node's shape: the cached one, or solved now when its record changed since.
+__device__ __forceinline__ ShapeCache currentShape(const DevTree& t, const AnalysisArgs& aa, uint32_t node) {
+    if (!t.shapeStale[node]) return aa.shapes[node];
+    const ShapeCache sh = solveShape(t, aa, node, aa.rollup[node]);
+    t.shapeStale[node] = 0;
+    return sh;
+}
+
+// Appends the wave's scored nodes to the work queue (one atomic per wave; the
+// queue holds maxNodes entries).
+__device__ inline void queueScored(const DevTree& t, const AnalysisArgs& aa, uint32_t* cursor, uint32_t node,
+                                   bool q) {
+    const uint64_t m = __ballot(q);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t at = 0;
+    if (laneId() == uint32_t(leader)) at = atomicAdd(cursor, uint32_t(__popcll(m)));
+    at = __shfl(at, leader);
+    const uint32_t slot = at + __popcll(m & lanesBelow());
+    if (q && slot < t.maxNodes) aa.queue[slot] = node;
+}
+
+// Empties the dirty list (every workgroup has read it) and hands the work queue's
+// other counter, cleared, to the next launch.
+__device__ void resetDirty(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity) {
+    for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x) t.dirtyListed[t.dirtyList[e]] = 0;
+    if (threadIdx.x == 0) {
+        // (the control words share a line with the live counter: atomics, not stores)
+        atomicExch(t.dirtyCount, 0u);
+        atomicExch(aa.queueCtl + (parity ^ 1u), 0u);
+        atomicExch(aa.queueCtl + kQueueParity, parity ^ 1u);
+        atomicExch(aa.queueCtl + kQueueReaders, 0u);
+    }
+}
+
+// One slot of the device-memory path: its partial plus its marked children's
+// final sums (their slots listed first in the slot's eight child links).
 __device__ __forceinline__ void rollSlot(uint32_t sl, RollupSums* sSums, uint32_t* sClose, const uint32_t* sChild,
                                          const uint32_t* sNk) {
     RollupSums P = sSums[sl];
@@ -577,87 +641,136 @@ __device__ __forceinline__ void rollSlot(uint32_t sl, RollupSums* sSums, uint32_
     sClose[sl] = ca;
 }
 
-// dst += v in LDS, exactly (modulo 2^128 per value) and safe against other lanes
-// adding to the same record: each value as 32-bit limbs, lowest first, a limb's
-// carry out taken from the value its atomic add found. The ten values (S1, S2, n)
-// go limb by limb together, so a record costs four rounds of LDS atomics.
-__device__ __forceinline__ void ldsAddSums(RollupSums* dst, const RollupSums& v) {
-    uint32_t* w = reinterpret_cast<uint32_t*>(dst);
-    uint32_t x[38];
-    memcpy(x, &v, sizeof(x));  // s1: words 0..11, s2: 12..35, n: 36..37
-    uint32_t carry[10];
-    for (int j = 0; j < 10; ++j) carry[j] = 0;
-    for (int k = 0; k < 4; ++k) {
-        for (int j = 0; j < 10; ++j) {
-            if (j == 9 && k >= 2) continue;  // n: two limbs
-            const int word = j < 9 ? 4 * j + k : 36 + k;
-            const uint64_t add = uint64_t(x[word]) + carry[j];
-            const uint32_t lo = uint32_t(add);
-            const uint32_t old = atomicAdd(&w[word], lo);
-            carry[j] = uint32_t(add >> 32) + uint32_t((uint64_t(old) + lo) >> 32);
-        }
-    }
+// The LDS path keeps a slot's sums as 64-bit accumulators of limbs: each of the
+// nine __int128 sums as bits 0..41, 42..83 and 84..127, then n. A marked node
+// adds its final sums to its parent's with non-returning LDS atomics; a limb
+// accumulator takes up to 2^19 additions of 44-bit limbs without overflow, and
+// the sum recombined modulo 2^128 is the exact __int128 sum.
+constexpr uint32_t kLimbs = 28;
+constexpr uint64_t kLimbMask = (1ull << 42) - 1ull;
+
+__device__ __forceinline__ void limbsOf(i128 v, uint64_t* out) {
+    const unsigned __int128 u = (unsigned __int128)v;
+    out[0] = uint64_t(u) & kLimbMask;
+    out[1] = uint64_t(u >> 42) & kLimbMask;
+    out[2] = uint64_t(u >> 84);
 }
 
-// The slot phases over slot arrays in LDS (kLds: up to kLdsSlots marked nodes)
-// or in device memory: placement by level, partials, the sums of marked
-// descendants, then the slots' records and the shapes. In LDS a slot's final sum
-// is its partial plus the partials of every marked node below it, so each marked
-// node adds its partial to each marked ancestor (exact LDS atomics, all nodes at
-// once: no pass per level); in device memory the levels go deepest first, a slot
-// adding its marked children's final sums, with a barrier per level.
-template <bool kLds>
-__device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t stamp,
-                                          uint32_t M, const uint32_t* markBits, const uint32_t* changedBits,
-                                          const uint32_t* count, const uint32_t* start, uint32_t* cursor,
-                                          uint32_t* leaves, RollupSums* sSums, uint32_t* sNode, uint32_t* sFc,
-                                          uint32_t* sClose, uint32_t* sChild, uint32_t* sNk, uint32_t* sUp) {
+__device__ __forceinline__ i128 fromLimbs(const uint64_t* l) {
+    return i128((unsigned __int128)l[0] + ((unsigned __int128)l[1] << 42) + ((unsigned __int128)l[2] << 84));
+}
+
+// Min / max of floats over the 8 lanes of a group (every lane active).
+__device__ __forceinline__ float fdpp(float v, int ctrl) {
+    const int i = __float_as_int(v);
+    int r;
+    switch (ctrl) {
+        case kDppXor1: r = __builtin_amdgcn_update_dpp(0, i, kDppXor1, 0xF, 0xF, false); break;
+        case kDppXor2: r = __builtin_amdgcn_update_dpp(0, i, kDppXor2, 0xF, 0xF, false); break;
+        default: r = __builtin_amdgcn_update_dpp(0, i, kDppHalfMirror, 0xF, 0xF, false); break;
+    }
+    return __int_as_float(r);
+}
+__device__ __forceinline__ float groupMinF(float v) {
+    v = fminf(v, fdpp(v, kDppXor1));
+    v = fminf(v, fdpp(v, kDppXor2));
+    return fminf(v, fdpp(v, kDppHalfMirror));
+}
+__device__ __forceinline__ float groupMaxF(float v) {
+    v = fmaxf(v, fdpp(v, kDppXor1));
+    v = fmaxf(v, fdpp(v, kDppXor2));
+    return fmaxf(v, fdpp(v, kDppHalfMirror));
+}
+
+// shapeOf of a node's root-frame record by the 8 lanes of a group (each holding
+// the record): lane q < 6 forms numerator q, the group shares them, and every
+// lane finishes the solve (the same operations as shapeOf, so the same result).
+__device__ __forceinline__ ShapeCache shapeOfGroup(const DevTree& t, uint32_t node, const RollupSums& R,
+                                                   uint32_t sub) {
+    ShapeCache sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.n = R.n;
+    if (R.n < 3) return sh;  // uniform over the group
+    const uint32_t ox = 2u * t.nx[node] + 1u, oy = 2u * t.ny[node] + 1u, oz = 2u * t.nz[node] + 1u;
+    const int s = int(R.e) - 1;
+    const i128 c0 = R.s1[0] - shl(mulU32(i128(R.n), ox), s);
+    const i128 c1 = R.s1[1] - shl(mulU32(i128(R.n), oy), s);
+    const i128 c2 = R.s1[2] - shl(mulU32(i128(R.n), oz), s);
+    const int maxS1 = max(bitsI128(c0), max(bitsI128(c1), bitsI128(c2)));
+    const int q = int(sub < 6 ? sub : 0);
+    const int a = q < 3 ? 0 : (q < 5 ? 1 : 2);
+    const int b = q < 3 ? q : (q < 5 ? q - 2 : 2);
+    const i128 s2 = centredS2Of(pick6(R.s2[0], R.s2[1], R.s2[2], R.s2[3], R.s2[4], R.s2[5], q),
+                                pick3(R.s1[0], R.s1[1], R.s1[2], a), pick3(R.s1[0], R.s1[1], R.s1[2], b), R.n,
+                                pick3(ox, oy, oz, a), pick3(ox, oy, oz, b), s);
+    const int maxS2 = int(groupMax8(uint32_t(sub < 6 ? bitsI128(s2) : 0)));
+    const double mine = shapeNumOf(pick3(c0, c1, c2, a), pick3(c0, c1, c2, b), s2, R.n,
+                                   shapeExact(R.n, maxS2, maxS1));
+    double num[6];
+    const uint32_t base = laneId() & ~7u;
+    for (int k = 0; k < 6; ++k) num[k] = __shfl(mine, int(base) + k);
+    shapeFinish(num, R.n, R.e, sh);
+    return sh;
+}
+
+// The projected extent of a node by the 8 lanes of a group, a corner each (the
+// minima and maxima of projectedExtent over the same values).
+__device__ __forceinline__ void extentGroup(const DevTree& t, uint32_t node, const remo_shared_uniforms& u,
+                                            uint32_t sub, float& dx, float& dy) {
+    float sx = 0.0f, sy = 0.0f;
+    const bool in = cornerScreen(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, int(sub),
+                                 sx, sy);
+    const float minX = groupMinF(in ? sx : 1e30f), maxX = groupMaxF(in ? sx : -1e30f);
+    const float minY = groupMinF(in ? sy : 1e30f), maxY = groupMaxF(in ? sy : -1e30f);
+    const bool any = (uint32_t(__ballot(in) >> (laneId() & ~7u)) & 0xFFu) != 0u;
+    dx = any ? maxX - minX : 0.0f;
+    dy = any ? maxY - minY : 0.0f;
+}
+
+// The slot phases over slot arrays in LDS (kLds: up to kLdsSlots marked nodes,
+// limb accumulators) or in device memory (__int128 sums, a pass per level):
+// placement by level, partials, the levels, then the records, shapes and (kFinal)
+// the finalize of the marked nodes, 8 lanes per node.
+template <bool kFinal, bool kLds>
+__device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t M,
+                                          const uint32_t* markBits, const uint32_t* count, const uint32_t* start,
+                                          uint32_t* cursor, uint64_t (*sLimbs)[kLimbs], RollupSums* sSums,
+                                          uint32_t* sNode, uint32_t* sFc, uint32_t* sClose, uint32_t* sUp,
+                                          uint32_t* sChild, uint32_t* sNk, uint32_t* queueCursor) {
     const uint32_t tid = threadIdx.x;
     // slots by level; a slot's first child carries the level in its top byte
     for (uint32_t k = tid; k < M; k += kRollThreads) {
-        const uint32_t v = aa.markList[k];
-        const uint32_t p = v & 0xFFFFFFu, L = v >> 24;
+        const uint4 v = aa.markList[k];  // (node | level << 24, parent, first child)
+        const uint32_t p = v.x & 0xFFFFFFu, L = v.x >> 24;
         const uint32_t pos = atomicAdd(&cursor[L], 1u);
         aa.slotOf[p] = pos;
         sNode[pos] = p;
-        sFc[pos] = uint32_t(t.firstChild[p]) | (L << 24);
-        if constexpr (kLds) sUp[pos] = t.parent[p];  // made the parent's slot below
+        sFc[pos] = v.z | (L << 24);
+        if constexpr (kLds) sUp[pos] = v.y;  // the parent's node; its slot below
     }
     __syncthreads();
-    // partials: 8 lanes per slot, one child each. Every lane loads what any of
-    // the three cases needs and picks after, so the group makes one memory round
-    // trip instead of one per case.
+    // partials: 8 lanes per slot, one child each
     const uint32_t sub = tid & 7u;
     for (uint32_t grp = tid >> 3; grp < M; grp += kRollThreads / 8) {
         const uint32_t fcw = sFc[grp];
         const uint32_t L = fcw >> 24, child = (fcw & 0xFFFFFFu) + sub;
         const bool mk = (markBits[child >> 5] >> (child & 31u)) & 1u;
-        const bool ch = !mk && ((changedBits[child >> 5] >> (child & 31u)) & 1u);
-        const uint32_t slotv = aa.slotOf[child];
         const RollupSums R = aa.rollup[child];
         const uint32_t caR = aa.closeAt[child];
-        const LeafIn lf = loadLeaf(t, child);
-        uint32_t up = kNoSlot;
+        uint32_t slotv = 0, up = kNoSlot;
         if constexpr (kLds) {
             if (sub == 0) {
                 const uint32_t par = sUp[grp];
                 if (par != kNone) up = aa.slotOf[par];  // every ancestor of a marked node is marked
             }
-        }
-        RollupSums F;
-        leafSumsOf(lf, D, F);
-        const uint32_t caF = lf.count == 0 ? 0u : lf.last + aa.margin;
-        if (ch) {  // a leaf whose sums changed: its record, and its shape below
-            aa.rollup[child] = F;
-            aa.closeAt[child] = caF;
-            aa.rollStamp[child] = stamp;
-            aa.markList[atomicAdd(leaves, 1u)] = child;
+        } else {
+            slotv = aa.slotOf[child];
         }
         RollupSums T;
-        for (int i = 0; i < 3; ++i) T.s1[i] = mk ? i128(0) : (ch ? F.s1[i] : R.s1[i]);
-        for (int q = 0; q < 6; ++q) T.s2[q] = mk ? i128(0) : (ch ? F.s2[q] : R.s2[q]);
-        T.n = mk ? 0ull : (ch ? F.n : R.n);
-        uint32_t ca = mk ? 0u : (ch ? caF : caR);
+        for (int i = 0; i < 3; ++i) T.s1[i] = mk ? i128(0) : R.s1[i];
+        for (int q = 0; q < 6; ++q) T.s2[q] = mk ? i128(0) : R.s2[q];
+        T.n = mk ? 0ull : R.n;
+        uint32_t ca = mk ? 0u : caR;
         uint32_t gbits = 0;
         if constexpr (!kLds) {
             // the marked children's slots, packed to the front of the slot's links
@@ -671,31 +784,37 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         P.e = 20 + D - L;
         P.pad = 0;
         ca = groupMax8(ca);
-        if (sub == 0) {
+        if constexpr (kLds) {
+            // the partial as limbs, a value per lane (lanes 0 and 1 take two)
+            for (int v = 0; v < 9; ++v)
+                if (sub == uint32_t(v & 7)) limbsOf(v < 3 ? P.s1[v] : P.s2[v - 3], &sLimbs[grp][3 * v]);
+            if (sub == 1) sLimbs[grp][27] = P.n;
+            if (sub == 0) {
+                sClose[grp] = ca;
+                sUp[grp] = up;
+            }
+        } else if (sub == 0) {
             sSums[grp] = P;
             sClose[grp] = ca;
-            if constexpr (kLds)
-                sUp[grp] = up;
-            else
-                sNk[grp] = __popc(gbits);
+            sNk[grp] = __popc(gbits);
         }
     }
     __syncthreads();
     if constexpr (kLds) {
-        // each marked node's partial into every marked ancestor (M <= kRollThreads:
-        // one slot per thread; the partials are read before any is added to)
-        const bool mine = tid < M;
-        RollupSums own;
-        uint32_t caOwn = 0, up = kNoSlot;
-        if (mine) {
-            own = sSums[tid];
-            caOwn = sClose[tid];
-            up = sUp[tid];
-        }
-        __syncthreads();
-        for (uint32_t a = up; a != kNoSlot; a = sUp[a]) {
-            ldsAddSums(sSums + a, own);
-            atomicMax(&sClose[a], caOwn);
+        // deepest level first, each slot adds its final sums to its parent's
+        // (M <= kRollThreads: a slot per thread)
+        const uint32_t Lt = tid < M ? sFc[tid] >> 24 : 0u;
+        const uint32_t up = tid < M ? sUp[tid] : kNoSlot;
+        for (int L = int(D) - 1; L >= 1; --L) {
+            if (count[L] == 0) continue;  // uniform: the barrier below is skipped by every thread
+            if (tid < M && Lt == uint32_t(L) && up != kNoSlot) {
+                // (nothing adds to this level's slots now: their children added before)
+#pragma unroll
+                for (uint32_t j = 0; j < kLimbs; ++j)
+                    atomicAdd((unsigned long long*)&sLimbs[up][j], (unsigned long long)sLimbs[tid][j]);
+                atomicMax(&sClose[up], sClose[tid]);
+            }
+            __syncthreads();
         }
     } else {
         for (int L = int(D) - 1; L >= 0; --L) {
@@ -706,89 +825,107 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
             __syncthreads();
         }
     }
-    __syncthreads();
-    // the slots' records, and the shapes of the slots and of the changed leaves
-    const uint32_t total = M + *leaves;
-    for (uint32_t k = tid; k < total; k += kRollThreads) {
-        uint32_t node;
-        RollupSums S;
+    // the slots' records, shapes and results, 8 lanes per slot: the first 32
+    // slots of a pass on waves 0..3 (a wave per SIMD), the next 32 on waves 4..7
+    const uint32_t current = t.g->batchletIndex;
+    const remo_shared_uniforms& u = aa.frame->u;
+    const uint32_t wave = tid >> 6;
+    const uint32_t inPass = (wave >= 4 ? 32u : 0u) + (wave & 3u) * 8u + ((tid >> 3) & 7u);
+    for (uint32_t base = 0; base < M; base += kRollThreads / 8) {
+        const uint32_t k = base + inPass;  // the same for the 8 lanes of a group
+        bool q = false;
+        uint32_t node = 0;
         if (k < M) {
             node = sNode[k];
-            S = sSums[k];
-            aa.rollup[node] = S;
-            aa.closeAt[node] = sClose[k];
-            aa.rollStamp[node] = stamp;
-        } else {
-            node = aa.markList[k - M];
-            S = aa.rollup[node];
+            RollupSums S;
+            if constexpr (kLds) {
+                const uint64_t* l = sLimbs[k];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) S.s1[i] = fromLimbs(l + 3 * i);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) S.s2[i] = fromLimbs(l + 9 + 3 * i);
+                S.n = l[27];
+                S.e = 20 + D - (sFc[k] >> 24);
+                S.pad = 0;
+            } else {
+                S = sSums[k];
+            }
+            const uint32_t ca = sClose[k];
+            // the record, a value per lane
+            RollupSums* r = aa.rollup + node;
+            for (int v = 0; v < 9; ++v) {
+                if (sub != uint32_t(v & 7)) continue;
+                if (v < 3)
+                    r->s1[v] = S.s1[v];
+                else
+                    r->s2[v - 3] = S.s2[v - 3];
+            }
+            if (sub == 1) {
+                r->n = S.n;
+                r->e = S.e;
+                r->pad = 0;
+            }
+            const ShapeCache sh = shapeOfGroup(t, node, S, sub);
+            float dx = 0.0f, dy = 0.0f;
+            if constexpr (kFinal) extentGroup(t, node, u, sub, dx, dy);
+            if (sub == 0) {
+                aa.closeAt[node] = ca;
+                aa.shapes[node] = sh;
+                t.shapeStale[node] = 0;
+                if constexpr (kFinal) {
+                    finalizeExtent(t, aa, u, node, sh, current >= ca, dx, dy);
+                    q = aa.results[node].score > 0.0f;
+                }
+            }
         }
-        solveShape(t, aa, node, S, stamp);
+        if constexpr (kFinal) queueScored(t, aa, queueCursor, node, q);
     }
 }
 
-__global__ __launch_bounds__(kRollThreads) void kRollupDirty(DevTree t, AnalysisArgs aa) {
-    __shared__ uint32_t markBits[kMarkBits / 32];     // marked (rolled-up) inner nodes
-    __shared__ uint32_t changedBits[kMarkBits / 32];  // leaves whose sums changed
-    __shared__ RollupSums lSums[kLdsSlots];
+template <bool kFinal>
+__device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity,
+                            uint32_t readers) {
+    __shared__ uint32_t markBits[kMarkBits / 32];  // marked (rolled-up) inner nodes
+    __shared__ uint64_t lLimbs[kLdsSlots][kLimbs];
     __shared__ uint32_t lNode[kLdsSlots], lFc[kLdsSlots], lClose[kLdsSlots], lUp[kLdsSlots];
     __shared__ uint32_t count[kLevels], start[kLevels], cursor[kLevels];
-    __shared__ uint32_t marked, stampS, leaves;
+    __shared__ uint32_t marked, lastReader;
     const uint32_t tid = threadIdx.x;
-    Globals* g = t.g;
-    const uint32_t D = g->maxLevel;
-    for (uint32_t k = tid; k < kMarkBits / 128; k += kRollThreads) {
+    const uint32_t D = t.g->maxLevel;
+    for (uint32_t k = tid; k < kMarkBits / 128; k += kRollThreads)
         reinterpret_cast<uint4*>(markBits)[k] = make_uint4(0, 0, 0, 0);
-        reinterpret_cast<uint4*>(changedBits)[k] = make_uint4(0, 0, 0, 0);
-    }
     if (tid < kLevels) count[tid] = 0;
-    if (tid == 0) {
-        marked = 0;
-        leaves = 0;
-        stampS = *aa.stamp + 1u;
-        *aa.stamp = stampS;
-        *aa.queueCount = 0;
-    }
+    if (tid == 0) marked = 0;
     __syncthreads();
-    const uint32_t stamp = stampS;
-    // (pools beyond the bitmap always take the full roll-up, which empties the list)
-    const uint32_t nd = t.maxNodes <= kMarkBits ? min(*t.dirtyCount, t.maxNodes) : 0u;
-    // mark: an inner node on the list (one that split) is rolled up itself and its
-    // children count as changed; a leaf is changed and rolled up from its parent.
-    // A mark carries the node's level in its top byte.
+    // mark: a node on the list that is inner (it split) is rolled up itself, a
+    // leaf from its parent; every ancestor with it. A mark carries the node's
+    // level, its parent and its first child.
     for (uint32_t e = tid; e < nd; e += kRollThreads) {
         const uint32_t node = t.dirtyList[e];
-        t.dirtyListed[node] = 0;
-        const int32_t fc = t.firstChild[node];
-        const uint32_t lvl = t.level[node];
-        uint32_t p, lp;
-        if (fc >= 0) {
-            for (uint32_t c = uint32_t(fc); c < uint32_t(fc) + 8u; ++c) atomicOr(&changedBits[c >> 5], 1u << (c & 31u));
-            p = node;
-            lp = lvl;
-        } else {
-            atomicOr(&changedBits[node >> 5], 1u << (node & 31u));
+        const bool inner = t.firstChild[node] >= 0;
+        uint32_t lp = t.level[node];
+        uint32_t p = node;
+        if (!inner) {
             p = t.parent[node];
-            lp = lvl - 1u;
-        }
-        if (p == kNone) {  // the root while it is a leaf
-            RollupSums S;
-            leafRecord(t, aa, node, D, stamp, S);
-            solveShape(t, aa, node, S, stamp);
-            continue;
+            --lp;
         }
         while (p != kNone) {
             const uint32_t next = t.parent[p];
+            const uint32_t fc = uint32_t(t.firstChild[p]);
             const uint32_t bit = 1u << (p & 31u);
             if (atomicOr(&markBits[p >> 5], bit) & bit) break;  // marked, with its ancestors
-            aa.markList[atomicAdd(&marked, 1u)] = p | (lp << 24);
+            aa.markList[atomicAdd(&marked, 1u)] = make_uint4(p | (lp << 24), next, fc, 0u);
             atomicAdd(&count[lp], 1u);
             p = next;
             --lp;
         }
     }
     __syncthreads();
+    // the dirty list is read: count this workgroup (from a wave with no loads in
+    // the partial pass below when fewer than 57 nodes are marked)
+    uint32_t readBefore = 0;
+    if (readers > 1 && tid == kRollThreads - 64) readBefore = atomicAdd(aa.queueCtl + kQueueReaders, 1u);
     if (tid == 0) {
-        *t.dirtyCount = 0;
         uint32_t acc = 0;
         for (uint32_t L = 0; L < kLevels; ++L) {
             start[L] = acc;
@@ -798,49 +935,90 @@ __global__ __launch_bounds__(kRollThreads) void kRollupDirty(DevTree t, Analysis
     }
     __syncthreads();
     const uint32_t M = marked;
+    uint32_t* queueCursor = aa.queueCtl + parity;
     if (M <= min(aa.ldsSlots, kLdsSlots))  // uniform
-        rollSlots<true>(t, aa, D, stamp, M, markBits, changedBits, count, start, cursor, &leaves, lSums, lNode, lFc,
-                        lClose, nullptr, nullptr, lUp);
+        rollSlots<kFinal, true>(t, aa, D, M, markBits, count, start, cursor, lLimbs, nullptr, lNode, lFc, lClose,
+                                lUp, nullptr, nullptr, queueCursor);
     else
-        rollSlots<false>(t, aa, D, stamp, M, markBits, changedBits, count, start, cursor, &leaves, aa.slotSums,
-                         aa.slotNode, aa.slotFc, aa.slotClose, aa.slotChild, aa.slotNk, nullptr);
+        rollSlots<kFinal, false>(t, aa, D, M, markBits, count, start, cursor, nullptr, aa.slotSums, aa.slotNode,
+                                 aa.slotFc, aa.slotClose, nullptr, aa.slotChild, aa.slotNk, queueCursor);
+    if (tid == kRollThreads - 64) lastReader = readers == 1 || readBefore == readers - 1u;
+    __syncthreads();
+    if (lastReader) resetDirty(t, aa, nd, parity);
 }
 
-// Appends the wave's scored nodes to the work queue (one atomic per wave).
-__device__ inline void queueScored(const AnalysisArgs& aa, uint32_t node, bool q) {
-    const uint64_t m = __ballot(q);
-    if (m == 0) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t at = 0;
-    if (laneId() == uint32_t(leader)) at = atomicAdd(aa.queueCount, uint32_t(__popcll(m)));
-    at = __shfl(at, leader);
-    if (q) aa.queue[at + __popcll(m & lanesBelow())] = node;
-}
-
-// Every node's closure, shape (the cached one: the roll-up that changed a node's
-// sums solved it), coverage and score, and the work queue of scored nodes.
-__global__ __launch_bounds__(kBlock) void kFinalize(DevTree t, AnalysisArgs aa) {
-    Globals* g = t.g;
-    const uint32_t n = g->numNodes;
-    const uint32_t current = g->batchletIndex;
+// A sweep workgroup: its ranges are 512-node ranges b, b + kSweepBlocks, ...
+__device__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity, uint32_t readers,
+                           uint32_t n) {
+    constexpr uint32_t kWords = kRollThreads / 32;
+    __shared__ uint32_t marks[kSweepRanges * kWords];
+    __shared__ uint32_t lastReader;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t b = blockIdx.x - 1u;
+    const uint32_t nRanges = (n + kRollThreads - 1) / kRollThreads;
+    for (uint32_t k = tid; k < kSweepRanges * kWords; k += kRollThreads) marks[k] = 0;
+    __syncthreads();
+    // the inner nodes of this workgroup's ranges that workgroup 0 rolls up
+    for (uint32_t e = tid; e < nd; e += kRollThreads) {
+        const uint32_t node = t.dirtyList[e];
+        uint32_t p = t.firstChild[node] >= 0 ? node : t.parent[node];
+        while (p != kNone) {
+            const uint32_t r = p / kRollThreads, k = r / kSweepBlocks;
+            if (r % kSweepBlocks == b && k < kSweepRanges) {
+                const uint32_t i = p % kRollThreads;
+                atomicOr(&marks[k * kWords + i / 32], 1u << (i & 31u));
+            }
+            p = t.parent[p];
+        }
+    }
+    __syncthreads();
+    uint32_t readBefore = 0;
+    if (tid == 0) readBefore = atomicAdd(aa.queueCtl + kQueueReaders, 1u);
+    const uint32_t current = t.g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t node = base + threadIdx.x;
+    uint32_t* queueCursor = aa.queueCtl + parity;
+    for (uint32_t k = 0, r = b; r < nRanges; ++k, r += kSweepBlocks) {
+        const uint32_t node = r * kRollThreads + tid;
+        const bool mk = k < kSweepRanges && ((marks[k * kWords + tid / 32] >> (tid & 31u)) & 1u);
         bool q = false;
-        if (node < n) {
-            finalizeNode(t, aa, u, node, current >= aa.closeAt[node]);
+        if (node < n && !mk) {
+            const ShapeCache sh = currentShape(t, aa, node);
+            finalizeNode(t, aa, u, node, sh, current >= aa.closeAt[node]);
             q = aa.results[node].score > 0.0f;
         }
-        queueScored(aa, node, q);
+        queueScored(t, aa, queueCursor, node, q);
+    }
+    if (tid == 0) lastReader = readBefore == readers - 1u;
+    __syncthreads();
+    if (lastReader) resetDirty(t, aa, nd, parity);
+}
+
+// kSweep: the one-launch Analysis (workgroup 0 the roll-up, 1 + kSweepBlocks
+// workgroups); otherwise the roll-up alone (one workgroup), whose finalize
+// follows as kFinalizeLearned.
+template <bool kSweep>
+__global__ __launch_bounds__(kRollThreads) void kAnalysisStep(DevTree t, AnalysisArgs aa) {
+    const uint32_t n = t.g->numNodes;
+    // (pools beyond the bitmap always take the full roll-up, which empties the list)
+    const uint32_t nd = t.maxNodes <= kMarkBits ? min(*t.dirtyCount, t.maxNodes) : 0u;
+    const uint32_t parity = aa.queueCtl[kQueueParity];
+    // the workgroups that read the dirty list: the roll-up and each sweep
+    // workgroup with a range
+    const uint32_t readers = kSweep ? 1u + min(kSweepBlocks, (n + kRollThreads - 1) / kRollThreads) : 1u;
+    if (blockIdx.x == 0) {
+        rollupBlock<kSweep>(t, aa, nd, parity, readers);
+    } else if constexpr (kSweep) {
+        if (blockIdx.x < readers) sweepBlock(t, aa, nd, parity, readers, n);
     }
 }
 
-// kFinalize with the learned score, 16 nodes per wave (the Analysis trees hold
-// some 10^4 nodes, so a wave per 16 keeps more of the chip busy than a wave per
-// 64 running four MLP tiles in a row): lanes 0..15 write their node's 13 inputs
-// into LDS, the wave runs one MLP tile on the matrix cores (the same tile as
-// kMlpInfer), and lanes 0..15 finalise their nodes, open nodes taking the
-// prediction.
+// The finalize with the learned score, after the roll-up alone, 16 nodes
+// per wave (the Analysis trees hold some 10^4 nodes, so a wave per 16 keeps more
+// of the chip busy than a wave per 64 running four MLP tiles in a row): lanes
+// 0..15 write their node's 13 inputs into LDS, the wave runs one MLP tile on the
+// matrix cores (the same tile as kMlpInfer), and lanes 0..15 finalise their
+// nodes, open nodes taking the prediction. A leaf whose record changed since its
+// shape was solved is solved here.
 // `arrival` is the batch fraction of the inputs, or negative to read it from the
 // frame parameters (a captured graph, whose arguments are fixed at capture).
 __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, const uint16_t* wb,
@@ -854,6 +1032,8 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
     const uint32_t current = g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
     if (arrival < 0.0f) arrival = aa.frame->arrival;
+    // the roll-up before this launch flipped the parity: its queue is the other one
+    uint32_t* queueCursor = aa.queueCtl + (aa.queueCtl[kQueueParity] ^ 1u);
     const uint32_t wave = threadIdx.x / 64, lane = laneId();
     // every lane's weight fragments, in registers for the whole kernel (no barrier)
     MlpFrag frag;
@@ -874,10 +1054,11 @@ __global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisAr
         }
         bool q = false;
         if (mine) {
-            finalizeNode(t, aa, u, node, current >= aa.closeAt[node], &pred[wave][lane * kMlpOut]);
+            const ShapeCache shape = currentShape(t, aa, node);
+            finalizeNode(t, aa, u, node, shape, current >= aa.closeAt[node], &pred[wave][lane * kMlpOut]);
             q = aa.results[node].score > 0.0f;
         }
-        queueScored(aa, node, q);
+        queueScored(t, aa, queueCursor, node, q);
     }
 }
 
@@ -1190,8 +1371,8 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.closed = static_cast<uint8_t*>(p);
     A(&p, 4 * maxNodes, "work queue");
     a.queue = static_cast<uint32_t*>(p);
-    A(&p, 16, "work queue count");
-    a.queueCount = static_cast<uint32_t*>(p);
+    A(&p, 16, "work queue counters");
+    a.queueCtl = static_cast<uint32_t*>(p);
     A(&p, maxNodes, "visible flags");
     a.visible = static_cast<uint8_t*>(p);
     A(&p, maxNodes, "large flags");
@@ -1206,14 +1387,12 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.frame = static_cast<FrameParams*>(p);
     A(&p, 4 * maxNodes, "closing batches");
     a.closeAt = static_cast<uint32_t*>(p);
-    A(&p, 4 * maxNodes, "roll-up stamps");
-    a.rollStamp = static_cast<uint32_t*>(p);
-    A(&p, 4 * maxNodes, "shape stamps");
-    a.shapeStamp = static_cast<uint32_t*>(p);
+    A(&p, 4 * maxNodes, "stale shape flags");
+    a.shapeStale = static_cast<uint32_t*>(p);
     A(&p, sizeof(ShapeCache) * maxNodes, "shape cache");
     a.shapes = static_cast<ShapeCache*>(p);
-    A(&p, 4 * maxNodes, "marked nodes");
-    a.markList = static_cast<uint32_t*>(p);
+    A(&p, 16 * maxNodes, "marked nodes");
+    a.markList = static_cast<uint4*>(p);
     const uint64_t slots = maxNodes < kMarkBits ? maxNodes : kMarkBits;
     A(&p, sizeof(RollupSums) * slots, "roll-up slot sums");
     a.slotSums = static_cast<RollupSums*>(p);
@@ -1229,11 +1408,9 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.slotNk = static_cast<uint32_t*>(p);
     A(&p, 4 * maxNodes, "roll-up slot of each node");
     a.slotOf = static_cast<uint32_t*>(p);
-    A(&p, 16, "analysis stamp");
-    a.stamp = static_cast<uint32_t*>(p);
-    // stamps start at 0 and only grow, so no cached shape can match a later roll-up
-    if (rc == REMO_OK && (hipMemset(a.stamp, 0, 16) != hipSuccess ||
-                          hipMemset(a.shapeStamp, 0, 4 * maxNodes) != hipSuccess))
+    // (the first Analysis rolls up every node, which marks every record changed)
+    if (rc == REMO_OK && (hipMemset(a.queueCtl, 0, 16) != hipSuccess ||
+                          hipMemset(a.shapeStale, 0, 4 * maxNodes) != hipSuccess))
         rc = REMO_ERR_DEVICE;
     if (rc == REMO_OK) rc = mlpAllocate(a, alloc);
     return rc;
@@ -1263,8 +1440,8 @@ void setFrame(AnalysisState& a, const remo_shared_uniforms& u, float arrival, bo
 }
 
 AnalysisArgs argsOf(const AnalysisState& a) {
-    return AnalysisArgs{a.rollup, a.closed, a.results, a.queue, a.queueCount, a.frame, a.closureMargin,
-                        a.closeAt, a.rollStamp, a.shapeStamp, a.shapes, a.markList, a.stamp,
+    return AnalysisArgs{a.rollup, a.closed, a.results, a.queue, a.queueCtl, a.frame, a.closureMargin,
+                        a.closeAt, a.shapes, a.markList,
                         a.slotSums, a.slotNode, a.slotFc, a.slotClose, a.slotChild, a.slotNk, a.slotOf,
                         a.rollLdsSlots};
 }
@@ -1294,17 +1471,19 @@ void launchFullRollup(AnalysisState& a, const DevTree& t, uint32_t maxLevel, hip
 }
 
 // The per-frame Analysis kernels (no host work, so they can be captured): the
-// incremental roll-up of what changed since the last Analysis, then the finalize
-// pass. The learned score's arrival fraction goes in as an argument when
-// arrival >= 0, else it is read from the frame parameters.
+// incremental roll-up of what changed since the last Analysis with the finalize
+// of every node in the same launch, or with the learned score the roll-up alone
+// and then the learned finalize. The learned score's arrival fraction goes in as
+// an argument when arrival >= 0, else it is read from the frame parameters.
 void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, float arrival, hipStream_t s) {
-    const uint32_t nb = a.shape.nodeBlocks;
     const AnalysisArgs aa = argsOf(a);
-    hipLaunchKernelGGL(kRollupDirty, dim3(1), dim3(kRollThreads), 0, s, t, aa);
-    if (learned)
-        hipLaunchKernelGGL(kFinalizeLearned, dim3(nb), dim3(kBlock), 0, s, t, aa, a.mlpWeights, arrival);
-    else
-        hipLaunchKernelGGL(kFinalize, dim3(nb), dim3(kBlock), 0, s, t, aa);
+    if (learned) {
+        hipLaunchKernelGGL(kAnalysisStep<false>, dim3(1), dim3(kRollThreads), 0, s, t, aa);
+        hipLaunchKernelGGL(kFinalizeLearned, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, aa, a.mlpWeights,
+                           arrival);
+    } else {
+        hipLaunchKernelGGL(kAnalysisStep<true>, dim3(1 + kSweepBlocks), dim3(kRollThreads), 0, s, t, aa);
+    }
 }
 
 // The Rasterize kernels alone.

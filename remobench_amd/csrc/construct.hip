@@ -24,7 +24,7 @@
 
 #include <algorithm>
 
-#include "remolod_internal.h"
+#include "remolod_analysis.h"
 
 namespace remo_hip {
 
@@ -396,7 +396,9 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
             gi = atomicAdd(&g->numGrids, 1u);
             gridAt = atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes);
         }
-        atomicMax(&g->maxLevel, lvl);
+        // the deepest level as it stands (a split that deepens the tree makes the
+        // next Analysis roll up every node, whatever frame these records take)
+        const uint32_t D = max(atomicMax(&g->maxLevel, lvl), lvl);
         for (uint32_t i = 0; i < 8; ++i) {
             const uint32_t c = c0 + i;
             const uint32_t cnt = octCount[i];
@@ -421,6 +423,15 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
             bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
             remo_node_accum* a = t.accums + c;
             *a = remo_node_accum{};
+            // an empty leaf's roll-up record: no points, closed from the start
+            RollupSums* r = t.rollup + c;
+            for (int k = 0; k < 3; ++k) r->s1[k] = 0;
+            for (int k = 0; k < 6; ++k) r->s2[k] = 0;
+            r->n = 0;
+            r->e = 20 + D - lvl;
+            r->pad = 0;
+            t.closeAt[c] = 0;
+            markRecordChanged(t, c);
         }
         uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
         bo[0] = bo[1] = make_uint4(0, 0, 0, 0);

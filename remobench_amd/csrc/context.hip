@@ -476,6 +476,10 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         (void)e;
         return rc;
     }
+    t.rollup = c->analysis.rollup;
+    t.closeAt = c->analysis.closeAt;
+    t.shapeStale = c->analysis.shapeStale;
+    t.closureMargin = c->analysis.closureMargin;
     *out = c;
     return remo_reset(c);
 }

@@ -24,6 +24,48 @@ struct RollupSums {
     uint32_t pad;
 };
 
+// Left shift of a possibly negative __int128 (two's complement, no UB).
+__device__ __forceinline__ __int128 shl(__int128 x, int k) { return __int128((unsigned __int128)x << k); }
+
+// a * b modulo 2^128 for a small unsigned factor (a 64x32 and a 32x32 product of
+// limbs instead of the general 128x128 expansion).
+__device__ __forceinline__ __int128 mulU32(__int128 a, uint32_t b) {
+    const unsigned __int128 u = (unsigned __int128)a;
+    const uint64_t lo = uint64_t(u), hi = uint64_t(u >> 64);
+    const unsigned __int128 plo = (unsigned __int128)lo * b;
+    return __int128(plo + ((unsigned __int128)(hi * b) << 64));
+}
+
+// A leaf's record from its accumulator. The accumulator's sums are in units
+// 2^-(20+L) centred on the leaf (L its level); in the root frame (units
+// 2^-(20+D), D the deepest level) a point is x = c + 2^(D-L) u with the leaf's
+// centre c = odd * 2^(19+D-L), so S1 = 2^(D-L) Q1 + n c and
+// S2_ab = 4^(D-L) Q2_ab + c_a 2^(D-L) Q1_b + 2^(D-L) Q1_a c_b + n c_a c_b.
+struct LeafFrame {
+    int sh, s;         // D - L and 19 + D - L
+    uint32_t odd[3];   // 2 x + 1 per axis, x the leaf's integer coordinate
+};
+
+__device__ __forceinline__ LeafFrame leafFrame(uint32_t D, uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
+    return LeafFrame{int(D - level), int(19 + D - level), {2u * x + 1u, 2u * y + 1u, 2u * z + 1u}};
+}
+
+// Value f of the record: S1[f] for f < 3, S2[f - 3] (xx xy xz yy yz zz) after.
+__device__ __forceinline__ __int128 leafRecordValue(const LeafFrame& F, const int64_t q1[3], int64_t q2, uint64_t n,
+                                                    int f) {
+    const __int128 nn = __int128(n);
+    if (f < 3) return shl(__int128(q1[f]), F.sh) + shl(mulU32(nn, F.odd[f]), F.s);
+    const int q = f - 3;
+    const int a = q < 3 ? 0 : (q < 5 ? 1 : 2);
+    const int b = q < 3 ? q : (q < 5 ? q - 2 : 2);
+    const __int128 qa = shl(__int128(q1[a]), F.sh), qb = shl(__int128(q1[b]), F.sh);
+    const __int128 cross = mulU32(qb, F.odd[a]) + mulU32(qa, F.odd[b]);
+    return shl(__int128(q2), 2 * F.sh) + shl(cross, F.s) + shl(mulU32(mulU32(nn, F.odd[a]), F.odd[b]), 2 * F.s);
+}
+
+// Marks a node's record as changed since its shape was solved.
+__device__ __forceinline__ void markRecordChanged(const DevTree& t, uint32_t node) { t.shapeStale[node] = 1u; }
+
 // Nearest double to an __int128 (two halves, one rounding each).
 __host__ __device__ inline double i128ToDouble(__int128 x) {
     const bool neg = x < 0;
@@ -80,10 +122,9 @@ struct AnalysisState {
     RollupSums* rollup = nullptr;
     uint8_t* closed = nullptr;        // closure flag per node (last Analysis)
     uint32_t* closeAt = nullptr;      // batch index from which the node is closed (max over its leaves)
-    uint32_t* rollStamp = nullptr;    // Analysis stamp of the node's last roll-up write
-    uint32_t* shapeStamp = nullptr;   // stamp of the sums its cached shape was computed from
+    uint32_t* shapeStale = nullptr;   // 1: the node's record changed since its cached shape was solved
     ShapeCache* shapes = nullptr;     // [maxNodes] cached analytic shape per node
-    uint32_t* markList = nullptr;     // [maxNodes] the incremental roll-up's marked nodes, then changed leaves
+    uint4* markList = nullptr;        // [maxNodes] the incremental roll-up's marked nodes
     // the incremental roll-up's slots in device memory, for calls that mark more
     // nodes than its LDS holds ([min(maxNodes, kMarkBits)], child links x8)
     RollupSums* slotSums = nullptr;
@@ -93,14 +134,15 @@ struct AnalysisState {
     uint32_t* slotChild = nullptr;
     uint32_t* slotNk = nullptr;
     uint32_t* slotOf = nullptr;       // [maxNodes] slot of each marked node
-    uint32_t* stamp = nullptr;        // [1] Analysis counter (device; never reset)
     bool needFull = true;             // the next Analysis rolls up every node
     uint32_t rollLdsSlots = 384;      // incremental roll-up: marked nodes kept in LDS at most
     uint32_t lastD = 0;               // tree depth of the last roll-up
     uint32_t closureMargin = 8;       // batches untouched before a leaf is closed
     float arrival = 0.0f;             // fraction of the cloud's batches ingested (last Analysis)
     uint32_t* queue = nullptr;        // work queue of nodes (score > 0)
-    uint32_t* queueCount = nullptr;
+    // [0], [1] the queue's counters (a launch appends to the one [2] names, then
+    // flips [2]), [3] the workgroups of the launch that have read the dirty list
+    uint32_t* queueCtl = nullptr;
     uint8_t* visible = nullptr;
     uint8_t* large = nullptr;
     DrawItem* drawList = nullptr;

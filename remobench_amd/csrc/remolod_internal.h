@@ -101,6 +101,8 @@ struct Globals {
     uint32_t innerWithSumsAcc, pad3;
 };
 
+struct RollupSums;  // remolod_analysis.h
+
 // Everything a kernel needs, passed by value (one kernel argument).
 struct DevTree {
     // node structure-of-arrays, capacity maxNodes
@@ -161,6 +163,13 @@ struct DevTree {
     uint32_t* freeGrids;    // [maxGrids]
     Globals* g;
     remo_node_accum* accums;
+    // Analysis' per-node roll-up records, written where a leaf's sums change
+    // (Accumulate's finish, a split's new children) and by Analysis for inner
+    // nodes; a record whose shape is not yet solved is flagged stale.
+    RollupSums* rollup;     // [maxNodes]
+    uint32_t* closeAt;      // [maxNodes] batch index from which the node is closed
+    uint32_t* shapeStale;   // [maxNodes]
+    uint32_t closureMargin; // batches untouched before a leaf is closed
 };
 
 // The stats sweep reduces per workgroup into statsPartials, then one workgroup
