@@ -176,14 +176,17 @@ struct Bounds {
 };
 
 // nodeBounds (remolod_render.cu:30-38); cubeMin = 0, 2^level is exact.
-__device__ __forceinline__ Bounds nodeBounds(const DevTree& t, uint32_t node, float cubeSize) {
+__device__ __forceinline__ Bounds nodeBoundsOf(uint32_t level, uint32_t x, uint32_t y, uint32_t z, float cubeSize) {
     Bounds b;
-    const float size = cubeSize / float(1u << t.level[node]);
-    b.mn[0] = 0.0f + float(t.nx[node]) * size;
-    b.mn[1] = 0.0f + float(t.ny[node]) * size;
-    b.mn[2] = 0.0f + float(t.nz[node]) * size;
+    const float size = cubeSize / float(1u << level);
+    b.mn[0] = 0.0f + float(x) * size;
+    b.mn[1] = 0.0f + float(y) * size;
+    b.mn[2] = 0.0f + float(z) * size;
     for (int i = 0; i < 3; ++i) b.mx[i] = b.mn[i] + size;
     return b;
+}
+__device__ __forceinline__ Bounds nodeBounds(const DevTree& t, uint32_t node, float cubeSize) {
+    return nodeBoundsOf(t.level[node], t.nx[node], t.ny[node], t.nz[node], cubeSize);
 }
 
 // projectedExtent (remolod_render.cu:40-70).
@@ -232,7 +235,7 @@ struct AnalysisArgs {
     uint32_t margin;
     uint32_t* closeAt;
     ShapeCache* shapes;
-    uint4* markList;             // marked nodes (node | level << 24, parent, first child)
+    uint4* markList;             // marked nodes: (node | level << 24, parent, first child), (x, y, z)
     // device slots of the incremental roll-up when its marked nodes exceed LDS
     RollupSums* slotSums;
     uint32_t* slotNode;
@@ -573,8 +576,10 @@ constexpr uint32_t kSweepBlocks = 128;
 // ranges a sweep workgroup marks in LDS (pools up to kMarkBits nodes; larger
 // ones take the full roll-up every time, so nothing on them is marked)
 constexpr uint32_t kSweepRanges = (kMarkBits + kRollThreads * kSweepBlocks - 1) / (kRollThreads * kSweepBlocks);
-// the work-queue control words: cursors 0 and 1, the parity, the readers counter
-constexpr uint32_t kQueueParity = 2, kQueueReaders = 3;
+// the work-queue control words: cursors 0 and 1 (the queue's front), the parity,
+// the readers counter, and the count of the queue's back segment (the rolled-up
+// nodes, written from the end of the queue by workgroup 0) for parity 0 and 1
+constexpr uint32_t kQueueParity = 2, kQueueReaders = 3, kQueueBack = 4;
 
 __device__ __forceinline__ void addSums(RollupSums& P, const RollupSums& T) {
     for (int i = 0; i < 3; ++i) P.s1[i] += T.s1[i];
@@ -647,6 +652,9 @@ __device__ __forceinline__ void rollSlot(uint32_t sl, RollupSums* sSums, uint32_
 // accumulator takes up to 2^19 additions of 44-bit limbs without overflow, and
 // the sum recombined modulo 2^128 is the exact __int128 sum.
 constexpr uint32_t kLimbs = 28;
+// a slot's row of limbs padded to an odd number of 8-byte words, so that slots
+// 4 apart do not share LDS banks (the ancestor pass adds from up to 384 lanes)
+constexpr uint32_t kLimbStride = 29;
 constexpr uint64_t kLimbMask = (1ull << 42) - 1ull;
 
 __device__ __forceinline__ void limbsOf(i128 v, uint64_t* out) {
@@ -660,95 +668,108 @@ __device__ __forceinline__ i128 fromLimbs(const uint64_t* l) {
     return i128((unsigned __int128)l[0] + ((unsigned __int128)l[1] << 42) + ((unsigned __int128)l[2] << 84));
 }
 
-// Min / max of floats over the 8 lanes of a group (every lane active).
-__device__ __forceinline__ float fdpp(float v, int ctrl) {
-    const int i = __float_as_int(v);
-    int r;
-    switch (ctrl) {
-        case kDppXor1: r = __builtin_amdgcn_update_dpp(0, i, kDppXor1, 0xF, 0xF, false); break;
-        case kDppXor2: r = __builtin_amdgcn_update_dpp(0, i, kDppXor2, 0xF, 0xF, false); break;
-        default: r = __builtin_amdgcn_update_dpp(0, i, kDppHalfMirror, 0xF, 0xF, false); break;
-    }
-    return __int_as_float(r);
+// Min / max of floats over the 4 lanes of a quad (every lane active).
+__device__ __forceinline__ float quadF(float v, bool mx) {
+    const float a = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kDppXor1, 0xF, 0xF, false));
+    v = mx ? fmaxf(v, a) : fminf(v, a);
+    const float b = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kDppXor2, 0xF, 0xF, false));
+    return mx ? fmaxf(v, b) : fminf(v, b);
 }
-__device__ __forceinline__ float groupMinF(float v) {
-    v = fminf(v, fdpp(v, kDppXor1));
-    v = fminf(v, fdpp(v, kDppXor2));
-    return fminf(v, fdpp(v, kDppHalfMirror));
-}
-__device__ __forceinline__ float groupMaxF(float v) {
-    v = fmaxf(v, fdpp(v, kDppXor1));
-    v = fmaxf(v, fdpp(v, kDppXor2));
-    return fmaxf(v, fdpp(v, kDppHalfMirror));
+__device__ __forceinline__ uint32_t quadMax(uint32_t v) {
+    v = max(v, dpp32<kDppXor1>(v));
+    return max(v, dpp32<kDppXor2>(v));
 }
 
-// shapeOf of a node's root-frame record by the 8 lanes of a group (each holding
-// the record): lane q < 6 forms numerator q, the group shares them, and every
-// lane finishes the solve (the same operations as shapeOf, so the same result).
-__device__ __forceinline__ ShapeCache shapeOfGroup(const DevTree& t, uint32_t node, const RollupSums& R,
-                                                   uint32_t sub) {
+// shapeOf of a node's root-frame record by the 4 lanes of a quad (each holding
+// the record; o* the node's odd factors 2 x + 1): lane j forms numerators j and
+// j + 4 (when < 6), the quad shares them, and every lane finishes the solve with
+// the same operations as shapeOf, so the result is the same.
+__device__ __forceinline__ ShapeCache shapeOfQuad(const RollupSums& R, uint32_t ox, uint32_t oy, uint32_t oz,
+                                                  uint32_t sub) {
     ShapeCache sh;
     memset(&sh, 0, sizeof(sh));
     sh.n = R.n;
-    if (R.n < 3) return sh;  // uniform over the group
-    const uint32_t ox = 2u * t.nx[node] + 1u, oy = 2u * t.ny[node] + 1u, oz = 2u * t.nz[node] + 1u;
+    if (R.n < 3) return sh;  // uniform over the quad
     const int s = int(R.e) - 1;
     const i128 c0 = R.s1[0] - shl(mulU32(i128(R.n), ox), s);
     const i128 c1 = R.s1[1] - shl(mulU32(i128(R.n), oy), s);
     const i128 c2 = R.s1[2] - shl(mulU32(i128(R.n), oz), s);
     const int maxS1 = max(bitsI128(c0), max(bitsI128(c1), bitsI128(c2)));
-    const int q = int(sub < 6 ? sub : 0);
-    const int a = q < 3 ? 0 : (q < 5 ? 1 : 2);
-    const int b = q < 3 ? q : (q < 5 ? q - 2 : 2);
-    const i128 s2 = centredS2Of(pick6(R.s2[0], R.s2[1], R.s2[2], R.s2[3], R.s2[4], R.s2[5], q),
-                                pick3(R.s1[0], R.s1[1], R.s1[2], a), pick3(R.s1[0], R.s1[1], R.s1[2], b), R.n,
-                                pick3(ox, oy, oz, a), pick3(ox, oy, oz, b), s);
-    const int maxS2 = int(groupMax8(uint32_t(sub < 6 ? bitsI128(s2) : 0)));
-    const double mine = shapeNumOf(pick3(c0, c1, c2, a), pick3(c0, c1, c2, b), s2, R.n,
-                                   shapeExact(R.n, maxS2, maxS1));
+    i128 s2[2];
+    int bits = 0;
+    for (int h = 0; h < 2; ++h) {
+        const int q = int(sub) + 4 * h < 6 ? int(sub) + 4 * h : 0;
+        const int a = q < 3 ? 0 : (q < 5 ? 1 : 2);
+        const int b = q < 3 ? q : (q < 5 ? q - 2 : 2);
+        s2[h] = centredS2Of(pick6(R.s2[0], R.s2[1], R.s2[2], R.s2[3], R.s2[4], R.s2[5], q),
+                            pick3(R.s1[0], R.s1[1], R.s1[2], a), pick3(R.s1[0], R.s1[1], R.s1[2], b), R.n,
+                            pick3(ox, oy, oz, a), pick3(ox, oy, oz, b), s);
+        if (int(sub) + 4 * h < 6) bits = max(bits, bitsI128(s2[h]));
+    }
+    const bool exact = shapeExact(R.n, int(quadMax(uint32_t(bits))), maxS1);
+    double mine[2];
+    for (int h = 0; h < 2; ++h) {
+        const int q = int(sub) + 4 * h < 6 ? int(sub) + 4 * h : 0;
+        const int a = q < 3 ? 0 : (q < 5 ? 1 : 2);
+        const int b = q < 3 ? q : (q < 5 ? q - 2 : 2);
+        mine[h] = shapeNumOf(pick3(c0, c1, c2, a), pick3(c0, c1, c2, b), s2[h], R.n, exact);
+    }
+    const int base = int(laneId() & ~3u);
     double num[6];
-    const uint32_t base = laneId() & ~7u;
-    for (int k = 0; k < 6; ++k) num[k] = __shfl(mine, int(base) + k);
+    for (int k = 0; k < 4; ++k) num[k] = __shfl(mine[0], base + k);
+    for (int k = 4; k < 6; ++k) num[k] = __shfl(mine[1], base + k - 4);
     shapeFinish(num, R.n, R.e, sh);
     return sh;
 }
 
-// The projected extent of a node by the 8 lanes of a group, a corner each (the
+// The projected extent of a node by the 4 lanes of a quad, two corners each (the
 // minima and maxima of projectedExtent over the same values).
-__device__ __forceinline__ void extentGroup(const DevTree& t, uint32_t node, const remo_shared_uniforms& u,
-                                            uint32_t sub, float& dx, float& dy) {
-    float sx = 0.0f, sy = 0.0f;
-    const bool in = cornerScreen(nodeBounds(t, node, t.octreeSize), u.transform_frozen, u.width, u.height, int(sub),
-                                 sx, sy);
-    const float minX = groupMinF(in ? sx : 1e30f), maxX = groupMaxF(in ? sx : -1e30f);
-    const float minY = groupMinF(in ? sy : 1e30f), maxY = groupMaxF(in ? sy : -1e30f);
-    const bool any = (uint32_t(__ballot(in) >> (laneId() & ~7u)) & 0xFFu) != 0u;
+__device__ __forceinline__ void extentQuad(const Bounds& bd, const remo_shared_uniforms& u, uint32_t sub, float& dx,
+                                           float& dy) {
+    float minX = 1e30f, minY = 1e30f, maxX = -1e30f, maxY = -1e30f;
+    bool in = false;
+    for (int h = 0; h < 2; ++h) {
+        float sx, sy;
+        if (!cornerScreen(bd, u.transform_frozen, u.width, u.height, int(sub) + 4 * h, sx, sy)) continue;
+        minX = fminf(minX, sx);
+        maxX = fmaxf(maxX, sx);
+        minY = fminf(minY, sy);
+        maxY = fmaxf(maxY, sy);
+        in = true;
+    }
+    minX = quadF(minX, false);
+    maxX = quadF(maxX, true);
+    minY = quadF(minY, false);
+    maxY = quadF(maxY, true);
+    const bool any = (uint32_t(__ballot(in) >> (laneId() & ~3u)) & 0xFu) != 0u;
     dx = any ? maxX - minX : 0.0f;
     dy = any ? maxY - minY : 0.0f;
 }
 
 // The slot phases over slot arrays in LDS (kLds: up to kLdsSlots marked nodes,
 // limb accumulators) or in device memory (__int128 sums, a pass per level):
-// placement by level, partials, the levels, then the records, shapes and (kFinal)
-// the finalize of the marked nodes, 8 lanes per node.
+// placement by level, partials, the sums of marked descendants, then the
+// records, shapes and (kFinal) the finalize of the marked nodes, 4 lanes per node.
 template <bool kFinal, bool kLds>
 __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t M,
                                           const uint32_t* markBits, const uint32_t* count, const uint32_t* start,
-                                          uint32_t* cursor, uint64_t (*sLimbs)[kLimbs], RollupSums* sSums,
+                                          uint32_t* cursor, uint64_t (*sLimbs)[kLimbStride], RollupSums* sSums,
                                           uint32_t* sNode, uint32_t* sFc, uint32_t* sClose, uint32_t* sUp,
-                                          uint32_t* sChild, uint32_t* sNk, uint32_t* queueCursor) {
+                                          uint4* sXyz, uint32_t* sChild, uint32_t* sNk, uint32_t* queueCursor,
+                                          uint32_t* backCount, const remo_shared_uniforms& u, uint32_t current) {
     const uint32_t tid = threadIdx.x;
-    // slots by level; a slot's first child carries the level in its top byte
-    for (uint32_t k = tid; k < M; k += kRollThreads) {
-        const uint4 v = aa.markList[k];  // (node | level << 24, parent, first child)
-        const uint32_t p = v.x & 0xFFFFFFu, L = v.x >> 24;
-        const uint32_t pos = atomicAdd(&cursor[L], 1u);
-        aa.slotOf[p] = pos;
-        sNode[pos] = p;
-        sFc[pos] = v.z | (L << 24);
-        if constexpr (kLds) sUp[pos] = v.y;  // the parent's node; its slot below
+    if constexpr (!kLds) {
+        // slots by level (the LDS slots are the marks themselves, in marking order)
+        for (uint32_t k = tid; k < M; k += kRollThreads) {
+            const uint4 v = aa.markList[2 * k];  // (node | level << 24, parent, first child)
+            const uint32_t p = v.x & 0xFFFFFFu, L = v.x >> 24;
+            const uint32_t pos = atomicAdd(&cursor[L], 1u);
+            aa.slotOf[p] = pos;
+            sNode[pos] = p;
+            sFc[pos] = v.z | (L << 24);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // partials: 8 lanes per slot, one child each
     const uint32_t sub = tid & 7u;
     for (uint32_t grp = tid >> 3; grp < M; grp += kRollThreads / 8) {
@@ -801,20 +822,22 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
     }
     __syncthreads();
     if constexpr (kLds) {
-        // deepest level first, each slot adds its final sums to its parent's
-        // (M <= kRollThreads: a slot per thread)
-        const uint32_t Lt = tid < M ? sFc[tid] >> 24 : 0u;
-        const uint32_t up = tid < M ? sUp[tid] : kNoSlot;
-        for (int L = int(D) - 1; L >= 1; --L) {
-            if (count[L] == 0) continue;  // uniform: the barrier below is skipped by every thread
-            if (tid < M && Lt == uint32_t(L) && up != kNoSlot) {
-                // (nothing adds to this level's slots now: their children added before)
+        // a slot's final sums are its partial plus the partials of every marked
+        // node below it: each adds its partial to each marked ancestor at once
+        // (M <= kRollThreads: a slot per thread; the partials are read first)
+        uint64_t own[kLimbs];
+        uint32_t caOwn = 0, up = kNoSlot;
+        if (tid < M) {
 #pragma unroll
-                for (uint32_t j = 0; j < kLimbs; ++j)
-                    atomicAdd((unsigned long long*)&sLimbs[up][j], (unsigned long long)sLimbs[tid][j]);
-                atomicMax(&sClose[up], sClose[tid]);
-            }
-            __syncthreads();
+            for (uint32_t j = 0; j < kLimbs; ++j) own[j] = sLimbs[tid][j];
+            caOwn = sClose[tid];
+            up = sUp[tid];
+        }
+        __syncthreads();
+        for (uint32_t a = up; a != kNoSlot; a = sUp[a]) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLimbs; ++j) atomicAdd((unsigned long long*)&sLimbs[a][j], (unsigned long long)own[j]);
+            atomicMax(&sClose[a], caOwn);
         }
     } else {
         for (int L = int(D) - 1; L >= 0; --L) {
@@ -825,50 +848,54 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
             __syncthreads();
         }
     }
-    // the slots' records, shapes and results, 8 lanes per slot: the first 32
-    // slots of a pass on waves 0..3 (a wave per SIMD), the next 32 on waves 4..7
-    const uint32_t current = t.g->batchletIndex;
-    const remo_shared_uniforms& u = aa.frame->u;
-    const uint32_t wave = tid >> 6;
-    const uint32_t inPass = (wave >= 4 ? 32u : 0u) + (wave & 3u) * 8u + ((tid >> 3) & 7u);
-    for (uint32_t base = 0; base < M; base += kRollThreads / 8) {
-        const uint32_t k = base + inPass;  // the same for the 8 lanes of a group
+    __syncthreads();
+    // the slots' records, shapes and results, 4 lanes per slot: the first 64
+    // slots of a pass on waves 0..3 (a wave per SIMD), the next 64 on waves 4..7
+    const uint32_t wave = tid >> 6, qs = tid & 3u;
+    const uint32_t inPass = (wave >= 4 ? 64u : 0u) + (wave & 3u) * 16u + ((tid >> 2) & 15u);
+    for (uint32_t base = 0; base < M; base += kRollThreads / 4) {
+        const uint32_t k = base + inPass;  // the same for the 4 lanes of a quad
         bool q = false;
         uint32_t node = 0;
         if (k < M) {
             node = sNode[k];
+            const uint32_t L = sFc[k] >> 24;
+            uint4 xyz;
             RollupSums S;
             if constexpr (kLds) {
+                xyz = sXyz[k];
                 const uint64_t* l = sLimbs[k];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) S.s1[i] = fromLimbs(l + 3 * i);
 #pragma unroll
                 for (int i = 0; i < 6; ++i) S.s2[i] = fromLimbs(l + 9 + 3 * i);
                 S.n = l[27];
-                S.e = 20 + D - (sFc[k] >> 24);
+                S.e = 20 + D - L;
                 S.pad = 0;
             } else {
+                xyz = make_uint4(t.nx[node], t.ny[node], t.nz[node], 0u);
                 S = sSums[k];
             }
             const uint32_t ca = sClose[k];
             // the record, a value per lane
             RollupSums* r = aa.rollup + node;
+#pragma unroll
             for (int v = 0; v < 9; ++v) {
-                if (sub != uint32_t(v & 7)) continue;
+                if (qs != uint32_t(v & 3)) continue;
                 if (v < 3)
                     r->s1[v] = S.s1[v];
                 else
                     r->s2[v - 3] = S.s2[v - 3];
             }
-            if (sub == 1) {
+            if (qs == 1) {
                 r->n = S.n;
                 r->e = S.e;
                 r->pad = 0;
             }
-            const ShapeCache sh = shapeOfGroup(t, node, S, sub);
+            const ShapeCache sh = shapeOfQuad(S, 2u * xyz.x + 1u, 2u * xyz.y + 1u, 2u * xyz.z + 1u, qs);
             float dx = 0.0f, dy = 0.0f;
-            if constexpr (kFinal) extentGroup(t, node, u, sub, dx, dy);
-            if (sub == 0) {
+            if constexpr (kFinal) extentQuad(nodeBoundsOf(L, xyz.x, xyz.y, xyz.z, t.octreeSize), u, qs, dx, dy);
+            if (qs == 0) {
                 aa.closeAt[node] = ca;
                 aa.shapes[node] = sh;
                 t.shapeStale[node] = 0;
@@ -878,7 +905,18 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
                 }
             }
         }
-        if constexpr (kFinal) queueScored(t, aa, queueCursor, node, q);
+        if constexpr (kFinal) {
+            // the queue's back segment, counted in LDS
+            const uint64_t m = __ballot(q);
+            if (m != 0) {
+                const int leader = __ffsll((unsigned long long)m) - 1;
+                uint32_t at = 0;
+                if (laneId() == uint32_t(leader)) at = atomicAdd(backCount, uint32_t(__popcll(m)));
+                at = __shfl(at, leader);
+                const uint32_t slot = at + __popcll(m & lanesBelow());
+                if (q && slot < t.maxNodes) aa.queue[t.maxNodes - 1u - slot] = node;
+            }
+        }
     }
 }
 
@@ -886,20 +924,36 @@ template <bool kFinal>
 __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity,
                             uint32_t readers) {
     __shared__ uint32_t markBits[kMarkBits / 32];  // marked (rolled-up) inner nodes
-    __shared__ uint64_t lLimbs[kLdsSlots][kLimbs];
+    __shared__ uint64_t lLimbs[kLdsSlots][kLimbStride];
+    __shared__ remo_shared_uniforms lU;  // the frame's uniforms, read at the start
+    __shared__ uint32_t lCurrent;
     __shared__ uint32_t lNode[kLdsSlots], lFc[kLdsSlots], lClose[kLdsSlots], lUp[kLdsSlots];
+    __shared__ uint4 lXyz[kLdsSlots];
     __shared__ uint32_t count[kLevels], start[kLevels], cursor[kLevels];
-    __shared__ uint32_t marked, lastReader;
+    __shared__ uint32_t marked, lastReader, backCount;
     const uint32_t tid = threadIdx.x;
     const uint32_t D = t.g->maxLevel;
     for (uint32_t k = tid; k < kMarkBits / 128; k += kRollThreads)
         reinterpret_cast<uint4*>(markBits)[k] = make_uint4(0, 0, 0, 0);
     if (tid < kLevels) count[tid] = 0;
-    if (tid == 0) marked = 0;
+    if (tid == 0) {
+        marked = 0;
+        backCount = 0;
+    }
     __syncthreads();
+    // the frame's uniforms into LDS, by the top threads (which rarely have a node
+    // to climb from) beside the marking below: the finalize at the end of this
+    // workgroup's path would otherwise wait for them there
+    constexpr uint32_t kUWords = sizeof(remo_shared_uniforms) / 4;
+    static_assert(sizeof(remo_shared_uniforms) % 4 == 0 && kUWords < kRollThreads, "uniforms copy as words");
+    if (tid >= kRollThreads - kUWords) {
+        const uint32_t w = tid - (kRollThreads - kUWords);
+        reinterpret_cast<uint32_t*>(&lU)[w] = reinterpret_cast<const uint32_t*>(&aa.frame->u)[w];
+        if (w == 0) lCurrent = t.g->batchletIndex;
+    }
     // mark: a node on the list that is inner (it split) is rolled up itself, a
     // leaf from its parent; every ancestor with it. A mark carries the node's
-    // level, its parent and its first child.
+    // level, parent, first child and coordinates (loaded beside the climb's loads).
     for (uint32_t e = tid; e < nd; e += kRollThreads) {
         const uint32_t node = t.dirtyList[e];
         const bool inner = t.firstChild[node] >= 0;
@@ -912,9 +966,19 @@ __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t n
         while (p != kNone) {
             const uint32_t next = t.parent[p];
             const uint32_t fc = uint32_t(t.firstChild[p]);
+            const uint4 xyz = make_uint4(t.nx[p], t.ny[p], t.nz[p], 0u);
             const uint32_t bit = 1u << (p & 31u);
             if (atomicOr(&markBits[p >> 5], bit) & bit) break;  // marked, with its ancestors
-            aa.markList[atomicAdd(&marked, 1u)] = make_uint4(p | (lp << 24), next, fc, 0u);
+            const uint32_t m = atomicAdd(&marked, 1u);
+            aa.markList[2 * m] = make_uint4(p | (lp << 24), next, fc, 0u);
+            aa.markList[2 * m + 1] = xyz;
+            aa.slotOf[p] = m;  // the LDS slot (the device path places by level)
+            if (m < kLdsSlots) {
+                lNode[m] = p;
+                lFc[m] = fc | (lp << 24);
+                lUp[m] = next;  // the parent's node; its slot in the partial pass
+                lXyz[m] = xyz;
+            }
             atomicAdd(&count[lp], 1u);
             p = next;
             --lp;
@@ -938,12 +1002,14 @@ __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t n
     uint32_t* queueCursor = aa.queueCtl + parity;
     if (M <= min(aa.ldsSlots, kLdsSlots))  // uniform
         rollSlots<kFinal, true>(t, aa, D, M, markBits, count, start, cursor, lLimbs, nullptr, lNode, lFc, lClose,
-                                lUp, nullptr, nullptr, queueCursor);
+                                lUp, lXyz, nullptr, nullptr, queueCursor, &backCount, lU, lCurrent);
     else
         rollSlots<kFinal, false>(t, aa, D, M, markBits, count, start, cursor, nullptr, aa.slotSums, aa.slotNode,
-                                 aa.slotFc, aa.slotClose, nullptr, aa.slotChild, aa.slotNk, queueCursor);
+                                 aa.slotFc, aa.slotClose, nullptr, nullptr, aa.slotChild, aa.slotNk, queueCursor,
+                                 &backCount, lU, lCurrent);
     if (tid == kRollThreads - 64) lastReader = readers == 1 || readBefore == readers - 1u;
     __syncthreads();
+    if (tid == 0) aa.queueCtl[kQueueBack + parity] = min(backCount, t.maxNodes);
     if (lastReader) resetDirty(t, aa, nd, parity);
 }
 
@@ -972,8 +1038,6 @@ __device__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd
         }
     }
     __syncthreads();
-    uint32_t readBefore = 0;
-    if (tid == 0) readBefore = atomicAdd(aa.queueCtl + kQueueReaders, 1u);
     const uint32_t current = t.g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
     uint32_t* queueCursor = aa.queueCtl + parity;
@@ -988,7 +1052,10 @@ __device__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd
         }
         queueScored(t, aa, queueCursor, node, q);
     }
-    if (tid == 0) lastReader = readBefore == readers - 1u;
+    // counted as a reader only now (it read the list before its climbs), so that
+    // the reset falls to a sweep workgroup rather than to workgroup 0, which
+    // counts itself after its marking and finishes last
+    if (tid == 0) lastReader = atomicAdd(aa.queueCtl + kQueueReaders, 1u) == readers - 1u;
     __syncthreads();
     if (lastReader) resetDirty(t, aa, nd, parity);
 }
@@ -1371,7 +1438,7 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.closed = static_cast<uint8_t*>(p);
     A(&p, 4 * maxNodes, "work queue");
     a.queue = static_cast<uint32_t*>(p);
-    A(&p, 16, "work queue counters");
+    A(&p, 32, "work queue counters");
     a.queueCtl = static_cast<uint32_t*>(p);
     A(&p, maxNodes, "visible flags");
     a.visible = static_cast<uint8_t*>(p);
@@ -1391,7 +1458,7 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     a.shapeStale = static_cast<uint32_t*>(p);
     A(&p, sizeof(ShapeCache) * maxNodes, "shape cache");
     a.shapes = static_cast<ShapeCache*>(p);
-    A(&p, 16 * maxNodes, "marked nodes");
+    A(&p, 32 * maxNodes, "marked nodes");
     a.markList = static_cast<uint4*>(p);
     const uint64_t slots = maxNodes < kMarkBits ? maxNodes : kMarkBits;
     A(&p, sizeof(RollupSums) * slots, "roll-up slot sums");
@@ -1409,7 +1476,7 @@ int analysisAllocate(AnalysisState& a, uint64_t maxNodes, const AllocFn& alloc) 
     A(&p, 4 * maxNodes, "roll-up slot of each node");
     a.slotOf = static_cast<uint32_t*>(p);
     // (the first Analysis rolls up every node, which marks every record changed)
-    if (rc == REMO_OK && (hipMemset(a.queueCtl, 0, 16) != hipSuccess ||
+    if (rc == REMO_OK && (hipMemset(a.queueCtl, 0, 32) != hipSuccess ||
                           hipMemset(a.shapeStale, 0, 4 * maxNodes) != hipSuccess))
         rc = REMO_ERR_DEVICE;
     if (rc == REMO_OK) rc = mlpAllocate(a, alloc);

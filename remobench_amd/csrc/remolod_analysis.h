@@ -124,7 +124,7 @@ struct AnalysisState {
     uint32_t* closeAt = nullptr;      // batch index from which the node is closed (max over its leaves)
     uint32_t* shapeStale = nullptr;   // 1: the node's record changed since its cached shape was solved
     ShapeCache* shapes = nullptr;     // [maxNodes] cached analytic shape per node
-    uint4* markList = nullptr;        // [maxNodes] the incremental roll-up's marked nodes
+    uint4* markList = nullptr;        // [2 maxNodes] the incremental roll-up's marked nodes
     // the incremental roll-up's slots in device memory, for calls that mark more
     // nodes than its LDS holds ([min(maxNodes, kMarkBits)], child links x8)
     RollupSums* slotSums = nullptr;
@@ -140,8 +140,10 @@ struct AnalysisState {
     uint32_t closureMargin = 8;       // batches untouched before a leaf is closed
     float arrival = 0.0f;             // fraction of the cloud's batches ingested (last Analysis)
     uint32_t* queue = nullptr;        // work queue of nodes (score > 0)
-    // [0], [1] the queue's counters (a launch appends to the one [2] names, then
-    // flips [2]), [3] the workgroups of the launch that have read the dirty list
+    // [0], [1] the counters of the queue's front (a launch appends to the one [2]
+    // names, then flips [2]), [3] the workgroups of the launch that have read the
+    // dirty list, [4], [5] the counts of the queue's back segment (the rolled-up
+    // nodes, from the queue's end backwards) by the same parity
     uint32_t* queueCtl = nullptr;
     uint8_t* visible = nullptr;
     uint8_t* large = nullptr;
