@@ -119,6 +119,8 @@ __device__ __forceinline__ i128 centredS2(const RollupSums& R, const uint32_t od
     return centredS2Of(R.s2[q], R.s1[a], R.s1[b], R.n, odd[a], odd[b], int(R.e) - 1);
 }
 
+__device__ __forceinline__ i128 int128Zero() { return i128(0); }
+
 // Value i of three or six, for an index known only at run time, as selects
 // (an array indexed at run time would go to scratch memory).
 template <typename T>
@@ -166,6 +168,17 @@ __device__ __forceinline__ i128 groupSum8(i128 v) {
     v += dpp128<kDppXor1>(v);
     v += dpp128<kDppXor2>(v);
     v += dpp128<kDppHalfMirror>(v);
+    return v;
+}
+
+template <int kCtrl>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+    return uint64_t(dpp32<kCtrl>(uint32_t(v))) | (uint64_t(dpp32<kCtrl>(uint32_t(v >> 32))) << 32);
+}
+__device__ __forceinline__ uint64_t groupSum8U64(uint64_t v) {
+    v += dpp64<kDppXor1>(v);
+    v += dpp64<kDppXor2>(v);
+    v += dpp64<kDppHalfMirror>(v);
     return v;
 }
 
@@ -752,7 +765,7 @@ __device__ __forceinline__ void extentQuad(const Bounds& bd, const remo_shared_u
 // records, shapes and (kFinal) the finalize of the marked nodes, 4 lanes per node.
 template <bool kFinal, bool kLds>
 __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t M,
-                                          const uint32_t* markBits, const uint32_t* count, const uint32_t* start,
+                                          const uint32_t* markBits, const uint32_t* count, uint32_t* start,
                                           uint32_t* cursor, uint64_t (*sLimbs)[kLimbStride], RollupSums* sSums,
                                           uint32_t* sNode, uint32_t* sFc, uint32_t* sClose, uint32_t* sUp,
                                           uint4* sXyz, uint32_t* sChild, uint32_t* sNk, uint32_t* queueCursor,
@@ -760,6 +773,15 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
     const uint32_t tid = threadIdx.x;
     if constexpr (!kLds) {
         // slots by level (the LDS slots are the marks themselves, in marking order)
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (uint32_t L = 0; L < kLevels; ++L) {
+                start[L] = acc;
+                cursor[L] = acc;
+                acc += count[L];
+            }
+        }
+        __syncthreads();
         for (uint32_t k = tid; k < M; k += kRollThreads) {
             const uint4 v = aa.markList[2 * k];  // (node | level << 24, parent, first child)
             const uint32_t p = v.x & 0xFFFFFFu, L = v.x >> 24;
@@ -801,14 +823,18 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         RollupSums P;
         for (int i = 0; i < 3; ++i) P.s1[i] = groupSum8(T.s1[i]);
         for (int q = 0; q < 6; ++q) P.s2[q] = groupSum8(T.s2[q]);
-        P.n = uint64_t(groupSum8(i128(T.n)));
+        P.n = groupSum8U64(T.n);
         P.e = 20 + D - L;
         P.pad = 0;
         ca = groupMax8(ca);
         if constexpr (kLds) {
-            // the partial as limbs, a value per lane (lanes 0 and 1 take two)
-            for (int v = 0; v < 9; ++v)
-                if (sub == uint32_t(v & 7)) limbsOf(v < 3 ? P.s1[v] : P.s2[v - 3], &sLimbs[grp][3 * v]);
+            // the partial as limbs: lane j value j (selects, no branch per value),
+            // lane 0 also value 8, lane 1 the count
+            const i128 mine = pick3(pick3(P.s1[0], P.s1[1], P.s1[2], int(sub)),
+                                    pick3(P.s2[0], P.s2[1], P.s2[2], int(sub) - 3),
+                                    pick3(P.s2[3], P.s2[4], int128Zero(), int(sub) - 6), int(sub) / 3);
+            limbsOf(mine, &sLimbs[grp][3 * sub]);
+            if (sub == 0) limbsOf(P.s2[5], &sLimbs[grp][24]);
             if (sub == 1) sLimbs[grp][27] = P.n;
             if (sub == 0) {
                 sClose[grp] = ca;
@@ -989,15 +1015,6 @@ __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t n
     // the partial pass below when fewer than 57 nodes are marked)
     uint32_t readBefore = 0;
     if (readers > 1 && tid == kRollThreads - 64) readBefore = atomicAdd(aa.queueCtl + kQueueReaders, 1u);
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t L = 0; L < kLevels; ++L) {
-            start[L] = acc;
-            cursor[L] = acc;
-            acc += count[L];
-        }
-    }
-    __syncthreads();
     const uint32_t M = marked;
     uint32_t* queueCursor = aa.queueCtl + parity;
     if (M <= min(aa.ldsSlots, kLdsSlots))  // uniform
