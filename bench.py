@@ -332,17 +332,20 @@ def main() -> int:
                    "model": str(DEFAULT_MODEL.relative_to(ROOT))}
         log(f"learned score: {learned}")
     # The same steps in the sequence the C++ plugin's build runs per frame at one
-    # batch per step (RemolodHipPipeline::build, host/RemolodHipPipeline.cpp:110-170:
-    # build step, Accumulate, Analysis, then readStats -> remo_get_stats, whose
-    # per-node stats sweep the C-ABI step leaves out)
+    # batch per step (RemolodHipPipeline::build, host/RemolodHipPipeline.cpp:110-175:
+    # build step, the stats request (the per-node stats sweep the C-ABI step leaves
+    # out, and the read of the counters), Accumulate, Analysis, then the stats)
     plugin_path = None
     if analysis_ok and analysis_mode == L.REMO_ANALYSIS_ANALYTIC:
         import ctypes as C
         raw = L.Stats()
 
         def plugin_step():
-            step()
-            tree._ck(tree.lib.remo_get_stats(tree.h, C.byref(raw)), "stats")
+            _, consumed = tree.build_step(view)
+            tree._ck(tree.lib.remo_request_stats(tree.h), "stats request")
+            tree.accumulate(consumed)
+            tree.analysis(uniforms, analysis_mode)
+            tree._ck(tree.lib.remo_collect_stats(tree.h, C.byref(raw)), "stats")
 
         tree.reset()
         tree.sync()
@@ -353,8 +356,8 @@ def main() -> int:
         dt_plugin = time.perf_counter() - t0
         plugin_path = {"ms_per_step": round(dt_plugin * 1e3 / steps, 4),
                        "vs_c_abi_step": round(dt_plugin / dt, 4),
-                       "sequence": "remo_build_step, remo_accumulate, remo_analysis, remo_get_stats "
-                                   "(RemolodHipPipeline::build with readStats)"}
+                       "sequence": "remo_build_step, remo_request_stats, remo_accumulate, remo_analysis, "
+                                   "remo_collect_stats (RemolodHipPipeline::build)"}
         log(f"plugin build path: {plugin_path}")
     ingested = st["num_points_processed"]
     value = ingested / dt / 1e6

@@ -306,6 +306,18 @@ int remo_get_frame_shape(remo_ctx* ctx, remo_tune_entry* out);
  * Runs the per-node stats sweep (kernel_construct's closing sweep,
  * remolod_octree.cu:1248-1319) and reads the counters back (synchronous). */
 int remo_get_stats(remo_ctx* ctx, remo_stats* out);
+
+/* remo_get_stats in two halves, so that a caller issuing more work after a build
+ * step does not drain the stream to read the stats. remo_request_stats queues the
+ * stats sweep and the read of the counters behind the work issued so far (no
+ * synchronisation); remo_collect_stats waits for that read alone and fills `out`
+ * (the draw counters are the last render's, as in remo_get_stats). The reference's
+ * build kernel ends in the same sweep and its readStats copies the result
+ * (remolod_octree.cu:1248-1319, src/pipelines/RemolodPipeline.cpp:440-470); its
+ * plugin reads them once a frame, after the build, as RemolodHipPipeline::build
+ * does with this pair. Collecting without a request first is REMO_ERR_STATE. */
+int remo_request_stats(remo_ctx* ctx);
+int remo_collect_stats(remo_ctx* ctx, remo_stats* out);
 /* The last render's DeviceDiagnostics only -- draw items, samples drawn, draw
  * list overflow -- as RemolodPipeline::render reads them after kernel_render
  * (src/pipelines/RemolodPipeline.cpp:592-600): one small readback, no sweep. */

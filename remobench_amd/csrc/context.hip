@@ -146,6 +146,9 @@ struct remo_ctx {
     hipStream_t stream = nullptr;
     Globals* hostG = nullptr;  // pinned mirror
     uint32_t* hostScratch = nullptr;  // pinned, 16 words after the mirror
+    Globals* hostGQueued = nullptr;   // pinned, after the scratch: the counters remo_request_stats reads
+    hipEvent_t statsReady = nullptr;  // recorded after that read
+    bool statsQueued = false;
     uint64_t budgetTicks = 0;         // step budget in device wall-clock ticks
     const uint32_t* uploadedPtr = nullptr;  // ring cursor last read, and its value
     uint32_t uploadedSeen = 0;
@@ -242,6 +245,8 @@ void releaseAll(remo_ctx* c) {
     c->hostG = nullptr;
     if (c->built) (void)hipEventDestroy(c->built);
     c->built = nullptr;
+    if (c->statsReady) (void)hipEventDestroy(c->statsReady);
+    c->statsReady = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -258,8 +263,7 @@ float arrivalOf(const remo_ctx* c) {
     return c->batchesTotal ? std::min(1.0f, float(c->batchletIndex) / float(c->batchesTotal)) : 1.0f;
 }
 
-void fillStats(const remo_ctx* c, remo_stats* s) {
-    const Globals& g = *c->hostG;
+void fillStats(const remo_ctx* c, const Globals& g, remo_stats* s) {
     std::memset(s, 0, sizeof(*s));
     s->num_nodes = g.numNodes - 8u * g.numFreeBlocks;
     s->num_inner = g.sInner;
@@ -376,16 +380,19 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     c->shape.pointBlocks = std::max(8u, (cus * 8u) & ~7u);
     c->shape.nodeBlocks = std::max(8u, (cus * 2u) & ~7u);
 
+    static_assert(sizeof(Globals) % 8 == 0, "the pinned block holds two mirrors");
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->built, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&c->hostG), sizeof(Globals) + 64, hipHostMallocDefault) !=
+        hipEventCreateWithFlags(&c->statsReady, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&c->hostG), 2 * sizeof(Globals) + 64, hipHostMallocDefault) !=
             hipSuccess) {
         releaseAll(c);
         delete c;
         return REMO_ERR_DEVICE;
     }
-    std::memset(c->hostG, 0, sizeof(Globals) + 64);
+    std::memset(c->hostG, 0, 2 * sizeof(Globals) + 64);
     c->hostScratch = reinterpret_cast<uint32_t*>(c->hostG + 1);
+    c->hostGQueued = reinterpret_cast<Globals*>(reinterpret_cast<uint8_t*>(c->hostG + 1) + 64);
     int clockKHz = 0;
     if (hipDeviceGetAttribute(&clockKHz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || clockKHz <= 0)
         clockKHz = 100000;
@@ -1067,7 +1074,32 @@ int remo_get_stats(remo_ctx* c, remo_stats* out) {
     analysisQueueDiagnostics(c->analysis, c->hostScratch + 8, c->stream);
     CK(hipStreamSynchronize(c->stream), "device work");
     analysisTakeDiagnostics(c->analysis, c->hostScratch + 8);
-    fillStats(c, out);
+    fillStats(c, *c->hostG, out);
+    return REMO_OK;
+}
+
+int remo_request_stats(remo_ctx* c) {
+    if (!c) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    launchStatsSweep(c->t, c->shape, c->stream);
+    CK(hipGetLastError(), "stats sweep");
+    CK(hipMemcpyAsync(c->hostGQueued, c->t.g, sizeof(Globals), hipMemcpyDeviceToHost, c->stream),
+       "reading device counters");
+    CK(hipEventRecord(c->statsReady, c->stream), "stats event");
+    c->statsQueued = true;
+    return REMO_OK;
+}
+
+int remo_collect_stats(remo_ctx* c, remo_stats* out) {
+    if (!c || !out) return REMO_ERR_INVALID;
+    if (c->dead) return REMO_ERR_DEVICE;
+    if (!c->statsQueued) {
+        c->err = "remo_collect_stats without a remo_request_stats before it";
+        return REMO_ERR_STATE;
+    }
+    CK(hipEventSynchronize(c->statsReady), "device work");
+    c->statsQueued = false;
+    fillStats(c, *c->hostGQueued, out);
     return REMO_OK;
 }
 

@@ -150,6 +150,8 @@ bool RemolodHipPipeline::build(PointSource& source, const FrameContext& frame) {
     }
     m_batchesConsumed = consumed;
     source.setBatchesConsumed(consumed);
+    // the stats of this build, read while Accumulate and Analysis run
+    const bool statsQueued = remo_request_stats(m_ctx) == REMO_OK;
     if (m_opt.accumulate) {
         const int ra = remo_accumulate(m_ctx, consumed);
         if (ra != REMO_OK) {
@@ -161,7 +163,10 @@ bool RemolodHipPipeline::build(PointSource& source, const FrameContext& frame) {
         std::string e;
         if (!analyse(frame, m_opt.analysisMode, &e)) return false;
     }
-    readStats();
+    if (statsQueued && remo_collect_stats(m_ctx, &m_raw) == REMO_OK)
+        takeStats();
+    else
+        readStats();
     if (m_batchesTotal > 0 && m_batchesConsumed >= m_batchesTotal) {
         m_complete = true;
         return false;
@@ -235,9 +240,13 @@ void RemolodHipPipeline::render(const FrameContext& frame) {
 }
 
 void RemolodHipPipeline::readStats() {
+    if (remo_get_stats(m_ctx, &m_raw) != REMO_OK) return;
+    takeStats();
+}
+
+void RemolodHipPipeline::takeStats() {
     // RemolodPipeline::readStats (src/pipelines/RemolodPipeline.cpp:440-470); the
     // visible counts belong to render() and are left alone here.
-    if (remo_get_stats(m_ctx, &m_raw) != REMO_OK) return;
     m_batchesConsumed = m_raw.batchlet_index;
     m_stats.numPoints = m_raw.num_points;
     m_stats.numVoxels = m_raw.num_voxels;

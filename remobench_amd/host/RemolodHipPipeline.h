@@ -63,7 +63,8 @@ public:
     const std::string& lastError() const { return m_lastError; }
 
 private:
-    void readStats();
+    void readStats();   // remo_get_stats (synchronous), then takeStats
+    void takeStats();   // m_raw into m_stats
     void fail(const char* what, int rc);
 
     Options m_opt;

@@ -213,6 +213,8 @@ def load() -> C.CDLL:
         "remo_analysis": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_int]),
         "remo_render": (C.c_int, [C.c_void_p, P(SharedUniforms), C.c_void_p]),
         "remo_get_stats": (C.c_int, [C.c_void_p, P(Stats)]),
+        "remo_request_stats": (C.c_int, [C.c_void_p]),
+        "remo_collect_stats": (C.c_int, [C.c_void_p, P(Stats)]),
         "remo_get_render_diagnostics": (C.c_int, [C.c_void_p, P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]),
         "remo_last_error": (C.c_char_p, [C.c_void_p]),
         "remo_sync": (C.c_int, [C.c_void_p]),

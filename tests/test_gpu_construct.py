@@ -191,3 +191,38 @@ def test_voxel_backlog_segment_overflow(gpu):
             assert np.array_equal(tree.grid(it), o.grid(ko[k])), f"occupancy grid differs at {k}"
         if rt["first_child"] < 0 and rt["num_points"] > 0:
             assert np.array_equal(point_rows(tree.samples(it)), point_rows(o.samples(ko[k]))), k
+
+
+@pytest.mark.gpu
+def test_requested_stats_match_get_stats(gpu):
+    """remo_request_stats / remo_collect_stats (the plugin's per-frame read) give
+    remo_get_stats's numbers for the same tree, step by step while Accumulate and
+    Analysis are queued behind the request; collecting twice is an error."""
+    import ctypes as C
+
+    from remobench_amd import camera
+
+    points, meta = lidar_scene(1_048_576, seed=5)
+    batch = 131_072
+    nb = (len(points) + batch - 1) // batch
+    ring = PointRing(batch, nb)
+    ring.set_cloud(points, batch)
+    ring.pump()
+    tree = Tree(meta, slot_capacity=batch, ring_slots=nb, batches_per_step=1, step_budget_ms=-1.0)
+    u = camera.poses(points, meta.box_size, 320, 180)["overview"]
+    raw = L.Stats()
+    for step in range(nb):
+        _, consumed = tree.build_step(ring.view())
+        tree._ck(tree.lib.remo_request_stats(tree.h), "stats request")
+        tree.accumulate(consumed)
+        tree.analysis(u)
+        tree._ck(tree.lib.remo_collect_stats(tree.h, C.byref(raw)), "stats")
+        got = raw.as_dict()
+        want = tree.stats()
+        for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "num_voxels", "batchlet_index",
+                  "max_points_per_node", "max_depth", "allocated_bytes_persistent", "num_chunks_points"):
+            assert got[k] == want[k], (step, k, got[k], want[k])
+    assert got["num_points"] == len(points)
+    assert L.ERRORS[tree.lib.remo_collect_stats(tree.h, C.byref(raw))] == "REMO_ERR_STATE"
+    tree.close()
+    ring.close()
