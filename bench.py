@@ -333,8 +333,8 @@ def main() -> int:
         log(f"learned score: {learned}")
     # The same steps in the sequence the C++ plugin's build runs per frame at one
     # batch per step (RemolodHipPipeline::build, host/RemolodHipPipeline.cpp:110-175:
-    # build step, the stats request (the per-node stats sweep the C-ABI step leaves
-    # out, and the read of the counters), Accumulate, Analysis, then the stats)
+    # build step, Accumulate, the stats request (the per-node stats sweep the C-ABI
+    # step leaves out, and the read of the counters), Analysis, then the stats)
     plugin_path = None
     if analysis_ok and analysis_mode == L.REMO_ANALYSIS_ANALYTIC:
         import ctypes as C
@@ -342,8 +342,8 @@ def main() -> int:
 
         def plugin_step():
             _, consumed = tree.build_step(view)
-            tree._ck(tree.lib.remo_request_stats(tree.h), "stats request")
             tree.accumulate(consumed)
+            tree._ck(tree.lib.remo_request_stats(tree.h), "stats request")
             tree.analysis(uniforms, analysis_mode)
             tree._ck(tree.lib.remo_collect_stats(tree.h, C.byref(raw)), "stats")
 
@@ -356,7 +356,7 @@ def main() -> int:
         dt_plugin = time.perf_counter() - t0
         plugin_path = {"ms_per_step": round(dt_plugin * 1e3 / steps, 4),
                        "vs_c_abi_step": round(dt_plugin / dt, 4),
-                       "sequence": "remo_build_step, remo_request_stats, remo_accumulate, remo_analysis, "
+                       "sequence": "remo_build_step, remo_accumulate, remo_request_stats, remo_analysis, "
                                    "remo_collect_stats (RemolodHipPipeline::build)"}
         log(f"plugin build path: {plugin_path}")
     ingested = st["num_points_processed"]

@@ -315,7 +315,8 @@ int remo_get_stats(remo_ctx* ctx, remo_stats* out);
  * build kernel ends in the same sweep and its readStats copies the result
  * (remolod_octree.cu:1248-1319, src/pipelines/RemolodPipeline.cpp:440-470); its
  * plugin reads them once a frame, after the build, as RemolodHipPipeline::build
- * does with this pair. Collecting without a request first is REMO_ERR_STATE. */
+ * does with this pair (requested after Accumulate, collected after Analysis).
+ * Collecting without a request first is REMO_ERR_STATE. */
 int remo_request_stats(remo_ctx* ctx);
 int remo_collect_stats(remo_ctx* ctx, remo_stats* out);
 /* The last render's DeviceDiagnostics only -- draw items, samples drawn, draw

@@ -132,8 +132,7 @@ __device__ __forceinline__ T pick6(T v0, T v1, T v2, T v3, T v4, T v5, int i) {
     return i < 3 ? pick3(v0, v1, v2, i) : pick3(v3, v4, v5, i - 3);
 }
 
-__device__ __forceinline__ RollupSums centred(const DevTree& t, uint32_t node, const RollupSums& R) {
-    const uint32_t odd[3] = {2u * t.nx[node] + 1u, 2u * t.ny[node] + 1u, 2u * t.nz[node] + 1u};
+__device__ __forceinline__ RollupSums centredOdd(const RollupSums& R, const uint32_t odd[3]) {
     RollupSums S;
     for (int i = 0; i < 3; ++i) S.s1[i] = centredS1(R, odd, i);
     for (int q = 0; q < 6; ++q) S.s2[q] = centredS2(R, odd, q);
@@ -141,6 +140,10 @@ __device__ __forceinline__ RollupSums centred(const DevTree& t, uint32_t node, c
     S.e = R.e;
     S.pad = 0;
     return S;
+}
+__device__ __forceinline__ RollupSums centred(const DevTree& t, uint32_t node, const RollupSums& R) {
+    const uint32_t odd[3] = {2u * t.nx[node] + 1u, 2u * t.ny[node] + 1u, 2u * t.nz[node] + 1u};
+    return centredOdd(R, odd);
 }
 
 // Lane exchange within rows of 16 lanes by DPP (a VALU operation, no LDS
@@ -561,24 +564,27 @@ __global__ __launch_bounds__(kBlock) void kAnalysisLevel2(DevTree t, AnalysisArg
 // writes a folded leaf's record, a split writes its new children's (empty) ones.
 // So only inner nodes are rolled up here: the nodes on the dirty list that are
 // inner (they split) and every ancestor of a node on the list are marked (an LDS
-// bitmap, so each is taken once; ~53 per 1M-point step on the 10k-node tree) and
-// given a slot by level. For every slot at once, eight lanes take its children:
-// a child that is not marked adds its record to the slot's partial sum (summed
-// over the eight lanes by DPP), a marked one is linked to the slot. Then the
-// levels, deepest first, add each slot's marked children's final sums to its
-// partial (exact int128 sums: the result equals a full roll-up's). Last each slot's
-// record is written, its shape solved and the node finalised (with the learned
-// score the finalize is a separate launch).
+// bitmap, so each is taken once; ~53 per 1M-point step on the 10k-node tree).
+// Each mark is a slot. For every slot at once, eight lanes take its children: a
+// child that is not marked adds its record to the slot's partial sum (summed over
+// the eight lanes by DPP). A slot's final sums are its partial plus the partials
+// of every marked node below it, so each marked node adds its partial to each
+// marked ancestor (non-returning LDS atomics on 42-bit limbs, exact modulo 2^128:
+// the result equals a full roll-up's). More marked nodes than the LDS holds take
+// slots in device memory, placed by level, with a pass per level. Last each
+// slot's record is written and its shape, extent and score computed, 4 lanes per
+// node (with the learned score, a wave's 16 nodes are one MLP tile).
 //
-// A sweep workgroup takes ranges of 512 nodes. It marks the inner nodes of its
-// ranges the same way (every node on the dirty list climbs to the root), and
-// finalises every node of its ranges that is not marked, first solving the shape
-// of a leaf whose record changed since its shape was solved.
+// A sweep workgroup takes ranges of nodes. It marks the inner nodes of its ranges
+// the same way (every node on the dirty list climbs to the root), and finalises
+// every node of its ranges that is not marked, first solving the shape of a leaf
+// whose record changed since its shape was solved.
 //
 // The dirty list is read by every workgroup, so the last to have read it (a
-// counter) empties it. The work queue of scored nodes alternates between two
-// counters: a launch appends to the one the parity word names, and the last
-// reader clears the other and flips the parity for the next launch.
+// counter) empties it. The work queue of scored nodes: the sweep appends to the
+// queue's front, on one of two counters (the one the parity word names; the last
+// reader clears the other and flips the parity for the next launch), and
+// workgroup 0 fills its back segment from the end, counted in LDS.
 constexpr uint32_t kRollThreads = 512;
 constexpr uint32_t kLevels = kMaxDepth + 1;
 constexpr uint32_t kLdsSlots = 384;
@@ -586,9 +592,6 @@ static_assert(AnalysisState{}.rollLdsSlots == kLdsSlots, "default LDS slot limit
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 static_assert(kLdsSlots <= kRollThreads, "a slot per thread when the LDS slots push to their parents");
 constexpr uint32_t kSweepBlocks = 128;
-// ranges a sweep workgroup marks in LDS (pools up to kMarkBits nodes; larger
-// ones take the full roll-up every time, so nothing on them is marked)
-constexpr uint32_t kSweepRanges = (kMarkBits + kRollThreads * kSweepBlocks - 1) / (kRollThreads * kSweepBlocks);
 // the work-queue control words: cursors 0 and 1 (the queue's front), the parity,
 // the readers counter, and the count of the queue's back segment (the rolled-up
 // nodes, written from the end of the queue by workgroup 0) for parity 0 and 1
@@ -632,7 +635,7 @@ __device__ inline void queueScored(const DevTree& t, const AnalysisArgs& aa, uin
 
 // Empties the dirty list (every workgroup has read it) and hands the work queue's
 // other counter, cleared, to the next launch.
-__device__ void resetDirty(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity) {
+__device__ __forceinline__ void resetDirty(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity) {
     for (uint32_t e = threadIdx.x; e < nd; e += blockDim.x) t.dirtyListed[t.dirtyList[e]] = 0;
     if (threadIdx.x == 0) {
         // (the control words share a line with the live counter: atomics, not stores)
@@ -759,36 +762,58 @@ __device__ __forceinline__ void extentQuad(const Bounds& bd, const remo_shared_u
     dy = any ? maxY - minY : 0.0f;
 }
 
+// The Analysis launch's LDS, one allocation shared by its roles (LDS declared in
+// separate functions would be allocated side by side).
+struct alignas(16) AnalysisLds {
+    union {
+        uint32_t markBits[kMarkBits / 32];  // workgroup 0: the marked (rolled-up) inner nodes
+        struct {  // then, with the learned score, each wave's MLP tile (the sweep's too)
+            MlpWaveShared tile[kRollThreads / 64];
+            float pred[kRollThreads / 64][16 * kMlpOut];
+        } mlp;
+    };
+    uint64_t limbs[kLdsSlots][kLimbStride];
+    uint32_t node[kLdsSlots], fc[kLdsSlots], close[kLdsSlots], up[kLdsSlots];
+    uint4 xyz[kLdsSlots];  // (x, y, z, voxel count)
+    uint32_t count[kLevels], start[kLevels], cursor[kLevels];
+    uint32_t sweepMarks[64];
+    remo_shared_uniforms u;
+    uint32_t marked, lastReader, backCount, current;
+};
+static_assert(sizeof(AnalysisLds::mlp) <= sizeof(AnalysisLds::markBits), "the MLP tiles reuse the bitmap");
+
 // The slot phases over slot arrays in LDS (kLds: up to kLdsSlots marked nodes,
 // limb accumulators) or in device memory (__int128 sums, a pass per level):
 // placement by level, partials, the sums of marked descendants, then the
-// records, shapes and (kFinal) the finalize of the marked nodes, 4 lanes per node.
-template <bool kFinal, bool kLds>
+// records, shapes and finalize of the marked nodes, 4 lanes per node (with the
+// learned score, a wave's 16 nodes are one MLP tile).
+template <bool kLearned, bool kLds>
 __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& aa, uint32_t D, uint32_t M,
-                                          const uint32_t* markBits, const uint32_t* count, uint32_t* start,
-                                          uint32_t* cursor, uint64_t (*sLimbs)[kLimbStride], RollupSums* sSums,
-                                          uint32_t* sNode, uint32_t* sFc, uint32_t* sClose, uint32_t* sUp,
-                                          uint4* sXyz, uint32_t* sChild, uint32_t* sNk, uint32_t* queueCursor,
-                                          uint32_t* backCount, const remo_shared_uniforms& u, uint32_t current) {
+                                          AnalysisLds& L, RollupSums* sSums, uint32_t* sNode, uint32_t* sFc,
+                                          uint32_t* sClose, uint32_t* sChild, uint32_t* sNk, uint32_t* queueCursor,
+                                          const uint16_t* wb, float arrival) {
     const uint32_t tid = threadIdx.x;
+    uint64_t(*sLimbs)[kLimbStride] = L.limbs;
+    uint32_t* sUp = L.up;
+    const uint32_t* markBits = L.markBits;
     if constexpr (!kLds) {
         // slots by level (the LDS slots are the marks themselves, in marking order)
         if (tid == 0) {
             uint32_t acc = 0;
-            for (uint32_t L = 0; L < kLevels; ++L) {
-                start[L] = acc;
-                cursor[L] = acc;
-                acc += count[L];
+            for (uint32_t Lv = 0; Lv < kLevels; ++Lv) {
+                L.start[Lv] = acc;
+                L.cursor[Lv] = acc;
+                acc += L.count[Lv];
             }
         }
         __syncthreads();
         for (uint32_t k = tid; k < M; k += kRollThreads) {
             const uint4 v = aa.markList[2 * k];  // (node | level << 24, parent, first child)
-            const uint32_t p = v.x & 0xFFFFFFu, L = v.x >> 24;
-            const uint32_t pos = atomicAdd(&cursor[L], 1u);
+            const uint32_t p = v.x & 0xFFFFFFu, Lv = v.x >> 24;
+            const uint32_t pos = atomicAdd(&L.cursor[Lv], 1u);
             aa.slotOf[p] = pos;
             sNode[pos] = p;
-            sFc[pos] = v.z | (L << 24);
+            sFc[pos] = v.z | (Lv << 24);
         }
         __syncthreads();
     }
@@ -796,7 +821,7 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
     const uint32_t sub = tid & 7u;
     for (uint32_t grp = tid >> 3; grp < M; grp += kRollThreads / 8) {
         const uint32_t fcw = sFc[grp];
-        const uint32_t L = fcw >> 24, child = (fcw & 0xFFFFFFu) + sub;
+        const uint32_t Lv = fcw >> 24, child = (fcw & 0xFFFFFFu) + sub;
         const bool mk = (markBits[child >> 5] >> (child & 31u)) & 1u;
         const RollupSums R = aa.rollup[child];
         const uint32_t caR = aa.closeAt[child];
@@ -824,7 +849,7 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
         for (int i = 0; i < 3; ++i) P.s1[i] = groupSum8(T.s1[i]);
         for (int q = 0; q < 6; ++q) P.s2[q] = groupSum8(T.s2[q]);
         P.n = groupSum8U64(T.n);
-        P.e = 20 + D - L;
+        P.e = 20 + D - Lv;
         P.pad = 0;
         ca = groupMax8(ca);
         if constexpr (kLds) {
@@ -866,43 +891,46 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
             atomicMax(&sClose[a], caOwn);
         }
     } else {
-        for (int L = int(D) - 1; L >= 0; --L) {
-            const uint32_t cnt = count[L];
+        for (int Lv = int(D) - 1; Lv >= 0; --Lv) {
+            const uint32_t cnt = L.count[Lv];
             if (cnt == 0) continue;  // uniform: the barrier below is skipped by every thread
-            for (uint32_t sl = start[L] + tid; sl < start[L] + cnt; sl += kRollThreads)
+            for (uint32_t sl = L.start[Lv] + tid; sl < L.start[Lv] + cnt; sl += kRollThreads)
                 rollSlot(sl, sSums, sClose, sChild, sNk);
             __syncthreads();
         }
     }
-    __syncthreads();
+    __syncthreads();  // (the bitmap is free from here: the MLP tiles take its place)
     // the slots' records, shapes and results, 4 lanes per slot: the first 64
     // slots of a pass on waves 0..3 (a wave per SIMD), the next 64 on waves 4..7
-    const uint32_t wave = tid >> 6, qs = tid & 3u;
-    const uint32_t inPass = (wave >= 4 ? 64u : 0u) + (wave & 3u) * 16u + ((tid >> 2) & 15u);
+    const remo_shared_uniforms& u = L.u;
+    const uint32_t wave = tid >> 6, qs = tid & 3u, qi = (tid >> 2) & 15u;
+    const uint32_t inPass = (wave >= 4 ? 64u : 0u) + (wave & 3u) * 16u + qi;
     for (uint32_t base = 0; base < M; base += kRollThreads / 4) {
         const uint32_t k = base + inPass;  // the same for the 4 lanes of a quad
-        bool q = false;
-        uint32_t node = 0;
-        if (k < M) {
+        const bool live = k < M;
+        uint32_t node = 0, Lv = 0, ca = 0;
+        uint4 xyz = make_uint4(0, 0, 0, 0);
+        RollupSums S;
+        ShapeCache sh;
+        float dx = 0.0f, dy = 0.0f;
+        if (live) {
             node = sNode[k];
-            const uint32_t L = sFc[k] >> 24;
-            uint4 xyz;
-            RollupSums S;
+            Lv = sFc[k] >> 24;
             if constexpr (kLds) {
-                xyz = sXyz[k];
+                xyz = L.xyz[k];
                 const uint64_t* l = sLimbs[k];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) S.s1[i] = fromLimbs(l + 3 * i);
 #pragma unroll
                 for (int i = 0; i < 6; ++i) S.s2[i] = fromLimbs(l + 9 + 3 * i);
                 S.n = l[27];
-                S.e = 20 + D - L;
+                S.e = 20 + D - Lv;
                 S.pad = 0;
             } else {
-                xyz = make_uint4(t.nx[node], t.ny[node], t.nz[node], 0u);
+                xyz = make_uint4(t.nx[node], t.ny[node], t.nz[node], t.numVoxels[node]);
                 S = sSums[k];
             }
-            const uint32_t ca = sClose[k];
+            ca = sClose[k];
             // the record, a value per lane
             RollupSums* r = aa.rollup + node;
 #pragma unroll
@@ -918,53 +946,61 @@ __device__ __forceinline__ void rollSlots(const DevTree& t, const AnalysisArgs& 
                 r->e = S.e;
                 r->pad = 0;
             }
-            const ShapeCache sh = shapeOfQuad(S, 2u * xyz.x + 1u, 2u * xyz.y + 1u, 2u * xyz.z + 1u, qs);
-            float dx = 0.0f, dy = 0.0f;
-            if constexpr (kFinal) extentQuad(nodeBoundsOf(L, xyz.x, xyz.y, xyz.z, t.octreeSize), u, qs, dx, dy);
-            if (qs == 0) {
-                aa.closeAt[node] = ca;
-                aa.shapes[node] = sh;
-                t.shapeStale[node] = 0;
-                if constexpr (kFinal) {
-                    finalizeExtent(t, aa, u, node, sh, current >= ca, dx, dy);
-                    q = aa.results[node].score > 0.0f;
-                }
-            }
+            sh = shapeOfQuad(S, 2u * xyz.x + 1u, 2u * xyz.y + 1u, 2u * xyz.z + 1u, qs);
+            extentQuad(nodeBoundsOf(Lv, xyz.x, xyz.y, xyz.z, t.octreeSize), u, qs, dx, dy);
         }
-        if constexpr (kFinal) {
-            // the queue's back segment, counted in LDS
-            const uint64_t m = __ballot(q);
-            if (m != 0) {
-                const int leader = __ffsll((unsigned long long)m) - 1;
-                uint32_t at = 0;
-                if (laneId() == uint32_t(leader)) at = atomicAdd(backCount, uint32_t(__popcll(m)));
-                at = __shfl(at, leader);
-                const uint32_t slot = at + __popcll(m & lanesBelow());
-                if (q && slot < t.maxNodes) aa.queue[t.maxNodes - 1u - slot] = node;
+        const float* pred = nullptr;
+        if constexpr (kLearned) {
+            // the wave's 16 quads are one MLP tile: quad i's node is row i (the
+            // weight fragments are loaded here, in flight beside the inputs)
+            auto& tile = L.mlp.tile[wave];
+            MlpFrag frag;
+            if (__ballot(live) != 0) mlpLoadFrag(wb, frag);
+            if (qs == 0) {
+                float f[kMlpIn];
+                if (live) {
+                    const uint32_t odd[3] = {2u * xyz.x + 1u, 2u * xyz.y + 1u, 2u * xyz.z + 1u};
+                    nodeFeaturesOf(Lv, true, xyz.w, centredOdd(S, odd), arrival, f);
+                }
+                for (uint32_t j = 0; j < 16; ++j) tile.in[qi][j] = (live && j < kMlpIn) ? f[j] : 0.0f;
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (__ballot(live) != 0) mlpInferTile(frag, tile.in, tile.act, 16, 0, L.mlp.pred[wave]);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the predictions are in LDS
+            pred = &L.mlp.pred[wave][qi * kMlpOut];
+        }
+        bool q = false;
+        if (live && qs == 0) {
+            aa.closeAt[node] = ca;
+            aa.shapes[node] = sh;
+            t.shapeStale[node] = 0;
+            finalizeExtent(t, aa, u, node, sh, L.current >= ca, dx, dy, pred);
+            q = aa.results[node].score > 0.0f;
+        }
+        // the queue's back segment, counted in LDS
+        const uint64_t m = __ballot(q);
+        if (m != 0) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t at = 0;
+            if (laneId() == uint32_t(leader)) at = atomicAdd(&L.backCount, uint32_t(__popcll(m)));
+            at = __shfl(at, leader);
+            const uint32_t slot = at + __popcll(m & lanesBelow());
+            if (q && slot < t.maxNodes) aa.queue[t.maxNodes - 1u - slot] = node;
         }
     }
 }
 
-template <bool kFinal>
-__device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity,
-                            uint32_t readers) {
-    __shared__ uint32_t markBits[kMarkBits / 32];  // marked (rolled-up) inner nodes
-    __shared__ uint64_t lLimbs[kLdsSlots][kLimbStride];
-    __shared__ remo_shared_uniforms lU;  // the frame's uniforms, read at the start
-    __shared__ uint32_t lCurrent;
-    __shared__ uint32_t lNode[kLdsSlots], lFc[kLdsSlots], lClose[kLdsSlots], lUp[kLdsSlots];
-    __shared__ uint4 lXyz[kLdsSlots];
-    __shared__ uint32_t count[kLevels], start[kLevels], cursor[kLevels];
-    __shared__ uint32_t marked, lastReader, backCount;
+template <bool kLearned>
+__device__ __forceinline__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity,
+                            uint32_t readers, AnalysisLds& L, const uint16_t* wb, float arrival) {
     const uint32_t tid = threadIdx.x;
     const uint32_t D = t.g->maxLevel;
     for (uint32_t k = tid; k < kMarkBits / 128; k += kRollThreads)
-        reinterpret_cast<uint4*>(markBits)[k] = make_uint4(0, 0, 0, 0);
-    if (tid < kLevels) count[tid] = 0;
+        reinterpret_cast<uint4*>(L.markBits)[k] = make_uint4(0, 0, 0, 0);
+    if (tid < kLevels) L.count[tid] = 0;
     if (tid == 0) {
-        marked = 0;
-        backCount = 0;
+        L.marked = 0;
+        L.backCount = 0;
     }
     __syncthreads();
     // the frame's uniforms into LDS, by the top threads (which rarely have a node
@@ -974,12 +1010,13 @@ __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t n
     static_assert(sizeof(remo_shared_uniforms) % 4 == 0 && kUWords < kRollThreads, "uniforms copy as words");
     if (tid >= kRollThreads - kUWords) {
         const uint32_t w = tid - (kRollThreads - kUWords);
-        reinterpret_cast<uint32_t*>(&lU)[w] = reinterpret_cast<const uint32_t*>(&aa.frame->u)[w];
-        if (w == 0) lCurrent = t.g->batchletIndex;
+        reinterpret_cast<uint32_t*>(&L.u)[w] = reinterpret_cast<const uint32_t*>(&aa.frame->u)[w];
+        if (w == 0) L.current = t.g->batchletIndex;
     }
     // mark: a node on the list that is inner (it split) is rolled up itself, a
     // leaf from its parent; every ancestor with it. A mark carries the node's
-    // level, parent, first child and coordinates (loaded beside the climb's loads).
+    // level, parent, first child, coordinates and voxel count (loaded beside the
+    // climb's loads).
     for (uint32_t e = tid; e < nd; e += kRollThreads) {
         const uint32_t node = t.dirtyList[e];
         const bool inner = t.firstChild[node] >= 0;
@@ -992,20 +1029,20 @@ __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t n
         while (p != kNone) {
             const uint32_t next = t.parent[p];
             const uint32_t fc = uint32_t(t.firstChild[p]);
-            const uint4 xyz = make_uint4(t.nx[p], t.ny[p], t.nz[p], 0u);
+            const uint4 xyz = make_uint4(t.nx[p], t.ny[p], t.nz[p], t.numVoxels[p]);
             const uint32_t bit = 1u << (p & 31u);
-            if (atomicOr(&markBits[p >> 5], bit) & bit) break;  // marked, with its ancestors
-            const uint32_t m = atomicAdd(&marked, 1u);
+            if (atomicOr(&L.markBits[p >> 5], bit) & bit) break;  // marked, with its ancestors
+            const uint32_t m = atomicAdd(&L.marked, 1u);
             aa.markList[2 * m] = make_uint4(p | (lp << 24), next, fc, 0u);
             aa.markList[2 * m + 1] = xyz;
             aa.slotOf[p] = m;  // the LDS slot (the device path places by level)
             if (m < kLdsSlots) {
-                lNode[m] = p;
-                lFc[m] = fc | (lp << 24);
-                lUp[m] = next;  // the parent's node; its slot in the partial pass
-                lXyz[m] = xyz;
+                L.node[m] = p;
+                L.fc[m] = fc | (lp << 24);
+                L.up[m] = next;  // the parent's node; its slot in the partial pass
+                L.xyz[m] = xyz;
             }
-            atomicAdd(&count[lp], 1u);
+            atomicAdd(&L.count[lp], 1u);
             p = next;
             --lp;
         }
@@ -1014,41 +1051,49 @@ __device__ void rollupBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t n
     // the dirty list is read: count this workgroup (from a wave with no loads in
     // the partial pass below when fewer than 57 nodes are marked)
     uint32_t readBefore = 0;
-    if (readers > 1 && tid == kRollThreads - 64) readBefore = atomicAdd(aa.queueCtl + kQueueReaders, 1u);
-    const uint32_t M = marked;
+    if (tid == kRollThreads - 64) readBefore = atomicAdd(aa.queueCtl + kQueueReaders, 1u);
+    const uint32_t M = L.marked;
     uint32_t* queueCursor = aa.queueCtl + parity;
     if (M <= min(aa.ldsSlots, kLdsSlots))  // uniform
-        rollSlots<kFinal, true>(t, aa, D, M, markBits, count, start, cursor, lLimbs, nullptr, lNode, lFc, lClose,
-                                lUp, lXyz, nullptr, nullptr, queueCursor, &backCount, lU, lCurrent);
+        rollSlots<kLearned, true>(t, aa, D, M, L, nullptr, L.node, L.fc, L.close, nullptr, nullptr, queueCursor, wb,
+                                  arrival);
     else
-        rollSlots<kFinal, false>(t, aa, D, M, markBits, count, start, cursor, nullptr, aa.slotSums, aa.slotNode,
-                                 aa.slotFc, aa.slotClose, nullptr, nullptr, aa.slotChild, aa.slotNk, queueCursor,
-                                 &backCount, lU, lCurrent);
-    if (tid == kRollThreads - 64) lastReader = readers == 1 || readBefore == readers - 1u;
+        rollSlots<kLearned, false>(t, aa, D, M, L, aa.slotSums, aa.slotNode, aa.slotFc, aa.slotClose, aa.slotChild,
+                                   aa.slotNk, queueCursor, wb, arrival);
+    if (tid == kRollThreads - 64) L.lastReader = readBefore == readers - 1u;
     __syncthreads();
-    if (tid == 0) aa.queueCtl[kQueueBack + parity] = min(backCount, t.maxNodes);
-    if (lastReader) resetDirty(t, aa, nd, parity);
+    if (tid == 0) aa.queueCtl[kQueueBack + parity] = min(L.backCount, t.maxNodes);
+    if (L.lastReader) resetDirty(t, aa, nd, parity);
 }
 
-// A sweep workgroup: its ranges are 512-node ranges b, b + kSweepBlocks, ...
-__device__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity, uint32_t readers,
-                           uint32_t n) {
-    constexpr uint32_t kWords = kRollThreads / 32;
-    __shared__ uint32_t marks[kSweepRanges * kWords];
-    __shared__ uint32_t lastReader;
+// A sweep workgroup: its ranges are node ranges b, b + kSweepBlocks, ... of 512
+// nodes (a node per thread) or, with the learned score, of 128 (16 per wave, one
+// MLP tile: the trees hold some 10^4 nodes, so a tile per wave keeps more of the
+// chip busy than four tiles in a row).
+template <bool kLearned>
+__device__ __forceinline__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd, uint32_t parity,
+                                           uint32_t readers,
+                           uint32_t n, AnalysisLds& L, const uint16_t* wb, float arrival) {
+    constexpr uint32_t kRange = kLearned ? 128u : kRollThreads;
+    constexpr uint32_t kWords = kRange / 32;
+    // ranges a workgroup marks in LDS (pools up to kMarkBits nodes; larger ones
+    // take the full roll-up every time, so nothing on them is marked)
+    constexpr uint32_t kRanges = (kMarkBits + kRange * kSweepBlocks - 1) / (kRange * kSweepBlocks);
+    static_assert(kRanges * kWords <= sizeof(AnalysisLds::sweepMarks) / 4, "the sweep's marks fit");
+    uint32_t* marks = L.sweepMarks;
     const uint32_t tid = threadIdx.x;
     const uint32_t b = blockIdx.x - 1u;
-    const uint32_t nRanges = (n + kRollThreads - 1) / kRollThreads;
-    for (uint32_t k = tid; k < kSweepRanges * kWords; k += kRollThreads) marks[k] = 0;
+    const uint32_t nRanges = (n + kRange - 1) / kRange;
+    for (uint32_t k = tid; k < kRanges * kWords; k += kRollThreads) marks[k] = 0;
     __syncthreads();
     // the inner nodes of this workgroup's ranges that workgroup 0 rolls up
     for (uint32_t e = tid; e < nd; e += kRollThreads) {
         const uint32_t node = t.dirtyList[e];
         uint32_t p = t.firstChild[node] >= 0 ? node : t.parent[node];
         while (p != kNone) {
-            const uint32_t r = p / kRollThreads, k = r / kSweepBlocks;
-            if (r % kSweepBlocks == b && k < kSweepRanges) {
-                const uint32_t i = p % kRollThreads;
+            const uint32_t r = p / kRange, k = r / kSweepBlocks;
+            if (r % kSweepBlocks == b && k < kRanges) {
+                const uint32_t i = p % kRange;
                 atomicOr(&marks[k * kWords + i / 32], 1u << (i & 31u));
             }
             p = t.parent[p];
@@ -1058,13 +1103,36 @@ __device__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd
     const uint32_t current = t.g->batchletIndex;
     const remo_shared_uniforms& u = aa.frame->u;
     uint32_t* queueCursor = aa.queueCtl + parity;
+    const uint32_t wave = tid >> 6, lane = laneId();
     for (uint32_t k = 0, r = b; r < nRanges; ++k, r += kSweepBlocks) {
-        const uint32_t node = r * kRollThreads + tid;
-        const bool mk = k < kSweepRanges && ((marks[k * kWords + tid / 32] >> (tid & 31u)) & 1u);
+        const uint32_t i = kLearned ? wave * 16 + lane : tid;  // within the range
+        const uint32_t node = r * kRange + i;
+        const bool mine = (!kLearned || lane < 16) && node < n;
+        const bool mk = mine && k < kRanges && ((marks[k * kWords + i / 32] >> (i & 31u)) & 1u);
+        const bool fin = mine && !mk;
+        ShapeCache sh;
+        if (fin) sh = currentShape(t, aa, node);
+        const float* pred = nullptr;
+        if constexpr (kLearned) {
+            auto& tile = L.mlp.tile[wave];
+            const uint32_t w0 = r * kRange + wave * 16;  // this wave's nodes
+            if (w0 < n) {  // uniform over the wave
+                MlpFrag frag;  // (loaded beside the inputs)
+                mlpLoadFrag(wb, frag);
+                if (lane < 16) {
+                    float f[kMlpIn];
+                    if (fin) nodeFeatures(t, node, centred(t, node, aa.rollup[node]), arrival, f);
+                    for (uint32_t j = 0; j < 16; ++j) tile.in[lane][j] = (fin && j < kMlpIn) ? f[j] : 0.0f;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                mlpInferTile(frag, tile.in, tile.act, n, w0, L.mlp.pred[wave]);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the predictions are in LDS
+            }
+            pred = &L.mlp.pred[wave][(lane & 15u) * kMlpOut];
+        }
         bool q = false;
-        if (node < n && !mk) {
-            const ShapeCache sh = currentShape(t, aa, node);
-            finalizeNode(t, aa, u, node, sh, current >= aa.closeAt[node]);
+        if (fin) {
+            finalizeNode(t, aa, u, node, sh, current >= aa.closeAt[node], pred);
             q = aa.results[node].score > 0.0f;
         }
         queueScored(t, aa, queueCursor, node, q);
@@ -1072,78 +1140,32 @@ __device__ void sweepBlock(const DevTree& t, const AnalysisArgs& aa, uint32_t nd
     // counted as a reader only now (it read the list before its climbs), so that
     // the reset falls to a sweep workgroup rather than to workgroup 0, which
     // counts itself after its marking and finishes last
-    if (tid == 0) lastReader = atomicAdd(aa.queueCtl + kQueueReaders, 1u) == readers - 1u;
+    if (tid == 0) L.lastReader = atomicAdd(aa.queueCtl + kQueueReaders, 1u) == readers - 1u;
     __syncthreads();
-    if (lastReader) resetDirty(t, aa, nd, parity);
+    if (L.lastReader) resetDirty(t, aa, nd, parity);
 }
 
-// kSweep: the one-launch Analysis (workgroup 0 the roll-up, 1 + kSweepBlocks
-// workgroups); otherwise the roll-up alone (one workgroup), whose finalize
-// follows as kFinalizeLearned.
-template <bool kSweep>
-__global__ __launch_bounds__(kRollThreads) void kAnalysisStep(DevTree t, AnalysisArgs aa) {
+// Analysis in one launch: workgroup 0 the roll-up, 1 + kSweepBlocks workgroups
+// in all. With the learned score (kLearned) `wb` holds its BF16 parameters and
+// `arrival` the batch fraction of its inputs, or a negative value to read it from
+// the frame parameters (a captured graph, whose arguments are fixed at capture).
+template <bool kLearned>
+__global__ __launch_bounds__(kRollThreads) void kAnalysisStep(DevTree t, AnalysisArgs aa, const uint16_t* wb,
+                                                              float arrival) {
+    __shared__ AnalysisLds L;
     const uint32_t n = t.g->numNodes;
     // (pools beyond the bitmap always take the full roll-up, which empties the list)
     const uint32_t nd = t.maxNodes <= kMarkBits ? min(*t.dirtyCount, t.maxNodes) : 0u;
     const uint32_t parity = aa.queueCtl[kQueueParity];
+    if (kLearned && arrival < 0.0f) arrival = aa.frame->arrival;
     // the workgroups that read the dirty list: the roll-up and each sweep
     // workgroup with a range
-    const uint32_t readers = kSweep ? 1u + min(kSweepBlocks, (n + kRollThreads - 1) / kRollThreads) : 1u;
-    if (blockIdx.x == 0) {
-        rollupBlock<kSweep>(t, aa, nd, parity, readers);
-    } else if constexpr (kSweep) {
-        if (blockIdx.x < readers) sweepBlock(t, aa, nd, parity, readers, n);
-    }
-}
-
-// The finalize with the learned score, after the roll-up alone, 16 nodes
-// per wave (the Analysis trees hold some 10^4 nodes, so a wave per 16 keeps more
-// of the chip busy than a wave per 64 running four MLP tiles in a row): lanes
-// 0..15 write their node's 13 inputs into LDS, the wave runs one MLP tile on the
-// matrix cores (the same tile as kMlpInfer), and lanes 0..15 finalise their
-// nodes, open nodes taking the prediction. A leaf whose record changed since its
-// shape was solved is solved here.
-// `arrival` is the batch fraction of the inputs, or negative to read it from the
-// frame parameters (a captured graph, whose arguments are fixed at capture).
-__global__ __launch_bounds__(kBlock) void kFinalizeLearned(DevTree t, AnalysisArgs aa, const uint16_t* wb,
-                                                           float arrival) {
-    static_assert(kBlock == kMlpInferThreads, "one MLP wave per finalize wave");
-    constexpr uint32_t kPerBlock = 16 * kMlpWavesPerBlock;
-    __shared__ MlpWaveShared sh[kMlpWavesPerBlock];
-    __shared__ float pred[kMlpWavesPerBlock][16 * kMlpOut];
-    Globals* g = t.g;
-    const uint32_t n = g->numNodes;
-    const uint32_t current = g->batchletIndex;
-    const remo_shared_uniforms& u = aa.frame->u;
-    if (arrival < 0.0f) arrival = aa.frame->arrival;
-    // the roll-up before this launch flipped the parity: its queue is the other one
-    uint32_t* queueCursor = aa.queueCtl + (aa.queueCtl[kQueueParity] ^ 1u);
-    const uint32_t wave = threadIdx.x / 64, lane = laneId();
-    // every lane's weight fragments, in registers for the whole kernel (no barrier)
-    MlpFrag frag;
-    if (blockIdx.x * kPerBlock + wave * 16 < n) mlpLoadFrag(wb, frag);
-    for (uint32_t base = blockIdx.x * kPerBlock; base < n; base += gridDim.x * kPerBlock) {
-        const uint32_t w0 = base + wave * 16;  // this wave's nodes
-        const uint32_t node = w0 + lane;
-        const bool mine = lane < 16 && node < n;
-        if (w0 < n) {
-            if (lane < 16) {
-                float f[kMlpIn];
-                if (node < n) nodeFeatures(t, node, centred(t, node, aa.rollup[node]), arrival, f);
-                for (uint32_t k = 0; k < 16; ++k) sh[wave].in[lane][k] = (node < n && k < kMlpIn) ? f[k] : 0.0f;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            mlpInferTile(frag, sh[wave].in, sh[wave].act, n, w0, pred[wave]);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the predictions are in LDS
-        }
-        bool q = false;
-        if (mine) {
-            const ShapeCache shape = currentShape(t, aa, node);
-            finalizeNode(t, aa, u, node, shape, current >= aa.closeAt[node], &pred[wave][lane * kMlpOut]);
-            q = aa.results[node].score > 0.0f;
-        }
-        queueScored(t, aa, queueCursor, node, q);
-    }
+    constexpr uint32_t kRange = kLearned ? 128u : kRollThreads;
+    const uint32_t readers = 1u + min(kSweepBlocks, (n + kRange - 1) / kRange);
+    if (blockIdx.x == 0)
+        rollupBlock<kLearned>(t, aa, nd, parity, readers, L, wb, arrival);
+    else if (blockIdx.x < readers)
+        sweepBlock<kLearned>(t, aa, nd, parity, readers, n, L, wb, arrival);
 }
 
 // Every node's record in its own frame (the roll-up dump). A record never
@@ -1556,18 +1578,16 @@ void launchFullRollup(AnalysisState& a, const DevTree& t, uint32_t maxLevel, hip
 
 // The per-frame Analysis kernels (no host work, so they can be captured): the
 // incremental roll-up of what changed since the last Analysis with the finalize
-// of every node in the same launch, or with the learned score the roll-up alone
-// and then the learned finalize. The learned score's arrival fraction goes in as
-// an argument when arrival >= 0, else it is read from the frame parameters.
+// of every node in the same launch. The learned score's arrival fraction goes in
+// as an argument when arrival >= 0, else it is read from the frame parameters.
 void launchAnalysis(AnalysisState& a, const DevTree& t, bool learned, float arrival, hipStream_t s) {
     const AnalysisArgs aa = argsOf(a);
-    if (learned) {
-        hipLaunchKernelGGL(kAnalysisStep<false>, dim3(1), dim3(kRollThreads), 0, s, t, aa);
-        hipLaunchKernelGGL(kFinalizeLearned, dim3(a.shape.nodeBlocks), dim3(kBlock), 0, s, t, aa, a.mlpWeights,
-                           arrival);
-    } else {
-        hipLaunchKernelGGL(kAnalysisStep<true>, dim3(1 + kSweepBlocks), dim3(kRollThreads), 0, s, t, aa);
-    }
+    if (learned)
+        hipLaunchKernelGGL(kAnalysisStep<true>, dim3(1 + kSweepBlocks), dim3(kRollThreads), 0, s, t, aa,
+                           a.mlpWeights, arrival);
+    else
+        hipLaunchKernelGGL(kAnalysisStep<false>, dim3(1 + kSweepBlocks), dim3(kRollThreads), 0, s, t, aa,
+                           static_cast<const uint16_t*>(nullptr), 0.0f);
 }
 
 // The Rasterize kernels alone.

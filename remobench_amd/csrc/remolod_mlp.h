@@ -261,13 +261,14 @@ struct alignas(16) MlpWaveShared {
 //   10     level / 20
 //   11     occupancy fill: inner nodes numVoxels / 128^3, leaves min(1, n / 50,000)
 //   12     arrival: the fraction of the cloud's batches ingested so far
-__device__ inline void nodeFeatures(const DevTree& t, uint32_t node, const RollupSums& S, float arrival,
-                                    float f[kMlpIn]) {
+// (level, whether the node is inner, its voxel count)
+__device__ inline void nodeFeaturesOf(uint32_t level, bool inner, uint32_t numVoxels, const RollupSums& S,
+                                      float arrival, float f[kMlpIn]) {
     for (uint32_t k = 0; k < kMlpIn; ++k) f[k] = 0.0f;
     const double dn = double(S.n);
     f[0] = float(log1p(dn) / log1p(50000.0));
-    f[10] = float(t.level[node]) / 20.0f;
-    f[11] = t.firstChild[node] >= 0 ? float(t.numVoxels[node]) / 2097152.0f : float(fmin(1.0, dn / 50000.0));
+    f[10] = float(level) / 20.0f;
+    f[11] = inner ? float(numVoxels) / 2097152.0f : float(fmin(1.0, dn / 50000.0));
     f[12] = arrival;
     if (S.n == 0) return;
     const double scale = ldexp(1.0, -int(S.e));
@@ -284,6 +285,11 @@ __device__ inline void nodeFeatures(const DevTree& t, uint32_t node, const Rollu
     const double trace = cov[0] + cov[3] + cov[5];
     if (!(trace > 0.0)) return;
     for (int k = 0; k < 6; ++k) f[1 + k] = float(cov[k] / trace);
+}
+
+__device__ inline void nodeFeatures(const DevTree& t, uint32_t node, const RollupSums& S, float arrival,
+                                    float f[kMlpIn]) {
+    nodeFeaturesOf(t.level[node], t.firstChild[node] >= 0, t.numVoxels[node], S, arrival, f);
 }
 
 // The context's copy of a trained model (BF16 parameters).

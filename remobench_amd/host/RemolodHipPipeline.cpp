@@ -150,8 +150,6 @@ bool RemolodHipPipeline::build(PointSource& source, const FrameContext& frame) {
     }
     m_batchesConsumed = consumed;
     source.setBatchesConsumed(consumed);
-    // the stats of this build, read while Accumulate and Analysis run
-    const bool statsQueued = remo_request_stats(m_ctx) == REMO_OK;
     if (m_opt.accumulate) {
         const int ra = remo_accumulate(m_ctx, consumed);
         if (ra != REMO_OK) {
@@ -159,6 +157,8 @@ bool RemolodHipPipeline::build(PointSource& source, const FrameContext& frame) {
             return false;
         }
     }
+    // the stats of this build and Accumulate, read while Analysis runs
+    const bool statsQueued = remo_request_stats(m_ctx) == REMO_OK;
     if (m_opt.analysisMode >= 0) {
         std::string e;
         if (!analyse(frame, m_opt.analysisMode, &e)) return false;

@@ -196,8 +196,8 @@ def test_voxel_backlog_segment_overflow(gpu):
 @pytest.mark.gpu
 def test_requested_stats_match_get_stats(gpu):
     """remo_request_stats / remo_collect_stats (the plugin's per-frame read) give
-    remo_get_stats's numbers for the same tree, step by step while Accumulate and
-    Analysis are queued behind the request; collecting twice is an error."""
+    remo_get_stats's numbers for the same tree, step by step with Analysis queued
+    behind the request; collecting twice is an error."""
     import ctypes as C
 
     from remobench_amd import camera
@@ -213,14 +213,15 @@ def test_requested_stats_match_get_stats(gpu):
     raw = L.Stats()
     for step in range(nb):
         _, consumed = tree.build_step(ring.view())
-        tree._ck(tree.lib.remo_request_stats(tree.h), "stats request")
         tree.accumulate(consumed)
+        tree._ck(tree.lib.remo_request_stats(tree.h), "stats request")
         tree.analysis(u)
         tree._ck(tree.lib.remo_collect_stats(tree.h, C.byref(raw)), "stats")
         got = raw.as_dict()
         want = tree.stats()
         for k in ("num_nodes", "num_inner", "num_leaves", "num_points", "num_voxels", "batchlet_index",
-                  "max_points_per_node", "max_depth", "allocated_bytes_persistent", "num_chunks_points"):
+                  "max_points_per_node", "max_depth", "allocated_bytes_persistent", "num_chunks_points",
+                  "sum_leaf_counts", "inner_with_sums", "num_accumulated", "morton_watermark"):
             assert got[k] == want[k], (step, k, got[k], want[k])
     assert got["num_points"] == len(points)
     assert L.ERRORS[tree.lib.remo_collect_stats(tree.h, C.byref(raw))] == "REMO_ERR_STATE"
