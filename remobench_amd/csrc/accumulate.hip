@@ -95,37 +95,57 @@ __global__ __launch_bounds__(kBlock) void kAccumPlan(DevTree t, AccumWork w, uin
     // after they are cleared
     unsigned long long sumCounts = 0;
     uint32_t innerSums = 0;
+    const uint32_t lane = laneId();
     for (uint32_t base0 = blockIdx.x * blockDim.x; base0 < n; base0 += gridDim.x * blockDim.x) {
         const uint32_t node = base0 + threadIdx.x;
+        uint32_t from = 0, to = 0, nt = 0;
         if (node < n) {
             remo_node_accum* a = t.accums + node;
-            w.nodeTiles[node] = 0;
             if (t.firstChild[node] >= 0) {
                 if (a->count != 0) *a = remo_node_accum{};
                 innerSums += a->count != 0 ? 1u : 0u;
             } else {
-                const uint32_t from = a->count, to = t.numPoints[node];
-                uint32_t after = from;
-                if (to > from) {
-                    const uint32_t nt = (to - from + AccumWork::kTile - 1) / AccumWork::kTile;
-                    const uint32_t base = atomicAdd(w.numTiles + parity, nt);
-                    if (uint64_t(base) + nt > w.tileCap) {
-                        // no room: mark the reserved slots below the capacity empty
-                        for (uint32_t k = 0; base + k < w.tileCap && k < nt; ++k)
-                            w.tiles[base + k] = make_uint4(kNone, 0, 0, 0);
-                    } else {
-                        for (uint32_t k = 0; k < nt; ++k) {
-                            const uint32_t b = from + k * AccumWork::kTile;
-                            w.tiles[base + k] = make_uint4(node, b, min(to, b + AccumWork::kTile), k);
-                        }
-                        w.nodeTiles[node] = nt;
-                        w.nodeBase[node] = base;
-                        w.planned[atomicAdd(w.numPlanned + parity, 1u)] = node;
-                        after = to;
-                    }
-                }
-                sumCounts += after;
+                from = a->count;
+                to = t.numPoints[node];
+                if (to > from) nt = (to - from + AccumWork::kTile - 1) / AccumWork::kTile;
             }
+        }
+        // the wave's tiles and planned leaves reserved with one atomic each (a
+        // returning atomic per leaf on these two counters queued at the memory side)
+        uint32_t incl = nt;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        const uint64_t planning = __ballot(nt != 0);
+        uint32_t tileAt = 0, leafAt = 0;
+        if (lane == 0 && total != 0) {
+            tileAt = atomicAdd(w.numTiles + parity, total);
+            leafAt = atomicAdd(w.numPlanned + parity, uint32_t(__popcll(planning)));
+        }
+        tileAt = __shfl(tileAt, 0);
+        leafAt = __shfl(leafAt, 0);
+        if (node < n && t.firstChild[node] < 0) {
+            uint32_t after = from;
+            if (nt != 0) {
+                const uint32_t base = tileAt + incl - nt;
+                if (uint64_t(base) + nt > w.tileCap) {
+                    // no room: mark the reserved slots below the capacity empty,
+                    // and the leaf's list entry with no tiles
+                    for (uint32_t k = 0; base + k < w.tileCap && k < nt; ++k)
+                        w.tiles[base + k] = make_uint4(kNone, 0, 0, 0);
+                    w.planned[leafAt + __popcll(planning & lanesBelow())] = make_uint4(kNone, 0, 0, 0);
+                } else {
+                    for (uint32_t k = 0; k < nt; ++k) {
+                        const uint32_t b = from + k * AccumWork::kTile;
+                        w.tiles[base + k] = make_uint4(node, b, min(to, b + AccumWork::kTile), k);
+                    }
+                    w.planned[leafAt + __popcll(planning & lanesBelow())] = make_uint4(node, base, nt, to);
+                    after = to;
+                }
+            }
+            sumCounts += after;
         }
     }
     sumCounts = waveSum(sumCounts);
@@ -169,9 +189,10 @@ struct LeafTotals {
     uint32_t count;
 };
 
-__device__ __forceinline__ void finishLeaf(const DevTree& t, const AccumWork& w, uint32_t node, uint32_t batchIndex,
-                           AccumPartial* sp, LeafTotals& tot) {
-    const uint32_t nt = w.nodeTiles[node], base = w.nodeBase[node];
+// `plan`: (leaf, first tile, tiles, count after the fold), as the plan listed it.
+__device__ __forceinline__ void finishLeaf(const DevTree& t, const AccumWork& w, uint4 plan, uint32_t batchIndex,
+                                           AccumPartial* sp, LeafTotals& tot) {
+    const uint32_t node = plan.x, base = plan.y, nt = plan.z;
     const uint32_t nLoad = min(nt, kMaxLeafTiles);
     constexpr uint32_t kVecs = sizeof(AccumPartial) / 8;
     const uint32_t f = threadIdx.x;
@@ -188,7 +209,7 @@ __device__ __forceinline__ void finishLeaf(const DevTree& t, const AccumWork& w,
         D = t.g->maxLevel;
     }
     if (f == kCountThread) {
-        const uint32_t from = a->count, to = w.tiles[base + nt - 1].z;
+        const uint32_t from = a->count, to = plan.w;
         a->count = to;
         tot.count = to;
         a->last_touched_batch = batchIndex;
@@ -392,7 +413,10 @@ __global__ __launch_bounds__(kBlock) void kAccumFinish(DevTree t, AccumWork w, u
         g->sumLeafCountsAcc = 0;
         g->innerWithSumsAcc = 0;
     }
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) finishLeaf(t, w, w.planned[k], batchIndex, sp, tot);
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint4 plan = w.planned[k];
+        if (plan.x != kNone) finishLeaf(t, w, plan, batchIndex, sp, tot);  // (uniform: a leaf per workgroup)
+    }
 }
 
 // The FP32-atomics comparator: every stored point of every leaf adds its

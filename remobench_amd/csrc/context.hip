@@ -462,8 +462,6 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
         w.tileCap = std::min<uint64_t>(w.tileCap, 1ull << 24);
         A(&w.tiles, w.tileCap, "accumulate tiles");
         A(&w.partials, w.tileCap, "accumulate partial sums");
-        A(&w.nodeTiles, N, "accumulate tiles per node");
-        A(&w.nodeBase, N, "accumulate first tile per node");
         A(&w.planned, N, "accumulate planned leaves");
         A(&w.numPlanned, 2, "accumulate planned leaf counts");
         A(&w.numTiles, 2, "accumulate tile counts");

@@ -191,9 +191,7 @@ struct AccumWork {
     static constexpr uint32_t kTile = 2048;  // points per tile
     uint4* tiles;            // (node, begin, end, index within node)
     AccumPartial* partials;  // one per tile
-    uint32_t* nodeTiles;     // [maxNodes] tiles of the node this launch
-    uint32_t* nodeBase;      // [maxNodes] first tile of the node
-    uint32_t* planned;       // [maxNodes] the leaves planned this launch
+    uint4* planned;          // [maxNodes] the leaves planned this launch (leaf, first tile, tiles, new count)
     uint32_t* numPlanned;    // [2] leaves planned, by launch parity (a launch clears the other's)
     uint32_t* numTiles;      // [2] tiles planned, by launch parity
     uint64_t tileCap;
