@@ -72,6 +72,54 @@ __device__ __forceinline__ T waveSum(T v) {
     return v;
 }
 
+// Wave reductions by DPP (VALU lane exchanges, no LDS round trip), the total in
+// lane 63: pairs and quads, the mirrored halves of each row of 8 and of 16 lanes
+// (every lane of a row then holds the row's total), row 0's total into row 1 and
+// row 2's into row 3 (row_bcast:15), rows 0-1's into rows 2-3 (row_bcast:31). A
+// 64-bit value moves as two 32-bit halves. The order of the additions is fixed,
+// so FP sums are deterministic.
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowMirror = 0x140;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dppU32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), kCtrl, kRowMask, 0xF, false));
+}
+template <int kCtrl, int kRowMask, typename T>
+__device__ __forceinline__ T dppMove(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        const uint32_t r = dppU32<kCtrl, kRowMask>(__builtin_bit_cast(uint32_t, v));
+        return __builtin_bit_cast(T, r);
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t r = uint64_t(dppU32<kCtrl, kRowMask>(uint32_t(u))) |
+                           (uint64_t(dppU32<kCtrl, kRowMask>(uint32_t(u >> 32))) << 32);
+        return __builtin_bit_cast(T, r);
+    }
+}
+// (rows outside the mask receive 0, the identity of both reductions below)
+template <typename T>
+__device__ __forceinline__ T waveTotalDpp(T v) {
+    v += dppMove<kQuadXor1, 0xF>(v);
+    v += dppMove<kQuadXor2, 0xF>(v);
+    v += dppMove<kRowHalfMirror, 0xF>(v);
+    v += dppMove<kRowMirror, 0xF>(v);
+    v += dppMove<kRowBcast15, 0xA>(v);
+    v += dppMove<kRowBcast31, 0xC>(v);
+    return v;
+}
+__device__ __forceinline__ uint64_t waveMaxDpp(uint64_t v) {
+    uint64_t o;
+    o = dppMove<kQuadXor1, 0xF>(v); v = o > v ? o : v;
+    o = dppMove<kQuadXor2, 0xF>(v); v = o > v ? o : v;
+    o = dppMove<kRowHalfMirror, 0xF>(v); v = o > v ? o : v;
+    o = dppMove<kRowMirror, 0xF>(v); v = o > v ? o : v;
+    o = dppMove<kRowBcast15, 0xA>(v); v = o > v ? o : v;
+    o = dppMove<kRowBcast31, 0xC>(v); v = o > v ? o : v;
+    return v;
+}
+
 __device__ __forceinline__ uint64_t waveMax(uint64_t v) {
     for (int off = 32; off > 0; off >>= 1) {
         const uint64_t o = __shfl_down(v, off);
@@ -355,21 +403,21 @@ __global__ __launch_bounds__(kBlock) void kAccumTiles(DevTree t, AccumWork w, ui
             lc[3] += cr * cr; lc[4] += cg * cg; lc[5] += cb * cb;
         }
         for (int i = 0; i < 9; ++i) {
-            const long long v = waveSum(q[i]);
-            const double d = waveSum(lg[i]);
-            if (lane == 0) {
+            const long long v = waveTotalDpp(q[i]);
+            const double d = waveTotalDpp(lg[i]);
+            if (lane == 63) {
                 sq[i][wave] = v;
                 sg[i][wave] = d;
             }
         }
         for (int i = 0; i < 6; ++i) {
-            const float v = waveSum(lc[i]);
-            if (lane == 0) sc[i][wave] = v;
+            const float v = waveTotalDpp(lc[i]);
+            if (lane == 63) sc[i][wave] = v;
         }
         {
             const uint64_t lm = (mortonSpread(bX) << 2) | (mortonSpread(bY) << 1) | mortonSpread(bZ);
-            const uint64_t v = waveMax(lm);
-            if (lane == 0) sm[wave] = v;
+            const uint64_t v = waveMaxDpp(lm);
+            if (lane == 63) sm[wave] = v;
         }
         __syncthreads();
         if (threadIdx.x < 25) {
