@@ -880,6 +880,7 @@ __global__ __launch_bounds__(kBlock) void kVoxelSample(DevTree t, BatchRef b) {
                 node = rec.x + ((((X >> s2) & 1u) << 2) | (((Y >> s2) & 1u) << 1) | ((Z >> s2) & 1u));
                 lvl += 1;
             }
+            t.pointLeaf[i] = node | (lvl << kTopLevelShift);  // for the insertion (the tree stays as it is)
         }
         // The climb, from the leaf up: one loop with a single entry and a
         // uniform trip count, so every lane runs each step's staging together. A
@@ -1092,7 +1093,7 @@ constexpr uint32_t kInsertPerThread = 8;
 constexpr uint32_t kInsertTile = kBlock * kInsertPerThread;
 static_assert(kInsertTile <= kAggSlots, "a tile's keys fit the aggregation table");
 
-__global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
+__global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b, uint32_t leavesKnown) {
     __shared__ BlockAgg agg;
     const uint32_t batchSize = batchSizeOf(t, b);
     Globals* g = t.g;
@@ -1121,7 +1122,15 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b) {
             Y[j] = quant28(p[j].y, size) >> 8;
             Z[j] = quant28(p[j].z, size) >> 8;
         }
-        descendN(t.firstChild, t.topCell, X, Y, Z, valid, leaf, level);
+        if (leavesKnown) {  // (uniform) the voxel sampler's descent of the same points
+            for (uint32_t j = 0; j < kInsertPerThread; ++j) {
+                const uint32_t v = valid[j] ? t.pointLeaf[tile * kInsertTile + j * kBlock + threadIdx.x] : 0u;
+                leaf[j] = v & kTopNodeMask;
+                level[j] = v >> kTopLevelShift;
+            }
+        } else {
+            descendN(t.firstChild, t.topCell, X, Y, Z, valid, leaf, level);
+        }
         for (uint32_t j = 0; j < kInsertPerThread; ++j) {
             slotOf[j] = kEmpty32;
             if (!valid[j]) {
@@ -1395,8 +1404,8 @@ void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s) {
     hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
 }
 
-void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s) {
-    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b);
+void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool leavesKnown) {
+    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, b, leavesKnown ? 1u : 0u);
 }
 
 void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool closeBatch) {
@@ -1657,7 +1666,7 @@ void launchCollapse(const DevTree& t, uint32_t numChosen, BatchRef empty, uint32
     hipLaunchKernelGGL(kCollapseFinish, dim3(std::max(1u, std::min(numChosen, 1024u))), dim3(kBlock), 0, s, t,
                        numChosen, refPoints, refVoxels);
     hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
-    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
+    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);  // (no voxel sampling ran over these points)
     hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);
 }
 
@@ -1674,7 +1683,7 @@ void launchDeepenSplit(const DevTree& t, uint32_t numChosen, BatchRef empty, con
     hipLaunchKernelGGL(kSpillAndZero, dim3(1024), dim3(kBlock), 0, s, t, 0u);
     hipLaunchKernelGGL(kVoxelSample, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
     hipLaunchKernelGGL(kAllocChunks, dim3(ls.nodeBlocks), dim3(kBlock), 0, s, t);
-    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty);
+    hipLaunchKernelGGL(kInsertPoints, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 1u);
     hipLaunchKernelGGL(kInsertVoxels, dim3(ls.pointBlocks), dim3(kBlock), 0, s, t, empty, 0u);
 }
 

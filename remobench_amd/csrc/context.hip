@@ -438,6 +438,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.arena, t.arenaCapacity, "the persistent octree store");
     A(&t.chunkQueue, t.chunkQueueCap, "the chunk queue");
     A(&t.spill, t.spillCap, "spilled points");
+    A(&t.pointLeaf, uint64_t(cfg.slot_capacity) + t.spillCap, "the leaf of each point of a batch");
     A(&t.backlogPts, t.backlogCap, "the voxel backlog");
     A(&t.backlogNode, t.backlogCap, "the voxel backlog targets");
     A(&t.backlogFill, uint64_t(kBacklogShards) * kShardStride, "the voxel backlog segment counters");
@@ -613,7 +614,7 @@ int remo_build_step(remo_ctx* c, const remo_batch_view* v, uint32_t* consumed) {
         }
         {
             Scope sc(c->prof, kScopeInsertPoints, s);
-            launchInsertPoints(t, b, c->shape, s);
+            launchInsertPoints(t, b, c->shape, s, true);
         }
         {
             Scope sc(c->prof, kScopeInsertVoxels, s);

@@ -133,6 +133,8 @@ struct DevTree {
     uint32_t* chunkQueue;   // recycled point chunks (remolod_octree.cu:349-360, :763-770)
     uint64_t chunkQueueCap;
     // per-batch scratch
+    uint32_t* pointLeaf;    // [slot capacity + spill capacity] each point's leaf | its level << kTopLevelShift,
+                            // from the voxel sampler's descent, for the insertion that follows it
     remo_point* spill;
     uint32_t spillCap;
     uint32_t backlogCap;
@@ -328,7 +330,9 @@ void launchFixedExpand(const DevTree& t, BatchRef b, const LaunchShape& ls, hipS
 inline uint64_t fixedPyramidEntries(uint32_t d) { return ((1ull << (3 * (d + 1))) - 1ull) / 7ull; }
 void launchVoxelSample(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
 void launchAllocChunks(const DevTree& t, const LaunchShape& ls, hipStream_t s);
-void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s);
+// leavesKnown: kVoxelSample ran over the same points since the tree last changed,
+// so their leaves are taken from pointLeaf instead of a descent.
+void launchInsertPoints(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool leavesKnown);
 // closeBatch: also the batch loop's closing update (batch index, processed points)
 void launchInsertVoxels(const DevTree& t, BatchRef b, const LaunchShape& ls, hipStream_t s, bool closeBatch = true);
 // Refinement's local deepening: list the leaves that may split (records of 24 B:
