@@ -326,6 +326,9 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
     // blocks and grids a collapse pooled go first (kSpillAndZero settles the counts)
     const uint32_t pooledBlocks = g->numFreeBlocks;
     const uint32_t pooledGrids = g->numFreeGrids;
+    // the deepest level before this launch: a split below it makes the next
+    // Analysis roll up every node, so the children's records may take any frame
+    const uint32_t D0 = g->maxLevel;
     const bool staged = nc <= kRankSlots;
     if (staged) {
         for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
@@ -334,25 +337,32 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         }
     }
     __syncthreads();
-    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
+    // 8 lanes per candidate: each computes the same rank and child block, lane i
+    // initialises child i, lane 0 does the node's own work
+    const uint32_t sub = threadIdx.x & 7u;
+    for (uint32_t ci = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; ci < nc; ci += (gridDim.x * blockDim.x) >> 3) {
         const uint32_t node = t.candidates[ci];
+        // the rank in key order: the group's lanes take every eighth candidate
         uint32_t rank = 0;
         if (staged) {
             const uint4 mine = keys[ci];
-            for (uint32_t k = 0; k < nc; ++k) rank += keyLess4(keys[k], mine) ? 1u : 0u;
+            for (uint32_t k = sub; k < nc; k += 8) rank += keyLess4(keys[k], mine) ? 1u : 0u;
         } else {
-            for (uint32_t k = 0; k < nc; ++k) rank += keyLess(t, t.candidates[k], node) ? 1u : 0u;
+            for (uint32_t k = sub; k < nc; k += 8) rank += keyLess(t, t.candidates[k], node) ? 1u : 0u;
         }
+        for (int m = 1; m < 8; m <<= 1) rank += uint32_t(__shfl_xor(int(rank), m));
         // the lowest ranks take the pooled blocks, the rest extend the pool's
         // used range: the node pool accepts a prefix of the key order
         const bool fromPool = rank < pooledBlocks;
         const uint64_t childOffset = fromPool ? uint64_t(t.freeBlocks[pooledBlocks - 1 - rank])
                                               : uint64_t(base) + 8ull * (rank - pooledBlocks);
         if (!fromPool && childOffset + 8 > t.maxNodes) {
-            g->nodePoolOverflow = 1;  // stays a leaf; its points stay where they are
-            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
-            bo[0] = make_uint4(0, 0, 0, 0);
-            bo[1] = make_uint4(0, 0, 0, 0);
+            if (sub == 0) {
+                g->nodePoolOverflow = 1;  // stays a leaf; its points stay where they are
+                uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
+                bo[0] = make_uint4(0, 0, 0, 0);
+                bo[1] = make_uint4(0, 0, 0, 0);
+            }
             continue;
         }
         const uint32_t c0 = uint32_t(childOffset);
@@ -371,6 +381,45 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
         const uint64_t gridOff0 = gno0 != kNone ? t.gridOffset[gno0] : 0ull;
         const uint32_t octCount[8] = {sLo.x + bLo.x, sLo.y + bLo.y, sLo.z + bLo.z, sLo.w + bLo.w,
                                       sHi.x + bHi.x, sHi.y + bHi.y, sHi.z + bHi.z, sHi.w + bHi.w};
+        // child `sub`, by its own lane (the children's many stores were one
+        // lane's long sequence of store instructions)
+        {
+            const uint32_t c = c0 + sub;
+            const uint32_t cnt = octCount[0] * (sub == 0) + octCount[1] * (sub == 1) + octCount[2] * (sub == 2) +
+                                 octCount[3] * (sub == 3) + octCount[4] * (sub == 4) + octCount[5] * (sub == 5) +
+                                 octCount[6] * (sub == 6) + octCount[7] * (sub == 7);
+            t.firstChild[c] = -1;
+            t.walk[c] = make_uint2(kNone, 0u);
+            t.parent[c] = node;
+            t.level[c] = lvl;
+            t.nx[c] = 2 * X + ((sub >> 2) & 1u);
+            t.ny[c] = 2 * Y + ((sub >> 1) & 1u);
+            t.nz[c] = 2 * Z + (sub & 1u);
+            t.counter[c] = cnt;
+            t.numPoints[c] = 0;
+            t.numVoxels[c] = 0;
+            t.numVoxelsStored[c] = 0;
+            t.gridNo[c] = kNone;
+            const bool pend = cnt > kMaxPointsPerNode && lvl < uint32_t(kMaxDepth);
+            t.pending[c] = pend ? 1u : 0u;
+            if (pend) atomicAdd(&g->numPending, 1u);
+            uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * c);
+            so[0] = so[1] = make_uint4(0, 0, 0, 0);
+            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * c);
+            bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
+            remo_node_accum* a = t.accums + c;
+            *a = remo_node_accum{};
+            // an empty leaf's roll-up record: no points, closed from the start
+            RollupSums* r = t.rollup + c;
+            for (int k = 0; k < 3; ++k) r->s1[k] = 0;
+            for (int k = 0; k < 6; ++k) r->s2[k] = 0;
+            r->n = 0;
+            r->e = 20 + max(D0, lvl) - lvl;
+            r->pad = 0;
+            t.closeAt[c] = 0;
+            markRecordChanged(t, c);
+        }
+        if (sub != 0) continue;
         // the split's counter reservations, issued together: each is a returning
         // atomic on one global counter, and waiting for one before issuing the
         // next made a round's splits a chain of five memory round trips
@@ -396,43 +445,7 @@ __global__ __launch_bounds__(1024) void kSplitApply(DevTree t) {
             gi = atomicAdd(&g->numGrids, 1u);
             gridAt = atomicAdd((unsigned long long*)&g->arenaOffset, (unsigned long long)kGridAllocBytes);
         }
-        // the deepest level as it stands (a split that deepens the tree makes the
-        // next Analysis roll up every node, whatever frame these records take)
-        const uint32_t D = max(atomicMax(&g->maxLevel, lvl), lvl);
-        for (uint32_t i = 0; i < 8; ++i) {
-            const uint32_t c = c0 + i;
-            const uint32_t cnt = octCount[i];
-            t.firstChild[c] = -1;
-            t.walk[c] = make_uint2(kNone, 0u);
-            t.parent[c] = node;
-            t.level[c] = lvl;
-            t.nx[c] = 2 * X + ((i >> 2) & 1u);
-            t.ny[c] = 2 * Y + ((i >> 1) & 1u);
-            t.nz[c] = 2 * Z + (i & 1u);
-            t.counter[c] = cnt;
-            t.numPoints[c] = 0;
-            t.numVoxels[c] = 0;
-            t.numVoxelsStored[c] = 0;
-            t.gridNo[c] = kNone;
-            const bool pend = cnt > kMaxPointsPerNode && lvl < uint32_t(kMaxDepth);
-            t.pending[c] = pend ? 1u : 0u;
-            if (pend) atomicAdd(&g->numPending, 1u);
-            uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * c);
-            so[0] = so[1] = make_uint4(0, 0, 0, 0);
-            uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * c);
-            bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
-            remo_node_accum* a = t.accums + c;
-            *a = remo_node_accum{};
-            // an empty leaf's roll-up record: no points, closed from the start
-            RollupSums* r = t.rollup + c;
-            for (int k = 0; k < 3; ++k) r->s1[k] = 0;
-            for (int k = 0; k < 6; ++k) r->s2[k] = 0;
-            r->n = 0;
-            r->e = 20 + D - lvl;
-            r->pad = 0;
-            t.closeAt[c] = 0;
-            markRecordChanged(t, c);
-        }
+        atomicMax(&g->maxLevel, lvl);
         uint4* bo = reinterpret_cast<uint4*>(t.batchOct + 8ull * node);
         bo[0] = bo[1] = make_uint4(0, 0, 0, 0);
         uint4* so = reinterpret_cast<uint4*>(t.storedOct + 8ull * node);
