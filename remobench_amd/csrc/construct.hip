@@ -977,6 +977,16 @@ __device__ __forceinline__ T waveExclusiveSum(T v, T& total) {
     return inc - v;
 }
 
+// The lane that owns item e of a wave whose lanes own consecutive ranges
+// starting at `before` (an exclusive prefix sum): the last lane whose range
+// starts at or below e. Every lane of the wave must take part.
+__device__ __forceinline__ uint32_t ownerLane(uint32_t before, uint32_t e) {
+    uint32_t o = 0;
+    for (uint32_t step = 32; step; step >>= 1)
+        if (uint32_t(__shfl(int(before), int(o + step))) <= e) o += step;
+    return o;
+}
+
 // allocatePointChunks + allocateVoxelChunks (remolod_octree.cu:742-792, :885-916).
 // Point chunks come from the recycled queue while the in-use count is below the
 // pool's high-water mark, from the persistent store beyond it. A wave reserves
@@ -1057,39 +1067,62 @@ __global__ __launch_bounds__(kBlock) void kAllocChunks(DevTree t) {
             if (lane == 0) waveOff = atomicAdd((unsigned long long*)&g->arenaOffset, bytesTotal);
             waveOff = __shfl(waveOff, 0);
         }
-        uint64_t off = waveOff + bytesBefore;
-        uint32_t f = 0;
-        for (uint32_t k = 0; k < add; ++k) {
-            const int64_t qi = first + k;
-            uint32_t ref = 0;
-            if (qi >= poolSize) {
-                const uint64_t o = off + uint64_t(f++) * kChunkAllocBytes;
-                if (o + kChunkAllocBytes <= t.arenaCapacity)
-                    ref = uint32_t(o >> 4);
+        // the references, a lane per chunk: the wave's point chunks are numbered
+        // 0..addTotal-1 in lane order, then its voxel chunks likewise (a loop over
+        // its own chunks per lane put a queue load and a store per chunk in one
+        // dependent chain)
+        const uint64_t offFresh = waveOff + bytesBefore;
+        for (uint32_t e0 = 0; e0 < addTotal; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const uint32_t o = ownerLane(addBefore, e);
+            const uint32_t oBefore = __shfl(addBefore, int(o)), oHave = __shfl(have, int(o));
+            const uint64_t oOff = __shfl(offFresh, int(o));
+            if (e < addTotal) {
+                const uint32_t k = e - oBefore;
+                const int64_t qi = waveFirst + int64_t(e);
+                uint32_t ref = 0;
+                if (qi >= poolSize) {
+                    // the owner's fresh chunks are its last ones, in order
+                    const int64_t oFirst = waveFirst + int64_t(oBefore);
+                    const int64_t oFreshFrom = oFirst > poolSize ? oFirst : poolSize;
+                    const uint64_t at = oOff + uint64_t(qi - oFreshFrom) * kChunkAllocBytes;
+                    if (at + kChunkAllocBytes <= t.arenaCapacity)
+                        ref = uint32_t(at >> 4);
+                    else
+                        g->allocOverflow = 1;
+                } else if (uint64_t(qi) < t.chunkQueueCap) {
+                    ref = t.chunkQueue[qi];
+                }
+                const uint32_t slot = oHave + k;
+                if (slot < kPointDir)
+                    t.pointDir[uint64_t(base + o) * kPointDir + slot] = ref;
                 else
-                    g->allocOverflow = 1;
-            } else if (uint64_t(qi) < t.chunkQueueCap) {
-                ref = t.chunkQueue[qi];
+                    g->dirOverflow = 1;
             }
-            const uint32_t slot = have + k;
-            if (slot < kPointDir)
-                t.pointDir[uint64_t(i) * kPointDir + slot] = ref;
-            else
-                g->dirOverflow = 1;
         }
-        off += uint64_t(fresh) * kChunkAllocBytes;
-        if (vAdd) {
-            for (uint32_t k = 0; k < vAdd; ++k) {
-                const uint64_t o = off + uint64_t(k) * kChunkAllocBytes;
-                const uint32_t slot = vHave + k;
-                if (o + kChunkAllocBytes > t.arenaCapacity) {
+        uint32_t vTotal;
+        const uint32_t vBefore = waveExclusiveSum(vAdd, vTotal);
+        const uint64_t offVoxel = offFresh + uint64_t(fresh) * kChunkAllocBytes;
+        for (uint32_t e0 = 0; e0 < vTotal; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const uint32_t o = ownerLane(vBefore, e);
+            const uint32_t oBefore = __shfl(vBefore, int(o)), oHave = __shfl(vHave, int(o));
+            const uint32_t oGrid = __shfl(gi, int(o));
+            const uint64_t oOff = __shfl(offVoxel, int(o));
+            if (e < vTotal) {
+                const uint32_t k = e - oBefore;
+                const uint64_t at = oOff + uint64_t(k) * kChunkAllocBytes;
+                const uint32_t slot = oHave + k;
+                if (at + kChunkAllocBytes > t.arenaCapacity) {
                     g->allocOverflow = 1;
                 } else if (slot < kVoxelDir) {
-                    t.voxelDir[uint64_t(gi) * kVoxelDir + slot] = uint32_t(o >> 4);
+                    t.voxelDir[uint64_t(oGrid) * kVoxelDir + slot] = uint32_t(at >> 4);
                 } else {
                     g->dirOverflow = 1;
                 }
             }
+        }
+        if (vAdd) {
             t.voxelChunks[gi] = vHave + vAdd;
         }
     }
