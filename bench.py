@@ -64,8 +64,10 @@ def algorithmic_bytes(scope: str, t: dict) -> float:
 
 
 def cpu_baseline(points, meta, batch: int, sample_batches: int) -> dict:
-    """The CPU oracle (a serial transcription of the reference kernels) on the
-    first `sample_batches` batches of the same workload."""
+    """The CPU oracle (a transcription of the reference kernels) on the first
+    `sample_batches` batches of the same workload: on one core, and threaded
+    over the cores this process may use (its per-point passes and per-leaf
+    folds; same results)."""
     import numpy as np
 
     from oracle.oracle import OracleTree
@@ -75,21 +77,31 @@ def cpu_baseline(points, meta, batch: int, sample_batches: int) -> dict:
     sizes = np.array([min(batch, n - i * batch) for i in range(nb)], dtype=np.uint32)
     from remobench_amd import lib as L
 
-    o = OracleTree(meta.box_size, batches_per_step=1)
     slots = points[:n]
     uniforms = L.SharedUniforms()
-    t0 = time.perf_counter()
-    done = 0
-    while done < nb:
-        done = o.build_step(slots, sizes, nb, batch, nb)
-        o.accumulate(done)
-        o.analyse(uniforms)
-    dt = time.perf_counter() - t0
-    o.close()
-    return {"value": round(n / dt / 1e6, 3), "unit": "M pts/s", "cores": 1, "kind": "port",
+
+    def timed(threads: int) -> float:
+        o = OracleTree(meta.box_size, batches_per_step=1, threads=threads)
+        t0 = time.perf_counter()
+        done = 0
+        while done < nb:
+            done = o.build_step(slots, sizes, nb, batch, nb)
+            o.accumulate(done)
+            o.analyse(uniforms)
+        dt = time.perf_counter() - t0
+        o.close()
+        return dt
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    dt1 = timed(1)
+    dtn = timed(cores)
+    sample = f"first {nb} of the workload's batches ({n:,} points), one step per batch (Update+Accumulate+Analysis)"
+    return {"value": round(n / dt1 / 1e6, 3), "unit": "M pts/s", "cores": 1, "kind": "port",
             "cpu": cpu_model(), "nproc": os.cpu_count(),
-            "sample": f"first {nb} of the workload's batches ({n:,} points), one step per batch "
-                      f"(Update+Accumulate+Analysis), serial C++ oracle, {dt:.1f} s"}
+            "sample": f"{sample}, serial C++ oracle, {dt1:.1f} s",
+            "all_cores": {"value": round(n / dtn / 1e6, 3), "unit": "M pts/s", "cores": cores, "kind": "port",
+                          "sample": f"{sample}, the oracle threaded over {cores} cores "
+                                    f"(descents, voxel sampling, leaf folds; Analysis serial), {dtn:.1f} s"}}
 
 
 # Kernels behind each timing scope (a scope launch may run several kernels).

@@ -43,6 +43,7 @@ def load():
     sig = {
         "oracle_create": (C.c_void_p, [P(C.c_float), C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
         "oracle_destroy": (None, [C.c_void_p]),
+        "oracle_set_threads": (None, [C.c_void_p, C.c_uint32]),
         "oracle_reset": (None, [C.c_void_p]),
         "oracle_build_step": (C.c_uint32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                            C.c_uint32, C.c_uint32]),
@@ -79,10 +80,15 @@ def load():
 class OracleTree:
     """The reference's construct/accumulate path, restated on the CPU."""
 
-    def __init__(self, box_size, max_nodes=200_000, persistent_bytes=1 << 62, batches_per_step=20, fixed_depth=0):
+    def __init__(self, box_size, max_nodes=200_000, persistent_bytes=1 << 62, batches_per_step=20, fixed_depth=0,
+                 threads=1):
+        """threads: threads for the per-point passes and the per-leaf folds; every
+        result is the serial one."""
         self.lib = load()
         bs = (C.c_float * 3)(*[float(v) for v in box_size])
         self.h = self.lib.oracle_create(bs, max_nodes, persistent_bytes, batches_per_step, fixed_depth)
+        if threads != 1:
+            self.lib.oracle_set_threads(self.h, threads)
 
     def close(self):
         if self.h:

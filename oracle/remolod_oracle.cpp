@@ -30,10 +30,12 @@
 //     reach below it, remolod_octree.cu:182).
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -119,6 +121,10 @@ struct oracle_tree {
     uint64_t persistentCapacity;
     uint32_t batchesPerStep;
     uint32_t fixedDepth = 0;  // 0: iterative expand; else the analytic-bucketing arm
+    // Threads for the per-point passes (descents, voxel sampling) and the
+    // per-leaf folds; 1 runs everything on the calling thread. Every result is
+    // the serial one: the passes keep point order wherever it is observable.
+    uint32_t threads = 1;
 
     std::vector<ONode> nodes;
     std::vector<std::vector<uint32_t>> grids;
@@ -147,7 +153,15 @@ struct oracle_tree {
     uint64_t mortonWatermark = 0, numAccumulated = 0, sumLeafCounts = 0;
     uint32_t pointsFolded = 0, leavesFolded = 0, innerWithSums = 0;
 
-    // per-batch scratch
+    // per-batch scratch (kept from batch to batch, so its memory is faulted in once)
+    std::vector<uint32_t> leafOf;  // leaf of each point of the batch + spill list
+    struct Visit {
+        uint32_t point, node, cell;  // a point's visit of a gridded node: the grid cell
+    };
+    std::vector<std::vector<Visit>> filed;             // threaded voxel sampling: [part][owner]
+    std::vector<std::vector<uint32_t>> filedPoints;    // threaded descents: [part][owner] point indices
+    std::vector<std::vector<remo_point>> freshVoxels;  // [owner] its new voxels, in point order
+    std::vector<std::vector<uint32_t>> freshNodes;
     std::vector<remo_point> spilled;
     std::vector<uint32_t> spilling;
     std::vector<remo_point> backlogVoxels;
@@ -233,25 +247,87 @@ struct oracle_tree {
         return cur;
     }
 
+    // Parts a pass over `n` items is cut into (one per thread; small passes stay
+    // on the calling thread).
+    uint32_t partsFor(size_t n) const { return n < 16384 ? 1u : std::max(threads, 1u); }
+
+    // f(part) for part in [0, parts), a thread each.
+    template <class F>
+    static void eachPart(uint32_t parts, F&& f) {
+        if (parts <= 1) {
+            f(0u);
+            return;
+        }
+        std::vector<std::thread> pool;
+        pool.reserve(parts);
+        for (uint32_t k = 0; k < parts; ++k) pool.emplace_back([&f, k] { f(k); });
+        for (std::thread& th : pool) th.join();
+    }
+
+    // The part that owns node id in a threaded pass (a multiplicative hash: ids
+    // modulo the part count would follow the octant, children being allocated
+    // in blocks of 8, and a scene's points crowd into few octants).
+    static uint32_t ownerOf(uint32_t id, uint32_t parts) { return (uint32_t(id * 0x9E3779B1u) >> 8) % parts; }
+
+    // Item i of a batch pass: the batch's points, then the spill list.
+    const remo_point& batchPoint(const remo_point* pts, uint32_t n, size_t i) const {
+        return i < n ? pts[i] : spilled[i - n];
+    }
+
+    // leafOf[i] = the leaf item i descends to, for the first `total` items.
+    // Threaded, each part also files its items under the part that owns their
+    // leaf, so that an owner can visit its leaves' items in order.
+    void descendAll(const remo_point* pts, uint32_t n, size_t total) {
+        leafOf.resize(total);
+        const uint32_t P = partsFor(total);
+        if (P > 1) filedPoints.resize(size_t(P) * P);
+        eachPart(P, [&](uint32_t part) {
+            if (P > 1)
+                for (uint32_t owner = 0; owner < P; ++owner) filedPoints[size_t(part) * P + owner].clear();
+            for (size_t i = total * part / P, e = total * (part + 1) / P; i < e; ++i) {
+                uint32_t q20[3], q28[3];
+                quantise(batchPoint(pts, n, i), octreeSize, q20, q28);
+                leafOf[i] = descend(q20);
+                if (P > 1) filedPoints[size_t(part) * P + ownerOf(leafOf[i], P)].push_back(uint32_t(i));
+            }
+        });
+    }
+
+    // f(i) for every item the descent filed under `owner`, in item order.
+    template <class F>
+    void ownedItems(uint32_t P, uint32_t owner, F&& f) const {
+        if (P == 1) {
+            for (size_t i = 0; i < leafOf.size(); ++i) f(i);
+            return;
+        }
+        for (uint32_t part = 0; part < P; ++part)
+            for (uint32_t i : filedPoints[size_t(part) * P + owner]) f(size_t(i));
+    }
+
     // doCounting (remolod_octree.cu:142-301). Serial: the warp-aggregated
     // atomicAdd becomes one increment per point, which crosses 50,000 exactly
     // once per leaf, as the grouped version does.
     bool doCounting(const remo_point* pts, uint32_t n, uint32_t countIteration) {
         spilling.clear();
-        auto countPoint = [&](const remo_point& p) {
-            uint32_t q20[3], q28[3];
-            quantise(p, octreeSize, q20, q28);
-            ONode& leaf = nodes[descend(q20)];
-            if (leaf.countIteration < countIteration) {
-                const uint32_t old = leaf.counter;
-                leaf.counter = old + 1;
-                if (old <= kMaxPointsPerNode && old + 1 > kMaxPointsPerNode)
-                    spilling.push_back(uint32_t(&leaf - nodes.data()));
-            }
-        };
-        for (uint32_t i = 0; i < n; ++i) countPoint(pts[i]);
-        const size_t nSpilledBefore = spilled.size();
-        for (size_t i = 0; i < nSpilledBefore; ++i) countPoint(spilled[i]);
+        // the descents in parallel; then each part increments the counters of
+        // the leaves it owns in point order
+        const size_t total = size_t(n) + spilled.size();
+        descendAll(pts, n, total);
+        const uint32_t P = partsFor(total);
+        std::vector<std::vector<uint32_t>> crossed(P);
+        eachPart(P, [&](uint32_t part) {
+            ownedItems(P, part, [&](size_t i) {
+                const uint32_t id = leafOf[i];
+                ONode& leaf = nodes[id];
+                if (leaf.countIteration < countIteration) {
+                    const uint32_t old = leaf.counter;
+                    leaf.counter = old + 1;
+                    if (old <= kMaxPointsPerNode && old + 1 > kMaxPointsPerNode) crossed[part].push_back(id);
+                }
+            });
+        });
+        // (the list is put in key order below)
+        for (const std::vector<uint32_t>& c : crossed) spilling.insert(spilling.end(), c.begin(), c.end());
 
         // Split candidates in canonical key order; the node pool accepts a prefix
         // (the reference accepts a prefix of its atomic order, :309-321).
@@ -440,8 +516,71 @@ struct oracle_tree {
                 cur = uint32_t(child);
             }
         };
-        for (uint32_t i = 0; i < n; ++i) traverse(pts[i]);
-        for (size_t i = 0; i < spilled.size(); ++i) traverse(spilled[i]);
+        const size_t total = size_t(n) + spilled.size();
+        const uint32_t P = partsFor(total);
+        if (P == 1) {
+            for (uint32_t i = 0; i < n; ++i) traverse(pts[i]);
+            for (size_t i = 0; i < spilled.size(); ++i) traverse(spilled[i]);
+            return;
+        }
+        // Threaded: each part descends a contiguous range of the points and files
+        // every (point, node, cell) it visits under the node's owner part; each owner then replays its visits part by part, so a
+        // node's cells are tested in point order and the first point to reach a
+        // cell still gives the voxel its colour; the new voxels of a node reach
+        // the backlog in point order, which is all the insertion reads.
+        filed.resize(size_t(P) * P);
+        freshVoxels.resize(P);
+        freshNodes.resize(P);
+        eachPart(P, [&](uint32_t part) {
+            for (uint32_t owner = 0; owner < P; ++owner) filed[size_t(part) * P + owner].clear();
+            for (size_t i = total * part / P, e = total * (part + 1) / P; i < e; ++i) {
+                uint32_t q20[3], q28[3];
+                quantise(batchPoint(pts, n, i), octreeSize, q20, q28);
+                uint32_t cur = 0;
+                for (int level = 0; level < kMaxDepth; ++level) {
+                    const ONode& nd = nodes[cur];
+                    if (nd.grid >= 0) {
+                        const uint32_t div = 1u << ((kMaxDepth + 1) - nd.level);
+                        const uint32_t cx = (q28[0] / div) % kGridSize;
+                        const uint32_t cy = (q28[1] / div) % kGridSize;
+                        const uint32_t cz = (q28[2] / div) % kGridSize;
+                        filed[size_t(part) * P + ownerOf(cur, P)].push_back(
+                            Visit{uint32_t(i), cur, cx + cy * kGridSize + cz * kGridSize * kGridSize});
+                    }
+                    const uint32_t cx = (q20[0] >> (kMaxDepth - level - 1)) & 1u;
+                    const uint32_t cy = (q20[1] >> (kMaxDepth - level - 1)) & 1u;
+                    const uint32_t cz = (q20[2] >> (kMaxDepth - level - 1)) & 1u;
+                    const int32_t child = nd.children[(cx << 2) | (cy << 1) | cz];
+                    if (child < 0) break;
+                    cur = uint32_t(child);
+                }
+            }
+        });
+        eachPart(P, [&](uint32_t owner) {
+            freshVoxels[owner].clear();
+            freshNodes[owner].clear();
+            for (uint32_t part = 0; part < P; ++part) {
+                for (const Visit& v : filed[size_t(part) * P + owner]) {
+                    ONode& nd = nodes[v.node];
+                    uint32_t& word = grids[size_t(nd.grid)][v.cell / 32];
+                    const uint32_t mask = 1u << (v.cell % 32);
+                    if (word & mask) continue;
+                    word |= mask;
+                    nd.numVoxels += 1;
+                    remo_point vox = cellCentre(nd, v.cell % kGridSize, (v.cell / kGridSize) % kGridSize,
+                                                v.cell / (kGridSize * kGridSize));
+                    vox.color = batchPoint(pts, n, v.point).color;
+                    freshVoxels[owner].push_back(vox);
+                    freshNodes[owner].push_back(v.node);
+                }
+            }
+        });
+        for (uint32_t owner = 0; owner < P; ++owner) {
+            backlogVoxels.insert(backlogVoxels.end(), freshVoxels[owner].begin(), freshVoxels[owner].end());
+            backlogTargets.insert(backlogTargets.end(), freshNodes[owner].begin(), freshNodes[owner].end());
+            newVoxels += freshNodes[owner].size();
+            totNewVoxels += freshNodes[owner].size();
+        }
     }
 
     // allocatePointChunks (remolod_octree.cu:742-792)
@@ -473,25 +612,33 @@ struct oracle_tree {
 
     // insertPoints (remolod_octree.cu:794-883)
     void insertPoints(const remo_point* pts, uint32_t n) {
-        auto insert = [&](const remo_point& p) {
-            uint32_t q20[3], q28[3];
-            quantise(p, octreeSize, q20, q28);
-            ONode& leaf = nodes[descend(q20)];
-            leaf.points.push_back(p);
-            leaf.numPoints += 1;
-        };
-        for (uint32_t i = 0; i < n; ++i) insert(pts[i]);
+        // the descents in parallel; then each part appends to the leaves it
+        // owns, in point order
+        const size_t total = size_t(n) + spilled.size();
+        descendAll(pts, n, total);
         if (spilled.size() > size_t(kSpillDropIndex) + 1) spillOverflow = true;
-        for (size_t i = 0; i < spilled.size(); ++i) insert(spilled[i]);
+        const uint32_t P = partsFor(total);
+        eachPart(P, [&](uint32_t part) {
+            ownedItems(P, part, [&](size_t i) {
+                ONode& leaf = nodes[leafOf[i]];
+                leaf.points.push_back(batchPoint(pts, n, i));
+                leaf.numPoints += 1;
+            });
+        });
     }
 
     // insertVoxels (remolod_octree.cu:918-938)
     void insertVoxels() {
-        for (size_t i = 0; i < backlogVoxels.size(); ++i) {
-            ONode& t = nodes[backlogTargets[i]];
-            t.voxels.push_back(backlogVoxels[i]);
-            t.numVoxelsStored += 1;
-        }
+        // each part appends to the nodes it owns, in backlog order
+        const uint32_t P = partsFor(backlogVoxels.size());
+        eachPart(P, [&](uint32_t part) {
+            for (size_t i = 0; i < backlogVoxels.size(); ++i) {
+                if (ownerOf(backlogTargets[i], P) != part) continue;
+                ONode& t = nodes[backlogTargets[i]];
+                t.voxels.push_back(backlogVoxels[i]);
+                t.numVoxelsStored += 1;
+            }
+        });
     }
 
     // Refinement's local deepening (README.md:81, wiki/02_KernelPipeline.md:232-236;
@@ -868,63 +1015,91 @@ struct oracle_tree {
 
     // kernel_accumulate (kernels/remolod/remolod_accum.cu:47-212) + the build's
     // node-centred int64 fixed point, and kernel_accum_verify (:214-233).
+    // Folds leaf nd's new points into its accumulator; returns how many it folded
+    // (0: nothing new, or an inner node, whose sums are cleared) and raises mmax
+    // to the largest Morton code among them.
+    uint32_t foldNode(ONode& nd, uint32_t batchIndex, uint64_t& mmax) const {
+        const double invSize = 1.0 / double(octreeSize);
+        const double fGrid = double(1u << kMaxDepth);
+        remo_node_accum& a = nd.acc;
+        if (!nd.isLeaf()) {
+            if (a.count != 0) std::memset(&a, 0, sizeof(a));
+            return 0;
+        }
+        const uint32_t from = a.count, to = nd.numPoints;
+        if (to <= from) return 0;
+        const double levelScale = double(1ull << nd.level);
+        for (uint32_t i = from; i < to; ++i) {
+            const remo_point& p = nd.points[i];
+            volatile double lx = (double(p.x) - 0.0) * invSize;
+            volatile double ly = (double(p.y) - 0.0) * invSize;
+            volatile double lz = (double(p.z) - 0.0) * invSize;
+            const uint32_t maxCoord = (1u << kMaxDepth) - 1u;
+            const uint32_t X = std::min(uint32_t(fGrid * lx), maxCoord);
+            const uint32_t Y = std::min(uint32_t(fGrid * ly), maxCoord);
+            const uint32_t Z = std::min(uint32_t(fGrid * lz), maxCoord);
+            const uint64_t m = (mortonSpread(X) << 2) | (mortonSpread(Y) << 1) | mortonSpread(Z);
+            mmax = std::max(mmax, m);
+            double x = lx * levelScale, y = ly * levelScale, z = lz * levelScale;
+            x = x - double(nd.X);
+            y = y - double(nd.Y);
+            z = z - double(nd.Z);
+            x = std::fmin(std::fmax(x, 0.0), 1.0);
+            y = std::fmin(std::fmax(y, 0.0), 1.0);
+            z = std::fmin(std::fmax(z, 0.0), 1.0);
+            a.g[0] += x; a.g[1] += y; a.g[2] += z;
+            a.g[3] += x * x; a.g[4] += x * y; a.g[5] += x * z;
+            a.g[6] += y * y; a.g[7] += y * z; a.g[8] += z * z;
+            const float cr = float((p.color >> 0) & 0xFFu) * (1.0f / 255.0f);
+            const float cg = float((p.color >> 8) & 0xFFu) * (1.0f / 255.0f);
+            const float cb = float((p.color >> 16) & 0xFFu) * (1.0f / 255.0f);
+            a.c[0] += cr; a.c[1] += cg; a.c[2] += cb;
+            a.c[3] += cr * cr; a.c[4] += cg * cg; a.c[5] += cb * cb;
+            // fixed point: q = floor(l * 2^20) - 2^19, l in [0,1]
+            const int64_t qx = int64_t(std::floor(x * 1048576.0)) - 524288;
+            const int64_t qy = int64_t(std::floor(y * 1048576.0)) - 524288;
+            const int64_t qz = int64_t(std::floor(z * 1048576.0)) - 524288;
+            a.q1[0] += qx; a.q1[1] += qy; a.q1[2] += qz;
+            a.q2[0] += qx * qx; a.q2[1] += qx * qy; a.q2[2] += qx * qz;
+            a.q2[3] += qy * qy; a.q2[4] += qy * qz; a.q2[5] += qz * qz;
+        }
+        a.count = to;
+        a.last_touched_batch = batchIndex;
+        return to - from;
+    }
+
     void accumulate(uint32_t batchIndex) {
         pointsFolded = leavesFolded = 0;
         sumLeafCounts = 0;
         innerWithSums = 0;
-        const double invSize = 1.0 / double(octreeSize);
-        const double fGrid = double(1u << kMaxDepth);
-        for (ONode& nd : nodes) {
-            remo_node_accum& a = nd.acc;
-            if (!nd.isLeaf()) {
-                if (a.count != 0) std::memset(&a, 0, sizeof(a));
-                continue;
+        // leaves in parallel (a shared cursor over blocks of 16 nodes: a batch's
+        // points crowd into few leaves), the totals combined after
+        struct Part {
+            uint64_t mmax = 0, points = 0;
+            uint32_t leaves = 0;
+        };
+        const uint32_t P = partsFor(size_t(nodes.size()) * 1024u);
+        std::vector<Part> parts(P);
+        std::atomic<size_t> cursor{0};
+        eachPart(P, [&](uint32_t part) {
+            Part& r = parts[part];
+            for (size_t b = cursor.fetch_add(16); b < nodes.size(); b = cursor.fetch_add(16)) {
+                for (size_t k = b, e = std::min(b + 16, nodes.size()); k < e; ++k) {
+                    uint64_t mmax = 0;
+                    const uint32_t folded = foldNode(nodes[k], batchIndex, mmax);
+                    if (!folded) continue;
+                    nodes[k].acc.morton_max = std::max(nodes[k].acc.morton_max, mmax);
+                    r.mmax = std::max(r.mmax, mmax);
+                    r.points += folded;
+                    r.leaves += 1;
+                }
             }
-            const uint32_t from = a.count, to = nd.numPoints;
-            if (to <= from) continue;
-            const double levelScale = double(1ull << nd.level);
-            uint64_t mmax = 0;
-            for (uint32_t i = from; i < to; ++i) {
-                const remo_point& p = nd.points[i];
-                volatile double lx = (double(p.x) - 0.0) * invSize;
-                volatile double ly = (double(p.y) - 0.0) * invSize;
-                volatile double lz = (double(p.z) - 0.0) * invSize;
-                const uint32_t maxCoord = (1u << kMaxDepth) - 1u;
-                const uint32_t X = std::min(uint32_t(fGrid * lx), maxCoord);
-                const uint32_t Y = std::min(uint32_t(fGrid * ly), maxCoord);
-                const uint32_t Z = std::min(uint32_t(fGrid * lz), maxCoord);
-                const uint64_t m = (mortonSpread(X) << 2) | (mortonSpread(Y) << 1) | mortonSpread(Z);
-                mmax = std::max(mmax, m);
-                double x = lx * levelScale, y = ly * levelScale, z = lz * levelScale;
-                x = x - double(nd.X);
-                y = y - double(nd.Y);
-                z = z - double(nd.Z);
-                x = std::fmin(std::fmax(x, 0.0), 1.0);
-                y = std::fmin(std::fmax(y, 0.0), 1.0);
-                z = std::fmin(std::fmax(z, 0.0), 1.0);
-                a.g[0] += x; a.g[1] += y; a.g[2] += z;
-                a.g[3] += x * x; a.g[4] += x * y; a.g[5] += x * z;
-                a.g[6] += y * y; a.g[7] += y * z; a.g[8] += z * z;
-                const float cr = float((p.color >> 0) & 0xFFu) * (1.0f / 255.0f);
-                const float cg = float((p.color >> 8) & 0xFFu) * (1.0f / 255.0f);
-                const float cb = float((p.color >> 16) & 0xFFu) * (1.0f / 255.0f);
-                a.c[0] += cr; a.c[1] += cg; a.c[2] += cb;
-                a.c[3] += cr * cr; a.c[4] += cg * cg; a.c[5] += cb * cb;
-                // fixed point: q = floor(l * 2^20) - 2^19, l in [0,1]
-                const int64_t qx = int64_t(std::floor(x * 1048576.0)) - 524288;
-                const int64_t qy = int64_t(std::floor(y * 1048576.0)) - 524288;
-                const int64_t qz = int64_t(std::floor(z * 1048576.0)) - 524288;
-                a.q1[0] += qx; a.q1[1] += qy; a.q1[2] += qz;
-                a.q2[0] += qx * qx; a.q2[1] += qx * qy; a.q2[2] += qx * qz;
-                a.q2[3] += qy * qy; a.q2[4] += qy * qz; a.q2[5] += qz * qz;
-            }
-            a.count = to;
-            a.last_touched_batch = batchIndex;
-            a.morton_max = std::max(a.morton_max, mmax);
-            mortonWatermark = std::max(mortonWatermark, mmax);
-            numAccumulated += to - from;
-            pointsFolded += to - from;
-            leavesFolded += 1;
+        });
+        for (const Part& r : parts) {
+            mortonWatermark = std::max(mortonWatermark, r.mmax);
+            numAccumulated += r.points;
+            pointsFolded += uint32_t(r.points);
+            leavesFolded += r.leaves;
         }
         for (const ONode& nd : nodes) {
             if (nd.isLeaf())
@@ -1336,6 +1511,8 @@ oracle_tree* oracle_create(const float boxSize[3], uint32_t maxNodes, uint64_t p
 }
 
 void oracle_destroy(oracle_tree* t) { delete t; }
+// Threads for the per-point passes and the per-leaf folds (results unchanged).
+void oracle_set_threads(oracle_tree* t, uint32_t threads) { t->threads = threads ? threads : 1u; }
 void oracle_reset(oracle_tree* t) { t->reset(); }
 
 uint32_t oracle_build_step(oracle_tree* t, const remo_point* slots, const uint32_t* batchSizes,
