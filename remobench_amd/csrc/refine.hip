@@ -712,9 +712,9 @@ int runRefine(RefineState& r, const DevTree& t, uint32_t numNodes, float budgetM
     // not depend on each other; the rest stay dirty); at least one node runs. The
     // kernels get the budget less the call's own cost (the marking kernels, two
     // readbacks, the launches: a decaying peak of what recent calls measured),
-    // with 15 % kept back for the model's error, so the call's wall time holds it.
+    // with 20 % kept back for the model's error, so the call's wall time holds it.
     const double kernelBudget =
-        budgetMs > 0.0f ? std::max(0.0, (double(budgetMs) - r.overheadMs) * 0.85) : double(budgetMs);
+        budgetMs > 0.0f ? std::max(0.0, (double(budgetMs) - r.overheadMs) * 0.80) : double(budgetMs);
     double est = 0.0;
     int lo = deepest;
     uint32_t taken = 0, lastCount = 0;
