@@ -240,8 +240,8 @@ def main() -> int:
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        log("bench.py measures one GPU: there is no multi-GPU path (a tree is built on one GPU; "
-            "a larger cloud would be cut into spatial shards, one tree per GPU); run with --gpus 1")
+        log("bench.py measures one GPU: there is no multi-GPU path (several GPUs would each own the "
+            "subtrees below a fixed octree level by Morton prefix); run with --gpus 1")
         return 2
 
     from remobench_amd import lib as L
