@@ -782,7 +782,7 @@ __device__ __forceinline__ void flushStage(const DevTree& t, VoxelStage& st, uin
         const uint64_t slot = uint64_t(shard) * t.backlogSegCap + base + k;
         if (held) {
             t.backlogPts[slot] = st.pts[k];
-            t.backlogNode[slot] = h ? node1 : node0;
+            t.backlogGrid[slot] = h ? grid1 : grid0;
             t.backlogRank[slot] = h ? gb1 + rank1 : gb0 + rank0;
         } else if (in) {
             // a dropped voxel still counts (no backlog entry will)
@@ -1217,6 +1217,8 @@ __global__ __launch_bounds__(kBlock) void kInsertPoints(DevTree t, BatchRef b, u
 
 // insertVoxels (remolod_octree.cu:918-938): every backlog entry goes to slot
 // start(grid, segment) + rank, both taken before (the voxel flush, kAllocChunks).
+// An entry carries its node's grid number, not the node (one dependent load
+// fewer between the entry and its store).
 __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, uint32_t closeBatch) {
     __shared__ uint32_t prefix[kBacklogShards + 1];
     Globals* g = t.g;
@@ -1241,7 +1243,7 @@ __global__ __launch_bounds__(kBlock) void kInsertVoxels(DevTree t, BatchRef b, u
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         const uint64_t at = backlogSlot(t, prefix, i);
-        const uint32_t gi = t.gridNo[t.backlogNode[at]];
+        const uint32_t gi = t.backlogGrid[at];
         if (gi == kNone) continue;  // not reached: sampled nodes have grids
         const uint32_t seg = uint32_t(at / t.backlogSegCap);
         const uint32_t slot = t.voxelShardStart[uint64_t(gi) * kBacklogShards + seg] + t.backlogRank[at];

@@ -440,7 +440,7 @@ int remo_create(const remo_config* cfgIn, const remo_cloud_meta* meta, remo_ctx*
     A(&t.spill, t.spillCap, "spilled points");
     A(&t.pointLeaf, uint64_t(cfg.slot_capacity) + t.spillCap, "the leaf of each point of a batch");
     A(&t.backlogPts, t.backlogCap, "the voxel backlog");
-    A(&t.backlogNode, t.backlogCap, "the voxel backlog targets");
+    A(&t.backlogGrid, t.backlogCap, "the voxel backlog targets (grid numbers)");
     A(&t.backlogFill, uint64_t(kBacklogShards) * kShardStride, "the voxel backlog segment counters");
     A(&t.backlogRank, t.backlogCap, "the voxel backlog ranks");
     A(&t.voxelShardCount, uint64_t(kBacklogShards) * t.maxGrids, "voxel counts per grid and backlog segment");

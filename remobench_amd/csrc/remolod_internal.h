@@ -139,7 +139,7 @@ struct DevTree {
     uint32_t spillCap;
     uint32_t backlogCap;
     remo_point* backlogPts;  // [backlogCap]: segment s holds [s * backlogSegCap, (s + 1) * backlogSegCap)
-    uint32_t* backlogNode;
+    uint32_t* backlogGrid;   // [backlogCap] the grid number of each entry's node
     uint32_t* backlogFill;   // [kBacklogShards * kShardStride] entries reserved per segment
     uint32_t backlogSegCap;  // backlogCap / kBacklogShards
     uint32_t* backlogRank;   // [backlogCap] an entry's rank among its segment's entries of its grid
