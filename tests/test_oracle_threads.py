@@ -45,3 +45,18 @@ def test_threaded_oracle_matches_serial():
     # the build split nodes, spilled points and sampled voxels
     s = serial.stats()
     assert s["total_nodes_split"] > 8 and s["total_spilled_points"] > 0 and s["total_new_voxels"] > 0
+
+    def same(what):
+        a, b = _state(serial), _state(threaded)
+        assert a[:3] == b[:3], f"stats, node table or accumulators differ after {what}"
+        for i, (x, y) in enumerate(zip(a[3], b[3])):
+            assert x == y, f"node {i}: points, voxels or grid differ after {what}"
+
+    # Refinement's deepening and collapse run the same passes (the spill list
+    # descended, sampled and inserted with an empty batch)
+    d1, d2 = serial.deepen(8, 20_000), threaded.deepen(8, 20_000)
+    assert d1 == d2 and d1["splits"] > 0 and d1["points_moved"] >= 16_384
+    same("deepening")
+    c1, c2 = serial.collapse(4, 50_000), threaded.collapse(4, 50_000)
+    assert c1 == c2 and c1["collapsed"] > 0 and c1["points_moved"] >= 16_384
+    same("collapse")
